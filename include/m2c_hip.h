@@ -1,0 +1,136 @@
+/*
+ * m2c_hip.h -- C ABI of the MI355X-native explicit finite-volume update.
+ *
+ * Plain C linkage, plain pointers and sizes. Every call returns an m2c_status code (0 = success)
+ * instead of calling exit() as the reference does; fatal numerical conditions of the reference
+ * are reported through m2c_get_status().
+ *
+ * Host arrays crossing this ABI use the reference's natural layout without ghosts:
+ * k-j-i order, x fastest, 5 primitive variables (rho, u, v, w, p) innermost for V
+ * (the interior of SpaceVariable3D, SpaceVariable.cpp:254-260). Material IDs are int32.
+ * On the device the context stores structure-of-arrays fields, x fastest, no ghost frame:
+ * ghost values are produced inside the kernels from the boundary rules.
+ */
+#ifndef M2C_HIP_H
+#define M2C_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M2C_MAX_MAT 8
+
+enum m2c_status_code {
+  M2C_OK = 0,
+  M2C_ERR_ARG = 1,        /* invalid argument or configuration */
+  M2C_ERR_HIP = 2,        /* HIP runtime error */
+  M2C_ERR_STATE = 3,      /* non-hyperbolic state (the reference exits: SpaceOperator.cpp:624-631) */
+  M2C_ERR_UNSUPPORTED = 4 /* option recognised but not provided by this library */
+};
+
+/* Equation-of-state kinds, VarFcnBase.h:40-41 */
+enum m2c_eos_kind { M2C_EOS_SG = 0, M2C_EOS_NASG = 1, M2C_EOS_MG = 2, M2C_EOS_JWL = 3, M2C_EOS_DUMMY = 5 };
+/* Boundary codes, IoData.h:266-267, plus PERIODIC (no reference counterpart) */
+enum m2c_bc { M2C_BC_INLET = 1, M2C_BC_OUTLET = 2, M2C_BC_SLIPWALL = 3, M2C_BC_STICKWALL = 4,
+              M2C_BC_SYMMETRY = 5, M2C_BC_OVERSET = 6, M2C_BC_PERIODIC = 7 };
+/* NavierStokes flux, IoData.h (SchemeData::Flux): ROE=0, LLF=1, HLLC=2, GODUNOV=3 */
+enum m2c_flux { M2C_FLUX_ROE = 0, M2C_FLUX_LLF = 1, M2C_FLUX_HLLC = 2, M2C_FLUX_GODUNOV = 3 };
+enum m2c_limiter { M2C_LIM_NONE = 0, M2C_LIM_GENMINMOD = 1, M2C_LIM_VANALBADA = 2 };
+enum m2c_ts { M2C_TS_FE = 0, M2C_TS_RK2 = 1, M2C_TS_RK3 = 2 };
+enum m2c_precision { M2C_PREC_FP64 = 0, M2C_PREC_MIXED = 1 };
+
+typedef struct {
+  int eos;                          /* m2c_eos_kind */
+  double rhomin, pmin, rhomax, pmax, failsafe_density; /* MaterialModelData, IoData.cpp:860-870 */
+  double p[8];                      /* SG: gamma, p_inf. NASG: gamma, pc, b, q.
+                                       MG: rho0, c0, Gamma0, s, e0. JWL: omega, A1, A2, R1, R2, rho0.
+                                       DUMMY: p0, rho0, e0. */
+} m2c_material;
+
+typedef struct {
+  int maxIts_main, maxIts_bracket, maxIts_shock, numSteps_rarefaction;
+  double tol_main, tol_shock, tol_rarefaction, min_pressure, failure_threshold, pressure_at_failure;
+  double surrogate_residual_tol;
+} m2c_riemann_params;                /* ExactRiemannSolverData, IoData.cpp:1345-1357 */
+
+typedef struct {
+  int nx, ny, nz;                   /* uniform Cartesian mesh (MeshGenerator.cpp:146-158) */
+  double x0, x1, y0, y1, z0, z1;
+  int bc[6];                        /* x0, xmax, y0, ymax, z0, zmax */
+  double inlet[5], outlet[5];       /* BoundaryConditions.Inlet / Outlet (primitive) */
+  int nmat;
+  int inactive_id;                  /* -1 when there is no inactive (dummy) material */
+  m2c_material mat[M2C_MAX_MAT];
+  int flux;                         /* m2c_flux */
+  double roe_delta;                 /* Harten entropy-fix coefficient */
+  int rec_linear;                   /* 0: constant, 1: linear reconstruction */
+  int limiter;                      /* m2c_limiter */
+  double gminmod_alpha;
+  int slope_zero_near_interface;
+  int ts;                           /* m2c_ts */
+  double cfl;
+  int mp_flux, mp_recon;            /* MultiPhase.Flux (0 numerical, 1 exact, 2 LLF), Recon (0/1) */
+  m2c_riemann_params riemann;
+  int precision;                    /* m2c_precision */
+  int device;                       /* HIP device ordinal */
+} m2c_config;
+
+typedef struct {
+  int error;                        /* sticky m2c_status_code of device-detected conditions */
+  long long reverted_faces;         /* reconstructed face states reverted to the cell value */
+  long long clipped_cells;
+  long long riemann_errors;
+  long long steps;
+  double t, dt, cfl;
+} m2c_status;
+
+typedef struct m2c_ctx m2c_ctx;
+
+/* Context life cycle. Replaces the SpaceOperator/Reconstructor/VarFcn/FluxFcn factories of
+ * Main.cpp:116-182 and the DataManagers3D allocations (SpaceVariable.cpp:49-156). */
+int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out);
+int m2c_ctx_destroy(m2c_ctx *ctx);
+
+/* State transfer (V: nz*ny*nx*5 doubles, id: nz*ny*nx int32 or NULL for all-zero). */
+int m2c_upload_state(m2c_ctx *ctx, const double *V, const int32_t *id);
+int m2c_download_state(m2c_ctx *ctx, double *V, int32_t *id);
+
+/* SpaceOperator::ComputeResidual (SpaceOperator.h:136-140) on the current state;
+ * R: nz*ny*nx*5 doubles (already divided by -volume, SpaceOperator.cpp:3394-3398). */
+int m2c_compute_residual(m2c_ctx *ctx, double *R);
+
+/* SpaceOperator::ComputeTimeStepSize (SpaceOperator.h:129-130). */
+int m2c_time_step_size(m2c_ctx *ctx, double *dt, double *cfl);
+
+/* TimeIntegratorBase::AdvanceOneTimeStep (TimeIntegrator.h:66-73) with the given dt. */
+int m2c_advance_dt(m2c_ctx *ctx, double dt);
+
+/* Main.cpp:514-621: repeat {dt = CFL dt; clamp to tmax; advance} while t < tmax and
+ * steps < maxsteps. Time and dt stay on the device; the host synchronises every
+ * `sync_every` steps (0 = only at the end). */
+int m2c_run(m2c_ctx *ctx, double tmax, int maxsteps, int sync_every);
+
+int m2c_get_status(m2c_ctx *ctx, m2c_status *st);
+int m2c_reset_time(m2c_ctx *ctx, double t);
+int m2c_synchronize(m2c_ctx *ctx);
+
+/* FluxFcnBase::ComputeNumericalFluxAtCellInterface (FluxFcnBase.h:65-69) for n face pairs
+ * (Vm, Vp: n*5 doubles, F: n*5 doubles), evaluated by the device kernels. */
+int m2c_flux_batch(m2c_ctx *ctx, int dir, int n, const double *Vm, const double *Vp, int id, double *F);
+
+/* Timing helper: runs `nsteps` CFL-adaptive time steps (no tmax) and reports the average
+ * duration (ms) of one fused stage-kernel launch and of one whole step, measured with HIP events
+ * recorded on the context's stream around every stage launch and around the whole loop. */
+int m2c_time_steps(m2c_ctx *ctx, int nsteps, double *ms_stage_kernel, double *ms_step);
+
+/* Raw device pointer of the primitive state (SoA, 5*nz*ny*nx doubles) for zero-copy callers. */
+int m2c_device_state(m2c_ctx *ctx, double **V_soa);
+
+const char *m2c_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,222 @@
+"""m2c_amd -- MI355X-native explicit finite-volume update of M2C.
+
+Python front end of the C ABI declared in include/m2c_hip.h (libm2c_hip.so, built in-tree for
+gfx950 by m2c_amd/build.py). The compute path is the HIP library; this module only marshals
+configuration and host arrays. Importing it without the built library raises ImportError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libm2c_hip.so")
+
+EOS_SG, EOS_NASG, EOS_MG, EOS_JWL, EOS_DUMMY = 0, 1, 2, 3, 5
+BC_INLET, BC_OUTLET, BC_SLIPWALL, BC_STICKWALL, BC_SYMMETRY, BC_OVERSET, BC_PERIODIC = 1, 2, 3, 4, 5, 6, 7
+FLUX_ROE, FLUX_LLF, FLUX_HLLC, FLUX_GODUNOV = 0, 1, 2, 3
+LIM_NONE, LIM_GENMINMOD, LIM_VANALBADA = 0, 1, 2
+TS_FE, TS_RK2, TS_RK3 = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+DBL_MAX = 1.7976931348623157e308
+
+
+class Material(C.Structure):
+    _fields_ = [("eos", C.c_int), ("rhomin", C.c_double), ("pmin", C.c_double), ("rhomax", C.c_double),
+                ("pmax", C.c_double), ("failsafe_density", C.c_double), ("p", C.c_double * 8)]
+
+
+class RiemannParams(C.Structure):
+    _fields_ = [("maxIts_main", C.c_int), ("maxIts_bracket", C.c_int), ("maxIts_shock", C.c_int),
+                ("numSteps_rarefaction", C.c_int)] + [(n, C.c_double) for n in (
+                    "tol_main", "tol_shock", "tol_rarefaction", "min_pressure", "failure_threshold",
+                    "pressure_at_failure", "surrogate_residual_tol")]
+
+
+class Config(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("x0", C.c_double), ("x1", C.c_double), ("y0", C.c_double), ("y1", C.c_double),
+                ("z0", C.c_double), ("z1", C.c_double),
+                ("bc", C.c_int * 6), ("inlet", C.c_double * 5), ("outlet", C.c_double * 5),
+                ("nmat", C.c_int), ("inactive_id", C.c_int), ("mat", Material * 8),
+                ("flux", C.c_int), ("roe_delta", C.c_double), ("rec_linear", C.c_int),
+                ("limiter", C.c_int), ("gminmod_alpha", C.c_double),
+                ("slope_zero_near_interface", C.c_int), ("ts", C.c_int), ("cfl", C.c_double),
+                ("mp_flux", C.c_int), ("mp_recon", C.c_int), ("riemann", RiemannParams),
+                ("precision", C.c_int), ("device", C.c_int)]
+
+
+class Status(C.Structure):
+    _fields_ = [("error", C.c_int), ("reverted_faces", C.c_longlong), ("clipped_cells", C.c_longlong),
+                ("riemann_errors", C.c_longlong), ("steps", C.c_longlong), ("t", C.c_double),
+                ("dt", C.c_double), ("cfl", C.c_double)]
+
+
+def material(kind, params, rhomin=0.0, pmin=-DBL_MAX, rhomax=DBL_MAX, pmax=DBL_MAX, failsafe_density=1e-10):
+    """MaterialModelData (IoData.cpp:860-906): EOS kind + parameters + cut-offs."""
+    m = Material()
+    m.eos = kind
+    m.rhomin, m.pmin, m.rhomax, m.pmax, m.failsafe_density = rhomin, pmin, rhomax, pmax, failsafe_density
+    for i, v in enumerate(params):
+        m.p[i] = v
+    return m
+
+
+def riemann_defaults():
+    """ExactRiemannSolverData defaults (IoData.cpp:1345-1357)."""
+    r = RiemannParams()
+    r.maxIts_main, r.maxIts_bracket, r.maxIts_shock, r.numSteps_rarefaction = 200, 100, 200, 200
+    r.tol_main, r.tol_shock, r.tol_rarefaction = 1e-4, 1e-12, 1e-8
+    r.min_pressure, r.failure_threshold, r.pressure_at_failure = -1e8, 0.2, 1e-8
+    r.surrogate_residual_tol = 1e-6
+    return r
+
+
+def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 0, 0, 1), flux=FLUX_HLLC,
+                limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True, inactive_id=-1, roe_delta=0.2,
+                gminmod_alpha=1.2, slope_zero=False, mp_flux=0, mp_recon=0, precision=PREC_FP64, device=0):
+    c = Config()
+    c.nx, c.ny, c.nz = nx, ny, nz
+    c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
+    for i in range(6):
+        c.bc[i] = bc[i]
+    for i in range(5):
+        c.inlet[i], c.outlet[i] = inlet[i], outlet[i]
+    c.nmat = len(mats)
+    for i, m in enumerate(mats):
+        c.mat[i] = m
+    c.inactive_id = inactive_id
+    c.flux, c.roe_delta = flux, roe_delta
+    c.rec_linear = int(rec_linear)
+    c.limiter, c.gminmod_alpha = limiter, gminmod_alpha
+    c.slope_zero_near_interface = int(slope_zero)
+    c.ts, c.cfl = ts, cfl
+    c.mp_flux, c.mp_recon = mp_flux, mp_recon
+    c.riemann = riemann_defaults()
+    c.precision, c.device = precision, device
+    return c
+
+
+_lib = None
+
+
+def lib():
+    """Load libm2c_hip.so (fails loudly when the HIP library has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libm2c_hip.so is not built; run m2c_amd/build.py (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.POINTER(C.c_double)
+        L.m2c_ctx_create.argtypes = [C.POINTER(Config), C.POINTER(P)]
+        L.m2c_ctx_destroy.argtypes = [P]
+        L.m2c_upload_state.argtypes = [P, D, C.POINTER(C.c_int32)]
+        L.m2c_download_state.argtypes = [P, D, C.POINTER(C.c_int32)]
+        L.m2c_compute_residual.argtypes = [P, D]
+        L.m2c_time_step_size.argtypes = [P, D, D]
+        L.m2c_advance_dt.argtypes = [P, C.c_double]
+        L.m2c_run.argtypes = [P, C.c_double, C.c_int, C.c_int]
+        L.m2c_get_status.argtypes = [P, C.POINTER(Status)]
+        L.m2c_reset_time.argtypes = [P, C.c_double]
+        L.m2c_synchronize.argtypes = [P]
+        L.m2c_flux_batch.argtypes = [P, C.c_int, C.c_int, D, D, C.c_int, D]
+        L.m2c_time_steps.argtypes = [P, C.c_int, D, D]
+        L.m2c_device_state.argtypes = [P, C.POINTER(D)]
+        L.m2c_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class M2CError(RuntimeError):
+    pass
+
+
+def _chk(rc, what):
+    if rc != OK:
+        raise M2CError(f"{what} failed with status {rc}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One simulation context on one GPU (SpaceOperator + TimeIntegrator drop-in)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = lib()
+        self.h = C.c_void_p()
+        _chk(self.L.m2c_ctx_create(C.byref(cfg), C.byref(self.h)), "m2c_ctx_create")
+        self.shape = (cfg.nz, cfg.ny, cfg.nx)
+
+    def close(self):
+        if self.h:
+            self.L.m2c_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, V, ID=None):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        assert V.shape == self.shape + (5,)
+        idp = None
+        if ID is not None:
+            ID = np.ascontiguousarray(ID, dtype=np.int32)
+            assert ID.shape == self.shape
+            idp = ID.ctypes.data_as(C.POINTER(C.c_int32))
+        _chk(self.L.m2c_upload_state(self.h, _dp(V), idp), "m2c_upload_state")
+
+    def download(self):
+        V = np.empty(self.shape + (5,), dtype=np.float64)
+        ID = np.empty(self.shape, dtype=np.int32)
+        _chk(self.L.m2c_download_state(self.h, _dp(V), ID.ctypes.data_as(C.POINTER(C.c_int32))),
+             "m2c_download_state")
+        return V, ID
+
+    def residual(self):
+        R = np.empty(self.shape + (5,), dtype=np.float64)
+        _chk(self.L.m2c_compute_residual(self.h, _dp(R)), "m2c_compute_residual")
+        return R
+
+    def time_step(self):
+        dt, cfl = C.c_double(), C.c_double()
+        _chk(self.L.m2c_time_step_size(self.h, C.byref(dt), C.byref(cfl)), "m2c_time_step_size")
+        return dt.value
+
+    def advance(self, dt):
+        _chk(self.L.m2c_advance_dt(self.h, dt), "m2c_advance_dt")
+
+    def run(self, tmax, maxsteps, sync_every=64):
+        _chk(self.L.m2c_run(self.h, tmax, maxsteps, sync_every), "m2c_run")
+        return self.status()
+
+    def reset_time(self, t=0.0):
+        _chk(self.L.m2c_reset_time(self.h, t), "m2c_reset_time")
+
+    def status(self):
+        s = Status()
+        _chk(self.L.m2c_get_status(self.h, C.byref(s)), "m2c_get_status")
+        return s
+
+    def flux(self, d, Vm, Vp, mid=0):
+        Vm = np.ascontiguousarray(np.atleast_2d(Vm), dtype=np.float64)
+        Vp = np.ascontiguousarray(np.atleast_2d(Vp), dtype=np.float64)
+        F = np.empty_like(Vm)
+        _chk(self.L.m2c_flux_batch(self.h, d, Vm.shape[0], _dp(Vm), _dp(Vp), mid, _dp(F)), "m2c_flux_batch")
+        return F
+
+    def time_steps(self, nsteps):
+        a, b = C.c_double(), C.c_double()
+        _chk(self.L.m2c_time_steps(self.h, nsteps, C.byref(a), C.byref(b)), "m2c_time_steps")
+        return a.value, b.value
+
+
+def version():
+    return lib().m2c_version().decode()

@@ -1,0 +1,63 @@
+"""Build the gfx950 shared library libm2c_hip.so in-tree with hipcc.
+
+The fused stage kernel is compiled once per (flux, equation-of-state) variant from
+csrc/stage_inst.hip; the objects are built in parallel and linked with the context code.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+OUT = os.path.join(HERE, "libm2c_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: no fused multiply-add contraction, so double-precision results equal the
+# reference's x86-64 arithmetic operation for operation.
+# -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
+FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
+         "-ffp-contract=off", "-fno-fast-math",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-result"]
+
+# (tag, flux code, eos kind) -- flux codes: ROE 0, LLF 1, HLLC 2; eos kind -1 = run-time dispatch
+VARIANTS = [("hllc_sg", 2, 0), ("hllc_gen", 2, -1), ("roe_gen", 0, -1), ("llf_gen", 1, -1)]
+UNITS = [("m2c_hip", "m2c_hip.hip", [])] + [
+    (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}"])
+    for t, f, e in VARIANTS]
+
+
+def _compile(unit, verbose):
+    name, src, defs = unit
+    obj = os.path.join(OBJ, name + ".o")
+    cmd = [HIPCC] + FLAGS + defs + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed on {src} ({name})")
+    return obj
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(len(UNITS), max(1, (os.cpu_count() or 2)))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda u: _compile(u, verbose), UNITS))
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc link failed for libm2c_hip.so")
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,592 @@
+// fv_kernels.hpp -- the fused finite-volume stage kernel and its helpers (gfx950).
+//
+// One launch performs one Runge-Kutta stage of SpaceOperator::ComputeResidual followed by the
+// stage combination of TimeIntegratorRK2/RK3/FE (TimeIntegrator.cpp:150-226, 291-428, 496-690):
+//
+//   ghost values      produced on the fly from the boundary rules (SpaceOperator.cpp:1667-1745,
+//                     Reconstructor.cpp:795-875, MultiPhaseOperator.cpp:796-819)
+//   MUSCL             Reconstructor.cpp:423-779, limiters Reconstructor.h:95-110
+//   face-state check  Reconstructor.cpp:719-775 + SpaceOperator.cpp:3198-3347 (a face state that
+//                     fails CheckState or would be clipped reverts to the cell value)
+//   face flux         FluxFcn*::ComputeNumericalFluxAtCellInterface, once per face
+//   accumulation      SpaceOperator.cpp:2423-2815 order (west, south, back subtracted, then east,
+//                     north, front added), then *1/(-vol) as Vector5D.h:174-185
+//   stage combine     U = (a*U_s + b*U_n) + c*dt*R, V = C2P(U), clip, CheckState
+//
+// Layout: a 32x8 block of (x,y) columns marches up a chunk of z planes. Plane k of the stencil
+// field, with a 2-cell halo in x and y, lives in LDS (structure of arrays); each thread keeps
+// V(k..k+2) of its own column in registers for the z direction. Slopes are computed once per
+// cell and direction, each face flux once per face (faces on a tile's x/y edge are also computed
+// by the neighbouring tile).
+#pragma once
+#include "physics.hpp"
+
+namespace m2c {
+
+constexpr int TX = 32, TY = 8, HALO = 2;
+constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
+constexpr int NTHR = TX * TY;
+constexpr int NSX = (TX + 2) * TY;             // x slopes: x in [-1, TX+1), y in [0, TY)
+constexpr int NSY = TX * (TY + 2);             // y slopes: x in [0, TX), y in [-1, TY+1)
+constexpr int NSL = NSX + NSY;
+constexpr int NFX = (TX + 1) * TY;             // x faces: x in [0, TX], y in [0, TY)
+constexpr int NFY = TX * (TY + 1);             // y faces: x in [0, TX), y in [0, TY]
+constexpr int NHALO = 2 * HALO * TY + 2 * HALO * TX;
+constexpr int SLOPE_ROUNDS = (NSL + NTHR - 1) / NTHR;
+
+enum { BC_INLET = 1, BC_OUTLET = 2, BC_SLIPWALL = 3, BC_STICKWALL = 4, BC_SYMMETRY = 5, BC_OVERSET = 6,
+       BC_PERIODIC = 7 };
+
+// status bits written by the kernels
+enum { ST_BAD_STATE = 1u, ST_CLIP_INTERMEDIATE = 2u, ST_NAN_FLUX = 4u, ST_NEG_C2 = 8u, ST_RIEMANN = 16u };
+
+struct Grid {
+  int nx, ny, nz;
+  long long N;                 // nx*ny*nz
+  int bc[6];
+  double inlet[5], outlet[5];
+  double dx, dy, dz, ax, ay, az, inv_mvol;
+  double A[3], B[3];
+  int K[3];
+  int flux, limiter, rec_linear, slope_zero;
+  double alpha, roe_delta;
+  int inactive_id, nmat, mp_flux, mp_recon;
+  int has_interfaces;          // more than one active material
+};
+
+struct Status {
+  unsigned int bits;
+  unsigned int pad;
+  unsigned long long reverted, clipped, riemann_err;
+};
+
+// device-side run control of the time loop (Main.cpp:514-621)
+struct Clock {
+  double t, dt, tmax, cfl;
+  unsigned long long minbits;  // running min of dx/(|u|+c) as ordered bits of a positive double
+  int steps, maxsteps, active, pad;
+};
+
+struct StageArgs {
+  const double *S;             // stencil field (V if IN_CONS==0, U_s otherwise), SoA 5*N
+  const double *Vn;            // primitive state at t_n (for U_n = P2C(V_n))
+  const double *Us;            // previous stage conservative state
+  double *Out;                 // U_out, V_out or R (SoA 5*N)
+  const int8_t *ID;
+  const double *Fi;            // interface fluxes, 3 directions x 10 x Nf (may be null)
+  long long Nf;                // faces per direction in Fi: (nx+1)*(ny+1)*(nz+1)
+  Clock *clk;
+  Status *st;
+  double a, b, c;              // U = (a*U_s + b*U_n) + (c*dt)*R
+  int first;                   // 1: first stage (U = U_n + dt*R)
+  int kchunk;
+  int ignore_active;           // 1: run even if the clock is inactive (direct API calls)
+};
+
+// Each translation unit that launches kernels holds its own copy of these tables (internal
+// linkage) and exports an upload function for it.
+static __constant__ Grid c_grid;
+static __constant__ DevEos c_eos[8];
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// image index of a ghost cell (SpaceOperator.cpp:449-480); periodic wraps
+__device__ __forceinline__ int image_idx(int i, int n, bool per) {
+  if (i < 0) return per ? ((i % n) + n) % n : -i - 1;
+  if (i >= n) return per ? i % n : n + (i - n) - 1;
+  return i;
+}
+
+// Load the stencil value of (possibly ghost) cell (i,j,k).
+template <int IN_CONS, int EK>
+__device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const int8_t *__restrict__ ID, int i,
+                                           int j, int k, double *V, int &id, unsigned &flags) {
+  const Grid &g = c_grid;
+  int oi = (i < 0) ? 0 : (i >= g.nx ? 1 : -1);
+  int oj = (j < 0) ? 2 : (j >= g.ny ? 3 : -1);
+  int ok = (k < 0) ? 4 : (k >= g.nz ? 5 : -1);
+  int nout = (oi >= 0) + (oj >= 0) + (ok >= 0);
+  int side = oi >= 0 ? oi : (oj >= 0 ? oj : ok);
+  int bc = (nout == 1) ? g.bc[side] : 0;
+  int ii = clampi(image_idx(i, g.nx, oi >= 0 && g.bc[oi] == BC_PERIODIC), 0, g.nx - 1);
+  int jj = clampi(image_idx(j, g.ny, oj >= 0 && g.bc[oj] == BC_PERIODIC), 0, g.ny - 1);
+  int kk = clampi(image_idx(k, g.nz, ok >= 0 && g.bc[ok] == BC_PERIODIC), 0, g.nz - 1);
+  long long idx = ((long long)kk * g.ny + jj) * g.nx + ii;
+  id = ID[idx];
+  if (bc == BC_INLET || bc == BC_OUTLET) {
+    const double *s = (bc == BC_INLET) ? g.inlet : g.outlet;
+#pragma unroll
+    for (int p = 0; p < 5; p++) V[p] = s[p];
+    return;
+  }
+  double raw[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) raw[p] = S[p * g.N + idx];
+  if (IN_CONS) {
+    const DevEos &m = c_eos[id];
+    eos_c2p<EK>(m, raw, V);
+    if (eos_clip<double, EK>(m, V)) flags |= ST_CLIP_INTERMEDIATE;
+  } else {
+#pragma unroll
+    for (int p = 0; p < 5; p++) V[p] = raw[p];
+  }
+  if (bc == BC_SYMMETRY || bc == BC_SLIPWALL) {
+    if (side < 2) V[1] *= -1.0;
+    else if (side < 4) V[2] *= -1.0;
+    else V[3] *= -1.0;
+  } else if (bc == BC_STICKWALL) {
+    V[1] = -1.0 * V[1]; V[2] = -1.0 * V[2]; V[3] = -1.0 * V[3];
+  }
+}
+
+template <int DIM> __device__ __forceinline__ double limited_slope(double dq0, double dq1) {
+  const Grid &g = c_grid;
+  switch (g.limiter) {
+  case LIM_GENMINMOD: {
+    double A = g.A[DIM], B = g.B[DIM], al = g.alpha;
+    if (dq0 < 0.0 && dq1 < 0.0) return smax(al * dq0, smax(B / (A + 1.0) * (dq0 + dq1), al * dq1));
+    if (dq0 > 0.0 && dq1 > 0.0) return smin(al * dq0, smin(B / (A + 1.0) * (dq0 + dq1), al * dq1));
+    return 0.0;
+  }
+  case LIM_VANALBADA: {
+    if (dq0 * dq1 <= 0) return 0.0;
+    if (dabs(dq0 * dq1) < 1e-16) return 0.0;
+    double d0k = dq0, d1k = dq1;
+    for (int t = 1; t < g.K[DIM]; t++) { d0k *= dq0; d1k *= dq1; }
+    return g.B[DIM] * (dq1 * d0k + dq0 * d1k) / (d0k + g.A[DIM] * d1k);
+  }
+  default: return 0.5 * (dq0 + dq1);
+  }
+}
+
+// Slopes of one cell in direction DIM from its two neighbours.
+template <int DIM>
+__device__ __forceinline__ void cell_slopes(const double *vl, const double *vc, const double *vr, int idl, int idc,
+                                            int idr, double *sig) {
+  const Grid &g = c_grid;
+  if (!g.rec_linear || idc == g.inactive_id || (g.slope_zero && (idc != idl || idc != idr))) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) sig[p] = 0.0;
+    return;
+  }
+  bool zl = (idl == g.inactive_id), zr = (idr == g.inactive_id);
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    double dq0 = zl ? 0.0 : vc[p] - vl[p];
+    double dq1 = zr ? 0.0 : vr[p] - vc[p];
+    sig[p] = limited_slope<DIM>(dq0, dq1);
+  }
+}
+
+// Face state v +/- sigma/2 with the revert rule; returns 1 if reverted.
+template <int EK, bool PLUS>
+__device__ __forceinline__ int face_state(const double *vc, const double *sig, int id, double *out) {
+#pragma unroll
+  for (int p = 0; p < 5; p++) out[p] = PLUS ? vc[p] + 0.5 * sig[p] : vc[p] - 0.5 * sig[p];
+  const DevEos &m = c_eos[id];
+  if (id != c_grid.inactive_id && (eos_bad<double, EK>(m, out) || eos_would_clip<double, EK>(m, out))) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = vc[p];
+    return 1;
+  }
+  return 0;
+}
+
+// Face state of a ghost cell from the boundary rule (Reconstructor.cpp:820-869).
+template <int DIM>
+__device__ __forceinline__ void ghost_face(int bc, const double *img_face, const double *vghost, double *out) {
+  if (bc == BC_SYMMETRY || bc == BC_SLIPWALL) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = (p == DIM + 1) ? -img_face[p] : img_face[p];
+  } else if (bc == BC_STICKWALL) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = (p >= 1 && p <= 3) ? -img_face[p] : img_face[p];
+  } else {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = vghost[p];
+  }
+}
+
+template <int FLUX, int EK, int DIR>
+__device__ __forceinline__ void numerical_flux(const double *Vm, const double *Vp, int id, double *F) {
+  const DevEos &m = c_eos[id];
+  if (FLUX == FLUX_HLLC) flux_hllc<double, EK, DIR>(m, Vm, Vp, F);
+  else if (FLUX == FLUX_ROE) flux_roe<double, EK, DIR>(m, Vm, Vp, c_grid.roe_delta, F);
+  else flux_llf2<double, EK, EK, DIR>(m, m, Vm, Vp, F);
+}
+
+// Flux of one face between cells with IDs idm (minus) and idp (plus). Interface faces solved by
+// the exact Riemann pass return 1 (their two fluxes are read from the interface-flux arrays).
+template <int FLUX, int EK, int DIR>
+__device__ __forceinline__ int face_flux(const double *Vm, const double *Vp, const double *cm, const double *cp,
+                                         int idm, int idp, double *F) {
+  const Grid &g = c_grid;
+  if (idm == idp && idp != g.inactive_id) {
+    numerical_flux<FLUX, EK, DIR>(Vm, Vp, idp, F);
+    return 0;
+  }
+#pragma unroll
+  for (int p = 0; p < 5; p++) F[p] = 0.0;
+  if (EK < 0 && idm != idp && idm != g.inactive_id && idp != g.inactive_id) {
+    if (g.mp_flux == 2) {   // LLF at the material interface (SpaceOperator.cpp:2493-2501)
+      const double *am = g.mp_recon ? Vm : cm, *ap = g.mp_recon ? Vp : cp;
+      flux_llf2<double, -1, -1, DIR>(c_eos[idm], c_eos[idp], am, ap, F);
+      return 0;
+    }
+    return 1;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ bool is_interface(int idm, int idp) {
+  const Grid &g = c_grid;
+  return idm != idp && idm != g.inactive_id && idp != g.inactive_id && g.mp_flux != 2;
+}
+
+// face (i-1/2, j, k) of direction 0 (and likewise for y, z) is keyed by its plus cell (i,j,k)
+__device__ __forceinline__ long long face_key(int i, int j, int k) {
+  const Grid &g = c_grid;
+  return ((long long)k * (g.ny + 1) + j) * (g.nx + 1) + i;
+}
+
+__device__ __forceinline__ void warp_min_atomic(unsigned long long *dst, double v) {
+  // wavefront min (64 lanes), then one atomic per wave
+  unsigned long long b = __double_as_longlong(v);
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off, 64);
+    b = o < b ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(dst, b);
+}
+
+// x (DIR 0) or y (DIR 1) face of the tile whose plus cell is at local (x, y); the flux goes to
+// sF[p*NF + fi].
+template <int FLUX, int EK, int DIR>
+__device__ __forceinline__ void tile_face(double (*sV)[LY][LX], int (*sID)[LX], const double *sS, double *sF,
+                                          int x, int y, int fi, int i0, int j0, unsigned long long &nrev) {
+  const Grid &g = c_grid;
+  constexpr int SOFF = DIR == 0 ? 0 : NSX;
+  constexpr int NF = DIR == 0 ? NFX : NFY;
+  const int xm = x - (DIR == 0), ym = y - (DIR == 1);
+  const int gpl = DIR == 0 ? i0 + x : j0 + y;
+  const int gn = DIR == 0 ? g.nx : g.ny;
+  const bool inside = (DIR == 0) ? (i0 + x <= g.nx && j0 + y < g.ny) : (i0 + x < g.nx && j0 + y <= g.ny);
+  double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (inside) {
+    int sm, sp;
+    if (DIR == 0) { sm = y * (TX + 2) + (xm + 1); sp = y * (TX + 2) + (x + 1); }
+    else { sm = (ym + 1) * TX + x; sp = (y + 1) * TX + x; }
+    double vcm[5], vcp[5], sgm[5], sgp[5], fm[5], fp[5];
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+      vcm[p] = sV[p][ym + HALO][xm + HALO];
+      vcp[p] = sV[p][y + HALO][x + HALO];
+      sgm[p] = sS[p * NSL + SOFF + sm];
+      sgp[p] = sS[p * NSL + SOFF + sp];
+    }
+    const int idm = sID[ym + HALO][xm + HALO], idp = sID[y + HALO][x + HALO];
+    const bool per = g.bc[2 * DIR] == BC_PERIODIC;
+    if (gpl == 0 && !per) {
+      nrev += face_state<EK, false>(vcp, sgp, idp, fp);
+      ghost_face<DIR>(g.bc[2 * DIR], fp, vcm, fm);
+    } else if (gpl == gn && !per) {
+      nrev += face_state<EK, true>(vcm, sgm, idm, fm);
+      ghost_face<DIR>(g.bc[2 * DIR + 1], fm, vcp, fp);
+    } else {
+      nrev += face_state<EK, true>(vcm, sgm, idm, fm);
+      nrev += face_state<EK, false>(vcp, sgp, idp, fp);
+    }
+    face_flux<FLUX, EK, DIR>(fm, fp, vcm, vcp, idm, idp, F);
+  }
+#pragma unroll
+  for (int p = 0; p < 5; p++) sF[p * NF + fi] = F[p];
+}
+
+// MODE 0: residual only (Out = R). MODE 1: intermediate stage (Out = U). MODE 2: final stage
+// (Out = V, plus the CFL reduction for the next step).
+template <int IN_CONS, int MODE, int FLUX, int EK>
+__global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
+  const Grid &g = c_grid;
+  if (!a.ignore_active && !a.clk->active) return;
+
+  __shared__ double sV[5][LY][LX];
+  __shared__ int sID[LY][LX];
+  __shared__ double sS[5 * NSL];              // slopes
+  __shared__ double sFx[5 * NFX];             // x-face fluxes
+  __shared__ double sFy[5 * NFY];             // y-face fluxes
+  __shared__ unsigned sFlags;
+
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int kbeg = blockIdx.z * a.kchunk;
+  const int kend = min(kbeg + a.kchunk, g.nz);
+  const int gi = i0 + tx, gj = j0 + ty;
+  const bool own = gi < g.nx && gj < g.ny;
+  // columns beyond the domain edge hold the ghost values there (fetch_cell applies the boundary
+  // rules), so a partial tile's neighbours of the last cells are correct
+  const int ci = min(gi, g.nx + 1), cj = min(gj, g.ny + 1);
+  const bool zper = g.bc[4] == BC_PERIODIC;
+  unsigned flags = 0;
+  unsigned long long nrev = 0;
+  double cfl_min = 1.7976931348623157e308;
+  if (tid == 0) sFlags = 0;
+
+  const double dt = a.clk->dt;
+  const double cdt = a.c * dt;
+
+  // ---- z prologue: sigma_z(kbeg), flux F_{kbeg-1/2}
+  double w0[5], w1[5], w2[5], sz0[5], sz1[5], Flo[5];
+  int id0, id1, id2;
+  int lo_iface = 0;
+  {
+    double vm2[5], vm1[5];
+    int idm2, idm1;
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 2, vm2, idm2, flags);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 1, vm1, idm1, flags);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg, w0, id0, flags);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 1, w1, id1, flags);
+    double szm[5];
+    cell_slopes<2>(vm2, vm1, w0, idm2, idm1, id0, szm);
+    cell_slopes<2>(vm1, w0, w1, idm1, id0, id1, sz0);
+    double fm[5], fp[5];
+    nrev += face_state<EK, false>(w0, sz0, id0, fp);
+    if (kbeg == 0 && !zper) {
+      ghost_face<2>(g.bc[4], fp, vm1, fm);
+    } else {
+      nrev += face_state<EK, true>(vm1, szm, idm1, fm);
+    }
+    lo_iface = face_flux<FLUX, EK, 2>(fm, fp, vm1, w0, idm1, id0, Flo);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 2, w2, id2, flags);
+  }
+
+  for (int k = kbeg; k < kend; k++) {
+    // ---- prefetch V(k+3) for the next plane (consumed at the end of the iteration)
+    double w3[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int id3 = 0;
+    if (k + 1 < kend) fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 3, w3, id3, flags);
+
+    // ---- plane k into LDS: own cells from registers, halo from memory
+#pragma unroll
+    for (int p = 0; p < 5; p++) sV[p][ty + HALO][tx + HALO] = w0[p];
+    sID[ty + HALO][tx + HALO] = id0;
+    if (tid < NHALO) {
+      int hx, hy;
+      if (tid < 2 * HALO * TY) {            // x halo: 4 columns x TY rows
+        int c = tid % (2 * HALO), r = tid / (2 * HALO);
+        hx = (c < HALO) ? c - HALO : TX + (c - HALO);
+        hy = r;
+      } else {                              // y halo: 4 rows x TX columns
+        int t = tid - 2 * HALO * TY;
+        int r = t / TX, c = t % TX;
+        hy = (r < HALO) ? r - HALO : TY + (r - HALO);
+        hx = c;
+      }
+      int hi = min(i0 + hx, g.nx + 1), hj = min(j0 + hy, g.ny + 1);
+      double hv[5];
+      int hid;
+      fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, k, hv, hid, flags);
+#pragma unroll
+      for (int p = 0; p < 5; p++) sV[p][hy + HALO][hx + HALO] = hv[p];
+      sID[hy + HALO][hx + HALO] = hid;
+    }
+    // ---- z slope of k+1 (registers)
+    cell_slopes<2>(w0, w1, w2, id0, id1, id2, sz1);
+    __syncthreads();
+
+    // ---- slopes in x and y, once per cell and direction
+#pragma unroll
+    for (int r = 0; r < SLOPE_ROUNDS; r++) {
+      int it = tid + r * NTHR;
+      if (it < NSX) {
+        int x = it % (TX + 2) - 1, y = it / (TX + 2);
+        double vl[5], vc[5], vr[5], sg[5];
+#pragma unroll
+        for (int p = 0; p < 5; p++) {
+          vl[p] = sV[p][y + HALO][x - 1 + HALO];
+          vc[p] = sV[p][y + HALO][x + HALO];
+          vr[p] = sV[p][y + HALO][x + 1 + HALO];
+        }
+        cell_slopes<0>(vl, vc, vr, sID[y + HALO][x - 1 + HALO], sID[y + HALO][x + HALO],
+                       sID[y + HALO][x + 1 + HALO], sg);
+#pragma unroll
+        for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
+      } else if (it < NSL) {
+        int t = it - NSX;
+        int x = t % TX, y = t / TX - 1;
+        double vl[5], vc[5], vr[5], sg[5];
+#pragma unroll
+        for (int p = 0; p < 5; p++) {
+          vl[p] = sV[p][y - 1 + HALO][x + HALO];
+          vc[p] = sV[p][y + HALO][x + HALO];
+          vr[p] = sV[p][y + 1 + HALO][x + HALO];
+        }
+        cell_slopes<1>(vl, vc, vr, sID[y - 1 + HALO][x + HALO], sID[y + HALO][x + HALO],
+                       sID[y + 1 + HALO][x + HALO], sg);
+#pragma unroll
+        for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
+      }
+    }
+    __syncthreads();
+
+    // ---- x and y face fluxes: the west/south face of every column, then the east column of
+    //      x faces and the north row of y faces
+    tile_face<FLUX, EK, 0>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
+    tile_face<FLUX, EK, 1>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
+    if (tid < TY) {
+      tile_face<FLUX, EK, 0>(sV, sID, sS, sFx, TX, tid, tid * (TX + 1) + TX, i0, j0, nrev);
+    } else if (tid < TY + TX) {
+      int x = tid - TY;
+      tile_face<FLUX, EK, 1>(sV, sID, sS, sFy, x, TY, TY * TX + x, i0, j0, nrev);
+    }
+
+    // ---- z face k+1/2 (registers)
+    double Fhi[5];
+    int hi_iface;
+    {
+      double fm[5], fp[5];
+      nrev += face_state<EK, true>(w0, sz0, id0, fm);
+      if (k + 1 == g.nz && !zper) {
+        ghost_face<2>(g.bc[5], fm, w1, fp);
+      } else {
+        nrev += face_state<EK, false>(w1, sz1, id1, fp);
+      }
+      hi_iface = face_flux<FLUX, EK, 2>(fm, fp, w0, w1, id0, id1, Fhi);
+    }
+    __syncthreads();
+
+    // ---- accumulate the residual of cell (gi, gj, k) and finish the stage
+    if (own) {
+      double Fw[5], Fe[5], Fs[5], Fn[5], Fb[5], Ft[5];
+      const int fw = ty * (TX + 1) + tx, fe = fw + 1;
+      const int fs = ty * TX + tx, fn = fs + TX;
+#pragma unroll
+      for (int p = 0; p < 5; p++) {
+        Fw[p] = sFx[p * NFX + fw];
+        Fe[p] = sFx[p * NFX + fe];
+        Fs[p] = sFy[p * NFY + fs];
+        Fn[p] = sFy[p * NFY + fn];
+        Fb[p] = Flo[p];
+        Ft[p] = Fhi[p];
+      }
+      if (EK < 0 && g.has_interfaces && a.Fi) {
+        // interface faces: the plus cell takes localflux2 of its low face, the minus cell
+        // localflux1 of its high face (SpaceOperator.cpp:2548-2556)
+        const int me = sID[ty + HALO][tx + HALO];
+        const long long Nf = a.Nf;
+        if (is_interface(sID[ty + HALO][tx - 1 + HALO], me)) {
+          const double *f = a.Fi + 0 * 10 * Nf + face_key(gi, gj, k);
+#pragma unroll
+          for (int p = 0; p < 5; p++) Fw[p] = f[(5 + p) * Nf];
+        }
+        if (is_interface(me, sID[ty + HALO][tx + 1 + HALO])) {
+          const double *f = a.Fi + 0 * 10 * Nf + face_key(gi + 1, gj, k);
+#pragma unroll
+          for (int p = 0; p < 5; p++) Fe[p] = f[p * Nf];
+        }
+        if (is_interface(sID[ty - 1 + HALO][tx + HALO], me)) {
+          const double *f = a.Fi + 1 * 10 * Nf + face_key(gi, gj, k);
+#pragma unroll
+          for (int p = 0; p < 5; p++) Fs[p] = f[(5 + p) * Nf];
+        }
+        if (is_interface(me, sID[ty + 1 + HALO][tx + HALO])) {
+          const double *f = a.Fi + 1 * 10 * Nf + face_key(gi, gj + 1, k);
+#pragma unroll
+          for (int p = 0; p < 5; p++) Fn[p] = f[p * Nf];
+        }
+        if (lo_iface) {
+          const double *f = a.Fi + 2 * 10 * Nf + face_key(gi, gj, k);
+#pragma unroll
+          for (int p = 0; p < 5; p++) Fb[p] = f[(5 + p) * Nf];
+        }
+        if (hi_iface) {
+          const double *f = a.Fi + 2 * 10 * Nf + face_key(gi, gj, k + 1);
+#pragma unroll
+          for (int p = 0; p < 5; p++) Ft[p] = f[p * Nf];
+        }
+      }
+      double R[5];
+#pragma unroll
+      for (int p = 0; p < 5; p++) {
+        double s = 0.0;
+        s -= Fw[p] * g.ax;
+        s -= Fs[p] * g.ay;
+        s -= Fb[p] * g.az;
+        s += Fe[p] * g.ax;
+        s += Fn[p] * g.ay;
+        s += Ft[p] * g.az;
+        R[p] = s * g.inv_mvol;
+        if (!dfinite(R[p])) flags |= ST_NAN_FLUX;
+      }
+      const long long idx = ((long long)k * g.ny + gj) * g.nx + gi;
+      if (MODE == 0) {
+#pragma unroll
+        for (int p = 0; p < 5; p++) a.Out[p * g.N + idx] = R[p];
+      } else {
+        const DevEos &m = c_eos[id0];
+        double vn[5], un[5], U[5];
+        if (IN_CONS) {
+#pragma unroll
+          for (int p = 0; p < 5; p++) vn[p] = a.Vn[p * g.N + idx];
+        } else {
+#pragma unroll
+          for (int p = 0; p < 5; p++) vn[p] = w0[p];
+        }
+        eos_p2c<double, EK>(m, vn, un);
+        if (a.first) {
+#pragma unroll
+          for (int p = 0; p < 5; p++) U[p] = un[p] + dt * R[p];
+        } else {
+#pragma unroll
+          for (int p = 0; p < 5; p++) {
+            double us = a.Us[p * g.N + idx];
+            double t = a.a * us + a.b * un[p];
+            U[p] = t + cdt * R[p];
+          }
+        }
+        double V[5];
+        eos_c2p<EK>(m, U, V);
+        bool clipped = eos_clip<double, EK>(m, V);
+        if (eos_bad<double, EK>(m, V)) flags |= ST_BAD_STATE;
+        if (MODE == 1) {
+          if (clipped) flags |= ST_CLIP_INTERMEDIATE;
+#pragma unroll
+          for (int p = 0; p < 5; p++) a.Out[p * g.N + idx] = U[p];
+        } else {
+#pragma unroll
+          for (int p = 0; p < 5; p++) a.Out[p * g.N + idx] = V[p];
+          // CFL contribution of V(n+1) (SpaceOperator.cpp:2128-2198)
+          if (id0 != g.inactive_id) {
+            double c2 = eos_c2<double, EK>(m, V[0], eos_e<double, EK>(m, V[0], V[4]));
+            if (c2 < 0) flags |= ST_NEG_C2;
+            double cs = dsqrt(c2);
+            double lf = dabs(V[1]) + cs, lg = dabs(V[2]) + cs, lh = dabs(V[3]) + cs;
+            double loc = smin(g.dx / lf, smin(g.dy / lg, g.dz / lh));
+            if (loc < cfl_min) cfl_min = loc;
+          }
+          if (clipped) atomicAdd(&a.st->clipped, 1ull);
+        }
+      }
+    }
+
+    // ---- shift the z window
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+      Flo[p] = Fhi[p];
+      sz0[p] = sz1[p];
+      w0[p] = w1[p];
+      w1[p] = w2[p];
+      w2[p] = w3[p];
+    }
+    lo_iface = hi_iface;
+    id0 = id1; id1 = id2; id2 = id3;
+    __syncthreads();
+  }
+  if (MODE == 2) warp_min_atomic(&a.clk->minbits, cfl_min);
+  if (flags) atomicOr(&sFlags, flags);
+  if (nrev) atomicAdd(&a.st->reverted, nrev);
+  __syncthreads();
+  if (tid == 0 && sFlags) atomicOr(&a.st->bits, sFlags);
+}
+
+}  // namespace m2c

@@ -1,0 +1,73 @@
+"""Synthetic workloads of the benchmark configurations (initial conditions + solver settings).
+
+Each builder returns (settings, V, ID): `settings` is a dict of solver options understood by
+m2c_amd.make_config, V the primitive state (nz, ny, nx, 5) and ID the material IDs (nz, ny, nx).
+
+  sod_1d            400x1x1 two-state Riemann problem (gamma=1.4, van Albada, HLLC, RK2, CFL 0.5)
+  periodic_modes    N^3 periodic gas: rho=p=1 plus 16 seeded Fourier modes of amplitude 1e-2 on
+                    density and velocity and a Gaussian pressure pulse (amplitude 0.5, sigma 8
+                    cells) at the centre; gamma=1.4, van Albada, HLLC, RK3, CFL 0.4
+The periodic boundary and the Fourier-mode initial state have no counterpart in the reference's
+input format (IoData.h:266-267 has no periodic code); they are the documented extension.
+"""
+import numpy as np
+
+from . import (BC_INLET, BC_OUTLET, BC_PERIODIC, BC_SYMMETRY, EOS_SG, FLUX_HLLC, LIM_VANALBADA, TS_RK2,
+               TS_RK3, material)
+
+FOURIER_SEED = 2024
+
+
+def sod_1d(n=400):
+    settings = dict(nx=n, ny=1, nz=1, box=(0.0, 1.0, 0.0, 1.0 / n, 0.0, 1.0 / n),
+                    bc=[BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_SYMMETRY, BC_SYMMETRY, BC_SYMMETRY],
+                    mats=[material(EOS_SG, [1.4, 0.0])], inlet=(1.0, 0.0, 0.0, 0.0, 1.0),
+                    outlet=(0.125, 0.0, 0.0, 0.0, 0.1), flux=FLUX_HLLC, limiter=LIM_VANALBADA, ts=TS_RK2,
+                    cfl=0.5)
+    x = (np.arange(n) + 0.5) / n
+    V = np.zeros((1, 1, n, 5))
+    V[..., 0] = np.where(x < 0.5, 1.0, 0.125)
+    V[..., 4] = np.where(x < 0.5, 1.0, 0.1)
+    return settings, V, np.zeros((1, 1, n), dtype=np.int32)
+
+
+def fourier_modes(seed=FOURIER_SEED, nmodes=16, kmax=4):
+    """Integer wave vectors in [-kmax, kmax]^3 minus the origin and uniform phases per field."""
+    rng = np.random.default_rng(seed)
+    ks = []
+    while len(ks) < nmodes:
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        if np.any(k != 0):
+            ks.append(k)
+    phases = rng.uniform(0.0, 2.0 * np.pi, size=(nmodes, 4))
+    return np.array(ks), phases
+
+
+def periodic_modes(n=256, ny=None, nz=None, amp=1e-2, pulse=0.5, sigma=8.0, seed=FOURIER_SEED):
+    nx = n
+    ny = n if ny is None else ny
+    nz = n if nz is None else nz
+    settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, float(nx), 0.0, float(ny), 0.0, float(nz)),
+                    bc=[BC_PERIODIC] * 6, mats=[material(EOS_SG, [1.4, 0.0])], flux=FLUX_HLLC,
+                    limiter=LIM_VANALBADA, ts=TS_RK3, cfl=0.4)
+    ks, ph = fourier_modes(seed)
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny, (np.arange(nx) + 0.5) / nx,
+                          indexing="ij")
+    V = np.zeros((nz, ny, nx, 5))
+    V[..., 0] = 1.0
+    V[..., 4] = 1.0
+    for m in range(len(ks)):
+        arg = 2.0 * np.pi * (ks[m, 0] * x + ks[m, 1] * y + ks[m, 2] * z)
+        for f, comp in enumerate((0, 1, 2, 3)):
+            V[..., comp] += amp * np.sin(arg + ph[m, f])
+    cz, cy, cx = nz / 2.0, ny / 2.0, nx / 2.0
+    r2 = ((x * nx - cx) ** 2 + (y * ny - cy) ** 2 + (z * nz - cz) ** 2)
+    V[..., 4] += pulse * np.exp(-r2 / (2.0 * sigma * sigma))
+    return settings, V, np.zeros((nz, ny, nx), dtype=np.int32)
+
+
+def config_from(settings, **over):
+    from . import make_config
+    s = dict(settings)
+    s.update(over)
+    return make_config(s.pop("nx"), s.pop("ny"), s.pop("nz"), s.pop("box"), s.pop("bc"), s.pop("mats"), **s)

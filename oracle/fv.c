@@ -1,0 +1,556 @@
+/*
+ * oracle/fv.c -- CPU restatement of M2C's explicit finite-volume update (test infrastructure).
+ *
+ * Functions follow, in order:
+ *   mesh + MUSCL coefficients   MeshGenerator.cpp:146-158, SpaceOperator.cpp:129-183,
+ *                               Reconstructor.cpp:53-121 (A, B, k of Zeng 2016)
+ *   ghost fill                  SpaceOperator.cpp:428-516 (ghost list), :1667-1745 (BCs),
+ *                               MultiPhaseOperator.cpp:796-819 (ghost IDs)
+ *   reconstruction              Reconstructor.cpp:332-904, limiters Reconstructor.h:95-110
+ *   reconstructed-state check   SpaceOperator.cpp:3198-3347
+ *   face loop + accumulation    SpaceOperator.cpp:2323-2856, residual :3351-3409
+ *   time-step size              SpaceOperator.cpp:2128-2227
+ *   Runge-Kutta stages          TimeIntegrator.cpp:150-226, 291-428, 496-690
+ *   time loop                   Main.cpp:514-621
+ * The residual sums each cell's six face contributions in the reference's loop order
+ * (west, south, back subtracted, then east, north, front added) and then multiplies by
+ * 1/(-vol) as Vec5D::operator/= does (Vector5D.h:174-185). The result does not depend on the
+ * number of OpenMP threads.
+ * The periodic boundary code is an extension with no reference counterpart; it is pinned only
+ * by the tests' invariants (translation invariance, conservation).
+ */
+#include "m2c_oracle.h"
+#include "flux.h"
+#include "oracle_internal.h"
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+long orc_ncells_ghosted(const orc_ctx *c) { return (long)c->gx * c->gy * c->gz; }
+
+static int slope_k(double A, double B) {          /* Reconstructor.cpp:103-121 */
+  int k;
+  double rhs;
+  for (k = 2; k < 1000; k++) {
+    rhs = 2.0 / (1.0 + pow((k - 1.0) / k, k - 1) / k) * (A < 1.0 ? A : 1.0);
+    if (B <= rhs) break;
+  }
+  return k;
+}
+
+orc_ctx *orc_create(const orc_config *cfg) {
+  orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+  c->cfg = *cfg;
+  for (int m = 0; m < cfg->nmat; m++) orc_eos_finalize(&c->cfg.mat[m]);
+  c->nx = cfg->nx; c->ny = cfg->ny; c->nz = cfg->nz;
+  c->gx = c->nx + 2; c->gy = c->ny + 2; c->gz = c->nz + 2;
+  c->dx = (cfg->x1 - cfg->x0) / cfg->nx;
+  c->dy = (cfg->y1 - cfg->y0) / cfg->ny;
+  c->dz = (cfg->z1 - cfg->z0) / cfg->nz;
+  c->vol = c->dx * c->dy * c->dz;
+  c->area[0] = c->dy * c->dz;
+  c->area[1] = c->dx * c->dz;
+  c->area[2] = c->dx * c->dy;
+  double d[3] = {c->dx, c->dy, c->dz};
+  for (int a = 0; a < 3; a++) {
+    c->A[a] = (d[a] + d[a]) / (d[a] + d[a]);
+    c->B[a] = 2.0 * d[a] / (d[a] + d[a]);
+    c->K[a] = (cfg->limiter == ORC_LIM_VANALBADA) ? slope_k(c->A[a], c->B[a]) : 0;
+  }
+  long n = orc_ncells_ghosted(c);
+  for (int f = 0; f < 6; f++) c->face[f] = (double *)calloc(n * 5, sizeof(double));
+  c->Un = (double *)calloc(n * 5, sizeof(double));
+  c->U1 = (double *)calloc(n * 5, sizeof(double));
+  c->R = (double *)calloc(n * 5, sizeof(double));
+  c->V1 = (double *)calloc(n * 5, sizeof(double));
+  c->V2 = (double *)calloc(n * 5, sizeof(double));
+  long nfx = (long)(c->nx + 1) * c->ny * c->nz, nfy = (long)c->nx * (c->ny + 1) * c->nz,
+       nfz = (long)c->nx * c->ny * (c->nz + 1);
+  c->Fx = (double *)calloc(nfx * 10, sizeof(double));
+  c->Fy = (double *)calloc(nfy * 10, sizeof(double));
+  c->Fz = (double *)calloc(nfz * 10, sizeof(double));
+  c->ferr = (int *)calloc((size_t)(c->nz + 1), sizeof(int));
+#ifdef _OPENMP
+  if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
+#endif
+  return c;
+}
+
+void orc_destroy(orc_ctx *c) {
+  if (!c) return;
+  for (int f = 0; f < 6; f++) free(c->face[f]);
+  free(c->Un); free(c->U1); free(c->R); free(c->V1); free(c->V2);
+  free(c->Fx); free(c->Fy); free(c->Fz); free(c->ferr);
+  orc_ls_free(c);
+  free(c);
+}
+
+void orc_counters(const orc_ctx *c, long *rev, long *clip, long *rerr) {
+  if (rev) *rev = c->n_reverted;
+  if (clip) *clip = c->n_clipped;
+  if (rerr) *rerr = c->n_riemann_err;
+}
+
+void orc_numerical_flux(const orc_ctx *c, int dir, const double *Vm, const double *Vp, int id, double *F) {
+  const orc_eos *m = &c->cfg.mat[id];
+  switch (c->cfg.flux) {
+  case ORC_FLUX_ROE: orc_flux_roe(m, dir, Vm, Vp, c->cfg.roe_delta, F); break;
+  case ORC_FLUX_LLF: orc_flux_llf2(m, m, dir, Vm, Vp, F); break;
+  case ORC_FLUX_GODUNOV: orc_flux_godunov(c, dir, Vm, Vp, id, F); break;
+  default: orc_flux_hllc(m, dir, Vm, Vp, F); break;
+  }
+}
+
+/* ------------------------------------------------------------------ ghost fill ---------- */
+
+/* image index of a ghost at -1 / n (SpaceOperator.cpp:449-480), periodic wraps */
+static inline int image_of(int i, int n, int periodic) {
+  if (i < 0) return periodic ? n + i : -i - 1;
+  if (i >= n) return periodic ? i - n : n + (i - n) - 1;
+  return i;
+}
+
+void orc_apply_bc(const orc_ctx *c, double *V) {
+  const orc_config *g = &c->cfg;
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int side = 0; side < 6; side++) {
+    int dim = side / 2, bc = g->bc[side];
+    int n0 = dim == 0 ? ny : nx, n1 = dim == 2 ? ny : nz;
+    for (int b = 0; b < n1; b++)
+      for (int a = 0; a < n0; a++) {
+        int i, j, k;
+        int gidx = (side % 2 == 0) ? -1 : (dim == 0 ? nx : dim == 1 ? ny : nz);
+        if (dim == 0) { i = gidx; j = a; k = b; }
+        else if (dim == 1) { i = a; j = gidx; k = b; }
+        else { i = a; j = b; k = gidx; }
+        double *v = V + 5 * ORC_IDX(c, i, j, k);
+        int ii = i, jj = j, kk = k;
+        if (dim == 0) ii = image_of(i, nx, bc == ORC_BC_PERIODIC);
+        if (dim == 1) jj = image_of(j, ny, bc == ORC_BC_PERIODIC);
+        if (dim == 2) kk = image_of(k, nz, bc == ORC_BC_PERIODIC);
+        const double *im = V + 5 * ORC_IDX(c, ii, jj, kk);
+        switch (bc) {
+        case ORC_BC_INLET: for (int p = 0; p < 5; p++) v[p] = g->inlet[p]; break;
+        case ORC_BC_OUTLET: for (int p = 0; p < 5; p++) v[p] = g->outlet[p]; break;
+        case ORC_BC_SLIPWALL: case ORC_BC_SYMMETRY:
+          for (int p = 0; p < 5; p++) v[p] = im[p];
+          v[dim + 1] *= -1.0;
+          break;
+        case ORC_BC_STICKWALL:
+          v[0] = im[0]; v[1] = -1.0 * im[1]; v[2] = -1.0 * im[2]; v[3] = -1.0 * im[3]; v[4] = im[4];
+          break;
+        case ORC_BC_PERIODIC: for (int p = 0; p < 5; p++) v[p] = im[p]; break;
+        default: break;
+        }
+      }
+  }
+}
+
+void orc_apply_bc_id(const orc_ctx *c, int *ID) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int side = 0; side < 6; side++) {
+    int dim = side / 2, per = c->cfg.bc[side] == ORC_BC_PERIODIC;
+    int n0 = dim == 0 ? ny : nx, n1 = dim == 2 ? ny : nz;
+    for (int b = 0; b < n1; b++)
+      for (int a = 0; a < n0; a++) {
+        int i, j, k;
+        int gidx = (side % 2 == 0) ? -1 : (dim == 0 ? nx : dim == 1 ? ny : nz);
+        if (dim == 0) { i = gidx; j = a; k = b; }
+        else if (dim == 1) { i = a; j = gidx; k = b; }
+        else { i = a; j = b; k = gidx; }
+        int ii = dim == 0 ? image_of(i, nx, per) : i;
+        int jj = dim == 1 ? image_of(j, ny, per) : j;
+        int kk = dim == 2 ? image_of(k, nz, per) : k;
+        ID[ORC_IDX(c, i, j, k)] = ID[ORC_IDX(c, ii, jj, kk)];
+      }
+  }
+}
+
+/* ------------------------------------------------------------------ reconstruction ------ */
+
+static inline double gen_minmod(double A, double B, double alpha, double dq0, double dq1) {
+  /* Reconstructor.h:95-102 */
+  if (dq0 < 0.0 && dq1 < 0.0) {
+    double a = alpha * dq0, b = B / (A + 1.0) * (dq0 + dq1), cc = alpha * dq1;
+    double m = (b < cc) ? cc : b;            /* std::max(b, cc) */
+    return (a < m) ? m : a;                  /* std::max(a, m)  */
+  } else if (dq0 > 0.0 && dq1 > 0.0) {
+    double a = alpha * dq0, b = B / (A + 1.0) * (dq0 + dq1), cc = alpha * dq1;
+    double m = (cc < b) ? cc : b;            /* std::min(b, cc) */
+    return (m < a) ? m : a;                  /* std::min(a, m)  */
+  }
+  return 0.0;
+}
+
+static inline double van_albada(double A, double B, int k, double dq0, double dq1, int use_pow) {
+  /* Reconstructor.h:104-110 */
+  if (dq0 * dq1 <= 0) return 0.0;
+  if (fabs(dq0 * dq1) < 1e-16) return 0.0;
+  double d0k, d1k;
+  if (use_pow) { d0k = pow(dq0, k); d1k = pow(dq1, k); }
+  else {
+    d0k = dq0; d1k = dq1;
+    for (int t = 1; t < k; t++) { d0k *= dq0; d1k *= dq1; }
+  }
+  return B * (dq1 * d0k + dq0 * d1k) / (d0k + A * d1k);
+}
+
+/* Reconstruct one interior cell: Reconstructor.cpp:423-779 (primitive variables). */
+static void recon_cell(orc_ctx *c, const double *V, const int *ID, int i, int j, int k, long *nrev) {
+  const orc_config *g = &c->cfg;
+  long me = ORC_IDX(c, i, j, k);
+  const double *v = V + 5 * me;
+  double *out[6];
+  for (int f = 0; f < 6; f++) out[f] = c->face[f] + 5 * me;
+  int myid = ID[me];
+  if (g->rec_type == ORC_REC_CONSTANT || myid == g->inactive_id) {
+    for (int f = 0; f < 6; f++) memcpy(out[f], v, 5 * sizeof(double));
+    return;
+  }
+  long nb[6] = {ORC_IDX(c, i - 1, j, k), ORC_IDX(c, i + 1, j, k), ORC_IDX(c, i, j - 1, k),
+                ORC_IDX(c, i, j + 1, k), ORC_IDX(c, i, j, k - 1), ORC_IDX(c, i, j, k + 1)};
+  double sig[3][5];
+  for (int d = 0; d < 3; d++) {
+    long lo = nb[2 * d], hi = nb[2 * d + 1];
+    for (int p = 0; p < 5; p++) {
+      double dq0 = (ID[lo] == g->inactive_id) ? 0.0 : v[p] - V[5 * lo + p];
+      double dq1 = (ID[hi] == g->inactive_id) ? 0.0 : V[5 * hi + p] - v[p];
+      double s;
+      switch (g->limiter) {
+      case ORC_LIM_GENMINMOD: s = gen_minmod(c->A[d], c->B[d], g->gminmod_alpha, dq0, dq1); break;
+      case ORC_LIM_VANALBADA: s = van_albada(c->A[d], c->B[d], c->K[d], dq0, dq1, g->vanalbada_pow); break;
+      default: s = 0.5 * (dq0 + dq1); break;
+      }
+      sig[d][p] = s;
+    }
+    if (g->slope_zero_near_interface && (myid != ID[lo] || myid != ID[hi]))
+      for (int p = 0; p < 5; p++) sig[d][p] = 0.0;
+  }
+  for (int p = 0; p < 5; p++) {
+    out[0][p] = v[p] - 0.5 * sig[0][p];
+    out[1][p] = v[p] + 0.5 * sig[0][p];
+    out[2][p] = v[p] - 0.5 * sig[1][p];
+    out[3][p] = v[p] + 0.5 * sig[1][p];
+    out[4][p] = v[p] - 0.5 * sig[2][p];
+    out[5][p] = v[p] + 0.5 * sig[2][p];
+  }
+  /* Reconstructor.cpp:719-775: the first face that fails CheckState reverts to the cell value */
+  const orc_eos *m = &g->mat[myid];
+  for (int f = 0; f < 6; f++)
+    if (orc_check_state(m, out[f])) {
+      memcpy(out[f], v, 5 * sizeof(double));
+      (*nrev)++;
+      break;
+    }
+}
+
+/* Ghost face states (Reconstructor.cpp:795-875); periodic faces copy the wrapped cell's face. */
+static void recon_ghost_faces(orc_ctx *c, const double *V) {
+  const orc_config *g = &c->cfg;
+  int n[3] = {c->nx, c->ny, c->nz};
+  for (int side = 0; side < 6; side++) {
+    int dim = side / 2, hi = side % 2, bc = g->bc[side];
+    int n0 = dim == 0 ? c->ny : c->nx, n1 = dim == 2 ? c->ny : c->nz;
+    int gface = hi ? 2 * dim : 2 * dim + 1;     /* face of the ghost pointing into the domain */
+    int iface = hi ? 2 * dim + 1 : 2 * dim;     /* matching face of the image cell */
+    for (int b = 0; b < n1; b++)
+      for (int a = 0; a < n0; a++) {
+        int ijk[3], im[3];
+        if (dim == 0) { ijk[1] = a; ijk[2] = b; }
+        else if (dim == 1) { ijk[0] = a; ijk[2] = b; }
+        else { ijk[0] = a; ijk[1] = b; }
+        ijk[dim] = hi ? n[dim] : -1;
+        im[0] = ijk[0]; im[1] = ijk[1]; im[2] = ijk[2];
+        im[dim] = image_of(ijk[dim], n[dim], bc == ORC_BC_PERIODIC);
+        long gi = ORC_IDX(c, ijk[0], ijk[1], ijk[2]), ii = ORC_IDX(c, im[0], im[1], im[2]);
+        double *dst = c->face[gface] + 5 * gi;
+        const double *src = c->face[iface] + 5 * ii;
+        switch (bc) {
+        case ORC_BC_INLET: case ORC_BC_OUTLET: case ORC_BC_OVERSET:
+          memcpy(dst, V + 5 * gi, 5 * sizeof(double)); break;
+        case ORC_BC_SYMMETRY: case ORC_BC_SLIPWALL:
+          for (int p = 0; p < 5; p++) dst[p] = (p == dim + 1) ? -src[p] : src[p];
+          break;
+        case ORC_BC_STICKWALL:
+          for (int p = 0; p < 5; p++) dst[p] = (p >= 1 && p <= 3) ? -src[p] : src[p];
+          break;
+        case ORC_BC_PERIODIC: {
+          /* the periodic image of ghost -1 is cell n-1; its face toward the domain is its
+           * high face (and vice versa) */
+          const double *ps = c->face[gface] + 5 * ii;
+          memcpy(dst, ps, 5 * sizeof(double));
+          break;
+        }
+        default: break;
+        }
+      }
+  }
+}
+
+/* SpaceOperator.cpp:3198-3347. Returns 1 on a hyperbolicity violation. */
+static int check_reconstructed(orc_ctx *c, const double *V, const int *ID, long *nclip) {
+  const orc_config *g = &c->cfg;
+  int err = 0;
+  long clipped = 0;
+#pragma omp parallel for reduction(+ : clipped) reduction(| : err) schedule(static)
+  for (int k = -1; k <= c->nz; k++)
+    for (int j = -1; j <= c->ny; j++)
+      for (int i = -1; i <= c->nx; i++) {
+        int boundary = (k == -1 || k == c->nz) + (j == -1 || j == c->ny) + (i == -1 || i == c->nx);
+        if (boundary >= 2) continue;
+        long me = ORC_IDX(c, i, j, k);
+        int myid = ID[me];
+        if (myid == g->inactive_id) continue;
+        const orc_eos *m = &g->mat[myid];
+        const double *v = V + 5 * me;
+        if (boundary == 0) {
+          for (int f = 0; f < 6; f++) {
+            double *fs = c->face[f] + 5 * me;
+            if (orc_clip(m, fs)) { memcpy(fs, v, 5 * sizeof(double)); clipped++; }
+          }
+          for (int f = 0; f < 6; f++) err |= orc_check_state(m, c->face[f] + 5 * me);
+        } else {
+          int f;
+          if (i == -1) f = 1; else if (i == c->nx) f = 0;
+          else if (j == -1) f = 3; else if (j == c->ny) f = 2;
+          else if (k == -1) f = 5; else f = 4;
+          double *fs = c->face[f] + 5 * me;
+          if (orc_clip(m, fs)) { memcpy(fs, v, 5 * sizeof(double)); clipped++; }
+          err |= orc_check_state(m, fs);
+        }
+      }
+  *nclip += clipped;
+  return err;
+}
+
+/* ------------------------------------------------------------------ residual ------------ */
+
+/* one face: localflux1 (to the minus cell) and localflux2 (to the plus cell) */
+static int face_flux(orc_ctx *c, int dir, const double *V, const int *ID, long lo, long hi,
+                     double *lf1, double *lf2) {
+  const orc_config *g = &c->cfg;
+  int nbid = ID[lo], myid = ID[hi];
+  const double *vm = c->face[2 * dir + 1] + 5 * lo;   /* vr / vt / vf of the minus cell */
+  const double *vp = c->face[2 * dir] + 5 * hi;       /* vl / vb / vk of the plus cell  */
+  if (nbid != myid) {
+    if (nbid != g->inactive_id && myid != g->inactive_id) {
+      const double *am = (g->mp_recon == ORC_REC_CONSTANT) ? V + 5 * lo : vm;
+      const double *ap = (g->mp_recon == ORC_REC_CONSTANT) ? V + 5 * hi : vp;
+      if (g->mp_flux == ORC_MPFLUX_LLF) {
+        orc_flux_llf2(&g->mat[nbid], &g->mat[myid], dir, am, ap, lf1);
+        memcpy(lf2, lf1, 5 * sizeof(double));
+        return 0;
+      }
+      return orc_interface_flux(c, dir, am, nbid, ap, myid, lo, hi, lf1, lf2);
+    }
+    memset(lf1, 0, 5 * sizeof(double));
+    memset(lf2, 0, 5 * sizeof(double));
+    return 0;
+  }
+  if (myid != g->inactive_id) {
+    orc_numerical_flux(c, dir, vm, vp, myid, lf1);
+    memcpy(lf2, lf1, 5 * sizeof(double));
+  } else {
+    memset(lf1, 0, 5 * sizeof(double));
+    memset(lf2, 0, 5 * sizeof(double));
+  }
+  return 0;
+}
+
+int orc_compute_residual(orc_ctx *c, const double *V, const int *ID, double *R) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  long nrev = 0;
+  orc_riemann_begin_stage(c);
+#pragma omp parallel for reduction(+ : nrev) schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) recon_cell(c, V, ID, i, j, k, &nrev);
+  c->n_reverted += nrev;
+  recon_ghost_faces(c, V);
+  if (check_reconstructed(c, V, ID, &c->n_clipped)) return -1;
+
+  long rerr = 0;
+#pragma omp parallel for reduction(+ : rerr) schedule(static)
+  for (int k = 0; k <= nz; k++) {
+    for (int j = 0; j <= ny; j++)
+      for (int i = 0; i <= nx; i++) {
+        if (j < ny && k < nz) {      /* x face i-1/2 */
+          double *F = c->Fx + 10 * (((long)k * ny + j) * (nx + 1) + i);
+          rerr += face_flux(c, 0, V, ID, ORC_IDX(c, i - 1, j, k), ORC_IDX(c, i, j, k), F, F + 5) != 0;
+        }
+        if (i < nx && k < nz) {      /* y face j-1/2 */
+          double *F = c->Fy + 10 * (((long)k * (ny + 1) + j) * nx + i);
+          rerr += face_flux(c, 1, V, ID, ORC_IDX(c, i, j - 1, k), ORC_IDX(c, i, j, k), F, F + 5) != 0;
+        }
+        if (i < nx && j < ny) {      /* z face k-1/2 */
+          double *F = c->Fz + 10 * (((long)k * ny + j) * nx + i);
+          rerr += face_flux(c, 2, V, ID, ORC_IDX(c, i, j, k - 1), ORC_IDX(c, i, j, k), F, F + 5) != 0;
+        }
+      }
+  }
+  c->n_riemann_err += rerr;
+  orc_riemann_end_stage(c);
+
+  const double ax = c->area[0], ay = c->area[1], az = c->area[2];
+  const double inv = 1.0 / (-c->vol);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const double *Fw = c->Fx + 10 * (((long)k * ny + j) * (nx + 1) + i) + 5;
+        const double *Fe = c->Fx + 10 * (((long)k * ny + j) * (nx + 1) + i + 1);
+        const double *Fs = c->Fy + 10 * (((long)k * (ny + 1) + j) * nx + i) + 5;
+        const double *Fn = c->Fy + 10 * (((long)k * (ny + 1) + j + 1) * nx + i);
+        const double *Fb = c->Fz + 10 * (((long)k * ny + j) * nx + i) + 5;
+        const double *Ft = c->Fz + 10 * (((long)(k + 1) * ny + j) * nx + i);
+        double *r = R + 5 * ORC_IDX(c, i, j, k);
+        for (int p = 0; p < 5; p++) {
+          double s = 0.0;
+          s -= Fw[p] * ax;
+          s -= Fs[p] * ay;
+          s -= Fb[p] * az;
+          s += Fe[p] * ax;
+          s += Fn[p] * ay;
+          s += Ft[p] * az;
+          r[p] = s * inv;
+        }
+      }
+  return (int)rerr;
+}
+
+/* ------------------------------------------------------------------ time step size ------ */
+
+int orc_time_step(const orc_ctx *c, const double *V, const int *ID, double *dt) {
+  const orc_config *g = &c->cfg;
+  double mn = DBL_MAX;
+  int bad = 0;
+#pragma omp parallel for reduction(min : mn) reduction(| : bad) schedule(static)
+  for (int k = 0; k < c->nz; k++)
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        long me = ORC_IDX(c, i, j, k);
+        int id = ID[me];
+        if (id == g->inactive_id) continue;
+        const orc_eos *m = &g->mat[id];
+        const double *v = V + 5 * me;
+        double c2 = orc_c2(m, v[0], orc_internal_energy(m, v[0], v[4]));
+        if (c2 < 0) { bad = 1; continue; }
+        double cs = sqrt(c2);
+        double lf = fabs(v[1]) + cs, lg = fabs(v[2]) + cs, lh = fabs(v[3]) + cs;
+        double a = c->dx / lf, b = c->dy / lg, cc = c->dz / lh;
+        double m2 = (cc < b) ? cc : b;
+        double m1 = (m2 < a) ? m2 : a;
+        mn = (m1 < mn) ? m1 : mn;
+      }
+  *dt = g->cfl * mn;
+  return bad ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ RK stages ----------- */
+
+static void p2c_all(orc_ctx *c, const double *V, const int *ID, double *U) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < c->nz; k++)
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        long me = ORC_IDX(c, i, j, k);
+        orc_p2c(&c->cfg.mat[ID[me]], V + 5 * me, U + 5 * me);
+      }
+}
+
+/* y = a*y + b*x over the interior (SpaceVariable.cpp:377-402) */
+static void axpby(orc_ctx *c, double *y, double a, double b, const double *x) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < c->nz; k++)
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        long me = 5 * ORC_IDX(c, i, j, k);
+        for (int p = 0; p < 5; p++) y[me + p] = a * y[me + p] + b * x[me + p];
+      }
+}
+
+/* C2P over the interior + ClipDensityAndPressure with CheckState (SpaceOperator.cpp:553-649).
+ * Returns the number of clipped cells, or -1 on a hyperbolicity violation. */
+static long c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V) {
+  long nclip = 0;
+  int bad = 0;
+#pragma omp parallel for reduction(+ : nclip) reduction(| : bad) schedule(static)
+  for (int k = 0; k < c->nz; k++)
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        long me = ORC_IDX(c, i, j, k);
+        const orc_eos *m = &c->cfg.mat[ID[me]];
+        orc_c2p(m, U + 5 * me, V + 5 * me);
+        nclip += orc_clip(m, V + 5 * me);
+        bad |= orc_check_state(m, V + 5 * me);
+      }
+  c->n_clipped += nclip;
+  return bad ? -1 : nclip;
+}
+
+int orc_advance(orc_ctx *c, double *V, const int *ID, double dt) {
+  long n = orc_ncells_ghosted(c);
+  double *R = c->R, *Un = c->Un, *U1 = c->U1, *V1 = c->V1, *V2 = c->V2;
+  int ts = c->cfg.ts;
+  int st = orc_compute_residual(c, V, ID, R);
+  if (st < 0) return -1;
+  p2c_all(c, V, ID, Un);
+  if (ts == ORC_TS_FE) {
+    axpby(c, Un, 1.0, dt, R);
+    if (c2p_clip(c, Un, ID, V) < 0) return -1;
+    orc_apply_bc(c, V);
+    return 0;
+  }
+  memset(U1, 0, n * 5 * sizeof(double));
+  axpby(c, U1, 0.0, 1.0, Un);
+  axpby(c, U1, 1.0, dt, R);
+  memcpy(V1, V, n * 5 * sizeof(double));        /* ghosts of V1 are refreshed by the BC below */
+  long cl = c2p_clip(c, U1, ID, V1);
+  if (cl < 0) return -1;
+  if (cl) p2c_all(c, V1, ID, U1);
+  orc_apply_bc(c, V1);
+  st = orc_compute_residual(c, V1, ID, R);
+  if (st < 0) return -1;
+  if (ts == ORC_TS_RK2) {
+    axpby(c, U1, 0.5, 0.5, Un);
+    axpby(c, U1, 1.0, 0.5 * dt, R);
+    if (c2p_clip(c, U1, ID, V) < 0) return -1;
+    orc_apply_bc(c, V);
+    return 0;
+  }
+  /* RK3 (Gottlieb-Shu), TimeIntegrator.cpp:583-649 */
+  axpby(c, U1, 0.25, 0.75, Un);
+  axpby(c, U1, 1.0, 0.25 * dt, R);
+  memcpy(V2, V1, n * 5 * sizeof(double));
+  cl = c2p_clip(c, U1, ID, V2);
+  if (cl < 0) return -1;
+  if (cl) p2c_all(c, V2, ID, U1);
+  orc_apply_bc(c, V2);
+  st = orc_compute_residual(c, V2, ID, R);
+  if (st < 0) return -1;
+  axpby(c, U1, 2.0 / 3.0, 1.0 / 3.0, Un);
+  axpby(c, U1, 1.0, 2.0 / 3.0 * dt, R);
+  if (c2p_clip(c, U1, ID, V) < 0) return -1;
+  orc_apply_bc(c, V);
+  return 0;
+}
+
+int orc_run(orc_ctx *c, double *V, int *ID, double tmax, int maxsteps, double *t_out, int *steps_out) {
+  double t = 0.0;
+  int step = 0;
+  while (t < tmax && step < maxsteps) {
+    step++;
+    double dt;
+    if (orc_time_step(c, V, ID, &dt) < 0) return -1;
+    if (t + dt > tmax) dt = tmax - t;
+    t += dt;
+    int st = orc_step_all(c, V, ID, dt, step);
+    if (st < 0) { *t_out = t; *steps_out = step; return -1; }
+  }
+  *t_out = t;
+  *steps_out = step;
+  return 0;
+}

@@ -1,0 +1,92 @@
+/*
+ * oracle/m2c_oracle.h -- CPU restatement of M2C's explicit finite-volume update.
+ *
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * load liboracle.so as the checker. The product (m2c_amd/, include/) never links it.
+ *
+ * Data layout mirrors the reference's SpaceVariable3D (SpaceVariable.cpp:254-260): array-of-
+ * structures, k-j-i, x fastest, one ghost layer at index -1 and N in every direction.
+ * Arrays passed in/out of this API hold the ghosted box: (nx+2)*(ny+2)*(nz+2) cells.
+ */
+#ifndef M2C_ORACLE_H
+#define M2C_ORACLE_H
+
+#include "eos.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_MAT 8
+
+/* boundary codes: IoData.h:266-267 (NONE..OVERSET), plus PERIODIC as an extension */
+enum { ORC_BC_NONE = 0, ORC_BC_INLET = 1, ORC_BC_OUTLET = 2, ORC_BC_SLIPWALL = 3, ORC_BC_STICKWALL = 4,
+       ORC_BC_SYMMETRY = 5, ORC_BC_OVERSET = 6, ORC_BC_PERIODIC = 7 };
+enum { ORC_REC_CONSTANT = 0, ORC_REC_LINEAR = 1 };
+enum { ORC_LIM_NONE = 0, ORC_LIM_GENMINMOD = 1, ORC_LIM_VANALBADA = 2 };
+enum { ORC_TS_FE = 0, ORC_TS_RK2 = 1, ORC_TS_RK3 = 2 };
+enum { ORC_MPFLUX_NUMERICAL = 0, ORC_MPFLUX_EXACT = 1, ORC_MPFLUX_LLF = 2 };
+
+typedef struct {
+  /* Riemann solver controls, IoData.cpp:1345-1357 (ExactRiemannSolverData) */
+  int maxIts_main, maxIts_bracket, maxIts_shock, numSteps_rarefaction;
+  double tol_main, tol_shock, tol_rarefaction, min_pressure, failure_threshold, pressure_at_failure;
+  double surrogate_residual_tol;
+} orc_riemann_cfg;
+
+typedef struct {
+  int nx, ny, nz;
+  double x0, x1, y0, y1, z0, z1;
+  int bc[6];                /* x0, xmax, y0, ymax, z0, zmax */
+  double inlet[5], outlet[5];
+  int nmat;                 /* number of entries in mat[], the last one may be the inactive dummy */
+  int inactive_id;          /* -1 if no inactive material */
+  orc_eos mat[ORC_MAX_MAT];
+  int flux;                 /* ORC_FLUX_* */
+  double roe_delta;         /* Harten entropy-fix coefficient (IoData.cpp:1158) */
+  int rec_type, limiter, slope_zero_near_interface;
+  double gminmod_alpha;
+  int vanalbada_pow;        /* 1: pow(dq,k) exactly as Reconstructor.h:107-108; 0: dq*dq */
+  int ts;                   /* ORC_TS_* */
+  double cfl;
+  int mp_flux, mp_recon;    /* MultiPhaseData flux / recon (IoData.cpp:1400-1416) */
+  orc_riemann_cfg riemann;
+  int nthreads;
+} orc_config;
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx *orc_create(const orc_config *cfg);
+void orc_destroy(orc_ctx *c);
+long orc_ncells_ghosted(const orc_ctx *c);
+
+/* Pointwise entry points (for pointwise parity tests). */
+void orc_numerical_flux(const orc_ctx *c, int dir, const double *Vm, const double *Vp, int id, double *F);
+
+/* SpaceOperator::ApplyBoundaryConditions (SpaceOperator.cpp:1667-1745) on a ghosted V. */
+void orc_apply_bc(const orc_ctx *c, double *V);
+/* MultiPhaseOperator::UpdateMaterialIDAtGhostNodes (MultiPhaseOperator.cpp:796-819). */
+void orc_apply_bc_id(const orc_ctx *c, int *ID);
+
+/* SpaceOperator::ComputeResidual (SpaceOperator.cpp:3351-3409), single stage.
+ * R is ghosted (only interior entries are meaningful). Returns the number of Riemann errors,
+ * or -1 if a reconstructed state violates hyperbolicity (the reference exits). */
+int orc_compute_residual(orc_ctx *c, const double *V, const int *ID, double *R);
+
+/* SpaceOperator::ComputeTimeStepSize (SpaceOperator.cpp:2202-2227). Returns 0 or -1 (c^2<0). */
+int orc_time_step(const orc_ctx *c, const double *V, const int *ID, double *dt);
+
+/* One time step (TimeIntegrator.cpp FE :150-226, RK2 :291-428, RK3 :496-690) for the
+ * Navier-Stokes part. V must have valid ghosts; on return V = V(n+1) with ghosts filled. */
+int orc_advance(orc_ctx *c, double *V, const int *ID, double dt);
+
+/* Main.cpp:514-621 time loop: runs until t >= tmax or nsteps steps. */
+int orc_run(orc_ctx *c, double *V, int *ID, double tmax, int maxsteps, double *t_out, int *steps_out);
+
+/* Diagnostics counters of the last call (reverted face states, clipped cells). */
+void orc_counters(const orc_ctx *c, long *reverted_faces, long *clipped_cells, long *riemann_errors);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,203 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+leg of bench.py as the checker. The product package never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+EOS_SG, EOS_NASG, EOS_MG, EOS_JWL, EOS_DUMMY = 0, 1, 2, 3, 5
+BC_INLET, BC_OUTLET, BC_SLIPWALL, BC_STICKWALL, BC_SYMMETRY, BC_OVERSET, BC_PERIODIC = 1, 2, 3, 4, 5, 6, 7
+FLUX_ROE, FLUX_LLF, FLUX_HLLC, FLUX_GODUNOV = 0, 1, 2, 3
+LIM_NONE, LIM_GENMINMOD, LIM_VANALBADA = 0, 1, 2
+TS_FE, TS_RK2, TS_RK3 = 0, 1, 2
+
+
+class Eos(C.Structure):
+    _fields_ = [("type", C.c_int)] + [(n, C.c_double) for n in (
+        "rhomin", "pmin", "rhomax", "pmax", "failsafe_density",
+        "gam", "pstiff", "gam1",
+        "pc", "b", "q", "gam_pc",
+        "rho0", "c0", "Gamma0", "s", "e0", "rho0_c0_c0", "Gamma0_over_2", "Gamma0_rho0",
+        "omega", "A1", "A2", "R1", "R2", "R1rho0", "R2rho0", "omega_over_R1rho0", "omega_over_R2rho0",
+        "d_p0", "d_rho0", "d_e0")]
+
+
+class RiemannCfg(C.Structure):
+    _fields_ = [("maxIts_main", C.c_int), ("maxIts_bracket", C.c_int), ("maxIts_shock", C.c_int),
+                ("numSteps_rarefaction", C.c_int)] + [(n, C.c_double) for n in (
+                    "tol_main", "tol_shock", "tol_rarefaction", "min_pressure", "failure_threshold",
+                    "pressure_at_failure", "surrogate_residual_tol")]
+
+
+class Config(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("x0", C.c_double), ("x1", C.c_double), ("y0", C.c_double), ("y1", C.c_double),
+                ("z0", C.c_double), ("z1", C.c_double),
+                ("bc", C.c_int * 6), ("inlet", C.c_double * 5), ("outlet", C.c_double * 5),
+                ("nmat", C.c_int), ("inactive_id", C.c_int), ("mat", Eos * 8),
+                ("flux", C.c_int), ("roe_delta", C.c_double),
+                ("rec_type", C.c_int), ("limiter", C.c_int), ("slope_zero_near_interface", C.c_int),
+                ("gminmod_alpha", C.c_double), ("vanalbada_pow", C.c_int),
+                ("ts", C.c_int), ("cfl", C.c_double),
+                ("mp_flux", C.c_int), ("mp_recon", C.c_int),
+                ("riemann", RiemannCfg), ("nthreads", C.c_int)]
+
+
+def default_riemann():
+    """ExactRiemannSolverData defaults, IoData.cpp:1345-1357."""
+    r = RiemannCfg()
+    r.maxIts_main, r.maxIts_bracket, r.maxIts_shock, r.numSteps_rarefaction = 200, 100, 200, 200
+    r.tol_main, r.tol_shock, r.tol_rarefaction = 1e-4, 1e-12, 1e-8
+    r.min_pressure, r.failure_threshold, r.pressure_at_failure = -1e8, 0.2, 1e-8
+    r.surrogate_residual_tol = 1e-6
+    return r
+
+
+def sg(gamma, pinf=0.0, rhomin=0.0, pmin=-1.7976931348623157e308):
+    e = Eos()
+    e.type = EOS_SG
+    e.gam, e.pstiff = gamma, pinf
+    e.rhomin, e.pmin = rhomin, pmin
+    e.rhomax = e.pmax = 1.7976931348623157e308
+    e.failsafe_density = 1e-10
+    return e
+
+
+def eos(kind, **kw):
+    e = Eos()
+    e.type = kind
+    e.rhomin, e.pmin = 0.0, -1.7976931348623157e308
+    e.rhomax = e.pmax = 1.7976931348623157e308
+    e.failsafe_density = 1e-10
+    for k, v in kw.items():
+        setattr(e, k, v)
+    return e
+
+
+def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 0, 0, 1),
+                flux=FLUX_HLLC, limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True,
+                vanalbada_pow=True, inactive_id=-1, nthreads=0, roe_delta=0.2, gminmod_alpha=1.2,
+                slope_zero=False, mp_flux=0, mp_recon=0):
+    c = Config()
+    c.nx, c.ny, c.nz = nx, ny, nz
+    c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
+    for i in range(6):
+        c.bc[i] = bc[i]
+    for i in range(5):
+        c.inlet[i] = inlet[i]
+        c.outlet[i] = outlet[i]
+    c.nmat = len(mats)
+    for i, m in enumerate(mats):
+        c.mat[i] = m
+    c.inactive_id = inactive_id
+    c.flux, c.roe_delta = flux, roe_delta
+    c.rec_type = 1 if rec_linear else 0
+    c.limiter, c.gminmod_alpha = limiter, gminmod_alpha
+    c.slope_zero_near_interface = int(slope_zero)
+    c.vanalbada_pow = int(vanalbada_pow)
+    c.ts, c.cfl = ts, cfl
+    c.mp_flux, c.mp_recon = mp_flux, mp_recon
+    c.riemann = default_riemann()
+    c.nthreads = nthreads
+    return c
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.POINTER(C.c_double)
+        I = C.POINTER(C.c_int)
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [C.POINTER(Config)]
+        L.orc_destroy.argtypes = [P]
+        L.orc_ncells_ghosted.restype = C.c_long
+        L.orc_ncells_ghosted.argtypes = [P]
+        L.orc_numerical_flux.argtypes = [P, C.c_int, D, D, C.c_int, D]
+        L.orc_apply_bc.argtypes = [P, D]
+        L.orc_apply_bc_id.argtypes = [P, I]
+        L.orc_compute_residual.argtypes = [P, D, I, D]
+        L.orc_time_step.argtypes = [P, D, I, D]
+        L.orc_advance.argtypes = [P, D, I, C.c_double]
+        L.orc_run.argtypes = [P, D, I, C.c_double, C.c_int, D, I]
+        L.orc_counters.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Oracle:
+    """Ghosted AoS state V[k+1, j+1, i+1, 5] and integer IDs, as the reference stores them."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = lib()
+        self.h = self.L.orc_create(C.byref(cfg))
+        self.shape = (cfg.nz + 2, cfg.ny + 2, cfg.nx + 2)
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def ghosted(self, V_int, ID_int=None):
+        """Embed interior arrays (nz,ny,nx,5) / (nz,ny,nx) into ghosted ones and fill ghosts."""
+        V = np.zeros(self.shape + (5,), dtype=np.float64)
+        V[1:-1, 1:-1, 1:-1] = V_int
+        ID = np.zeros(self.shape, dtype=np.int32)
+        if ID_int is not None:
+            ID[1:-1, 1:-1, 1:-1] = ID_int
+        self.L.orc_apply_bc_id(self.h, _ip(ID))
+        self.L.orc_apply_bc(self.h, _dp(V))
+        return V, ID
+
+    def flux(self, d, Vm, Vp, mid=0):
+        F = np.zeros(5)
+        Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+        Vp = np.ascontiguousarray(Vp, dtype=np.float64)
+        self.L.orc_numerical_flux(self.h, d, _dp(Vm), _dp(Vp), mid, _dp(F))
+        return F
+
+    def residual(self, V, ID):
+        R = np.zeros_like(V)
+        st = self.L.orc_compute_residual(self.h, _dp(V), _ip(ID), _dp(R))
+        return st, R
+
+    def time_step(self, V, ID):
+        dt = C.c_double()
+        st = self.L.orc_time_step(self.h, _dp(V), _ip(ID), C.byref(dt))
+        return st, dt.value
+
+    def advance(self, V, ID, dt):
+        return self.L.orc_advance(self.h, _dp(V), _ip(ID), dt)
+
+    def run(self, V, ID, tmax, maxsteps):
+        t = C.c_double()
+        n = C.c_int()
+        st = self.L.orc_run(self.h, _dp(V), _ip(ID), tmax, maxsteps, C.byref(t), C.byref(n))
+        return st, t.value, n.value
+
+    def counters(self):
+        a, b, c = C.c_long(), C.c_long(), C.c_long()
+        self.L.orc_counters(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
