@@ -83,6 +83,7 @@ def main():
     # stage-kernel duration with HIP events on the library's stream
     nprof = max(5, min(args.steps, 50))
     ms_stage, ms_step_ev = s.time_steps(nprof)
+    copy_gbs, copy_ms = M.stream_copy_GBs(s, 1 << 30, 20)
     s.close()
     achieved = BYTES_PER_CELL_UPDATE / 3.0 * cells / (ms_stage * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -93,7 +94,8 @@ def main():
             "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"periodic_modes_{args.n}^3_rk3_hllc_vanalbada_fp64", "cells": cells,
-                       "precision": "fp64", "hbm_frac_whole_step": value * BYTES_PER_CELL_UPDATE / 1e9 / HBM_PEAK_GBS},
+                       "precision": "fp64", "hbm_frac_whole_step": value * BYTES_PER_CELL_UPDATE / 1e9 / HBM_PEAK_GBS,
+                       "measured_copy_GBs": copy_gbs, "copy_kernel_ms_per_GiB": copy_ms},
             "roofline": roof}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(settings, V0, ID0, args.cpu_steps)

@@ -125,6 +125,12 @@ int m2c_flux_batch(m2c_ctx *ctx, int dir, int n, const double *Vm, const double 
  * recorded on the context's stream around every stage launch and around the whole loop. */
 int m2c_time_steps(m2c_ctx *ctx, int nsteps, double *ms_stage_kernel, double *ms_step);
 
+/* Measured HBM copy bandwidth: a plain device copy kernel moves `nbytes` (read + write) `reps`
+ * times between two buffers of that size; returns the average kernel duration (ms), timed with
+ * HIP events on the context's stream. Also the known-size calibration kernel for traffic
+ * counters. */
+int m2c_stream_copy(m2c_ctx *ctx, long long nbytes, int reps, double *ms_avg);
+
 /* Raw device pointer of the primitive state (SoA, 5*nz*ny*nx doubles) for zero-copy callers. */
 int m2c_device_state(m2c_ctx *ctx, double **V_soa);
 

@@ -124,6 +124,7 @@ def lib():
         L.m2c_flux_batch.argtypes = [P, C.c_int, C.c_int, D, D, C.c_int, D]
         L.m2c_time_steps.argtypes = [P, C.c_int, D, D]
         L.m2c_device_state.argtypes = [P, C.POINTER(D)]
+        L.m2c_stream_copy.argtypes = [P, C.c_longlong, C.c_int, D]
         L.m2c_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -216,6 +217,13 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         _chk(self.L.m2c_time_steps(self.h, nsteps, C.byref(a), C.byref(b)), "m2c_time_steps")
         return a.value, b.value
+
+
+def stream_copy_GBs(solver, nbytes=1 << 30, reps=20):
+    """Measured copy bandwidth (read + write bytes per second, GB/s) of a plain copy kernel."""
+    ms = C.c_double()
+    _chk(solver.L.m2c_stream_copy(solver.h, nbytes, reps, C.byref(ms)), "m2c_stream_copy")
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
 
 
 def version():

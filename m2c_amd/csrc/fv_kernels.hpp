@@ -360,32 +360,38 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
     fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 2, w2, id2, flags);
   }
 
-  for (int k = kbeg; k < kend; k++) {
-    // ---- prefetch V(k+3) for the next plane (consumed at the end of the iteration)
-    double w3[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    int id3 = 0;
-    if (k + 1 < kend) fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 3, w3, id3, flags);
+  // halo cell owned by this thread (x halo: 4 columns x TY rows, then y halo: 4 rows x TX columns)
+  int hx = 0, hy = 0;
+  if (tid < 2 * HALO * TY) {
+    int c = tid % (2 * HALO), r = tid / (2 * HALO);
+    hx = (c < HALO) ? c - HALO : TX + (c - HALO);
+    hy = r;
+  } else if (tid < NHALO) {
+    int t = tid - 2 * HALO * TY;
+    int r = t / TX, c = t % TX;
+    hy = (r < HALO) ? r - HALO : TY + (r - HALO);
+    hx = c;
+  }
+  const int hi = min(i0 + hx, g.nx + 1), hj = min(j0 + hy, g.ny + 1);
+  double hv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int hid = 0;
+  if (tid < NHALO) fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, kbeg, hv, hid, flags);
 
-    // ---- plane k into LDS: own cells from registers, halo from memory
+  for (int k = kbeg; k < kend; k++) {
+    // ---- prefetch V(k+3) of the column and the halo cell of plane k+1 (consumed at the end)
+    double w3[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double hvn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int id3 = 0, hidn = 0;
+    if (k + 1 < kend) {
+      fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 3, w3, id3, flags);
+      if (tid < NHALO) fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, k + 1, hvn, hidn, flags);
+    }
+
+    // ---- plane k into LDS: own cells and the halo cell from registers
 #pragma unroll
     for (int p = 0; p < 5; p++) sV[p][ty + HALO][tx + HALO] = w0[p];
     sID[ty + HALO][tx + HALO] = id0;
     if (tid < NHALO) {
-      int hx, hy;
-      if (tid < 2 * HALO * TY) {            // x halo: 4 columns x TY rows
-        int c = tid % (2 * HALO), r = tid / (2 * HALO);
-        hx = (c < HALO) ? c - HALO : TX + (c - HALO);
-        hy = r;
-      } else {                              // y halo: 4 rows x TX columns
-        int t = tid - 2 * HALO * TY;
-        int r = t / TX, c = t % TX;
-        hy = (r < HALO) ? r - HALO : TY + (r - HALO);
-        hx = c;
-      }
-      int hi = min(i0 + hx, g.nx + 1), hj = min(j0 + hy, g.ny + 1);
-      double hv[5];
-      int hid;
-      fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, k, hv, hid, flags);
 #pragma unroll
       for (int p = 0; p < 5; p++) sV[p][hy + HALO][hx + HALO] = hv[p];
       sID[hy + HALO][hx + HALO] = hid;
@@ -433,11 +439,17 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
     //      x faces and the north row of y faces
     tile_face<FLUX, EK, 0>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
     tile_face<FLUX, EK, 1>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
-    if (tid < TY) {
-      tile_face<FLUX, EK, 0>(sV, sID, sS, sFx, TX, tid, tid * (TX + 1) + TX, i0, j0, nrev);
-    } else if (tid < TY + TX) {
-      int x = tid - TY;
-      tile_face<FLUX, EK, 1>(sV, sID, sS, sFy, x, TY, TY * TX + x, i0, j0, nrev);
+    {
+      // the TY east x faces go to wave 0, the TX north y faces are spread over waves 1..3 so
+      // that every wave evaluates one extra flux
+      const int wave = tid >> 6, lane = tid & 63;
+      constexpr int YPER = (TX + (NTHR / 64 - 1) - 1) / (NTHR / 64 - 1);
+      if (wave == 0) {
+        if (lane < TY) tile_face<FLUX, EK, 0>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
+      } else {
+        const int x = (wave - 1) * YPER + lane;
+        if (lane < YPER && x < TX) tile_face<FLUX, EK, 1>(sV, sID, sS, sFy, x, TY, TY * TX + x, i0, j0, nrev);
+      }
     }
 
     // ---- z face k+1/2 (registers)
@@ -580,6 +592,9 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
     }
     lo_iface = hi_iface;
     id0 = id1; id1 = id2; id2 = id3;
+#pragma unroll
+    for (int p = 0; p < 5; p++) hv[p] = hvn[p];
+    hid = hidn;
     __syncthreads();
   }
   if (MODE == 2) warp_min_atomic(&a.clk->minbits, cfl_min);

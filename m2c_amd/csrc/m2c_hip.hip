@@ -128,6 +128,13 @@ static void launch_flux_batch(int dir, dim3 g, hipStream_t s, int n, const doubl
   else hipLaunchKernelGGL((k_flux_batch<FLUX, 2>), g, dim3(256), 0, s, n, a, b, id, f);
 }
 
+// Plain streaming copy (16-byte vectors) used to measure and calibrate HBM traffic.
+__global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                     long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // AoS host layout <-> SoA device layout
 __global__ void k_aos_to_soa(const double *in, double *out, long long N) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
@@ -556,6 +563,33 @@ int m2c_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const double *V
   HIPCHK(hipMemcpyAsync(F, d + 10 * (size_t)n, b, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipFree(d));
+  return M2C_OK;
+}
+
+int m2c_stream_copy(m2c_ctx *c, long long nbytes, int reps, double *ms_avg) {
+  if (!c || nbytes < 16 || reps < 1) return M2C_ERR_ARG;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  long long n = nbytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n * 16));
+  HIPCHK(hipMalloc(&b, n * 16));
+  HIPCHK(hipMemsetAsync(a, 0, n * 16, c->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int nb = 256 * 8 * 4;
+  hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, c->stream, a, b, n);   // warm
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, c->stream, a, b, n);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIPCHK(hipFree(a));
+  HIPCHK(hipFree(b));
+  if (ms_avg) *ms_avg = ms / reps;
   return M2C_OK;
 }
 

@@ -194,20 +194,44 @@ template <typename T> __device__ __forceinline__ T wavg(T w1, T f1, T w2, T f2) 
   return (w1 * f1 + w2 * f2) / (w1 + w2);
 }
 
-// HLLC star-region flux, FluxFcnHLLC.h:159-192
+// Physical flux with the state's e, p/rho and |u|^2 already known (the same values
+// FluxFcnBase.h:83-123 recomputes through ComputeTotalEnthalpyPerUnitMass).
 template <typename T, int EK, int DIR>
-__device__ __forceinline__ void hllc_fstar(const DevEos &m, const T *Vs, T S, T Sstar, T *F) {
+__device__ __forceinline__ void phys_flux_k(const DevEos &m, const T *V, T e, T por, T vel2, T *F) {
+  T H = (kind_of<EK>(m) == EOS_DUMMY) ? T(m.d_e0) : e + T(0.5) * vel2 + por;
+  T vn = V[DIR + 1];
+  T rhou = V[0] * vn;
+  F[0] = rhou;
+  F[1] = rhou * V[1];
+  F[2] = rhou * V[2];
+  F[3] = rhou * V[3];
+  F[DIR + 1] = rhou * vn + V[4];
+  F[4] = rhou * H;
+}
+
+// HLLC star-region flux, FluxFcnHLLC.h:159-192 (e, p/rho, |u|^2 of Vs passed in; P2C of
+// VarFcnBase.h:185-197 inlined with the same operations)
+template <typename T, int EK, int DIR>
+__device__ __forceinline__ void hllc_fstar(const DevEos &m, const T *Vs, T e, T por, T vel2, T S, T Sstar, T *F) {
   T velo = Vs[DIR + 1];
   T rhostar = Vs[0] * (S - velo) / (S - Sstar);
   T U[5], Us[5];
-  eos_p2c<T, EK>(m, Vs, U);
+  if (kind_of<EK>(m) == EOS_DUMMY) {
+    U[0] = T(m.d_rho0); U[1] = U[2] = U[3] = T(0); U[4] = T(m.d_rho0) * T(m.d_e0);
+  } else {
+    U[0] = Vs[0];
+    U[1] = Vs[0] * Vs[1];
+    U[2] = Vs[0] * Vs[2];
+    U[3] = Vs[0] * Vs[3];
+    U[4] = Vs[0] * (e + T(0.5) * vel2);
+  }
   Us[0] = T(1);
   Us[1] = Vs[1]; Us[2] = Vs[2]; Us[3] = Vs[3];
   Us[DIR + 1] = Sstar;
   Us[4] = U[4] / Vs[0] + (Sstar - velo) * (Sstar + Vs[4] / (Vs[0] * (S - velo)));
 #pragma unroll
   for (int i = 0; i < 5; i++) Us[i] *= rhostar;
-  phys_flux<T, EK, DIR>(m, Vs, F);
+  phys_flux_k<T, EK, DIR>(m, Vs, e, por, vel2, F);
 #pragma unroll
   for (int i = 0; i < 5; i++) F[i] += S * (Us[i] - U[i]);
 }
@@ -223,7 +247,8 @@ __device__ __forceinline__ void flux_hllc(const DevEos &m, const T *Vm, const T 
   T dp = Vp[4] - Vm[4];
   T vm = Vm[DIR + 1], vp = Vp[DIR + 1];
   T diff = (vp - vm) / (sqrt_rhom + sqrt_rhop);
-  T p_over_rho_hat = wavg<T>(sqrt_rhom, Vm[4] / Vm[0], sqrt_rhop, Vp[4] / Vp[0]) + T(0.5) * diff * diff;
+  T porm = Vm[4] / Vm[0], porp = Vp[4] / Vp[0];
+  T p_over_rho_hat = wavg<T>(sqrt_rhom, porm, sqrt_rhop, porp) + T(0.5) * diff * diff;
   T em = eos_e<T, EK>(m, Vm[0], Vm[4]);
   T ep = eos_e<T, EK>(m, Vp[0], Vp[4]);
   T de = ep - em;
@@ -246,17 +271,25 @@ __device__ __forceinline__ void flux_hllc(const DevEos &m, const T *Vm, const T 
   T u_hat = wavg<T>(sqrt_rhom, vm, sqrt_rhop, vp);
   T Sm = smin<T>(vm - cm, u_hat - c_hat);
   T Sp = smax<T>(vp + cp, u_hat + c_hat);
-  if (T(0) <= Sm) { phys_flux<T, EK, DIR>(m, Vm, F); return; }
-  if (T(0) >= Sp) { phys_flux<T, EK, DIR>(m, Vp, F); return; }
-  T Sstar = (Vp[4] - Vm[4] + Vm[0] * vm * (Sm - vm) - Vp[0] * vp * (Sp - vp))
+  // the state whose flux (or star flux) is taken: minus side for 0 <= Sm, plus side for
+  // 0 >= Sp, else by the sign of S* (a NaN S* takes the plus side, where the reference aborts)
+  T Sstar = T(0);
+  bool star = false, left;
+  if (T(0) <= Sm) left = true;
+  else if (T(0) >= Sp) left = false;
+  else {
+    Sstar = (Vp[4] - Vm[4] + Vm[0] * vm * (Sm - vm) - Vp[0] * vp * (Sp - vp))
           / (Vm[0] * (Sm - vm) - Vp[0] * (Sp - vp));
-  // one F* evaluation on the side selected by the sign of S* (a NaN S* takes the plus side,
-  // where the reference aborts)
+    star = true;
+    left = Sstar >= T(0);
+  }
   T Vs[5];
-  const bool left = Sstar >= T(0);
 #pragma unroll
   for (int i = 0; i < 5; i++) Vs[i] = left ? Vm[i] : Vp[i];
-  hllc_fstar<T, EK, DIR>(m, Vs, left ? Sm : Sp, Sstar, F);
+  const T es = left ? em : ep, pors = left ? porm : porp;
+  const T vel2 = Vs[1] * Vs[1] + Vs[2] * Vs[2] + Vs[3] * Vs[3];
+  if (star) hllc_fstar<T, EK, DIR>(m, Vs, es, pors, vel2, left ? Sm : Sp, Sstar, F);
+  else phys_flux_k<T, EK, DIR>(m, Vs, es, pors, vel2, F);
 }
 
 // Local Lax-Friedrichs / Rusanov, FluxFcnLLF.h:48-106 and :112-184 (two materials).
