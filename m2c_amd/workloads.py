@@ -7,6 +7,12 @@ m2c_amd.make_config, V the primitive state (nz, ny, nx, 5) and ID the material I
   periodic_modes    N^3 periodic gas: rho=p=1 plus 16 seeded Fourier modes of amplitude 1e-2 on
                     density and velocity and a Gaussian pressure pulse (amplitude 0.5, sigma 8
                     cells) at the centre; gamma=1.4, van Albada, HLLC, RK3, CFL 0.4
+  shock_bubble      two-material shock-bubble: stiffened-gas water (gamma=6.12, p_inf=3.43e8 Pa,
+                    rho=1000, p=1e5) around a level-set sphere of air (gamma=1.4, rho=1.2, p=1e5)
+                    of radius 24 cells centred at x=160 cells on a 384x192x192 grid; a planar shock
+                    in the water at x=64 cells with post-shock p=1e9 Pa; exact two-material Riemann
+                    solves at interface faces, level-set reinitialisation every 5 steps, HLLC+MUSCL
+                    (van Albada), RK2, CFL 0.5. `scale` shrinks every length for small test runs.
 The periodic boundary and the Fourier-mode initial state have no counterpart in the reference's
 input format (IoData.h:266-267 has no periodic code); they are the documented extension.
 """
@@ -64,6 +70,47 @@ def periodic_modes(n=256, ny=None, nz=None, amp=1e-2, pulse=0.5, sigma=8.0, seed
     r2 = ((x * nx - cx) ** 2 + (y * ny - cy) ** 2 + (z * nz - cz) ** 2)
     V[..., 4] += pulse * np.exp(-r2 / (2.0 * sigma * sigma))
     return settings, V, np.zeros((nz, ny, nx), dtype=np.int32)
+
+
+WATER = (6.12, 3.43e8)
+AIR = (1.4, 0.0)
+
+
+def sg_shock_state(gamma, pinf, rho1, p1, p2):
+    """Post-shock (rho2, u2) behind a right-running shock into stiffened gas at rest (rho1, p1).
+
+    Rankine-Hugoniot for p + p_inf, which obeys the perfect-gas relations."""
+    P1, P2 = p1 + pinf, p2 + pinf
+    g = gamma
+    rho2 = rho1 * ((g + 1.0) * P2 + (g - 1.0) * P1) / ((g - 1.0) * P2 + (g + 1.0) * P1)
+    u2 = np.sqrt((p2 - p1) * (1.0 / rho1 - 1.0 / rho2))
+    return rho2, u2
+
+
+def shock_bubble(nx=384, ny=192, nz=192, radius=24.0, cx=160.0, shock_x=64.0, p_shock=1e9, h=1.0,
+                 reinit_freq=5, scale=1.0):
+    if scale != 1.0:
+        nx, ny, nz = max(int(round(nx * scale)), 4), max(int(round(ny * scale)), 4), max(int(round(nz * scale)), 4)
+        radius, cx, shock_x = radius * scale, cx * scale, shock_x * scale
+    rho_w, p0 = 1000.0, 1e5
+    rho2, u2 = sg_shock_state(WATER[0], WATER[1], rho_w, p0, p_shock)
+    post = (rho2, u2, 0.0, 0.0, p_shock)
+    settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, nx * h, 0.0, ny * h, 0.0, nz * h),
+                    bc=[BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_SYMMETRY, BC_SYMMETRY, BC_SYMMETRY],
+                    mats=[material(EOS_SG, list(WATER)), material(EOS_SG, list(AIR))],
+                    inlet=post, outlet=(rho_w, 0.0, 0.0, 0.0, p0), flux=FLUX_HLLC, limiter=LIM_VANALBADA,
+                    ts=TS_RK2, cfl=0.5, ls_matid=1, reinit_freq=reinit_freq)
+    zc, yc, xc = np.meshgrid((np.arange(nz) + 0.5) * h, (np.arange(ny) + 0.5) * h, (np.arange(nx) + 0.5) * h,
+                             indexing="ij")
+    phi = np.sqrt((xc - cx * h) ** 2 + (yc - 0.5 * ny * h) ** 2 + (zc - 0.5 * nz * h) ** 2) - radius * h
+    ID = np.where(phi < 0.0, 1, 0).astype(np.int32)
+    V = np.zeros((nz, ny, nx, 5))
+    V[..., 0] = np.where(ID == 1, 1.2, rho_w)
+    V[..., 4] = p0
+    shocked = xc < shock_x * h
+    for c in range(5):
+        V[..., c] = np.where(shocked, post[c], V[..., c])
+    return settings, V, ID, phi
 
 
 def config_from(settings, **over):

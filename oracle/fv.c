@@ -76,6 +76,7 @@ orc_ctx *orc_create(const orc_config *cfg) {
 #ifdef _OPENMP
   if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
 #endif
+  orc_ls_init(c);
   return c;
 }
 
@@ -451,7 +452,7 @@ int orc_time_step(const orc_ctx *c, const double *V, const int *ID, double *dt) 
 
 /* ------------------------------------------------------------------ RK stages ----------- */
 
-static void p2c_all(orc_ctx *c, const double *V, const int *ID, double *U) {
+void orc_p2c_all(orc_ctx *c, const double *V, const int *ID, double *U) {
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < c->nz; k++)
     for (int j = 0; j < c->ny; j++)
@@ -462,7 +463,7 @@ static void p2c_all(orc_ctx *c, const double *V, const int *ID, double *U) {
 }
 
 /* y = a*y + b*x over the interior (SpaceVariable.cpp:377-402) */
-static void axpby(orc_ctx *c, double *y, double a, double b, const double *x) {
+void orc_axpby(orc_ctx *c, double *y, double a, double b, const double *x) {
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < c->nz; k++)
     for (int j = 0; j < c->ny; j++)
@@ -474,7 +475,7 @@ static void axpby(orc_ctx *c, double *y, double a, double b, const double *x) {
 
 /* C2P over the interior + ClipDensityAndPressure with CheckState (SpaceOperator.cpp:553-649).
  * Returns the number of clipped cells, or -1 on a hyperbolicity violation. */
-static long c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V) {
+long orc_c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V) {
   long nclip = 0;
   int bad = 0;
 #pragma omp parallel for reduction(+ : nclip) reduction(| : bad) schedule(static)
@@ -483,7 +484,7 @@ static long c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V) {
       for (int i = 0; i < c->nx; i++) {
         long me = ORC_IDX(c, i, j, k);
         const orc_eos *m = &c->cfg.mat[ID[me]];
-        orc_c2p(m, U + 5 * me, V + 5 * me);
+        if (U) orc_c2p(m, U + 5 * me, V + 5 * me);   /* U == NULL: clip V in place */
         nclip += orc_clip(m, V + 5 * me);
         bad |= orc_check_state(m, V + 5 * me);
       }
@@ -491,66 +492,3 @@ static long c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V) {
   return bad ? -1 : nclip;
 }
 
-int orc_advance(orc_ctx *c, double *V, const int *ID, double dt) {
-  long n = orc_ncells_ghosted(c);
-  double *R = c->R, *Un = c->Un, *U1 = c->U1, *V1 = c->V1, *V2 = c->V2;
-  int ts = c->cfg.ts;
-  int st = orc_compute_residual(c, V, ID, R);
-  if (st < 0) return -1;
-  p2c_all(c, V, ID, Un);
-  if (ts == ORC_TS_FE) {
-    axpby(c, Un, 1.0, dt, R);
-    if (c2p_clip(c, Un, ID, V) < 0) return -1;
-    orc_apply_bc(c, V);
-    return 0;
-  }
-  memset(U1, 0, n * 5 * sizeof(double));
-  axpby(c, U1, 0.0, 1.0, Un);
-  axpby(c, U1, 1.0, dt, R);
-  memcpy(V1, V, n * 5 * sizeof(double));        /* ghosts of V1 are refreshed by the BC below */
-  long cl = c2p_clip(c, U1, ID, V1);
-  if (cl < 0) return -1;
-  if (cl) p2c_all(c, V1, ID, U1);
-  orc_apply_bc(c, V1);
-  st = orc_compute_residual(c, V1, ID, R);
-  if (st < 0) return -1;
-  if (ts == ORC_TS_RK2) {
-    axpby(c, U1, 0.5, 0.5, Un);
-    axpby(c, U1, 1.0, 0.5 * dt, R);
-    if (c2p_clip(c, U1, ID, V) < 0) return -1;
-    orc_apply_bc(c, V);
-    return 0;
-  }
-  /* RK3 (Gottlieb-Shu), TimeIntegrator.cpp:583-649 */
-  axpby(c, U1, 0.25, 0.75, Un);
-  axpby(c, U1, 1.0, 0.25 * dt, R);
-  memcpy(V2, V1, n * 5 * sizeof(double));
-  cl = c2p_clip(c, U1, ID, V2);
-  if (cl < 0) return -1;
-  if (cl) p2c_all(c, V2, ID, U1);
-  orc_apply_bc(c, V2);
-  st = orc_compute_residual(c, V2, ID, R);
-  if (st < 0) return -1;
-  axpby(c, U1, 2.0 / 3.0, 1.0 / 3.0, Un);
-  axpby(c, U1, 1.0, 2.0 / 3.0 * dt, R);
-  if (c2p_clip(c, U1, ID, V) < 0) return -1;
-  orc_apply_bc(c, V);
-  return 0;
-}
-
-int orc_run(orc_ctx *c, double *V, int *ID, double tmax, int maxsteps, double *t_out, int *steps_out) {
-  double t = 0.0;
-  int step = 0;
-  while (t < tmax && step < maxsteps) {
-    step++;
-    double dt;
-    if (orc_time_step(c, V, ID, &dt) < 0) return -1;
-    if (t + dt > tmax) dt = tmax - t;
-    t += dt;
-    int st = orc_step_all(c, V, ID, dt, step);
-    if (st < 0) { *t_out = t; *steps_out = step; return -1; }
-  }
-  *t_out = t;
-  *steps_out = step;
-  return 0;
-}

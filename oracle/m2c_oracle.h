@@ -52,6 +52,11 @@ typedef struct {
   int mp_flux, mp_recon;    /* MultiPhaseData flux / recon (IoData.cpp:1400-1416) */
   orc_riemann_cfg riemann;
   int nthreads;
+  /* level set (LevelSetSchemeData / LevelSetReinitializationData, IoData.cpp:1205-1266) */
+  int ls_on, ls_matid;
+  int ls_bc[6];             /* 0 none, 1 zero-Neumann, 2 linear extrapolation, 3 non-negative, 7 periodic */
+  int reinit_freq, reinit_maxits;
+  double reinit_cfl, reinit_tol;
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;
@@ -80,8 +85,26 @@ int orc_time_step(const orc_ctx *c, const double *V, const int *ID, double *dt);
  * Navier-Stokes part. V must have valid ghosts; on return V = V(n+1) with ghosts filled. */
 int orc_advance(orc_ctx *c, double *V, const int *ID, double dt);
 
-/* Main.cpp:514-621 time loop: runs until t >= tmax or nsteps steps. */
-int orc_run(orc_ctx *c, double *V, int *ID, double tmax, int maxsteps, double *t_out, int *steps_out);
+/* Main.cpp:514-621 time loop: runs until t >= tmax or nsteps steps. phi (ghosted, may be NULL
+ * when the level set is off) is advanced with the flow; ID is updated by the level set. */
+int orc_run(orc_ctx *c, double *V, int *ID, double *phi, double tmax, int maxsteps, double *t_out, int *steps_out);
+
+/* LevelSetOperator::ApplyBoundaryConditions (LevelSetOperator.cpp:934-1033) on a ghosted phi. */
+void orc_apply_bc_phi(const orc_ctx *c, double *phi);
+
+/* LevelSetOperator::ComputeResidualFDM_FullDomain (LevelSetOperator.cpp:1066-1141). */
+void orc_ls_residual(orc_ctx *c, const double *V, const double *phi, double *R);
+
+/* LevelSetReinitializer::ReinitializeFullDomain (LevelSetReinitializer.cpp:74-172), FIXED first
+ * layer. Returns the number of iterations, or -1 on failure. */
+int orc_ls_reinitialize(orc_ctx *c, double *phi);
+
+/* MultiPhaseOperator::UpdateMaterialIDByLevelSet (MultiPhaseOperator.cpp:127-330) */
+void orc_update_id(orc_ctx *c, const double *phi0, const double *phi, int *ID);
+
+/* Exact two-material Riemann problem (ExactRiemannSolverBase::ComputeRiemannSolution). */
+int orc_riemann(const orc_ctx *c, const double *dir, const double *Vm, int idm, const double *Vp, int idp,
+                double *Vs, int *id, double *Vsm, double *Vsp);
 
 /* Diagnostics counters of the last call (reverted face states, clipped cells). */
 void orc_counters(const orc_ctx *c, long *reverted_faces, long *clipped_cells, long *riemann_errors);

@@ -21,11 +21,28 @@ struct orc_ctx {
   long n_reverted, n_clipped, n_riemann_err;
   /* multiphase state (oracle/multiphase.c) */
   int store_riemann;        /* 1 during the first stage of a step */
-  orc_rs_list rs[6];        /* left/right x, y, z (RiemannSolutions.h:14-19) */
-  void *ls;                 /* level-set state */
+  int rs_stamp;             /* time step of the stored Riemann solutions */
+  double *rsV[3];           /* per face (keyed by the plus cell): Vsm[5], Vsp[5] */
+  int *rsId[3];             /* per face: idm, idp, stamp */
+  double *phi_tmp, *phi1, *Rls, *phi0, *phibk, *rphi1, *sign, *Rre;
+  int *tag, *IDn;
+  double *xc[3];            /* cell-centre coordinates with one ghost on each side */
+  double *fd_l[3], *fd_r[3], *fd_r1y;   /* FD3 coefficients per index along each axis */
+  double reinit_cfl_state;  /* LevelSetReinitializer::cfl (persists, divided by 1.5 on retries) */
+  long n_reinit_iters, n_unresolved;
 };
 
 #define ORC_IDX(c, i, j, k) ((((long)(k) + 1) * (c)->gy + ((j) + 1)) * (long)(c)->gx + ((i) + 1))
+
+/* building blocks shared by fv.c and multiphase.c */
+void orc_p2c_all(orc_ctx *c, const double *V, const int *ID, double *U);
+void orc_axpby(orc_ctx *c, double *y, double a, double b, const double *x);
+long orc_c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V);
+void orc_rs_store(orc_ctx *c, int dir, long hi, const double *Vsm, int idm, const double *Vsp, int idp);
+int orc_riemann_solve(const orc_eos *mat, const orc_riemann_cfg *cfg, const double *dir, const double *Vm, int idl,
+                      const double *Vp, int idr, double *Vs, int *id, double *Vsm, double *Vsp);
+double orc_pow_quarter(double x);
+double orc_pow_fifth(double x);
 
 /* exact Riemann solver (oracle/riemann.c) */
 int orc_interface_flux(orc_ctx *c, int dir, const double *vm, int idm, const double *vp, int idp,
@@ -35,7 +52,8 @@ void orc_riemann_begin_stage(orc_ctx *c);
 void orc_riemann_end_stage(orc_ctx *c);
 
 /* level set + end-of-step tasks (oracle/multiphase.c) */
-int orc_step_all(orc_ctx *c, double *V, int *ID, double dt, int step);
+int orc_step_all(orc_ctx *c, double *V, int *ID, double *phi, double dt, int step);
 void orc_ls_free(orc_ctx *c);
+void orc_ls_init(orc_ctx *c);
 
 #endif

@@ -46,7 +46,10 @@ class Config(C.Structure):
                 ("gminmod_alpha", C.c_double), ("vanalbada_pow", C.c_int),
                 ("ts", C.c_int), ("cfl", C.c_double),
                 ("mp_flux", C.c_int), ("mp_recon", C.c_int),
-                ("riemann", RiemannCfg), ("nthreads", C.c_int)]
+                ("riemann", RiemannCfg), ("nthreads", C.c_int),
+                ("ls_on", C.c_int), ("ls_matid", C.c_int), ("ls_bc", C.c_int * 6),
+                ("reinit_freq", C.c_int), ("reinit_maxits", C.c_int),
+                ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double)]
 
 
 def default_riemann():
@@ -83,7 +86,8 @@ def eos(kind, **kw):
 def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 0, 0, 1),
                 flux=FLUX_HLLC, limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True,
                 vanalbada_pow=True, inactive_id=-1, nthreads=0, roe_delta=0.2, gminmod_alpha=1.2,
-                slope_zero=False, mp_flux=0, mp_recon=0):
+                slope_zero=False, mp_flux=0, mp_recon=0, ls_matid=None, ls_bc=(1, 1, 1, 1, 1, 1),
+                reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8, reinit_tol=2.0e-4):
     c = Config()
     c.nx, c.ny, c.nz = nx, ny, nz
     c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
@@ -105,6 +109,12 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
     c.mp_flux, c.mp_recon = mp_flux, mp_recon
     c.riemann = default_riemann()
     c.nthreads = nthreads
+    c.ls_on = int(ls_matid is not None)
+    c.ls_matid = ls_matid if ls_matid is not None else -1
+    for i in range(6):
+        c.ls_bc[i] = ls_bc[i]
+    c.reinit_freq, c.reinit_maxits = reinit_freq, reinit_maxits
+    c.reinit_cfl, c.reinit_tol = reinit_cfl, reinit_tol
     return c
 
 
@@ -129,7 +139,12 @@ def lib():
         L.orc_compute_residual.argtypes = [P, D, I, D]
         L.orc_time_step.argtypes = [P, D, I, D]
         L.orc_advance.argtypes = [P, D, I, C.c_double]
-        L.orc_run.argtypes = [P, D, I, C.c_double, C.c_int, D, I]
+        L.orc_run.argtypes = [P, D, I, D, C.c_double, C.c_int, D, I]
+        L.orc_apply_bc_phi.argtypes = [P, D]
+        L.orc_ls_residual.argtypes = [P, D, D, D]
+        L.orc_ls_reinitialize.argtypes = [P, D]
+        L.orc_update_id.argtypes = [P, D, D, I]
+        L.orc_riemann.argtypes = [P, D, D, C.c_int, D, C.c_int, D, I, D, D]
         L.orc_counters.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
         _lib = L
     return _lib
@@ -191,11 +206,43 @@ class Oracle:
     def advance(self, V, ID, dt):
         return self.L.orc_advance(self.h, _dp(V), _ip(ID), dt)
 
-    def run(self, V, ID, tmax, maxsteps):
+    def run(self, V, ID, tmax, maxsteps, phi=None):
         t = C.c_double()
         n = C.c_int()
-        st = self.L.orc_run(self.h, _dp(V), _ip(ID), tmax, maxsteps, C.byref(t), C.byref(n))
+        pp = _dp(phi) if phi is not None else None
+        st = self.L.orc_run(self.h, _dp(V), _ip(ID), pp, tmax, maxsteps, C.byref(t), C.byref(n))
         return st, t.value, n.value
+
+    def ghosted_phi(self, phi_int):
+        phi = np.zeros(self.shape, dtype=np.float64)
+        phi[1:-1, 1:-1, 1:-1] = phi_int
+        self.L.orc_apply_bc_phi(self.h, _dp(phi))
+        return phi
+
+    def apply_bc_phi(self, phi):
+        self.L.orc_apply_bc_phi(self.h, _dp(phi))
+
+    def ls_residual(self, V, phi):
+        R = np.zeros(self.shape, dtype=np.float64)
+        self.L.orc_ls_residual(self.h, _dp(V), _dp(phi), _dp(R))
+        return R
+
+    def reinitialize(self, phi):
+        return self.L.orc_ls_reinitialize(self.h, _dp(phi))
+
+    def update_id(self, phi0, phi, ID):
+        self.L.orc_update_id(self.h, _dp(phi0), _dp(phi), _ip(ID))
+
+    def riemann(self, d, Vm, idm, Vp, idp):
+        """Exact two-material Riemann solution along unit normal d: (status, Vs, id, Vsm, Vsp)."""
+        dd = np.ascontiguousarray(d, dtype=np.float64)
+        Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+        Vp = np.ascontiguousarray(Vp, dtype=np.float64)
+        Vs, Vsm, Vsp = np.zeros(5), np.zeros(5), np.zeros(5)
+        mid = C.c_int()
+        st = self.L.orc_riemann(self.h, _dp(dd), _dp(Vm), idm, _dp(Vp), idp, _dp(Vs), C.byref(mid),
+                                _dp(Vsm), _dp(Vsp))
+        return st, Vs, mid.value, Vsm, Vsp
 
     def counters(self):
         a, b, c = C.c_long(), C.c_long(), C.c_long()

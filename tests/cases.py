@@ -35,14 +35,22 @@ def oracle_config(settings, vanalbada_pow=True, nthreads=0, **over):
                           vanalbada_pow=vanalbada_pow, inactive_id=s.get("inactive_id", -1), nthreads=nthreads,
                           roe_delta=s.get("roe_delta", 0.2), gminmod_alpha=s.get("gminmod_alpha", 1.2),
                           slope_zero=s.get("slope_zero", False), mp_flux=s.get("mp_flux", 0),
-                          mp_recon=s.get("mp_recon", 0))
+                          mp_recon=s.get("mp_recon", 0), ls_matid=s.get("ls_matid"),
+                          ls_bc=s.get("ls_bc", (1, 1, 1, 1, 1, 1)), reinit_freq=s.get("reinit_freq", -1),
+                          reinit_maxits=s.get("reinit_maxits", 30), reinit_cfl=s.get("reinit_cfl", 0.8),
+                          reinit_tol=s.get("reinit_tol", 2.0e-4))
 
 
-def run_oracle(settings, V0, ID0, tmax=1e300, maxsteps=10, vanalbada_pow=True, **over):
+def run_oracle(settings, V0, ID0, tmax=1e300, maxsteps=10, vanalbada_pow=True, phi0=None, **over):
+    """Run the oracle; with phi0 the level set is advanced too and (.., ID, phi) are appended."""
     o = po.Oracle(oracle_config(settings, vanalbada_pow=vanalbada_pow, **over))
     V, ID = o.ghosted(V0, ID0)
-    st, t, n = o.run(V, ID, tmax, maxsteps)
-    return st, t, n, V[1:-1, 1:-1, 1:-1].copy(), o
+    if phi0 is None:
+        st, t, n = o.run(V, ID, tmax, maxsteps)
+        return st, t, n, V[1:-1, 1:-1, 1:-1].copy(), o
+    phi = o.ghosted_phi(phi0)
+    st, t, n = o.run(V, ID, tmax, maxsteps, phi=phi)
+    return (st, t, n, V[1:-1, 1:-1, 1:-1].copy(), o, ID[1:-1, 1:-1, 1:-1].copy(), phi[1:-1, 1:-1, 1:-1].copy())
 
 
 def smooth_state(nx, ny, nz, seed=1, amp=0.2):
