@@ -75,6 +75,13 @@ typedef struct {
   m2c_riemann_params riemann;
   int precision;                    /* m2c_precision */
   int device;                       /* HIP device ordinal */
+  /* level set (LevelSetSchemeData / LevelSetReinitializationData, IoData.cpp:1205-1266) */
+  int ls_matid;                     /* material tracked by the level set (phi < 0), -1 = none */
+  int ls_bc[6];                     /* 1 zero-Neumann, 2 linear extrapolation, 3 non-negative,
+                                       7 periodic (x0, xmax, y0, ymax, z0, zmax) */
+  int reinit_freq;                  /* reinitialise every reinit_freq steps (<= 0: never) */
+  int reinit_maxits;
+  double reinit_cfl, reinit_tol;
 } m2c_config;
 
 typedef struct {
@@ -84,6 +91,8 @@ typedef struct {
   long long riemann_errors;
   long long steps;
   double t, dt, cfl;
+  long long reinit_iterations;      /* pseudo-time iterations of all reinitialisations */
+  long long unresolved_cells;       /* cells updated by FixUnresolvedNodes */
 } m2c_status;
 
 typedef struct m2c_ctx m2c_ctx;
@@ -96,6 +105,11 @@ int m2c_ctx_destroy(m2c_ctx *ctx);
 /* State transfer (V: nz*ny*nx*5 doubles, id: nz*ny*nx int32 or NULL for all-zero). */
 int m2c_upload_state(m2c_ctx *ctx, const double *V, const int32_t *id);
 int m2c_download_state(m2c_ctx *ctx, double *V, int32_t *id);
+
+/* Level set of the material ls_matid (nz*ny*nx doubles, no ghosts): LevelSetOperator's Phi
+ * (LevelSetOperator.h:86-90). Upload before the first step when ls_matid >= 0. */
+int m2c_upload_levelset(m2c_ctx *ctx, const double *phi);
+int m2c_download_levelset(m2c_ctx *ctx, double *phi);
 
 /* SpaceOperator::ComputeResidual (SpaceOperator.h:136-140) on the current state;
  * R: nz*ny*nx*5 doubles (already divided by -volume, SpaceOperator.cpp:3394-3398). */
@@ -119,6 +133,14 @@ int m2c_synchronize(m2c_ctx *ctx);
 /* FluxFcnBase::ComputeNumericalFluxAtCellInterface (FluxFcnBase.h:65-69) for n face pairs
  * (Vm, Vp: n*5 doubles, F: n*5 doubles), evaluated by the device kernels. */
 int m2c_flux_batch(m2c_ctx *ctx, int dir, int n, const double *Vm, const double *Vp, int id, double *F);
+
+/* ExactRiemannSolverBase::ComputeRiemannSolution (ExactRiemannSolverBase.h:35-39) for n
+ * independent problems along the unit normal of axis dir[t] (0 x, 1 y, 2 z), evaluated by the device
+ * solver of the interface faces. Vm, Vp, Vs, Vsm, Vsp: n*5 doubles; idm, idp, id, status: n ints.
+ * status[t]: 0 success, 1 solver failure (counted as a Riemann error by the reference),
+ * -1 invalid input state (the reference exits), 2 rarefaction path longer than the device cache. */
+int m2c_riemann_batch(m2c_ctx *ctx, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
+                      const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status);
 
 /* Timing helper: runs `nsteps` CFL-adaptive time steps (no tmax) and reports the average
  * duration (ms) of one fused stage-kernel launch and of one whole step, measured with HIP events

@@ -44,13 +44,16 @@ class Config(C.Structure):
                 ("limiter", C.c_int), ("gminmod_alpha", C.c_double),
                 ("slope_zero_near_interface", C.c_int), ("ts", C.c_int), ("cfl", C.c_double),
                 ("mp_flux", C.c_int), ("mp_recon", C.c_int), ("riemann", RiemannParams),
-                ("precision", C.c_int), ("device", C.c_int)]
+                ("precision", C.c_int), ("device", C.c_int),
+                ("ls_matid", C.c_int), ("ls_bc", C.c_int * 6), ("reinit_freq", C.c_int),
+                ("reinit_maxits", C.c_int), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double)]
 
 
 class Status(C.Structure):
     _fields_ = [("error", C.c_int), ("reverted_faces", C.c_longlong), ("clipped_cells", C.c_longlong),
                 ("riemann_errors", C.c_longlong), ("steps", C.c_longlong), ("t", C.c_double),
-                ("dt", C.c_double), ("cfl", C.c_double)]
+                ("dt", C.c_double), ("cfl", C.c_double), ("reinit_iterations", C.c_longlong),
+                ("unresolved_cells", C.c_longlong)]
 
 
 def material(kind, params, rhomin=0.0, pmin=-DBL_MAX, rhomax=DBL_MAX, pmax=DBL_MAX, failsafe_density=1e-10):
@@ -75,7 +78,11 @@ def riemann_defaults():
 
 def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 0, 0, 1), flux=FLUX_HLLC,
                 limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True, inactive_id=-1, roe_delta=0.2,
-                gminmod_alpha=1.2, slope_zero=False, mp_flux=0, mp_recon=0, precision=PREC_FP64, device=0):
+                gminmod_alpha=1.2, slope_zero=False, mp_flux=0, mp_recon=0, precision=PREC_FP64, device=0,
+                ls_matid=None, ls_bc=(1, 1, 1, 1, 1, 1), reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8,
+                reinit_tol=2.0e-4):
+    """Solver options; the level-set ones follow LevelSetSchemeData / LevelSetReinitializationData
+    (IoData.cpp:1205-1266): ls_matid=None disables the level set."""
     c = Config()
     c.nx, c.ny, c.nz = nx, ny, nz
     c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
@@ -95,6 +102,11 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
     c.mp_flux, c.mp_recon = mp_flux, mp_recon
     c.riemann = riemann_defaults()
     c.precision, c.device = precision, device
+    c.ls_matid = -1 if ls_matid is None else ls_matid
+    for i in range(6):
+        c.ls_bc[i] = ls_bc[i]
+    c.reinit_freq, c.reinit_maxits = reinit_freq, reinit_maxits
+    c.reinit_cfl, c.reinit_tol = reinit_cfl, reinit_tol
     return c
 
 
@@ -125,6 +137,10 @@ def lib():
         L.m2c_time_steps.argtypes = [P, C.c_int, D, D]
         L.m2c_device_state.argtypes = [P, C.POINTER(D)]
         L.m2c_stream_copy.argtypes = [P, C.c_longlong, C.c_int, D]
+        I = C.POINTER(C.c_int)
+        L.m2c_riemann_batch.argtypes = [P, C.c_int, I, D, I, D, I, D, I, D, D, I]
+        L.m2c_upload_levelset.argtypes = [P, D]
+        L.m2c_download_levelset.argtypes = [P, D]
         L.m2c_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -180,6 +196,30 @@ class Solver:
         _chk(self.L.m2c_download_state(self.h, _dp(V), ID.ctypes.data_as(C.POINTER(C.c_int32))),
              "m2c_download_state")
         return V, ID
+
+    def upload_levelset(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        assert phi.shape == self.shape
+        _chk(self.L.m2c_upload_levelset(self.h, _dp(phi)), "m2c_upload_levelset")
+
+    def download_levelset(self):
+        phi = np.empty(self.shape, dtype=np.float64)
+        _chk(self.L.m2c_download_levelset(self.h, _dp(phi)), "m2c_download_levelset")
+        return phi
+
+    def riemann(self, dirs, Vm, idm, Vp, idp):
+        """Exact two-material Riemann solutions of n problems: (status, Vs, id, Vsm, Vsp)."""
+        Vm = np.ascontiguousarray(np.atleast_2d(Vm), dtype=np.float64)
+        Vp = np.ascontiguousarray(np.atleast_2d(Vp), dtype=np.float64)
+        n = Vm.shape[0]
+        ip = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+        d, im, iq = ip(dirs), ip(idm), ip(idp)
+        Vs, Vsm, Vsp = np.empty_like(Vm), np.empty_like(Vm), np.empty_like(Vm)
+        mid, st = np.empty(n, np.int32), np.empty(n, np.int32)
+        cp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        _chk(self.L.m2c_riemann_batch(self.h, n, cp(d), _dp(Vm), cp(im), _dp(Vp), cp(iq), _dp(Vs), cp(mid),
+                                      _dp(Vsm), _dp(Vsp), cp(st)), "m2c_riemann_batch")
+        return st, Vs, mid, Vsm, Vsp
 
     def residual(self):
         R = np.empty(self.shape + (5,), dtype=np.float64)

@@ -24,7 +24,7 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
 
 # (tag, flux code, eos kind) -- flux codes: ROE 0, LLF 1, HLLC 2; eos kind -1 = run-time dispatch
 VARIANTS = [("hllc_sg", 2, 0), ("hllc_gen", 2, -1), ("roe_gen", 0, -1), ("llf_gen", 1, -1)]
-UNITS = [("m2c_hip", "m2c_hip.hip", [])] + [
+UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", [])] + [
     (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}"])
     for t, f, e in VARIANTS]
 

@@ -38,7 +38,8 @@ enum { BC_INLET = 1, BC_OUTLET = 2, BC_SLIPWALL = 3, BC_STICKWALL = 4, BC_SYMMET
        BC_PERIODIC = 7 };
 
 // status bits written by the kernels
-enum { ST_BAD_STATE = 1u, ST_CLIP_INTERMEDIATE = 2u, ST_NAN_FLUX = 4u, ST_NEG_C2 = 8u, ST_RIEMANN = 16u };
+enum { ST_BAD_STATE = 1u, ST_CLIP_INTERMEDIATE = 2u, ST_NAN_FLUX = 4u, ST_NEG_C2 = 8u, ST_RIEMANN = 16u,
+       ST_RIEMANN_PATH = 32u, ST_LIST_OVERFLOW = 64u, ST_REINIT_FAIL = 128u };
 
 struct Grid {
   int nx, ny, nz;
@@ -81,6 +82,7 @@ struct StageArgs {
   int first;                   // 1: first stage (U = U_n + dt*R)
   int kchunk;
   int ignore_active;           // 1: run even if the clock is inactive (direct API calls)
+  int no_cfl;                  // 1: the final stage does not feed the CFL minimum (it is taken later)
 };
 
 // Each translation unit that launches kernels holds its own copy of these tables (internal
@@ -597,7 +599,7 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
     hid = hidn;
     __syncthreads();
   }
-  if (MODE == 2) warp_min_atomic(&a.clk->minbits, cfl_min);
+  if (MODE == 2 && !a.no_cfl) warp_min_atomic(&a.clk->minbits, cfl_min);
   if (flags) atomicOr(&sFlags, flags);
   if (nrev) atomicAdd(&a.st->reverted, nrev);
   __syncthreads();

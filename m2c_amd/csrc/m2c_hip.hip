@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/m2c_hip.h"
+#include "multiphase.hpp"
 #include "stage_launch.hpp"
 
 using namespace m2c;
@@ -39,6 +40,15 @@ struct m2c_ctx {
   int kchunk = 1;
   int variant = 1;             // 0 HLLC+SG, 1 HLLC generic EOS, 2 Roe, 3 LLF
   dim3 grid3;
+  // two materials / level set
+  bool iface = false;          // exact Riemann pass at material interfaces
+  bool ls = false;             // level set advected and material IDs updated
+  bool id_single = false;      // the only material a swept-out node can take is 0
+  MpParams mp;
+  MpLaunch L;
+  double *phiA = nullptr, *phiB = nullptr, *phiD = nullptr;
+  double *xc = nullptr, *fd = nullptr;   // coordinates and FD3 tables (device)
+  std::vector<void *> mp_bufs;
 };
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
@@ -145,7 +155,31 @@ __global__ void k_soa_to_aos(const double *in, double *out, long long N) {
     for (int p = 0; p < 5; p++) out[5 * i + p] = in[p * N + i];
 }
 
+__global__ void k_copy_gated(const double *src, double *dst, long long n, const Clock *clk) {
+  if (!clk->active) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- host helpers
+
+// derivative-at-0 weights of the 4-point interpolant through offsets d[0..3] with d[c] = 0
+// (closed form of the Taylor system solved by GradientCalculatorFD3.cpp:451-466)
+static void fd_weights(const double *d, int cidx, double *w) {
+  for (int k = 0; k < 4; k++) {
+    if (k == cidx) continue;
+    double num = 1.0, den = 1.0;
+    for (int l = 0; l < 4; l++)
+      if (l != k && l != cidx) num = num * (-d[l]);
+    for (int m = 0; m < 4; m++)
+      if (m != k) den = den * (d[k] - d[m]);
+    w[k] = num / den;
+  }
+  double sum = 0.0;
+  for (int m = 0; m < 4; m++)
+    if (m != cidx) sum = sum + 1.0 / d[m];
+  w[cidx] = -sum;
+}
 
 static void build_eos(const m2c_material &in, DevEos &m) {
   memset(&m, 0, sizeof(m));
@@ -195,6 +229,7 @@ static m2c_ctx *g_resident = nullptr;
 static int upload_constants(m2c_ctx *c) {
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_grid), &c->grid, sizeof(Grid)));
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_eos), c->eos, sizeof(c->eos)));
+  if ((c->iface || c->ls) && m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
   if (m2c_stage_upload_hllc_sg(c->grid, c->eos) || m2c_stage_upload_hllc_gen(c->grid, c->eos) ||
       m2c_stage_upload_roe_gen(c->grid, c->eos) || m2c_stage_upload_llf_gen(c->grid, c->eos))
     return M2C_ERR_HIP;
@@ -212,6 +247,7 @@ static int ensure_constants(m2c_ctx *c) {
 static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, const double *Vn, const double *Us,
                         double *Out, double a, double b, double cc, int first, int ignore_active) {
   StageArgs s;
+  s.no_cfl = c->ls ? 1 : 0;
   s.S = S; s.Vn = Vn; s.Us = Us; s.Out = Out; s.ID = c->ID;
   s.Fi = c->Fi; s.Nf = c->Nf;
   s.clk = c->clk; s.st = c->st;
@@ -231,39 +267,107 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
   return M2C_OK;
 }
 
+static void launch_cfl(m2c_ctx *c);
+
+static int mp_chk(int rc) { return rc ? M2C_ERR_HIP : M2C_OK; }
+
+// Interface Riemann pass on the stage field S (first stage: keep the Riemann solutions).
+static int iface_pass(m2c_ctx *c, const double *S, int in_cons, int store, int ignore_active) {
+  if (!c->iface) return M2C_OK;
+  return mp_chk(m2c_mp_launch_iface(c->L, S, in_cons, store && c->ls, ignore_active));
+}
+
+// End-of-step tasks of TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
+// 695-797): IDs of swept nodes, reinitialisation, states of cells whose material changed, clip,
+// then the CFL minimum of the updated field. `reinit_now`: the host's view of whether this step
+// is a reinitialisation step (the device decides again from its own step counter).
+static int end_of_step(m2c_ctx *c, bool reinit_now) {
+  if (mp_chk(m2c_mp_launch_id_remaining(c->L, c->id_single ? 1 : 0))) return M2C_ERR_HIP;
+  if (reinit_now) {
+    if (mp_chk(m2c_mp_launch_reinit(c->L, c->phiA, c->cfg.reinit_maxits, 0))) return M2C_ERR_HIP;
+    for (int attempt = 0; attempt < 64; attempt++) {
+      MpState h;
+      HIPCHK(hipMemcpyAsync(&h, c->L.ms, sizeof(MpState), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (!h.rs.retry) break;
+      if (mp_chk(m2c_mp_launch_reinit(c->L, c->phiA, c->cfg.reinit_maxits, 1))) return M2C_ERR_HIP;
+    }
+  }
+  if (mp_chk(m2c_mp_launch_state_update(c->L, c->V, 4))) return M2C_ERR_HIP;
+  launch_cfl(c);
+  return M2C_OK;
+}
+
 // One time step: the Runge-Kutta stages (TimeIntegrator.cpp FE :150-226, RK2 :291-428,
-// RK3 :496-690). `ev` (optional) receives a start/stop event pair per stage launch.
-static int enqueue_step(m2c_ctx *c, int ignore_active, std::vector<hipEvent_t> *ev) {
+// RK3 :496-690), with the level-set stages interleaved as there. `ev` (optional) receives a
+// start/stop event pair per stage launch. `time_step`: the step's number as the host counts it.
+static int enqueue_step(m2c_ctx *c, int ignore_active, std::vector<hipEvent_t> *ev, long long time_step = 0) {
   int ts = c->cfg.ts;
   int rc;
   auto rec = [&](int i) { if (ev) hipEventRecord((*ev)[i], c->stream); };
+  const bool reinit_now = c->ls && c->cfg.reinit_freq > 0 && time_step > 0 && time_step % c->cfg.reinit_freq == 0;
+  if (c->iface || c->ls) if (mp_chk(m2c_mp_step_begin(c->L))) return M2C_ERR_HIP;
+  const MpLaunch &L = c->L;
   if (ts == M2C_TS_FE) {
+    if ((rc = iface_pass(c, c->V, 0, 1, ignore_active))) return rc;
     rec(0);
-    rc = launch_stage(c, 0, 2, c->V, c->V, nullptr, c->V, 0, 0, 1.0, 1, ignore_active);
+    rc = launch_stage(c, 0, 2, c->V, c->V, nullptr, c->U1, 0, 0, 1.0, 1, ignore_active);
     rec(1);
-    return rc;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_copy_gated, dim3(2048), dim3(256), 0, c->stream, c->U1, c->V, 5 * c->N, c->clk);
+    if (c->ls) {
+      // Phi(n+1) = 1*Phi + dt R(Phi; V(n+1)) into B, IDs from sign(Phi(n)) and sign(Phi(n+1))
+      if (mp_chk(m2c_mp_launch_ls(L, 0, LS_X, 1, c->V, c->phiA, c->phiA, nullptr, c->phiB, 0, 0, 1.0, c->phiA)))
+        return M2C_ERR_HIP;
+      hipLaunchKernelGGL(k_copy_gated, dim3(2048), dim3(256), 0, c->stream, c->phiB, c->phiA, c->N, c->clk);
+      return end_of_step(c, reinit_now);
+    }
+    return M2C_OK;
   }
-  // stage 1: U1 = U_n + dt R(V_n)
+  // stage 1: U1 = U_n + dt R(V_n); Phi1 = (0*Phi1 + 1*Phi) + dt R(Phi; V_n)
+  if ((rc = iface_pass(c, c->V, 0, 1, ignore_active))) return rc;
   rec(0);
   rc = launch_stage(c, 0, 1, c->V, c->V, nullptr, c->U1, 0, 0, 1.0, 1, ignore_active);
   rec(1);
   if (rc) return rc;
+  if (c->ls && mp_chk(m2c_mp_launch_ls(L, 0, LS_FIRST, 0, c->V, c->phiA, c->phiA, nullptr, c->phiB, 0, 0, 1.0,
+                                       nullptr)))
+    return M2C_ERR_HIP;
+  if ((rc = iface_pass(c, c->U1, 1, 0, ignore_active))) return rc;
   if (ts == M2C_TS_RK2) {
-    // U(n+1) = 0.5 U1 + 0.5 U_n + 0.5 dt R(V1)
+    // U(n+1) = 0.5 U1 + 0.5 U_n + 0.5 dt R(V1); Phi(n+1) = (0.5 Phi + 0.5 Phi1) + 0.5 dt R(Phi1; V1)
     rec(2);
     rc = launch_stage(c, 1, 2, c->U1, c->V, c->U1, c->V, 0.5, 0.5, 0.5, 0, ignore_active);
     rec(3);
-    return rc;
+    if (rc) return rc;
+    if (c->ls) {
+      if (mp_chk(m2c_mp_launch_ls(L, 1, LS_AXBY, 1, c->U1, c->phiB, c->phiA, c->phiB, c->phiA, 0.5, 0.5, 0.5,
+                                  c->phiA)))
+        return M2C_ERR_HIP;
+      return end_of_step(c, reinit_now);
+    }
+    return M2C_OK;
   }
   // RK3: U2 = 0.25 U1 + 0.75 U_n + 0.25 dt R(V1); U(n+1) = 2/3 U2 + 1/3 U_n + 2/3 dt R(V2)
   rec(2);
   rc = launch_stage(c, 1, 1, c->U1, c->V, c->U1, c->U2, 0.25, 0.75, 0.25, 0, ignore_active);
   rec(3);
   if (rc) return rc;
+  if (c->ls && mp_chk(m2c_mp_launch_ls(L, 1, LS_AXBY, 0, c->U1, c->phiB, c->phiB, c->phiA, c->phiD, 0.25, 0.75, 0.25,
+                                       nullptr)))
+    return M2C_ERR_HIP;
+  if ((rc = iface_pass(c, c->U2, 1, 0, ignore_active))) return rc;
   rec(4);
   rc = launch_stage(c, 1, 2, c->U2, c->V, c->U2, c->V, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0, ignore_active);
   rec(5);
-  return rc;
+  if (rc) return rc;
+  if (c->ls) {
+    if (mp_chk(m2c_mp_launch_ls(L, 1, LS_AXBY, 1, c->U2, c->phiD, c->phiA, c->phiD, c->phiA, 1.0 / 3.0, 2.0 / 3.0,
+                                2.0 / 3.0, c->phiA)))
+      return M2C_ERR_HIP;
+    return end_of_step(c, reinit_now);
+  }
+  return M2C_OK;
 }
 
 static int nstages(const m2c_ctx *c) { return c->cfg.ts == M2C_TS_FE ? 1 : (c->cfg.ts == M2C_TS_RK2 ? 2 : 3); }
@@ -271,6 +375,147 @@ static int nstages(const m2c_ctx *c) { return c->cfg.ts == M2C_TS_FE ? 1 : (c->c
 static void launch_cfl(m2c_ctx *c) {
   int nb = (int)std::min<long long>((c->N + 255) / 256, 4096);
   hipLaunchKernelGGL(k_cfl_min, dim3(nb), dim3(256), 0, c->stream, c->V, c->ID, c->clk, c->st);
+}
+
+template <typename T> static int mp_alloc(m2c_ctx *c, T **p, size_t count, int fill) {
+  HIPCHK(hipMalloc((void **)p, sizeof(T) * (count ? count : 1)));
+  HIPCHK(hipMemset(*p, fill, sizeof(T) * (count ? count : 1)));
+  c->mp_bufs.push_back((void *)*p);
+  return M2C_OK;
+}
+
+// Buffers and tables of the interface pass and the level set (per-cell arrays without a ghost
+// frame; face arrays keyed by the face's plus cell).
+static int setup_multiphase(m2c_ctx *c) {
+  if (!c->iface && !c->ls) return M2C_OK;
+  const m2c_config *cfg = &c->cfg;
+  const Grid &g = c->grid;
+  MpParams &mp = c->mp;
+  memset(&mp, 0, sizeof(mp));
+  MpLaunch &L = c->L;
+  memset(&L, 0, sizeof(L));
+  const long long N = c->N;
+  const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  if (3 * Nf >= (1ll << 32)) return M2C_ERR_UNSUPPORTED;
+  c->Nf = Nf;
+  L.stream = c->stream;
+  L.N = N;
+  L.Nf = Nf;
+  L.flux = cfg->flux;
+  L.ID = c->ID;
+  L.st = c->st;
+  L.clk = c->clk;
+  mp.ls_matid = cfg->ls_matid;
+  for (int sd = 0; sd < 6; sd++) mp.ls_bc[sd] = cfg->ls_bc[sd];
+  mp.reinit_freq = cfg->reinit_freq;
+  mp.reinit_maxits = cfg->reinit_maxits;
+  mp.reinit_tol = cfg->reinit_tol;
+  mp.min_dx = std::min(g.dx, std::min(g.dy, g.dz));
+  mp.sign_factor = 1.0 * std::min(g.dx, std::min(g.dy, g.dz));
+  const m2c_riemann_params &rp = cfg->riemann;
+  mp.rcfg.maxIts_main = rp.maxIts_main; mp.rcfg.maxIts_bracket = rp.maxIts_bracket;
+  mp.rcfg.maxIts_shock = rp.maxIts_shock; mp.rcfg.numSteps_rarefaction = rp.numSteps_rarefaction;
+  mp.rcfg.tol_main = rp.tol_main; mp.rcfg.tol_shock = rp.tol_shock; mp.rcfg.tol_rarefaction = rp.tol_rarefaction;
+  mp.rcfg.min_pressure = rp.min_pressure; mp.rcfg.failure_threshold = rp.failure_threshold;
+  mp.rcfg.pressure_at_failure = rp.pressure_at_failure;
+  mp.path_cap = 1024;
+  for (int m = 0; m < cfg->nmat; m++) mp.failsafe_density[m] = cfg->mat[m].failsafe_density;
+  int rc;
+  if ((rc = mp_alloc(c, &L.ms, 1, 0))) return rc;
+  {
+    MpState h;
+    memset(&h, 0, sizeof(h));
+    h.rs.cfl = cfg->reinit_cfl;
+    HIPCHK(hipMemcpy(L.ms, &h, sizeof(h), hipMemcpyHostToDevice));
+  }
+  if (c->iface) {
+    long long nthr = std::min<long long>(16384, ((3 * Nf + 63) / 64) * 64);
+    L.riemann_threads = (int)std::max<long long>(64, nthr);
+    if ((rc = mp_alloc(c, &L.faces, (size_t)(3 * Nf), 0))) return rc;
+    if ((rc = mp_alloc(c, &c->Fi, (size_t)(30 * Nf), 0))) return rc;
+    L.Fi = c->Fi;
+    if ((rc = mp_alloc(c, &L.rsV, (size_t)(30 * Nf), 0))) return rc;
+    if ((rc = mp_alloc(c, &L.rsMeta, (size_t)(3 * Nf), 0))) return rc;
+    if ((rc = mp_alloc(c, &L.scratch, (size_t)6 * mp.path_cap * L.riemann_threads, 0))) return rc;
+  }
+  if (c->ls) {
+    if ((rc = mp_alloc(c, &c->phiA, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &c->phiB, (size_t)N, 0))) return rc;
+    if (cfg->ts == M2C_TS_RK3 && (rc = mp_alloc(c, &c->phiD, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.IDnew, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.rem, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.tagR, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.tag, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.sign, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.phibk, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.rphi1a, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.rphi1b, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.chg, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.cidx, (size_t)N, 0xff))) return rc;
+    if ((rc = mp_alloc(c, &L.lvl, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.cnt, (size_t)N, 0))) return rc;
+    if ((rc = mp_alloc(c, &L.Vnew, (size_t)(5 * N), 0))) return rc;
+    if ((rc = mp_alloc(c, &L.unres, (size_t)N, 0))) return rc;
+    // coordinates x[-1..n] (MeshGenerator.cpp:146-158, ghosts SpaceOperator.cpp:188-230) and the
+    // FD3 coefficients of every index along each axis (GradientCalculatorFD3.cpp:300-469)
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const double lo[3] = {cfg->x0, cfg->y0, cfg->z0}, h[3] = {g.dx, g.dy, g.dz};
+    std::vector<double> xc, fd;
+    size_t xo[3], lo_off[3], ro_off[3], r1y_off;
+    for (int d = 0; d < 3; d++) {
+      xo[d] = xc.size();
+      std::vector<double> x(n[d] + 2);
+      for (int i = 0; i < n[d]; i++) x[i + 1] = lo[d] + 0.5 * h[d] + i * h[d];
+      x[0] = x[1] - h[d];
+      x[n[d] + 1] = x[n[d]] + h[d];
+      xc.insert(xc.end(), x.begin(), x.end());
+      const double *xi = x.data() + 1;
+      lo_off[d] = fd.size();
+      for (int i = 0; i < n[d]; i++) {
+        double w[4];
+        if (i > 0) {
+          double dd[4] = {xi[i - 2] - xi[i], xi[i - 1] - xi[i], 0.0, xi[i + 1] - xi[i]};
+          fd_weights(dd, 2, w);
+        } else {
+          double dx = xi[i] - xi[i - 1];
+          w[0] = 0.0; w[1] = -1.0 / dx; w[2] = 1.0 / dx; w[3] = 0.0;
+        }
+        fd.insert(fd.end(), w, w + 4);
+      }
+      ro_off[d] = fd.size();
+      for (int i = 0; i < n[d]; i++) {
+        double w[4];
+        if (i < n[d] - 1) {
+          double dd[4] = {xi[i - 1] - xi[i], 0.0, xi[i + 1] - xi[i], xi[i + 2] - xi[i]};
+          fd_weights(dd, 1, w);
+        } else {
+          double dx = xi[i + 1] - xi[i];
+          w[0] = 0.0; w[1] = -1.0 / dx; w[2] = 1.0 / dx; w[3] = 0.0;
+        }
+        fd.insert(fd.end(), w, w + 4);
+      }
+    }
+    r1y_off = fd.size();
+    {
+      const double *yj = xc.data() + xo[1] + 1;
+      for (int j = 0; j < n[1]; j++) {
+        double dy = yj[j + 1] - yj[j];
+        double w[4] = {0.0, -1.0 / dy, 1.0 / dy, 0.0};
+        fd.insert(fd.end(), w, w + 4);
+      }
+    }
+    if ((rc = mp_alloc(c, &c->xc, xc.size(), 0))) return rc;
+    if ((rc = mp_alloc(c, &c->fd, fd.size(), 0))) return rc;
+    HIPCHK(hipMemcpy(c->xc, xc.data(), sizeof(double) * xc.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fd, fd.data(), sizeof(double) * fd.size(), hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; d++) {
+      mp.xc[d] = c->xc + xo[d];
+      mp.fdl[d] = c->fd + lo_off[d];
+      mp.fdr[d] = c->fd + ro_off[d];
+    }
+    mp.fdr1y = c->fd + r1y_off;
+  }
+  return M2C_OK;
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -328,9 +573,26 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   for (int m = 0; m < cfg->nmat; m++) all_sg = all_sg && cfg->mat[m].eos == M2C_EOS_SG;
   if (cfg->flux == M2C_FLUX_HLLC) c->variant = (all_sg && cfg->inactive_id < 0 && !g.has_interfaces) ? 0 : 1;
   else c->variant = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
-  if (g.has_interfaces && cfg->mp_flux != 2) {
+  c->iface = g.has_interfaces && cfg->mp_flux != 2;
+  c->ls = cfg->ls_matid >= 0;
+  int bad = 0;
+  if (c->iface && cfg->mp_recon != 0) bad = M2C_ERR_UNSUPPORTED;   // linear reconstruction at interfaces
+  if (c->ls) {
+    if (cfg->ls_matid >= cfg->nmat || cfg->ls_matid == cfg->inactive_id) bad = M2C_ERR_ARG;
+    for (int sd = 0; sd < 6; sd++) {
+      int b = cfg->ls_bc[sd];
+      if (b != 1 && b != 2 && b != 3 && b != M2C_BC_PERIODIC) bad = bad ? bad : M2C_ERR_ARG;
+      if ((b == M2C_BC_PERIODIC) != (cfg->bc[sd] == M2C_BC_PERIODIC)) bad = bad ? bad : M2C_ERR_ARG;
+    }
+    if (cfg->reinit_freq > 0 && (cfg->reinit_maxits < 1 || !(cfg->reinit_cfl > 0))) bad = bad ? bad : M2C_ERR_ARG;
+    int nuni = 0, only = -1;
+    for (int m = 0; m < cfg->nmat; m++)
+      if (m != cfg->ls_matid && m != cfg->inactive_id) { nuni++; only = m; }
+    c->id_single = (nuni == 1 && only == 0);
+  }
+  if (bad) {
     delete c;
-    return M2C_ERR_UNSUPPORTED;   // exact two-material Riemann pass: see m2c_interface.hip
+    return bad;
   }
 
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -345,6 +607,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   HIPCHK(hipMalloc(&c->clk, sizeof(Clock)));
   HIPCHK(hipMalloc(&c->st, sizeof(Status)));
   HIPCHK(hipMemset(c->st, 0, sizeof(Status)));
+  if (int rc = setup_multiphase(c)) return rc;
   if (upload_constants(c)) return M2C_ERR_HIP;
   hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(1), 0, c->stream, c->clk, 0.0, 0.0, cfg->cfl, 0);
 
@@ -364,8 +627,9 @@ int m2c_ctx_destroy(m2c_ctx *c) {
   if (!c) return M2C_OK;
   if (g_resident == c) g_resident = nullptr;
   hipStreamSynchronize(c->stream);
-  hipFree(c->V); hipFree(c->U1); hipFree(c->U2); hipFree(c->R); hipFree(c->Fi);
+  hipFree(c->V); hipFree(c->U1); hipFree(c->U2); hipFree(c->R);
   hipFree(c->ID); hipFree(c->clk); hipFree(c->st);
+  for (void *p : c->mp_bufs) hipFree(p);
   hipStreamDestroy(c->stream);
   delete c;
   return M2C_OK;
@@ -413,7 +677,9 @@ int m2c_download_state(m2c_ctx *c, double *V, int32_t *id) {
 int m2c_compute_residual(m2c_ctx *c, double *R) {
   if (!c) return M2C_ERR_ARG;
   if (int rc0 = ensure_constants(c)) return rc0;
-  int rc = launch_stage(c, 0, 0, c->V, c->V, nullptr, c->U1, 0, 0, 0, 0, 1);
+  int rc = iface_pass(c, c->V, 0, 0, 1);
+  if (rc) return rc;
+  rc = launch_stage(c, 0, 0, c->V, c->V, nullptr, c->U1, 0, 0, 0, 0, 1);
   if (rc) return rc;
   if (R) {
     hipLaunchKernelGGL(k_soa_to_aos, dim3(1024), dim3(256), 0, c->stream, c->U1, c->R, c->N);
@@ -445,7 +711,10 @@ int m2c_advance_dt(m2c_ctx *c, double dt) {
   if (int rc0 = ensure_constants(c)) return rc0;
   hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(1), 0, c->stream, c->clk, INFINITY, 1 << 30);
   hipLaunchKernelGGL(k_clock_fixed_dt, dim3(1), dim3(1), 0, c->stream, c->clk, dt);
-  int rc = enqueue_step(c, 1, nullptr);
+  Clock h;
+  HIPCHK(hipMemcpyAsync(&h, c->clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = enqueue_step(c, 1, nullptr, (long long)h.steps + 1);
   if (rc) return rc;
   hipLaunchKernelGGL(k_clock_finalize, dim3(1), dim3(1), 0, c->stream, c->clk, 1);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -458,8 +727,15 @@ int m2c_run(m2c_ctx *c, double tmax, int maxsteps, int sync_every) {
   hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(1), 0, c->stream, c->clk, tmax, maxsteps);
   launch_cfl(c);
   hipLaunchKernelGGL(k_clock_finalize, dim3(1), dim3(1), 0, c->stream, c->clk, 0);
+  long long step0 = 0;
+  if (c->ls) {
+    Clock h;
+    HIPCHK(hipMemcpyAsync(&h, c->clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    step0 = h.steps;
+  }
   for (int s = 0; s < maxsteps; s++) {
-    int rc = enqueue_step(c, 0, nullptr);
+    int rc = enqueue_step(c, 0, nullptr, step0 + s + 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_clock_finalize, dim3(1), dim3(1), 0, c->stream, c->clk, 1);
     if (sync_every > 0 && (s + 1) % sync_every == 0) {
@@ -486,9 +762,16 @@ int m2c_time_steps(m2c_ctx *c, int nsteps, double *ms_stage, double *ms_step) {
   launch_cfl(c);
   hipLaunchKernelGGL(k_clock_finalize, dim3(1), dim3(1), 0, c->stream, c->clk, 0);
   HIPCHK(hipEventRecord(a, c->stream));
+  long long step0 = 0;
+  if (c->ls) {
+    Clock h;
+    HIPCHK(hipMemcpyAsync(&h, c->clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    step0 = h.steps;
+  }
   for (int s = 0; s < nsteps; s++) {
     std::vector<hipEvent_t> sub(ev.begin() + 2 * ns * s, ev.begin() + 2 * ns * (s + 1));
-    int rc = enqueue_step(c, 0, &sub);
+    int rc = enqueue_step(c, 0, &sub, step0 + s + 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_clock_finalize, dim3(1), dim3(1), 0, c->stream, c->clk, 1);
   }
@@ -519,7 +802,8 @@ int m2c_get_status(m2c_ctx *c, m2c_status *out) {
   HIPCHK(hipMemcpyAsync(&k, c->clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   memset(out, 0, sizeof(*out));
-  out->error = (s.bits & (ST_BAD_STATE | ST_NAN_FLUX | ST_NEG_C2)) ? M2C_ERR_STATE
+  out->error = (s.bits & (ST_BAD_STATE | ST_NAN_FLUX | ST_NEG_C2 | ST_RIEMANN_PATH | ST_LIST_OVERFLOW |
+                          ST_REINIT_FAIL)) ? M2C_ERR_STATE
              : ((s.bits & ST_CLIP_INTERMEDIATE) ? M2C_ERR_UNSUPPORTED : M2C_OK);
   out->reverted_faces = (long long)s.reverted;
   out->clipped_cells = (long long)s.clipped;
@@ -528,6 +812,30 @@ int m2c_get_status(m2c_ctx *c, m2c_status *out) {
   out->t = k.t;
   out->dt = k.dt;
   out->cfl = c->cfg.cfl;
+  if (c->ls || c->iface) {
+    MpState h;
+    HIPCHK(hipMemcpy(&h, c->L.ms, sizeof(MpState), hipMemcpyDeviceToHost));
+    out->reinit_iterations = h.reinit_iters;
+    out->unresolved_cells = h.unresolved_total;
+  }
+  return M2C_OK;
+}
+
+int m2c_upload_levelset(m2c_ctx *c, const double *phi) {
+  if (!c || !phi) return M2C_ERR_ARG;
+  if (!c->ls) return M2C_ERR_ARG;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  HIPCHK(hipMemcpyAsync(c->phiA, phi, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return M2C_OK;
+}
+
+int m2c_download_levelset(m2c_ctx *c, double *phi) {
+  if (!c || !phi) return M2C_ERR_ARG;
+  if (!c->ls) return M2C_ERR_ARG;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  HIPCHK(hipMemcpyAsync(phi, c->phiA, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return M2C_OK;
 }
 
@@ -561,6 +869,57 @@ int m2c_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const double *V
   else launch_flux_batch<FLUX_LLF>(dir, gb, c->stream, n, d, d + 5 * (size_t)n, id, d + 10 * (size_t)n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(F, d + 10 * (size_t)n, b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d));
+  return M2C_OK;
+}
+
+int m2c_riemann_batch(m2c_ctx *c, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
+                      const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status) {
+  if (!c || n < 0 || (n > 0 && (!dir || !Vm || !idm || !Vp || !idp || !Vs || !id || !Vsm || !Vsp || !status)))
+    return M2C_ERR_ARG;
+  if (n == 0) return M2C_OK;
+  for (int t = 0; t < n; t++) {
+    if (dir[t] < 0 || dir[t] > 2) return M2C_ERR_ARG;
+    if (idm[t] < 0 || idm[t] >= c->cfg.nmat || idp[t] < 0 || idp[t] >= c->cfg.nmat) return M2C_ERR_ARG;
+  }
+  if (int rc0 = ensure_constants(c)) return rc0;
+  if (!c->iface && !c->ls) {
+    // the solver constants live with the interface kernels: upload them for this context
+    MpParams mp;
+    memset(&mp, 0, sizeof(mp));
+    const m2c_riemann_params &rp = c->cfg.riemann;
+    mp.rcfg.maxIts_main = rp.maxIts_main; mp.rcfg.maxIts_bracket = rp.maxIts_bracket;
+    mp.rcfg.maxIts_shock = rp.maxIts_shock; mp.rcfg.numSteps_rarefaction = rp.numSteps_rarefaction;
+    mp.rcfg.tol_main = rp.tol_main; mp.rcfg.tol_shock = rp.tol_shock; mp.rcfg.tol_rarefaction = rp.tol_rarefaction;
+    mp.rcfg.min_pressure = rp.min_pressure; mp.rcfg.failure_threshold = rp.failure_threshold;
+    mp.rcfg.pressure_at_failure = rp.pressure_at_failure;
+    mp.path_cap = 1024;
+    c->mp = mp;
+    if (m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
+  }
+  const int nthreads = (int)std::min<long long>(4096, ((long long)n + 63) / 64 * 64);
+  size_t b5 = sizeof(double) * 5 * (size_t)n, bi = sizeof(int) * (size_t)n;
+  size_t sbytes = sizeof(double) * 6 * (size_t)c->mp.path_cap * nthreads;
+  char *d = nullptr;
+  HIPCHK(hipMalloc(&d, 5 * b5 + 5 * bi + sbytes));
+  double *dVm = (double *)d, *dVp = dVm + 5 * (size_t)n, *dVs = dVp + 5 * (size_t)n, *dVsm = dVs + 5 * (size_t)n,
+         *dVsp = dVsm + 5 * (size_t)n;
+  int *ddir = (int *)(dVsp + 5 * (size_t)n), *didm = ddir + n, *didp = didm + n, *did = didp + n, *dst = did + n;
+  double *scratch = (double *)(dst + n + (n & 1));
+  HIPCHK(hipMemcpyAsync(dVm, Vm, b5, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dVp, Vp, b5, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(ddir, dir, bi, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(didm, idm, bi, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(didp, idp, bi, hipMemcpyHostToDevice, c->stream));
+  int rc = m2c_mp_launch_riemann_batch(c->stream, n, ddir, dVm, didm, dVp, didp, dVs, did, dVsm, dVsp, dst, scratch,
+                                       nthreads);
+  if (rc) { hipFree(d); return M2C_ERR_HIP; }
+  HIPCHK(hipMemcpyAsync(Vs, dVs, b5, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(Vsm, dVsm, b5, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(Vsp, dVsp, b5, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(id, did, bi, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(status, dst, bi, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipFree(d));
   return M2C_OK;

@@ -1,0 +1,1128 @@
+// multiphase.hip -- two-material interface faces, level set and end-of-step tasks (gfx950).
+//
+//   interface faces      SpaceOperator.cpp:2486-2542 (+ y/z analogues): exact two-material Riemann
+//                        solve along the mesh normal, clip of the one-sided star states, the
+//                        Riemann-solution store of the first stage (RiemannSolutions.h:12-30,
+//                        SpaceOperator.cpp:2518-2523) and the two one-sided numerical fluxes
+//   level-set stages     LevelSetOperator.cpp:1066-1141 (residual), GradientCalculatorFD3.cpp:54-171,
+//                        LevelSetOperator.cpp:934-1033 (ghost rules, evaluated on the fly) and the
+//                        stage combinations of TimeIntegrator.cpp:150-690
+//   material IDs         MultiPhaseOperator.cpp:127-330
+//   reinitialisation     LevelSetReinitializer.cpp:74-172, :297-430, :594-615, :953-1035, :1970-1999
+//                        (first layer FIXED)
+//   state update         MultiPhaseOperator.cpp:857-1028 (LocalUpdateByRiemannSolutions),
+//                        :1156-1320 (FixUnresolvedNodes, failsafe density)
+//
+// Device fields have no ghost frame: ghost values of V, ID and phi are produced by the boundary
+// rules where they are read. Interface faces are gathered once per step into a list (the material
+// IDs are fixed during a step); the Riemann pass runs one thread per listed face and writes both
+// one-sided fluxes into face-keyed arrays that the fused stage kernel reads.
+//
+// The reference updates the cells whose ID changed in place, in (k,j,i) order, so a cell sees the
+// updated velocity of an earlier changed neighbour. Here every changed cell is first computed from
+// the pre-update field; a cell whose computation reads a changed lower neighbour across a face that
+// carries a usable Riemann solution is recomputed once that neighbour is final (a few dependency
+// rounds, then an ordered single-thread pass for any chain that is still open), which reproduces
+// the in-order result exactly.
+#include <hip/hip_runtime.h>
+
+#include "fv_kernels.hpp"
+#include "multiphase.hpp"
+#include "riemann_dev.hpp"
+
+namespace m2c {
+
+static __constant__ MpParams c_mp;
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ long long cidx3(int i, int j, int k) {
+  return ((long long)k * c_grid.ny + j) * c_grid.nx + i;
+}
+
+__device__ __forceinline__ double XC(int d, int i) { return c_mp.xc[d][i + 1]; }
+
+// wave-aggregated append; every lane of the wavefront must call it
+__device__ __forceinline__ unsigned wave_append(unsigned *counter, bool pred) {
+  unsigned long long m = __ballot(pred);
+  const int lane = threadIdx.x & 63;
+  unsigned base = 0;
+  if (m) {
+    const int leader = __ffsll((long long)m) - 1;
+    if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
+    base = __shfl((int)base, leader, 64);
+  }
+  return base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *dst, double v) {
+  atomicMax(dst, (unsigned long long)__double_as_longlong(v));   // v >= 0: bits are ordered
+}
+
+__device__ __forceinline__ double bits_to_d(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// material ID of (possibly ghost) cell (i,j,k): the image cell's (MultiPhaseOperator.cpp:796-819)
+__device__ __forceinline__ int cell_id(const int8_t *ID, int i, int j, int k) {
+  const Grid &g = c_grid;
+  int ii = clampi(image_idx(i, g.nx, i < 0 ? g.bc[0] == BC_PERIODIC : g.bc[1] == BC_PERIODIC), 0, g.nx - 1);
+  int jj = clampi(image_idx(j, g.ny, j < 0 ? g.bc[2] == BC_PERIODIC : g.bc[3] == BC_PERIODIC), 0, g.ny - 1);
+  int kk = clampi(image_idx(k, g.nz, k < 0 ? g.bc[4] == BC_PERIODIC : g.bc[5] == BC_PERIODIC), 0, g.nz - 1);
+  return ID[cidx3(ii, jj, kk)];
+}
+
+// level set at (i,j,k) with at most one index one step outside the box: the face-ghost rules of
+// LevelSetOperator.cpp:934-1033 (1 zero-Neumann, 2 linear extrapolation, 3 non-negative,
+// 7 periodic)
+__device__ inline double phi_at(const double *phi, int i, int j, int k) {
+  const Grid &g = c_grid;
+  int n[3] = {g.nx, g.ny, g.nz};
+  int q[3] = {i, j, k};
+  int dim = -1;
+  if (i < 0 || i >= g.nx) dim = 0;
+  else if (j < 0 || j >= g.ny) dim = 1;
+  else if (k < 0 || k >= g.nz) dim = 2;
+  if (dim < 0) return phi[cidx3(i, j, k)];
+  const int gq = q[dim];
+  const int hi = gq >= n[dim];
+  const int bc = c_mp.ls_bc[2 * dim + hi];
+  int im[3] = {q[0], q[1], q[2]};
+  if (bc == BC_PERIODIC) im[dim] = hi ? 0 : n[dim] - 1;
+  else im[dim] = hi ? n[dim] - 1 : 0;
+  double v = phi[cidx3(im[0], im[1], im[2])];
+  if (bc == 2 || bc == 3) {
+    if (n[dim] >= 2) {
+      const int s = hi ? -1 : 1;
+      int a1[3] = {q[0], q[1], q[2]}, a2[3] = {q[0], q[1], q[2]};
+      a1[dim] = gq + s;
+      a2[dim] = gq + 2 * s;
+      double r = XC(dim, gq), r1 = XC(dim, gq + s), r2 = XC(dim, gq + 2 * s);
+      double f1 = phi[cidx3(a1[0], a1[1], a1[2])], f2 = phi[cidx3(a2[0], a2[1], a2[2])];
+      v = f1 + (f2 - f1) / (r2 - r1) * (r - r1);
+    }
+    if (bc == 3) {
+      double d2wall = 0.5 * dabs(XC(dim, im[dim]) - XC(dim, gq));
+      v = smax(-d2wall, v);
+    }
+  }
+  return v;
+}
+
+// velocity of a cell of the stage field S (primitive, or conservative -> ConservativeToPrimitive)
+template <int IN_CONS>
+__device__ __forceinline__ void cell_velocity(const double *S, const int8_t *ID, long long idx, double *u) {
+  const Grid &g = c_grid;
+  if (IN_CONS) {
+    const DevEos &m = c_eos[ID[idx]];
+    if (m.type == EOS_DUMMY) { u[0] = u[1] = u[2] = 0.0; return; }
+    double invRho = 1.0 / S[idx];
+    u[0] = S[g.N + idx] * invRho;
+    u[1] = S[2 * g.N + idx] * invRho;
+    u[2] = S[3 * g.N + idx] * invRho;
+  } else {
+    u[0] = S[g.N + idx];
+    u[1] = S[2 * g.N + idx];
+    u[2] = S[3 * g.N + idx];
+  }
+}
+
+// upwinded third-order level-set residual -(u . grad phi) of interior cell (i,j,k)
+// (LevelSetOperator.cpp:1066-1141 with the FD3 stencils of GradientCalculatorFD3.cpp:54-171)
+__device__ inline double ls_residual(const double *phi, int i, int j, int k, const double *u) {
+  const Grid &g = c_grid;
+  const int NX = g.nx, NY = g.ny, NZ = g.nz;
+  const double p0 = phi[cidx3(i, j, k)];
+  double a, b, cc, d, e, f;
+  if (i - 2 >= -1) {
+    const double *w = c_mp.fdl[0] + 4 * i;
+    a = w[0] * phi_at(phi, i - 2, j, k) + w[1] * phi_at(phi, i - 1, j, k) + w[2] * p0 + w[3] * phi_at(phi, i + 1, j, k);
+  } else
+    a = (p0 - phi_at(phi, i - 1, j, k)) / (XC(0, i) - XC(0, i - 1));
+  if (i + 2 <= NX) {
+    const double *w = c_mp.fdr[0] + 4 * i;
+    b = w[0] * phi_at(phi, i - 1, j, k) + w[1] * p0 + w[2] * phi_at(phi, i + 1, j, k) + w[3] * phi_at(phi, i + 2, j, k);
+  } else
+    b = (phi_at(phi, i + 1, j, k) - p0) / (XC(0, i + 1) - XC(0, i));
+  if (j - 2 >= -1) {
+    const double *w = c_mp.fdl[1] + 4 * j;
+    cc = w[0] * phi_at(phi, i, j - 2, k) + w[1] * phi_at(phi, i, j - 1, k) + w[2] * p0 + w[3] * phi_at(phi, i, j + 1, k);
+  } else
+    cc = (p0 - phi_at(phi, i, j - 1, k)) / (XC(1, j) - XC(1, j - 1));
+  if (j + 2 <= NY) {
+    // the right-biased y coefficients are third order only where the column index i < NY-1
+    // (GradientCalculatorFD3.cpp:415 tests i, not j)
+    const double *w = (i < NY - 1 && j < NY - 1) ? c_mp.fdr[1] + 4 * j : c_mp.fdr1y + 4 * j;
+    d = w[0] * phi_at(phi, i, j - 1, k) + w[1] * p0 + w[2] * phi_at(phi, i, j + 1, k) + w[3] * phi_at(phi, i, j + 2, k);
+  } else
+    d = (phi_at(phi, i, j + 1, k) - p0) / (XC(1, j + 1) - XC(1, j));
+  if (k - 2 >= -1) {
+    const double *w = c_mp.fdl[2] + 4 * k;
+    e = w[0] * phi_at(phi, i, j, k - 2) + w[1] * phi_at(phi, i, j, k - 1) + w[2] * p0 + w[3] * phi_at(phi, i, j, k + 1);
+  } else
+    e = (p0 - phi_at(phi, i, j, k - 1)) / (XC(2, k) - XC(2, k - 1));
+  if (k + 2 <= NZ) {
+    const double *w = c_mp.fdr[2] + 4 * k;
+    f = w[0] * phi_at(phi, i, j, k - 1) + w[1] * p0 + w[2] * phi_at(phi, i, j, k + 1) + w[3] * phi_at(phi, i, j, k + 2);
+  } else
+    f = (phi_at(phi, i, j, k + 1) - p0) / (XC(2, k + 1) - XC(2, k));
+  double r = (u[0] >= 0 ? a * u[0] : b * u[0]) + (u[1] >= 0 ? cc * u[1] : d * u[1]) + (u[2] >= 0 ? e * u[2] : f * u[2]);
+  return r * -1.0;
+}
+
+__device__ __forceinline__ bool step_active(const Clock *clk) { return clk->active != 0; }
+
+// ---------------------------------------------------------------- per-step bookkeeping
+
+__global__ void k_mp_step_begin(MpState *ms, const Clock *clk) {
+  if (!step_active(clk)) return;
+  ms->epoch += 1;
+  ms->nface = 0;
+  ms->nrem = 0;
+  ms->nchg = 0;
+  ms->nunres = 0;
+  ms->nseq = 0;
+}
+
+__global__ void k_mp_face_reset(MpState *ms) { ms->nface = 0; }
+
+// interface faces of the current IDs: (dir, plus cell) with different active materials on the two
+// sides; code = dir*Nf + face_key
+__global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *faces, MpState *ms, const Clock *clk,
+                                                     int ignore_active) {
+  if (!ignore_active && !step_active(clk)) return;
+  const Grid &g = c_grid;
+  const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  const long long total = 3 * Nf;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < total; base += (long long)gridDim.x * blockDim.x) {
+    long long code = base + threadIdx.x;
+    bool pred = false;
+    if (code < total) {
+      int dir = (int)(code / Nf);
+      long long key = code - dir * Nf;
+      int i = (int)(key % (g.nx + 1));
+      long long r = key / (g.nx + 1);
+      int j = (int)(r % (g.ny + 1));
+      int k = (int)(r / (g.ny + 1));
+      bool valid = (dir == 0) ? (j < g.ny && k < g.nz) : (dir == 1 ? (i < g.nx && k < g.nz) : (i < g.nx && j < g.ny));
+      if (valid) {
+        int idp = cell_id(ID, i, j, k);
+        int idm = cell_id(ID, i - (dir == 0), j - (dir == 1), k - (dir == 2));
+        pred = idm != idp && idm != g.inactive_id && idp != g.inactive_id && g.mp_flux != 2;
+      }
+    }
+    unsigned slot = wave_append(&ms->nface, pred);
+    if (pred) faces[slot] = (unsigned)code;
+  }
+}
+
+// ---------------------------------------------------------------- interface Riemann pass
+
+template <int FLUX, int DIR>
+__device__ __forceinline__ void iface_fluxes(const double *vm, const double *Vsm, int idm, const double *Vsp,
+                                             const double *vp, int idp, const double *Vmid, int midid, double *lf1,
+                                             double *lf2) {
+  if (c_grid.mp_flux == 1) {   // EXACT: physical flux of the state at xi = 0 to both sides
+    phys_flux<double, -1, DIR>(c_eos[midid], Vmid, lf1);
+#pragma unroll
+    for (int p = 0; p < 5; p++) lf2[p] = lf1[p];
+  } else {
+    numerical_flux<FLUX, -1, DIR>(vm, Vsm, idm, lf1);
+    numerical_flux<FLUX, -1, DIR>(Vsp, vp, idp, lf2);
+  }
+}
+
+// One thread per listed face (grid-stride). store != 0: keep the star states of this stage as
+// the step's Riemann solutions (first stage of a step).
+template <int FLUX, int IN_CONS>
+__global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *ID, const unsigned *faces,
+                                                 MpState *ms, double *Fi, double *rsV, unsigned long long *rsMeta,
+                                                 int store, double *scratch, Status *st, const Clock *clk,
+                                                 int ignore_active) {
+  if (!ignore_active && !step_active(clk)) return;
+  const Grid &g = c_grid;
+  const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  const int T = gridDim.x * blockDim.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned n = ms->nface;
+  RSolver S_;
+  S_.mat = c_eos;
+  S_.cfg = c_mp.rcfg;
+  S_.overflow = 0;
+  const int cap = c_mp.path_cap;
+  S_.path1.base = scratch + tid;
+  S_.path1.cap = cap;
+  S_.path1.stride = T;
+  S_.path1.n = 0;
+  S_.path3.base = scratch + (long long)3 * cap * T + tid;
+  S_.path3.cap = cap;
+  S_.path3.stride = T;
+  S_.path3.n = 0;
+  unsigned flags = 0;
+  unsigned long long nerr = 0;
+  const unsigned long long stamp = ms->epoch;
+  for (unsigned e = tid; e < n; e += T) {
+    const unsigned code = faces[e];
+    const int dir = (int)(code / Nf);
+    const long long key = code - (long long)dir * Nf;
+    const int i = (int)(key % (g.nx + 1));
+    const long long r = key / (g.nx + 1);
+    const int j = (int)(r % (g.ny + 1));
+    const int k = (int)(r / (g.ny + 1));
+    double vm[5], vp[5];
+    int idm, idp;
+    unsigned fl = 0;
+    fetch_cell<IN_CONS, -1>(S, ID, i - (dir == 0), j - (dir == 1), k - (dir == 2), vm, idm, fl);
+    fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp, idp, fl);
+    flags |= fl;
+    double nrm[3] = {0.0, 0.0, 0.0};
+    nrm[dir] = 1.0;
+    double Vmid[5], Vsm[5], Vsp[5];
+    int midid = 0;
+    int err = riemann_solve(S_, nrm, vm, idm, vp, idp, Vmid, &midid, Vsm, Vsp);
+    if (err < 0) flags |= ST_BAD_STATE;
+    if (err) nerr++;
+    eos_clip<double, -1>(c_eos[idm], Vsm);
+    eos_clip<double, -1>(c_eos[idp], Vsp);
+    if (store && err == 0) {
+#pragma unroll
+      for (int p = 0; p < 5; p++) {
+        rsV[((long long)dir * 10 + p) * Nf + key] = Vsm[p];
+        rsV[((long long)dir * 10 + 5 + p) * Nf + key] = Vsp[p];
+      }
+      rsMeta[(long long)dir * Nf + key] = (stamp << 16) | ((unsigned long long)(idm & 0xff) << 8) |
+                                          (unsigned long long)(idp & 0xff);
+    }
+    double lf1[5], lf2[5];
+    if (dir == 0) iface_fluxes<FLUX, 0>(vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
+    else if (dir == 1) iface_fluxes<FLUX, 1>(vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
+    else iface_fluxes<FLUX, 2>(vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
+    double *f = Fi + (long long)dir * 10 * Nf + key;
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+      f[p * Nf] = lf1[p];
+      f[(5 + p) * Nf] = lf2[p];
+    }
+  }
+  if (S_.overflow) flags |= ST_RIEMANN_PATH;
+  if (nerr) atomicAdd(&st->riemann_err, nerr);
+  if (flags & (ST_BAD_STATE | ST_RIEMANN_PATH | ST_CLIP_INTERMEDIATE)) atomicOr(&st->bits, flags);
+}
+
+// ---------------------------------------------------------------- level-set stages
+
+// out = t + (coef*dt) R(phiS; velocity of S), with t = 0*out + 1*X (LS_FIRST), a*X + b*Y (LS_AXBY)
+// or X (LS_X). FINAL != 0: also the material-ID update of the step (MultiPhaseOperator.cpp:127-213)
+// from sign(phi_n) = sign(X... phin) and the new value; out may alias phin in that case.
+template <int IN_CONS, int MODE, int FINAL>
+__global__ __launch_bounds__(256) void k_ls_stage(const double *S, const int8_t *IDn, const double *phiS,
+                                                   const double *X, const double *Y, double *out, double a, double b,
+                                                   double coef, const double *phin, int8_t *IDnew, unsigned *rem,
+                                                   MpState *ms, const Clock *clk) {
+  if (!step_active(clk)) return;
+  const Grid &g = c_grid;
+  const double cdt = coef * clk->dt;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < g.N; base += (long long)gridDim.x * blockDim.x) {
+    const long long idx = base + threadIdx.x;
+    bool swept_out = false;
+    if (idx < g.N) {
+      const int i = (int)(idx % g.nx);
+      const long long r = idx / g.nx;
+      const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+      double u[3];
+      cell_velocity<IN_CONS>(S, IDn, idx, u);
+      const double R = ls_residual(phiS, i, j, k, u);
+      double t;
+      if (MODE == LS_FIRST) {
+        double x = X[idx];
+        t = (x == 0.0) ? 0.0 * out[idx] + 1.0 * x : x;   // 0*old + 1*x differs from x only in the sign of zero
+      } else if (MODE == LS_AXBY) {
+        t = a * X[idx] + b * Y[idx];
+      } else {
+        t = X[idx];
+      }
+      const double pn = FINAL ? phin[idx] : 0.0;
+      const double v = t + cdt * R;
+      out[idx] = v;
+      if (FINAL) {
+        int id = IDn[idx];
+        int nid = id;
+        if (id != g.inactive_id && !(v * pn > 0.0)) {
+          if (v < 0) nid = c_mp.ls_matid;
+          else swept_out = true;
+        }
+        IDnew[idx] = (int8_t)nid;
+      }
+    }
+    if (FINAL) {
+      unsigned slot = wave_append(&ms->nrem, swept_out);
+      if (swept_out) rem[slot] = (unsigned)idx;
+    }
+  }
+}
+
+// nodes swept out of the level-set subdomain take a neighbour's untracked ID; when the only
+// candidate material is 0 every such node ends with ID 0 whatever the visiting order
+// (MultiPhaseOperator.cpp:214-318, orphans -> 0)
+__global__ __launch_bounds__(256) void k_id_remaining_single(const unsigned *rem, const MpState *ms, int8_t *IDnew,
+                                                              const Clock *clk) {
+  if (!step_active(clk)) return;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < ms->nrem; e += gridDim.x * blockDim.x)
+    IDnew[rem[e]] = 0;
+}
+
+// bitonic sort of up to MP_SORT_CAP 64-bit keys in LDS by one block of 1024 threads
+__device__ void block_sort(unsigned long long *s, int n) {
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int t = threadIdx.x + n; t < m; t += blockDim.x) s[t] = ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= m; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        int o = t ^ stride;
+        if (o > t) {
+          bool up = (t & size) == 0;
+          unsigned long long x = s[t], y = s[o];
+          if ((x > y) == up) { s[t] = y; s[o] = x; }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+// general case: the reference's passes in Int3 (i,j,k) order, one thread, Gauss-Seidel updates
+__global__ __launch_bounds__(1024) void k_id_remaining_seq(const unsigned *rem, MpState *ms, int8_t *IDnew,
+                                                            uint8_t *tagR, Status *st, const Clock *clk) {
+  if (!step_active(clk)) return;
+  const Grid &g = c_grid;
+  __shared__ unsigned long long s[MP_SORT_CAP];
+  __shared__ uint8_t gone[MP_SORT_CAP];
+  const int n = (int)ms->nrem;
+  if (n == 0) return;
+  if (n > MP_SORT_CAP) {
+    if (threadIdx.x == 0) atomicOr(&st->bits, (unsigned)ST_LIST_OVERFLOW);
+    return;
+  }
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    long long idx = rem[t];
+    int i = (int)(idx % g.nx);
+    long long r = idx / g.nx;
+    int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    s[t] = ((unsigned long long)i * g.ny + j) * g.nz + k;   // Int3(i,j,k) order (Vector3D.h:31-35)
+    tagR[idx] = 1;
+    gone[t] = 0;
+  }
+  __syncthreads();
+  block_sort(s, n);
+  if (threadIdx.x == 0) {
+    int left = n;
+    for (int iter = 0; iter < 100 && left > 0; iter++) {
+      int progress = 0;
+      for (int q = 0; q < n; q++) {
+        if (gone[q]) continue;
+        unsigned long long key = s[q];
+        int k = (int)(key % g.nz);
+        unsigned long long r = key / g.nz;
+        int j = (int)(r % g.ny), i = (int)(r / g.ny);
+        int best = -1;
+        for (int nk = k - 1; nk <= k + 1; nk++)
+          for (int nj = j - 1; nj <= j + 1; nj++)
+            for (int ni = i - 1; ni <= i + 1; ni++) {
+              if (ni < 0 || ni >= g.nx || nj < 0 || nj >= g.ny || nk < 0 || nk >= g.nz) continue;
+              long long nb = cidx3(ni, nj, nk);
+              if (tagR[nb]) continue;
+              int nid = IDnew[nb];
+              if (nid == g.inactive_id || nid == c_mp.ls_matid) continue;
+              if (best < 0 || nid < best) best = nid;
+            }
+        if (best < 0) continue;
+        long long me = cidx3(i, j, k);
+        IDnew[me] = (int8_t)best;
+        tagR[me] = 0;
+        gone[q] = 1;
+        left--;
+        progress++;
+      }
+      if (!progress) {
+        for (int q = 0; q < n; q++)
+          if (!gone[q]) {
+            unsigned long long key = s[q];
+            int k = (int)(key % g.nz);
+            unsigned long long r = key / g.nz;
+            int j = (int)(r % g.ny), i = (int)(r / g.ny);
+            IDnew[cidx3(i, j, k)] = 0;
+          }
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < n; t += blockDim.x) tagR[rem[t]] = 0;
+}
+
+// ---------------------------------------------------------------- reinitialisation
+
+__global__ void k_reinit_begin(MpState *ms, const Clock *clk, int retry) {
+  ReinitState &R = ms->rs;
+  if (!retry) {
+    int ts = clk->steps + 1;
+    R.active = step_active(clk) && c_mp.reinit_freq > 0 && ts % c_mp.reinit_freq == 0;
+    R.detected = 0;
+  }
+  R.retry = 0;
+  R.done = 0;
+  R.iter = c_mp.reinit_maxits;
+  R.maxres_bits = 0ull;
+  R.dphi_bits = 0x7FF0000000000000ull;   // +inf
+}
+
+// first-layer tags (interior), sign function and the failsafe copy (LevelSetReinitializer.cpp:74-100)
+__global__ __launch_bounds__(256) void k_reinit_tag(const double *phi, uint8_t *tag, double *sign, double *phibk,
+                                                     MpState *ms) {
+  if (!ms->rs.active) return;
+  const Grid &g = c_grid;
+  int det = 0;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % g.nx);
+    const long long r = idx / g.nx;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    const double p = phi[idx];
+    int t = 0;
+    if (p * phi_at(phi, i - 1, j, k) <= 0) t = 1;
+    else if (p * phi_at(phi, i + 1, j, k) <= 0) t = 1;
+    else if (p * phi_at(phi, i, j - 1, k) <= 0) t = 1;
+    else if (p * phi_at(phi, i, j + 1, k) <= 0) t = 1;
+    else if (p * phi_at(phi, i, j, k - 1) <= 0) t = 1;
+    else if (p * phi_at(phi, i, j, k + 1) <= 0) t = 1;
+    tag[idx] = (uint8_t)t;
+    det |= t;
+    const double f = c_mp.sign_factor;
+    sign[idx] = p / dsqrt<double>(p * p + f * f);
+    const double ob = phibk[idx];
+    phibk[idx] = 0.0 * ob + 1.0 * p;
+  }
+  if (__any(det) && (threadIdx.x & 63) == 0) atomicOr(&ms->rs.detected, 1);
+}
+
+__global__ void k_reinit_restore(double *phi, const double *phibk, const MpState *ms) {
+  const Grid &g = c_grid;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x)
+    phi[idx] = 0.0 * phi[idx] + 1.0 * phibk[idx];
+}
+
+// pseudo-time residual of the reinitialisation equation (LevelSetReinitializer.cpp:953-1035)
+__device__ inline double reinit_rhs(const double *phi, const uint8_t *tag, const double *sign, int i, int j, int k,
+                                    long long idx, double cfl, double *lres) {
+  *lres = 0.0;
+  if (tag[idx] != 0) return 0.0;
+  const double dt = cfl * c_mp.min_dx;
+  const double p = phi[idx];
+  double a = (p - phi_at(phi, i - 1, j, k)) / (XC(0, i) - XC(0, i - 1));
+  double b = (phi_at(phi, i + 1, j, k) - p) / (XC(0, i + 1) - XC(0, i));
+  double c = (p - phi_at(phi, i, j - 1, k)) / (XC(1, j) - XC(1, j - 1));
+  double d = (phi_at(phi, i, j + 1, k) - p) / (XC(1, j + 1) - XC(1, j));
+  double e = (p - phi_at(phi, i, j, k - 1)) / (XC(2, k) - XC(2, k - 1));
+  double f = (phi_at(phi, i, j, k + 1) - p) / (XC(2, k + 1) - XC(2, k));
+  double ap = smax(a, 0.0), am = smin(a, 0.0), bp = smax(b, 0.0), bm = smin(b, 0.0);
+  double cp = smax(c, 0.0), cm = smin(c, 0.0), dp = smax(d, 0.0), dm = smin(d, 0.0);
+  double ep = smax(e, 0.0), em = smin(e, 0.0), fp = smax(f, 0.0), fm = smin(f, 0.0);
+  double lr;
+  if (p >= 0)
+    lr = dsqrt<double>(smax(ap * ap, bm * bm) + smax(cp * cp, dm * dm) + smax(ep * ep, fm * fm)) - 1.0;
+  else
+    lr = dsqrt<double>(smax(am * am, bp * bp) + smax(cm * cm, dp * dp) + smax(em * em, fp * fp)) - 1.0;
+  *lres = dabs(lr);
+  if (dabs(lr) <= 1.0) return -dt * sign[idx] * lr;
+  return -dt * sign[idx] * lr / dabs(lr);
+}
+
+// RK3 stage 1 of reinitialisation iteration `it`: rphi1a = (0*rphi1a + 1*phi) + 1*R(phi)
+__global__ __launch_bounds__(256) void k_reinit_A(const double *phi, double *rphi1a, const uint8_t *tag,
+                                                   const double *sign, MpState *ms, int it) {
+  ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected || Rs.done) return;
+  if (it > 0 && bits_to_d(Rs.dphi_bits) < c_mp.reinit_tol) {   // converged after iteration it-1
+    if (blockIdx.x == 0 && threadIdx.x == 0) { Rs.done = 1; Rs.iter = it - 1; }
+    return;
+  }
+  const Grid &g = c_grid;
+  const double cfl = Rs.cfl;
+  double mx = 0.0;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % g.nx);
+    const long long r = idx / g.nx;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    double lres;
+    double R = reinit_rhs(phi, tag, sign, i, j, k, idx, cfl, &lres);
+    if (lres > mx) mx = lres;
+    double x = phi[idx];
+    double t = (x == 0.0) ? 0.0 * rphi1a[idx] + 1.0 * x : x;
+    rphi1a[idx] = t + 1.0 * R;
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = smax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0 && mx > 0.0) atomic_max_pos(&Rs.maxres_bits, mx);
+}
+
+// stage 2: stop if the residual of stage 1 is below tolerance, else
+// rphi1b = (0.25*rphi1a + 0.75*phi) + 0.25*R(rphi1a)
+__global__ __launch_bounds__(256) void k_reinit_B(const double *phi, const double *rphi1a, double *rphi1b,
+                                                   const uint8_t *tag, const double *sign, MpState *ms, int it) {
+  ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected || Rs.done) return;
+  if (bits_to_d(Rs.maxres_bits) < c_mp.reinit_tol) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { Rs.done = 1; Rs.iter = it; }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) Rs.dphi_bits = 0ull;
+  const Grid &g = c_grid;
+  const double cfl = Rs.cfl;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % g.nx);
+    const long long r = idx / g.nx;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    double lres;
+    double R = reinit_rhs(rphi1a, tag, sign, i, j, k, idx, cfl, &lres);
+    double t = 0.25 * rphi1a[idx] + 0.75 * phi[idx];
+    rphi1b[idx] = t + 0.25 * R;
+  }
+}
+
+// stage 3: phi = (1/3*phi + 2/3*rphi1b) + 2/3*R(rphi1b), and the maximum relative change
+__global__ __launch_bounds__(256) void k_reinit_C(double *phi, const double *rphi1b, const uint8_t *tag,
+                                                   const double *sign, MpState *ms) {
+  ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected || Rs.done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) Rs.maxres_bits = 0ull;
+  const Grid &g = c_grid;
+  const double cfl = Rs.cfl;
+  double mx = 0.0;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % g.nx);
+    const long long r = idx / g.nx;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    double lres;
+    double R = reinit_rhs(rphi1b, tag, sign, i, j, k, idx, cfl, &lres);
+    const double po = phi[idx];
+    double t = 1.0 / 3.0 * po + 2.0 / 3.0 * rphi1b[idx];
+    const double pn = t + 2.0 / 3.0 * R;
+    phi[idx] = pn;
+    double le = dabs(pn - po);
+    if (pn != 0) le /= dabs(pn);
+    mx = smax(mx, le);
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = smax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) atomic_max_pos(&Rs.dphi_bits, mx);
+}
+
+// after the last iteration: convergence on the last relative change, else the failsafe of
+// LevelSetReinitializer.cpp:144-169 (accept below 0.02, retry with cfl/1.5, fail below cfl 0.02)
+__global__ void k_reinit_end(MpState *ms, Status *st) {
+  ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected) return;
+  if (!Rs.done) {
+    if (bits_to_d(Rs.dphi_bits) < c_mp.reinit_tol) Rs.iter = c_mp.reinit_maxits - 1;
+    else Rs.iter = c_mp.reinit_maxits;
+  }
+  ms->reinit_iters += Rs.iter;
+  if (Rs.iter < c_mp.reinit_maxits) return;
+  if (bits_to_d(Rs.dphi_bits) < 0.02) return;
+  if (Rs.cfl < 0.02) {
+    atomicOr(&st->bits, (unsigned)ST_REINIT_FAIL);
+    return;
+  }
+  Rs.cfl = Rs.cfl / 1.5;
+  Rs.retry = 1;
+}
+
+// ---------------------------------------------------------------- state update after interface motion
+
+__global__ __launch_bounds__(256) void k_changed_list(const int8_t *IDn, const int8_t *IDnew, unsigned *chg,
+                                                       int *cidx, MpState *ms, const Clock *clk) {
+  if (!step_active(clk)) return;
+  const Grid &g = c_grid;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < g.N; base += (long long)gridDim.x * blockDim.x) {
+    long long idx = base + threadIdx.x;
+    bool pred = idx < g.N && IDnew[idx] != IDn[idx] && IDnew[idx] != g.inactive_id;
+    unsigned slot = wave_append(&ms->nchg, pred);
+    if (pred) {
+      chg[slot] = (unsigned)idx;
+      cidx[idx] = (int)slot;
+    }
+  }
+}
+
+// neighbour state for the state update: the value final at this point of the in-order sweep
+// (a changed, resolved, final lower neighbour contributes its new value; everything else the
+// field after the last stage, ghosts by the boundary rules)
+struct NbView {
+  const double *V;
+  const int8_t *IDn;
+  const int *cidx;
+  const int *lvl;
+  const int8_t *cnt;
+  const double *Vnew;
+  unsigned ncap;
+};
+
+// LocalUpdateByRiemannSolutions (MultiPhaseOperator.cpp:916-1028). `round` >= 0: lower changed
+// neighbours must be final before `round` (returns -2 if one is not); round < 0: round 0, the
+// pre-update field is used everywhere and *dep reports whether a lower changed neighbour was read.
+__device__ int local_update(const NbView &nv, const double *rsV, const unsigned long long *rsMeta,
+                            unsigned long long stamp, int i, int j, int k, int id, bool upwind, double *v, int round,
+                            bool *dep) {
+  const Grid &g = c_grid;
+  const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  int counter = 0;
+  double sum_weight = 0.0;
+#pragma unroll 1
+  for (int s = 0; s < 6; s++) {
+    const int dir = s >> 1, plus = s & 1;
+    const int fi = i + (dir == 0 && plus), fj = j + (dir == 1 && plus), fk = k + (dir == 2 && plus);
+    const long long key = ((long long)fk * (g.ny + 1) + fj) * (g.nx + 1) + fi;
+    const unsigned long long meta = rsMeta[(long long)dir * Nf + key];
+    if ((meta >> 16) != stamp) continue;
+    const int rid = plus ? (int)(meta & 0xff) : (int)((meta >> 8) & 0xff);
+    if (rid != id) continue;
+    const int ni = i + (dir == 0 ? (plus ? 1 : -1) : 0);
+    const int nj = j + (dir == 1 ? (plus ? 1 : -1) : 0);
+    const int nk = k + (dir == 2 ? (plus ? 1 : -1) : 0);
+    double vn[5];
+    bool used_new = false;
+    if (!plus && ni >= 0 && nj >= 0 && nk >= 0) {
+      const long long nidx = cidx3(ni, nj, nk);
+      const int ci = nv.cidx[nidx];
+      if (ci >= 0) {   // a changed lower neighbour
+        if (round < 0) {
+          if (upwind) *dep = true;
+        } else {
+          const int l = nv.lvl[ci];
+          if (l < 0 || l >= round) return -2;
+          if (nv.cnt[ci] > 0) {
+#pragma unroll
+            for (int p = 0; p < 5; p++) vn[p] = nv.Vnew[(long long)5 * ci + p];
+            used_new = true;
+          }
+        }
+      }
+    }
+    if (!used_new) {
+      int dummy;
+      unsigned fl = 0;
+      fetch_cell<0, -1>(nv.V, nv.IDn, ni, nj, nk, vn, dummy, fl);
+    }
+    const int comp = dir + 1;
+    if (upwind && !(plus ? vn[comp] < 0 : vn[comp] > 0)) continue;
+    double w;
+    if (upwind) {
+      double nrm = dsqrt<double>(vn[1] * vn[1] + vn[2] * vn[2] + vn[3] * vn[3]);
+      w = (plus ? -vn[comp] : vn[comp]) / nrm;
+    } else {
+      w = 1.0;
+    }
+    sum_weight += w;
+    const double *rs = rsV + ((long long)dir * 10 + (plus ? 5 : 0)) * Nf + key;
+    if (counter == 0) {
+#pragma unroll
+      for (int p = 0; p < 5; p++) v[p] = w * rs[p * Nf];
+    } else {
+#pragma unroll
+      for (int p = 0; p < 5; p++) v[p] += w * rs[p * Nf];
+    }
+    counter++;
+  }
+  if (sum_weight > 0.0) {
+    double inv = 1.0 / sum_weight;
+#pragma unroll
+    for (int p = 0; p < 5; p++) v[p] *= inv;
+  }
+  return counter;
+}
+
+__device__ __forceinline__ void load_v(const double *V, long long idx, double *v) {
+#pragma unroll
+  for (int p = 0; p < 5; p++) v[p] = V[p * c_grid.N + idx];
+}
+
+// upwinded pass, then the pass without upwinding (MultiPhaseOperator.cpp:881-892)
+__device__ int update_cell(const NbView &nv, const double *rsV, const unsigned long long *rsMeta,
+                           unsigned long long stamp, long long idx, int id, double *v, int round, bool *dep) {
+  const int i = (int)(idx % c_grid.nx);
+  const long long r = idx / c_grid.nx;
+  const int j = (int)(r % c_grid.ny), k = (int)(r / c_grid.ny);
+  load_v(nv.V, idx, v);
+  int cnt = local_update(nv, rsV, rsMeta, stamp, i, j, k, id, true, v, round, dep);
+  if (cnt == -2) return -2;
+  if (cnt == 0) cnt = local_update(nv, rsV, rsMeta, stamp, i, j, k, id, false, v, round, dep);
+  return cnt;
+}
+
+// round 0: every changed cell from the pre-update field
+__global__ __launch_bounds__(256) void k_state_round0(NbView nv, const unsigned *chg, const int8_t *IDnew,
+                                                       const double *rsV, const unsigned long long *rsMeta,
+                                                       MpState *ms, int *lvl, int8_t *cnt, double *Vnew,
+                                                       const Clock *clk) {
+  if (!step_active(clk)) return;
+  const unsigned n = ms->nchg;
+  const unsigned long long stamp = ms->epoch;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const long long idx = chg[e];
+    double v[5];
+    bool dep = false;
+    int c = update_cell(nv, rsV, rsMeta, stamp, idx, IDnew[idx], v, -1, &dep);
+    cnt[e] = (int8_t)(c > 0 ? 1 : 0);
+    lvl[e] = dep ? -1 : 0;
+#pragma unroll
+    for (int p = 0; p < 5; p++) Vnew[(long long)5 * e + p] = v[p];
+  }
+}
+
+// dependency round `round`: cells whose changed lower neighbours are all final
+__global__ __launch_bounds__(256) void k_state_round(NbView nv, const unsigned *chg, const int8_t *IDnew,
+                                                      const double *rsV, const unsigned long long *rsMeta,
+                                                      MpState *ms, int *lvl, int8_t *cnt, double *Vnew, int round,
+                                                      const Clock *clk) {
+  if (!step_active(clk)) return;
+  const unsigned n = ms->nchg;
+  const unsigned long long stamp = ms->epoch;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    if (lvl[e] >= 0) continue;
+    const long long idx = chg[e];
+    double v[5];
+    bool dep = false;
+    int c = update_cell(nv, rsV, rsMeta, stamp, idx, IDnew[idx], v, round, &dep);
+    if (c == -2) continue;
+#pragma unroll
+    for (int p = 0; p < 5; p++) Vnew[(long long)5 * e + p] = v[p];
+    cnt[e] = (int8_t)(c > 0 ? 1 : 0);
+    __threadfence();
+    lvl[e] = round;
+  }
+}
+
+// cells still open after the rounds, in (k,j,i) order by one thread
+__global__ __launch_bounds__(1024) void k_state_seq(NbView nv, const unsigned *chg, const int8_t *IDnew,
+                                                     const double *rsV, const unsigned long long *rsMeta, MpState *ms,
+                                                     int *lvl, int8_t *cnt, double *Vnew, Status *st,
+                                                     const Clock *clk) {
+  if (!step_active(clk)) return;
+  __shared__ unsigned long long s[MP_SORT_CAP];
+  __shared__ int ns;
+  if (threadIdx.x == 0) ns = 0;
+  __syncthreads();
+  const unsigned n = ms->nchg;
+  for (unsigned e = threadIdx.x; e < n; e += blockDim.x)
+    if (lvl[e] < 0) {
+      int q = atomicAdd(&ns, 1);
+      if (q < MP_SORT_CAP) s[q] = ((unsigned long long)chg[e] << 32) | (unsigned long long)e;
+    }
+  __syncthreads();
+  const int m = ns;
+  if (m == 0) return;
+  if (m > MP_SORT_CAP) {
+    if (threadIdx.x == 0) atomicOr(&st->bits, (unsigned)ST_LIST_OVERFLOW);
+    return;
+  }
+  block_sort(s, m);
+  if (threadIdx.x == 0) {
+    ms->nseq += m;
+    const int big = 1 << 30;
+    for (int q = 0; q < m; q++) {
+      const unsigned e = (unsigned)(s[q] & 0xffffffffull);
+      const long long idx = (long long)(s[q] >> 32);
+      double v[5];
+      bool dep = false;
+      int c = update_cell(nv, rsV, rsMeta, ms->epoch, idx, IDnew[idx], v, big, &dep);
+      for (int p = 0; p < 5; p++) Vnew[(long long)5 * e + p] = v[p];
+      cnt[e] = (int8_t)(c > 0 ? 1 : 0);
+      lvl[e] = big - 1;   // final for every later cell of this pass
+    }
+  }
+}
+
+// new values into V; unresolved cells are listed for FixUnresolvedNodes
+__global__ __launch_bounds__(256) void k_state_commit(double *V, const unsigned *chg, const int8_t *cnt,
+                                                       const double *Vnew, unsigned *unres, MpState *ms,
+                                                       const Clock *clk) {
+  if (!step_active(clk)) return;
+  const unsigned n = ms->nchg;
+  for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    unsigned e = base + threadIdx.x;
+    bool un = false;
+    if (e < n) {
+      const long long idx = chg[e];
+      if (cnt[e] > 0) {
+#pragma unroll
+        for (int p = 0; p < 5; p++) V[p * c_grid.N + idx] = Vnew[(long long)5 * e + p];
+      } else {
+        un = true;
+      }
+    }
+    unsigned slot = wave_append(&ms->nunres, un);
+    if (un) unres[slot] = chg[e];
+  }
+}
+
+// FixUnresolvedNodes with the failsafe density (MultiPhaseOperator.cpp:1156-1320), in (k,j,i)
+// order by one thread; `reset` carries over from node to node as in the reference
+__global__ __launch_bounds__(1024) void k_fix_unresolved(double *V, const int8_t *IDnew, const int *cidx,
+                                                          const int8_t *cnt, const unsigned *unres, MpState *ms,
+                                                          Status *st, const Clock *clk) {
+  if (!step_active(clk)) return;
+  const Grid &g = c_grid;
+  __shared__ unsigned long long s[MP_SORT_CAP];
+  const int n = (int)ms->nunres;
+  if (n == 0) return;
+  if (n > MP_SORT_CAP) {
+    if (threadIdx.x == 0) atomicOr(&st->bits, (unsigned)ST_LIST_OVERFLOW);
+    return;
+  }
+  for (int t = threadIdx.x; t < n; t += blockDim.x) s[t] = unres[t];
+  __syncthreads();
+  block_sort(s, n);
+  if (threadIdx.x != 0) return;
+  ms->unresolved_total += n;
+  bool reset = false;
+  for (int q = 0; q < n; q++) {
+    const long long me = (long long)s[q];
+    const int i = (int)(me % g.nx);
+    const long long r = me / g.nx;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    const int myid = IDnew[me];
+    double v[5];
+    load_v(V, me, v);
+    const double x0[3] = {XC(0, i), XC(1, j), XC(2, k)};
+    double sum_weight = 0.0, sum_weight2 = 0.0, vtmp[5] = {0, 0, 0, 0, 0};
+    for (int nk = k - 1; nk <= k + 1; nk++)
+      for (int nj = j - 1; nj <= j + 1; nj++)
+        for (int ni = i - 1; ni <= i + 1; ni++) {
+          if (ni < 0 || ni >= g.nx || nj < 0 || nj >= g.ny || nk < 0 || nk >= g.nz) continue;
+          const long long nb = cidx3(ni, nj, nk);
+          if (IDnew[nb] != myid) continue;
+          if (nb == me) continue;
+          const int ci = cidx[nb];
+          if (ci >= 0 && cnt[ci] == 0) continue;   // also unresolved
+          double vn[5];
+          load_v(V, nb, vn);
+          double v1[3] = {vn[1], vn[2], vn[3]};
+          double v1n = dsqrt<double>(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+          if (v1n != 0) { double iv = 1.0 / v1n; v1[0] *= iv; v1[1] *= iv; v1[2] *= iv; }
+          double d[3] = {x0[0] - XC(0, ni), x0[1] - XC(1, nj), x0[2] - XC(2, nk)};
+          double dn = dsqrt<double>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          double idn = 1.0 / dn;
+          d[0] *= idn; d[1] *= idn; d[2] *= idn;
+          double dot = d[0] * v1[0] + d[1] * v1[1] + d[2] * v1[2];
+          double weight = smax(0.0, dot);
+          if (weight > 0) {
+            sum_weight += weight;
+            if (reset) for (int p = 0; p < 5; p++) v[p] += weight * vn[p];
+            else { for (int p = 0; p < 5; p++) v[p] = weight * vn[p]; reset = true; }
+          }
+          for (int p = 0; p < 5; p++) vtmp[p] += 1.0 / dn * vn[p];
+          sum_weight2 += 1.0 / dn;
+        }
+    if (sum_weight > 0) {
+      double inv = 1.0 / sum_weight;
+      for (int p = 0; p < 5; p++) v[p] *= inv;
+    } else if (sum_weight2 > 0) {
+      for (int p = 0; p < 5; p++) v[p] = vtmp[p] / sum_weight2;
+    } else {
+      double density = 0.0;
+      for (int layer = 1; layer <= 10; layer++) {
+        for (int nk = k - layer; nk <= k + layer; nk++)
+          for (int nj = j - layer; nj <= j + layer; nj++)
+            for (int ni = i - layer; ni <= i + layer; ni++) {
+              if (ni < 0 || ni >= g.nx || nj < 0 || nj >= g.ny || nk < 0 || nk >= g.nz) continue;
+              const long long nb = cidx3(ni, nj, nk);
+              if (IDnew[nb] != myid) continue;
+              if (nb == me) continue;
+              const int ci = cidx[nb];
+              if (ci >= 0 && cnt[ci] == 0) continue;
+              double d[3] = {XC(0, ni) - x0[0], XC(1, nj) - x0[1], XC(2, nk) - x0[2]};
+              double dist = dsqrt<double>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+              sum_weight += 1.0 / dist;
+              density += (1.0 / dist) * V[nb];
+            }
+        if (sum_weight > 0) { v[0] = density / sum_weight; break; }
+      }
+      if (sum_weight == 0) v[0] = c_mp.failsafe_density[myid];
+    }
+    for (int p = 0; p < 5; p++) V[p * g.N + me] = v[p];
+  }
+}
+
+// ClipDensityAndPressure + CheckState of the cells whose material changed (unchanged cells were
+// clipped with the same material by the last stage); IDs committed; per-step maps cleared
+__global__ __launch_bounds__(256) void k_state_finish(double *V, int8_t *IDn, const int8_t *IDnew, const unsigned *chg,
+                                                       int *cidx, MpState *ms, Status *st, const Clock *clk) {
+  if (!step_active(clk)) return;
+  const Grid &g = c_grid;
+  const unsigned n = ms->nchg;
+  unsigned flags = 0;
+  unsigned long long nclip = 0;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const long long idx = chg[e];
+    const int id = IDnew[idx];
+    double v[5];
+    load_v(V, idx, v);
+    const DevEos &m = c_eos[id];
+    if (eos_clip<double, -1>(m, v)) nclip++;
+    if (eos_bad<double, -1>(m, v)) flags |= ST_BAD_STATE;
+#pragma unroll
+    for (int p = 0; p < 5; p++) V[p * g.N + idx] = v[p];
+    IDn[idx] = (int8_t)id;
+    cidx[idx] = -1;
+  }
+  if (nclip) atomicAdd(&st->clipped, nclip);
+  if (flags) atomicOr(&st->bits, flags);
+}
+
+// ComputeRiemannSolution for a batch of independent problems (pointwise checks): problem t uses
+// the unit normal of axis dir[t]
+__global__ __launch_bounds__(64) void k_riemann_batch(int n, const int *dir, const double *Vm, const int *idm,
+                                                       const double *Vp, const int *idp, double *Vs, int *id,
+                                                       double *Vsm, double *Vsp, int *status, double *scratch,
+                                                       int nthreads) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nthreads) return;
+  RSolver S_;
+  S_.mat = c_eos;
+  S_.cfg = c_mp.rcfg;
+  S_.overflow = 0;
+  const int cap = c_mp.path_cap;
+  S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
+  S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
+  S_.path3.n = 0;
+  for (int t = tid; t < n; t += nthreads) {
+    double nrm[3] = {0.0, 0.0, 0.0};
+    nrm[dir[t]] = 1.0;
+    int mid = 0;
+    S_.overflow = 0;
+    int r = riemann_solve(S_, nrm, Vm + 5 * t, idm[t], Vp + 5 * t, idp[t], Vs + 5 * t, &mid, Vsm + 5 * t, Vsp + 5 * t);
+    id[t] = mid;
+    status[t] = S_.overflow ? 2 : r;
+  }
+}
+
+static int hchk(hipError_t e) { return e == hipSuccess ? 0 : 2; }
+
+}  // namespace m2c
+
+using namespace m2c;
+
+extern "C" int m2c_mp_upload(const Grid &grid, const DevEos *eos, const MpParams &mp) {
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_grid), &grid, sizeof(Grid)) != hipSuccess) return 2;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_eos), eos, sizeof(DevEos) * 8) != hipSuccess) return 2;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_mp), &mp, sizeof(MpParams)) != hipSuccess) return 2;
+  return 0;
+}
+
+static int nblocks(long long n, int bs, int cap) {
+  long long b = (n + bs - 1) / bs;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_cons, int store, int ignore_active) {
+  const long long total = 3 * L.Nf;
+  hipLaunchKernelGGL(k_mp_face_reset, dim3(1), dim3(1), 0, L.stream, L.ms);
+  hipLaunchKernelGGL(k_iface_list, dim3(nblocks(total, 256, 8192)), dim3(256), 0, L.stream, L.ID, L.faces, L.ms,
+                     L.clk, ignore_active);
+  dim3 gr(L.riemann_threads / 64), bl(64);
+#define M2C_RIEMANN(F)                                                                                          \
+  if (in_cons)                                                                                                  \
+    hipLaunchKernelGGL((k_riemann<F, 1>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,   \
+                       store, L.scratch, L.st, L.clk, ignore_active);                                           \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_riemann<F, 0>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,   \
+                       store, L.scratch, L.st, L.clk, ignore_active);
+  if (L.flux == FLUX_HLLC) { M2C_RIEMANN(FLUX_HLLC) }
+  else if (L.flux == FLUX_ROE) { M2C_RIEMANN(FLUX_ROE) }
+  else { M2C_RIEMANN(FLUX_LLF) }
+#undef M2C_RIEMANN
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,
+                                           const double *Vp, const int *idp, double *Vs, int *id, double *Vsm,
+                                           double *Vsp, int *status, double *scratch, int nthreads) {
+  hipLaunchKernelGGL(k_riemann_batch, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Vs, id,
+                     Vsm, Vsp, status, scratch, nthreads);
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_step_begin(const MpLaunch &L) {
+  hipLaunchKernelGGL(k_mp_step_begin, dim3(1), dim3(1), 0, L.stream, L.ms, L.clk);
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_ls(const MpLaunch &L, int in_cons, int mode, int final_, const double *S,
+                                const double *phiS, const double *X, const double *Y, double *out, double a, double b,
+                                double coef, const double *phin) {
+  dim3 gr(nblocks(L.N, 256, 16384)), bl(256);
+#define M2C_LS(IC, MO, FI)                                                                                      \
+  hipLaunchKernelGGL((k_ls_stage<IC, MO, FI>), gr, bl, 0, L.stream, S, L.ID, phiS, X, Y, out, a, b, coef, phin, \
+                     L.IDnew, L.rem, L.ms, L.clk)
+  if (final_) {
+    if (in_cons) {
+      if (mode == LS_AXBY) M2C_LS(1, LS_AXBY, 1); else if (mode == LS_X) M2C_LS(1, LS_X, 1); else M2C_LS(1, LS_FIRST, 1);
+    } else {
+      if (mode == LS_AXBY) M2C_LS(0, LS_AXBY, 1); else if (mode == LS_X) M2C_LS(0, LS_X, 1); else M2C_LS(0, LS_FIRST, 1);
+    }
+  } else {
+    if (in_cons) {
+      if (mode == LS_AXBY) M2C_LS(1, LS_AXBY, 0); else if (mode == LS_X) M2C_LS(1, LS_X, 0); else M2C_LS(1, LS_FIRST, 0);
+    } else {
+      if (mode == LS_AXBY) M2C_LS(0, LS_AXBY, 0); else if (mode == LS_X) M2C_LS(0, LS_X, 0); else M2C_LS(0, LS_FIRST, 0);
+    }
+  }
+#undef M2C_LS
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_id_remaining(const MpLaunch &L, int single) {
+  if (single)
+    hipLaunchKernelGGL(k_id_remaining_single, dim3(64), dim3(256), 0, L.stream, L.rem, L.ms, L.IDnew, L.clk);
+  else
+    hipLaunchKernelGGL(k_id_remaining_seq, dim3(1), dim3(1024), 0, L.stream, L.rem, L.ms, L.IDnew, L.tagR, L.st,
+                       L.clk);
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_reinit(const MpLaunch &L, double *phi, int maxits, int retry) {
+  dim3 gr(nblocks(L.N, 256, 8192)), bl(256);
+  hipLaunchKernelGGL(k_reinit_begin, dim3(1), dim3(1), 0, L.stream, L.ms, L.clk, retry);
+  if (retry) hipLaunchKernelGGL(k_reinit_restore, gr, bl, 0, L.stream, phi, L.phibk, L.ms);
+  else hipLaunchKernelGGL(k_reinit_tag, gr, bl, 0, L.stream, phi, L.tag, L.sign, L.phibk, L.ms);
+  for (int it = 0; it < maxits; it++) {
+    hipLaunchKernelGGL(k_reinit_A, gr, bl, 0, L.stream, phi, L.rphi1a, L.tag, L.sign, L.ms, it);
+    hipLaunchKernelGGL(k_reinit_B, gr, bl, 0, L.stream, phi, L.rphi1a, L.rphi1b, L.tag, L.sign, L.ms, it);
+    hipLaunchKernelGGL(k_reinit_C, gr, bl, 0, L.stream, phi, L.rphi1b, L.tag, L.sign, L.ms);
+  }
+  hipLaunchKernelGGL(k_reinit_end, dim3(1), dim3(1), 0, L.stream, L.ms, L.st);
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_state_update(const MpLaunch &L, double *V, int rounds) {
+  dim3 grN(nblocks(L.N, 256, 16384)), bl(256);
+  hipLaunchKernelGGL(k_changed_list, grN, bl, 0, L.stream, L.ID, L.IDnew, L.chg, L.cidx, L.ms, L.clk);
+  NbView nv;
+  nv.V = V; nv.IDn = L.ID; nv.cidx = L.cidx; nv.lvl = L.lvl; nv.cnt = L.cnt; nv.Vnew = L.Vnew; nv.ncap = 0;
+  dim3 grc(1024);
+  hipLaunchKernelGGL(k_state_round0, grc, bl, 0, L.stream, nv, L.chg, L.IDnew, L.rsV, L.rsMeta, L.ms, L.lvl, L.cnt,
+                     L.Vnew, L.clk);
+  for (int r = 1; r <= rounds; r++)
+    hipLaunchKernelGGL(k_state_round, grc, bl, 0, L.stream, nv, L.chg, L.IDnew, L.rsV, L.rsMeta, L.ms, L.lvl, L.cnt,
+                       L.Vnew, r, L.clk);
+  hipLaunchKernelGGL(k_state_seq, dim3(1), dim3(1024), 0, L.stream, nv, L.chg, L.IDnew, L.rsV, L.rsMeta, L.ms, L.lvl,
+                     L.cnt, L.Vnew, L.st, L.clk);
+  hipLaunchKernelGGL(k_state_commit, grc, bl, 0, L.stream, V, L.chg, L.cnt, L.Vnew, L.unres, L.ms, L.clk);
+  hipLaunchKernelGGL(k_fix_unresolved, dim3(1), dim3(1024), 0, L.stream, V, L.IDnew, L.cidx, L.cnt, L.unres, L.ms,
+                     L.st, L.clk);
+  hipLaunchKernelGGL(k_state_finish, grc, bl, 0, L.stream, V, L.ID, L.IDnew, L.chg, L.cidx, L.ms, L.st, L.clk);
+  return hchk(hipGetLastError());
+}

@@ -1,0 +1,82 @@
+// multiphase.hpp -- shared types of the interface-face, level-set and end-of-step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fv_kernels.hpp"
+#include "riemann_dev.hpp"
+
+namespace m2c {
+
+constexpr int MP_SORT_CAP = 8192;   // ordered single-thread passes handle at most this many nodes
+enum { LS_FIRST = 0, LS_AXBY = 1, LS_X = 2 };
+
+// run-time constants of the level set and the interface solver
+struct MpParams {
+  int ls_matid;
+  int ls_bc[6];
+  int reinit_freq, reinit_maxits;
+  double reinit_tol;
+  double sign_factor;          // smoothing width of the sign function: 1 * min(dx, dy, dz)
+  double min_dx;               // min(dx, dy, dz): pseudo-time step = cfl * min_dx
+  const double *xc[3];         // cell-centre coordinates, index -1..n stored at [i+1]
+  const double *fdl[3], *fdr[3], *fdr1y;   // FD3 coefficients per index along each axis
+  RiemannCfg rcfg;
+  int path_cap;                // rarefaction path entries per side and thread
+  double failsafe_density[8];
+};
+
+struct ReinitState {
+  int active, detected, done, iter, retry, pad;
+  double cfl;                  // persists between calls (LevelSetReinitializer::cfl)
+  unsigned long long maxres_bits, dphi_bits;
+};
+
+// device-resident per-step counters
+struct MpState {
+  unsigned long long epoch;    // stamp of the Riemann solutions of the current step
+  unsigned nface, nrem, nchg, nunres;
+  int nseq, pad;
+  ReinitState rs;
+  long long reinit_iters, unresolved_total;
+};
+
+// device buffers and launch parameters of one context
+struct MpLaunch {
+  hipStream_t stream;
+  long long N, Nf;
+  int flux, riemann_threads;
+  int8_t *ID, *IDnew;
+  unsigned *faces;
+  MpState *ms;
+  double *Fi, *rsV;
+  unsigned long long *rsMeta;
+  double *scratch;
+  Status *st;
+  Clock *clk;
+  unsigned *rem;
+  uint8_t *tagR, *tag;
+  double *sign, *phibk, *rphi1a, *rphi1b;
+  unsigned *chg;
+  int *cidx, *lvl;
+  int8_t *cnt;
+  double *Vnew;
+  unsigned *unres;
+};
+
+}  // namespace m2c
+
+extern "C" {
+int m2c_mp_upload(const m2c::Grid &grid, const m2c::DevEos *eos, const m2c::MpParams &mp);
+int m2c_mp_step_begin(const m2c::MpLaunch &L);
+int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,
+                                const double *Vp, const int *idp, double *Vs, int *id, double *Vsm, double *Vsp,
+                                int *status, double *scratch, int nthreads);
+int m2c_mp_launch_iface(const m2c::MpLaunch &L, const double *S, int in_cons, int store, int ignore_active);
+int m2c_mp_launch_ls(const m2c::MpLaunch &L, int in_cons, int mode, int final_, const double *S, const double *phiS,
+                     const double *X, const double *Y, double *out, double a, double b, double coef,
+                     const double *phin);
+int m2c_mp_launch_id_remaining(const m2c::MpLaunch &L, int single);
+int m2c_mp_launch_reinit(const m2c::MpLaunch &L, double *phi, int maxits, int retry);
+int m2c_mp_launch_state_update(const m2c::MpLaunch &L, double *V, int rounds);
+}

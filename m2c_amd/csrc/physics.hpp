@@ -36,7 +36,8 @@ struct DevEos {
 template <typename T> __device__ __forceinline__ T dsqrt(T x) { return __builtin_sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
 template <typename T> __device__ __forceinline__ T dexp(T x) { return exp(x); }
-template <typename T> __device__ __forceinline__ T dabs(T x) { return x < T(0) ? -x : x; }
+template <typename T> __device__ __forceinline__ T dabs(T x) { return __builtin_fabs(x); }   // C fabs
+template <> __device__ __forceinline__ float dabs<float>(float x) { return __builtin_fabsf(x); }
 template <typename T> __device__ __forceinline__ bool dfinite(T x) { return __builtin_isfinite(x); }
 // std::min / std::max semantics (first argument wins ties)
 template <typename T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
