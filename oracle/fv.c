@@ -492,3 +492,12 @@ long orc_c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V) {
   return bad ? -1 : nclip;
 }
 
+
+/* ------------------------------------------------------------------ pointwise EOS ------- */
+
+void orc_eos_p2c(const orc_ctx *c, int id, const double *V, double *U) { orc_p2c(&c->cfg.mat[id], V, U); }
+void orc_eos_c2p(const orc_ctx *c, int id, const double *U, double *V) { orc_c2p(&c->cfg.mat[id], U, V); }
+double orc_eos_c2(const orc_ctx *c, int id, double rho, double p) {
+  const orc_eos *m = &c->cfg.mat[id];
+  return orc_c2(m, rho, orc_internal_energy(m, rho, p));
+}

@@ -106,6 +106,18 @@ void orc_update_id(orc_ctx *c, const double *phi0, const double *phi, int *ID);
 int orc_riemann(const orc_ctx *c, const double *dir, const double *Vm, int idm, const double *Vp, int idp,
                 double *Vs, int *id, double *Vsm, double *Vsp);
 
+/* Pointwise EOS of material id: PrimitiveToConservative / ConservativeToPrimitive
+ * (VarFcnBase.h:169-197) and the squared sound speed at (rho, p) (VarFcnBase.h:215-219). */
+void orc_eos_p2c(const orc_ctx *c, int id, const double *V, double *U);
+void orc_eos_c2p(const orc_ctx *c, int id, const double *U, double *V);
+double orc_eos_c2(const orc_ctx *c, int id, double rho, double p);
+
+/* Deterministic log/exp used by the Riemann solver's step control (oracle/riemann.c). */
+double orc_det_log(double x);
+double orc_det_exp(double y);
+double orc_pow_quarter(double x);
+double orc_pow_fifth(double x);
+
 /* Diagnostics counters of the last call (reverted face states, clipped cells). */
 void orc_counters(const orc_ctx *c, long *reverted_faces, long *clipped_cells, long *riemann_errors);
 

@@ -145,6 +145,13 @@ def lib():
         L.orc_ls_reinitialize.argtypes = [P, D]
         L.orc_update_id.argtypes = [P, D, D, I]
         L.orc_riemann.argtypes = [P, D, D, C.c_int, D, C.c_int, D, I, D, D]
+        L.orc_eos_p2c.argtypes = [P, C.c_int, D, D]
+        L.orc_eos_c2p.argtypes = [P, C.c_int, D, D]
+        L.orc_eos_c2.argtypes = [P, C.c_int, C.c_double, C.c_double]
+        L.orc_eos_c2.restype = C.c_double
+        for f in ("orc_det_log", "orc_det_exp", "orc_pow_quarter", "orc_pow_fifth"):
+            getattr(L, f).argtypes = [C.c_double]
+            getattr(L, f).restype = C.c_double
         L.orc_counters.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
         _lib = L
     return _lib
@@ -243,6 +250,21 @@ class Oracle:
         st = self.L.orc_riemann(self.h, _dp(dd), _dp(Vm), idm, _dp(Vp), idp, _dp(Vs), C.byref(mid),
                                 _dp(Vsm), _dp(Vsp))
         return st, Vs, mid.value, Vsm, Vsp
+
+    def p2c(self, V, mid=0):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        U = np.zeros(5)
+        self.L.orc_eos_p2c(self.h, mid, _dp(V), _dp(U))
+        return U
+
+    def c2p(self, U, mid=0):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        V = np.zeros(5)
+        self.L.orc_eos_c2p(self.h, mid, _dp(U), _dp(V))
+        return V
+
+    def c2(self, rho, p, mid=0):
+        return self.L.orc_eos_c2(self.h, mid, rho, p)
 
     def counters(self):
         a, b, c = C.c_long(), C.c_long(), C.c_long()
