@@ -56,7 +56,24 @@ def build(verbose=False, jobs=None):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc link failed for libm2c_hip.so")
     os.replace(tmp, OUT)
+    build_host_demo(verbose)
     return OUT
+
+
+DEMO = os.path.join(HERE, "m2c_demo")
+
+
+def build_host_demo(verbose=False):
+    """The C++ host-layer driver (host/m2c_demo.cpp), linked against libm2c_hip.so."""
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "host", "m2c_demo.cpp"),
+           "-L" + HERE, "-lm2c_hip", "-Wl,-rpath,$ORIGIN", "-o", DEMO]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed on host/m2c_demo.cpp")
+    return DEMO
 
 
 if __name__ == "__main__":

@@ -1,0 +1,174 @@
+// m2c_host.hpp -- C++ host layer over the C ABI of include/m2c_hip.h, shaped like the reference's
+// operator interfaces so that code written against them keeps its calls:
+//
+//   SpaceOperator           SpaceOperator.h:103-140 (ComputeResidual, ComputeTimeStepSize,
+//                           ConservativeToPrimitive-free state transfer)
+//   TimeIntegrator          TimeIntegrator.h:66-73 (AdvanceOneTimeStep) and the time loop of
+//                           Main.cpp:514-621
+//   ExactRiemannSolver      ExactRiemannSolverBase.h:35-39 (ComputeRiemannSolution)
+//   LevelSetOperator        LevelSetOperator.h:86-90 (the level set Phi of one material)
+//
+// Arrays are the interior of SpaceVariable3D (k-j-i, dof innermost, no ghosts). Fatal conditions
+// that make the reference call exit() throw m2c_host::Error carrying the status code; a program
+// that does not catch it terminates, as the reference does.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/m2c_hip.h"
+
+namespace m2c_host {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(const std::string &what, int c) : std::runtime_error(what + " (status " + std::to_string(c) + ")"), code(c) {}
+};
+
+inline void check(int rc, const char *what) {
+  if (rc != M2C_OK) throw Error(what, rc);
+}
+
+// Solver options in IoData's vocabulary (defaults of IoData.cpp for the fields on this path).
+inline m2c_config default_config(int nx, int ny, int nz, double x0, double x1, double y0, double y1, double z0,
+                                 double z1) {
+  m2c_config c = {};
+  c.nx = nx; c.ny = ny; c.nz = nz;
+  c.x0 = x0; c.x1 = x1; c.y0 = y0; c.y1 = y1; c.z0 = z0; c.z1 = z1;
+  for (int s = 0; s < 6; s++) { c.bc[s] = M2C_BC_SYMMETRY; c.ls_bc[s] = 1; }
+  c.inlet[0] = c.outlet[0] = 1.0;
+  c.inlet[4] = c.outlet[4] = 1.0;
+  c.nmat = 0;
+  c.inactive_id = -1;
+  c.flux = M2C_FLUX_HLLC;
+  c.roe_delta = 0.2;               // IoData.cpp:1158
+  c.rec_linear = 1;
+  c.limiter = M2C_LIM_VANALBADA;
+  c.gminmod_alpha = 1.2;
+  c.ts = M2C_TS_RK2;               // TimeIntegration default: Runge-Kutta 2
+  c.cfl = 0.5;
+  c.mp_flux = 0;                   // MultiPhaseData: Numerical flux, Constant reconstruction
+  c.mp_recon = 0;
+  m2c_riemann_params &r = c.riemann;   // ExactRiemannSolverData, IoData.cpp:1345-1357
+  r.maxIts_main = 200; r.maxIts_bracket = 100; r.maxIts_shock = 200; r.numSteps_rarefaction = 200;
+  r.tol_main = 1e-4; r.tol_shock = 1e-12; r.tol_rarefaction = 1e-8;
+  r.min_pressure = -1e8; r.failure_threshold = 0.2; r.pressure_at_failure = 1e-8;
+  r.surrogate_residual_tol = 1e-6;
+  c.precision = M2C_PREC_FP64;
+  c.device = 0;
+  c.ls_matid = -1;
+  c.reinit_freq = -1;              // LevelSetReinitializationData, IoData.cpp:1205-1212
+  c.reinit_maxits = 30;
+  c.reinit_cfl = 0.8;
+  c.reinit_tol = 2.0e-4;
+  return c;
+}
+
+inline int add_stiffened_gas(m2c_config &c, double gamma, double pinf, double failsafe_density = 1e-10) {
+  m2c_material &m = c.mat[c.nmat];
+  m = {};
+  m.eos = M2C_EOS_SG;
+  m.rhomin = 0.0; m.pmin = -1.7976931348623157e308;
+  m.rhomax = m.pmax = 1.7976931348623157e308;
+  m.failsafe_density = failsafe_density;
+  m.p[0] = gamma; m.p[1] = pinf;
+  return c.nmat++;
+}
+
+// One device context: the SpaceOperator / TimeIntegrator / LevelSetOperator of a subdomain.
+class Context {
+ public:
+  explicit Context(const m2c_config &cfg) : cfg_(cfg) { check(m2c_ctx_create(&cfg_, &ctx_), "m2c_ctx_create"); }
+  ~Context() { m2c_ctx_destroy(ctx_); }
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  m2c_ctx *get() const { return ctx_; }
+  const m2c_config &config() const { return cfg_; }
+  long long cells() const { return (long long)cfg_.nx * cfg_.ny * cfg_.nz; }
+
+ private:
+  m2c_config cfg_;
+  m2c_ctx *ctx_ = nullptr;
+};
+
+class SpaceOperator {
+ public:
+  explicit SpaceOperator(Context &c) : c_(c) {}
+  // state V (5 primitive dofs per cell) and material IDs
+  void SetState(const double *V, const int32_t *ID) { check(m2c_upload_state(c_.get(), V, ID), "m2c_upload_state"); }
+  void GetState(double *V, int32_t *ID) const { check(m2c_download_state(c_.get(), V, ID), "m2c_download_state"); }
+  // SpaceOperator::ComputeResidual(V, ID, R): R = -(sum of face fluxes)/volume on the current state
+  void ComputeResidual(double *R) { check(m2c_compute_residual(c_.get(), R), "ComputeResidual"); }
+  // SpaceOperator::ComputeTimeStepSize(V, ID, dt, cfl)
+  void ComputeTimeStepSize(double &dt, double &cfl) {
+    check(m2c_time_step_size(c_.get(), &dt, &cfl), "ComputeTimeStepSize");
+  }
+
+ private:
+  Context &c_;
+};
+
+class LevelSetOperator {
+ public:
+  explicit LevelSetOperator(Context &c) : c_(c) {}
+  void SetPhi(const double *phi) { check(m2c_upload_levelset(c_.get(), phi), "m2c_upload_levelset"); }
+  void GetPhi(double *phi) const { check(m2c_download_levelset(c_.get(), phi), "m2c_download_levelset"); }
+
+ private:
+  Context &c_;
+};
+
+class TimeIntegrator {
+ public:
+  explicit TimeIntegrator(Context &c) : c_(c) {}
+  // TimeIntegratorBase::AdvanceOneTimeStep with the given dt (level set, IDs and end-of-step
+  // tasks included)
+  void AdvanceOneTimeStep(double dt) {
+    check(m2c_advance_dt(c_.get(), dt), "AdvanceOneTimeStep");
+    status_check();
+  }
+  // Main.cpp:514-621: CFL-adaptive steps until tmax or maxsteps (dt and t stay on the device)
+  m2c_status Run(double tmax, int maxsteps, int sync_every = 64) {
+    check(m2c_run(c_.get(), tmax, maxsteps, sync_every), "m2c_run");
+    return status_check();
+  }
+  m2c_status Status() const {
+    m2c_status s;
+    check(m2c_get_status(c_.get(), &s), "m2c_get_status");
+    return s;
+  }
+
+ private:
+  m2c_status status_check() const {
+    m2c_status s = Status();
+    if (s.error != M2C_OK) throw Error("non-hyperbolic state or solver failure", s.error);
+    return s;
+  }
+  Context &c_;
+};
+
+// ExactRiemannSolverBase::ComputeRiemannSolution for mesh-axis normals (the default
+// MultiPhase.RiemannNormal = Mesh). Returns 0 on success, 1 on a solver failure (the reference
+// counts it); throws on an invalid input state (the reference exits).
+class ExactRiemannSolver {
+ public:
+  explicit ExactRiemannSolver(Context &c) : c_(c) {}
+  int ComputeRiemannSolution(const double *dir, const double *Vm, int idm, const double *Vp, int idp, double *Vs,
+                             int &id, double *Vsm, double *Vsp) {
+    int axis = -1;
+    for (int d = 0; d < 3; d++)
+      if (dir[d] == 1.0 && dir[(d + 1) % 3] == 0.0 && dir[(d + 2) % 3] == 0.0) axis = d;
+    if (axis < 0) throw Error("ComputeRiemannSolution: only mesh-axis normals are provided", M2C_ERR_UNSUPPORTED);
+    int st = 0;
+    check(m2c_riemann_batch(c_.get(), 1, &axis, Vm, &idm, Vp, &idp, Vs, &id, Vsm, Vsp, &st), "m2c_riemann_batch");
+    if (st < 0) throw Error("ComputeRiemannSolution: invalid input state", M2C_ERR_STATE);
+    return st == 0 ? 0 : 1;
+  }
+
+ private:
+  Context &c_;
+};
+
+}  // namespace m2c_host

@@ -47,3 +47,11 @@ def test_front_end_fails_loudly_without_library(monkeypatch):
     monkeypatch.setattr(M, "LIB_PATH", os.path.join(ROOT, "m2c_amd", "no_such_library.so"))
     with pytest.raises(ImportError):
         M.lib()
+
+
+def test_cpp_host_layer_compiles():
+    import subprocess
+    inc = os.path.join(ROOT, "m2c_amd", "host")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", inc, "-x", "c++", "-"],
+                       input='#include "m2c_host.hpp"\nint main() { return 0; }\n', capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
