@@ -55,6 +55,37 @@ def cpu_baseline(settings, V0, ID0, nsteps, phi0=None):
                       f"{el:.1f} s wall, C oracle -O3 with {threads} OpenMP threads"}
 
 
+def mixed_form(M, W, settings, V0, ID0, phi0, args, V64, value64):
+    """The mixed-precision form on the same workload and step count: throughput and the relative
+    L1 error per field of its final state against the FP64 run's."""
+    cells = V0.shape[0] * V0.shape[1] * V0.shape[2]
+    s = M.Solver(W.config_from(settings, precision=M.PREC_MIXED))
+    s.upload(V0, ID0)
+    if phi0 is not None:
+        s.upload_levelset(phi0)
+    if args.warmup:
+        s.run(1e300, args.warmup, sync_every=0)
+    s.L.m2c_synchronize(s.h)
+    t0 = time.perf_counter()
+    st = s.run(1e300, args.steps, sync_every=0)
+    s.L.m2c_synchronize(s.h)
+    el = time.perf_counter() - t0
+    V32, _ = s.download()
+    ms_stage, _ = s.time_steps(max(5, min(args.steps, 20)))
+    s.close()
+    names = ["rho", "u", "v", "w", "p"]
+    l1 = {}
+    for f in range(5):
+        den = float(np.abs(V64[..., f]).sum())
+        l1[names[f]] = float(np.abs(V32[..., f] - V64[..., f]).sum() / den) if den > 0 else 0.0
+    v = cells * args.steps / el
+    print(f"bench: mixed form {args.steps} steps in {el:.3f} s, L1(rho) {l1['rho']:.2e}", file=sys.stderr, flush=True)
+    return {"value": v, "unit": "cell-updates/s", "speedup_vs_fp64": v / value64, "status": st.error,
+            "l1_rel_err_vs_fp64": l1, "l1_bar": 1e-5, "avg_stage_launch_ms": ms_stage,
+            "form": "fp32 reconstruction/limiters/flux; fp64 fetch+C2P, accumulation, RK combination, CFL, "
+                    "exact Riemann"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +95,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0: 8 for the periodic case, 10 for the shock-bubble)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="periodic_modes", choices=["periodic_modes", "shock_bubble"])
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     args = ap.parse_args()
     if args.cpu_steps <= 0:
         args.cpu_steps = 10 if args.workload == "shock_bubble" else 8
@@ -100,6 +132,7 @@ def main():
         sys.exit(3)
     value = cells * args.steps / el
     reinit_its = st.reinit_iterations
+    V64, ID64 = s.download()
     print(f"bench: {args.steps} steps in {el:.3f} s (reinit iterations {reinit_its})", file=sys.stderr, flush=True)
     # stage-kernel duration with HIP events on the library's stream
     nprof = max(5, min(args.steps, 50))
@@ -126,6 +159,8 @@ def main():
                        "hbm_frac_whole_step": value * bpcu / 1e9 / HBM_PEAK_GBS,
                        "measured_copy_GBs": copy_gbs, "copy_kernel_ms_per_GiB": copy_ms},
             "roofline": roof}
+    if not args.no_mixed:
+        line["mixed"] = mixed_form(M, W, settings, V0, ID0, phi0 if mp else None, args, V64, value)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(settings, V0, ID0, args.cpu_steps, phi0 if mp else None)
     print(json.dumps(line), flush=True)

@@ -39,6 +39,10 @@ enum m2c_bc { M2C_BC_INLET = 1, M2C_BC_OUTLET = 2, M2C_BC_SLIPWALL = 3, M2C_BC_S
 enum m2c_flux { M2C_FLUX_ROE = 0, M2C_FLUX_LLF = 1, M2C_FLUX_HLLC = 2, M2C_FLUX_GODUNOV = 3 };
 enum m2c_limiter { M2C_LIM_NONE = 0, M2C_LIM_GENMINMOD = 1, M2C_LIM_VANALBADA = 2 };
 enum m2c_ts { M2C_TS_FE = 0, M2C_TS_RK2 = 1, M2C_TS_RK3 = 2 };
+/* FP64: the drop-in path, bit-identical to the reference's arithmetic. MIXED: reconstruction,
+ * limiters, face-state checks and fluxes in FP32; fetch with ConservativeToPrimitive (p(e)
+ * inversion), residual accumulation, RK stage combination, clipping, CFL and the exact interface
+ * Riemann solves in FP64. */
 enum m2c_precision { M2C_PREC_FP64 = 0, M2C_PREC_MIXED = 1 };
 
 typedef struct {

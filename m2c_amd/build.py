@@ -23,10 +23,14 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
          "-Wno-unused-result"]
 
 # (tag, flux code, eos kind) -- flux codes: ROE 0, LLF 1, HLLC 2; eos kind -1 = run-time dispatch
-VARIANTS = [("hllc_sg", 2, 0), ("hllc_gen", 2, -1), ("roe_gen", 0, -1), ("llf_gen", 1, -1)]
+# (tag, flux code, eos kind, arithmetic type) -- flux codes: ROE 0, LLF 1, HLLC 2; eos kind -1 = run-time
+# dispatch; float = the mixed-precision form
+VARIANTS = [("hllc_sg", 2, 0, "double"), ("hllc_gen", 2, -1, "double"), ("roe_gen", 0, -1, "double"),
+            ("llf_gen", 1, -1, "double"), ("hllc_sg_f32", 2, 0, "float"), ("hllc_gen_f32", 2, -1, "float"),
+            ("roe_gen_f32", 0, -1, "float"), ("llf_gen_f32", 1, -1, "float")]
 UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", [])] + [
-    (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}"])
-    for t, f, e in VARIANTS]
+    (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"])
+    for t, f, e, tc in VARIANTS]
 
 
 def _compile(unit, verbose):

@@ -141,52 +141,52 @@ __device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const i
   }
 }
 
-template <int DIM> __device__ __forceinline__ double limited_slope(double dq0, double dq1) {
+template <int DIM, typename TC> __device__ __forceinline__ TC limited_slope(TC dq0, TC dq1) {
   const Grid &g = c_grid;
   switch (g.limiter) {
   case LIM_GENMINMOD: {
-    double A = g.A[DIM], B = g.B[DIM], al = g.alpha;
-    if (dq0 < 0.0 && dq1 < 0.0) return smax(al * dq0, smax(B / (A + 1.0) * (dq0 + dq1), al * dq1));
-    if (dq0 > 0.0 && dq1 > 0.0) return smin(al * dq0, smin(B / (A + 1.0) * (dq0 + dq1), al * dq1));
-    return 0.0;
+    TC A = TC(g.A[DIM]), B = TC(g.B[DIM]), al = TC(g.alpha);
+    if (dq0 < TC(0) && dq1 < TC(0)) return smax(al * dq0, smax(B / (A + TC(1)) * (dq0 + dq1), al * dq1));
+    if (dq0 > TC(0) && dq1 > TC(0)) return smin(al * dq0, smin(B / (A + TC(1)) * (dq0 + dq1), al * dq1));
+    return TC(0);
   }
   case LIM_VANALBADA: {
-    if (dq0 * dq1 <= 0) return 0.0;
-    if (dabs(dq0 * dq1) < 1e-16) return 0.0;
-    double d0k = dq0, d1k = dq1;
+    if (dq0 * dq1 <= TC(0)) return TC(0);
+    if (dabs(dq0 * dq1) < TC(1e-16)) return TC(0);
+    TC d0k = dq0, d1k = dq1;
     for (int t = 1; t < g.K[DIM]; t++) { d0k *= dq0; d1k *= dq1; }
-    return g.B[DIM] * (dq1 * d0k + dq0 * d1k) / (d0k + g.A[DIM] * d1k);
+    return TC(g.B[DIM]) * (dq1 * d0k + dq0 * d1k) / (d0k + TC(g.A[DIM]) * d1k);
   }
-  default: return 0.5 * (dq0 + dq1);
+  default: return TC(0.5) * (dq0 + dq1);
   }
 }
 
 // Slopes of one cell in direction DIM from its two neighbours.
-template <int DIM>
-__device__ __forceinline__ void cell_slopes(const double *vl, const double *vc, const double *vr, int idl, int idc,
-                                            int idr, double *sig) {
+template <int DIM, typename TC>
+__device__ __forceinline__ void cell_slopes(const TC *vl, const TC *vc, const TC *vr, int idl, int idc, int idr,
+                                            TC *sig) {
   const Grid &g = c_grid;
   if (!g.rec_linear || idc == g.inactive_id || (g.slope_zero && (idc != idl || idc != idr))) {
 #pragma unroll
-    for (int p = 0; p < 5; p++) sig[p] = 0.0;
+    for (int p = 0; p < 5; p++) sig[p] = TC(0);
     return;
   }
   bool zl = (idl == g.inactive_id), zr = (idr == g.inactive_id);
 #pragma unroll
   for (int p = 0; p < 5; p++) {
-    double dq0 = zl ? 0.0 : vc[p] - vl[p];
-    double dq1 = zr ? 0.0 : vr[p] - vc[p];
-    sig[p] = limited_slope<DIM>(dq0, dq1);
+    TC dq0 = zl ? TC(0) : vc[p] - vl[p];
+    TC dq1 = zr ? TC(0) : vr[p] - vc[p];
+    sig[p] = limited_slope<DIM, TC>(dq0, dq1);
   }
 }
 
 // Face state v +/- sigma/2 with the revert rule; returns 1 if reverted.
-template <int EK, bool PLUS>
-__device__ __forceinline__ int face_state(const double *vc, const double *sig, int id, double *out) {
+template <int EK, bool PLUS, typename TC>
+__device__ __forceinline__ int face_state(const TC *vc, const TC *sig, int id, TC *out) {
 #pragma unroll
-  for (int p = 0; p < 5; p++) out[p] = PLUS ? vc[p] + 0.5 * sig[p] : vc[p] - 0.5 * sig[p];
+  for (int p = 0; p < 5; p++) out[p] = PLUS ? vc[p] + TC(0.5) * sig[p] : vc[p] - TC(0.5) * sig[p];
   const DevEos &m = c_eos[id];
-  if (id != c_grid.inactive_id && (eos_bad<double, EK>(m, out) || eos_would_clip<double, EK>(m, out))) {
+  if (id != c_grid.inactive_id && (eos_bad<TC, EK>(m, out) || eos_would_clip<TC, EK>(m, out))) {
 #pragma unroll
     for (int p = 0; p < 5; p++) out[p] = vc[p];
     return 1;
@@ -195,8 +195,8 @@ __device__ __forceinline__ int face_state(const double *vc, const double *sig, i
 }
 
 // Face state of a ghost cell from the boundary rule (Reconstructor.cpp:820-869).
-template <int DIM>
-__device__ __forceinline__ void ghost_face(int bc, const double *img_face, const double *vghost, double *out) {
+template <int DIM, typename TC>
+__device__ __forceinline__ void ghost_face(int bc, const TC *img_face, const TC *vghost, TC *out) {
   if (bc == BC_SYMMETRY || bc == BC_SLIPWALL) {
 #pragma unroll
     for (int p = 0; p < 5; p++) out[p] = (p == DIM + 1) ? -img_face[p] : img_face[p];
@@ -209,30 +209,30 @@ __device__ __forceinline__ void ghost_face(int bc, const double *img_face, const
   }
 }
 
-template <int FLUX, int EK, int DIR>
-__device__ __forceinline__ void numerical_flux(const double *Vm, const double *Vp, int id, double *F) {
+template <int FLUX, int EK, int DIR, typename TC = double>
+__device__ __forceinline__ void numerical_flux(const TC *Vm, const TC *Vp, int id, TC *F) {
   const DevEos &m = c_eos[id];
-  if (FLUX == FLUX_HLLC) flux_hllc<double, EK, DIR>(m, Vm, Vp, F);
-  else if (FLUX == FLUX_ROE) flux_roe<double, EK, DIR>(m, Vm, Vp, c_grid.roe_delta, F);
-  else flux_llf2<double, EK, EK, DIR>(m, m, Vm, Vp, F);
+  if (FLUX == FLUX_HLLC) flux_hllc<TC, EK, DIR>(m, Vm, Vp, F);
+  else if (FLUX == FLUX_ROE) flux_roe<TC, EK, DIR>(m, Vm, Vp, TC(c_grid.roe_delta), F);
+  else flux_llf2<TC, EK, EK, DIR>(m, m, Vm, Vp, F);
 }
 
 // Flux of one face between cells with IDs idm (minus) and idp (plus). Interface faces solved by
 // the exact Riemann pass return 1 (their two fluxes are read from the interface-flux arrays).
-template <int FLUX, int EK, int DIR>
-__device__ __forceinline__ int face_flux(const double *Vm, const double *Vp, const double *cm, const double *cp,
-                                         int idm, int idp, double *F) {
+template <int FLUX, int EK, int DIR, typename TC>
+__device__ __forceinline__ int face_flux(const TC *Vm, const TC *Vp, const TC *cm, const TC *cp, int idm, int idp,
+                                         TC *F) {
   const Grid &g = c_grid;
   if (idm == idp && idp != g.inactive_id) {
-    numerical_flux<FLUX, EK, DIR>(Vm, Vp, idp, F);
+    numerical_flux<FLUX, EK, DIR, TC>(Vm, Vp, idp, F);
     return 0;
   }
 #pragma unroll
-  for (int p = 0; p < 5; p++) F[p] = 0.0;
+  for (int p = 0; p < 5; p++) F[p] = TC(0);
   if (EK < 0 && idm != idp && idm != g.inactive_id && idp != g.inactive_id) {
     if (g.mp_flux == 2) {   // LLF at the material interface (SpaceOperator.cpp:2493-2501)
-      const double *am = g.mp_recon ? Vm : cm, *ap = g.mp_recon ? Vp : cp;
-      flux_llf2<double, -1, -1, DIR>(c_eos[idm], c_eos[idp], am, ap, F);
+      const TC *am = g.mp_recon ? Vm : cm, *ap = g.mp_recon ? Vp : cp;
+      flux_llf2<TC, -1, -1, DIR>(c_eos[idm], c_eos[idp], am, ap, F);
       return 0;
     }
     return 1;
@@ -263,9 +263,9 @@ __device__ __forceinline__ void warp_min_atomic(unsigned long long *dst, double 
 
 // x (DIR 0) or y (DIR 1) face of the tile whose plus cell is at local (x, y); the flux goes to
 // sF[p*NF + fi].
-template <int FLUX, int EK, int DIR>
-__device__ __forceinline__ void tile_face(double (*sV)[LY][LX], int (*sID)[LX], const double *sS, double *sF,
-                                          int x, int y, int fi, int i0, int j0, unsigned long long &nrev) {
+template <int FLUX, int EK, int DIR, typename TC>
+__device__ __forceinline__ void tile_face(TC (*sV)[LY][LX], int (*sID)[LX], const TC *sS, TC *sF, int x, int y, int fi,
+                                          int i0, int j0, unsigned long long &nrev) {
   const Grid &g = c_grid;
   constexpr int SOFF = DIR == 0 ? 0 : NSX;
   constexpr int NF = DIR == 0 ? NFX : NFY;
@@ -273,12 +273,12 @@ __device__ __forceinline__ void tile_face(double (*sV)[LY][LX], int (*sID)[LX], 
   const int gpl = DIR == 0 ? i0 + x : j0 + y;
   const int gn = DIR == 0 ? g.nx : g.ny;
   const bool inside = (DIR == 0) ? (i0 + x <= g.nx && j0 + y < g.ny) : (i0 + x < g.nx && j0 + y <= g.ny);
-  double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  TC F[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
   if (inside) {
     int sm, sp;
     if (DIR == 0) { sm = y * (TX + 2) + (xm + 1); sp = y * (TX + 2) + (x + 1); }
     else { sm = (ym + 1) * TX + x; sp = (y + 1) * TX + x; }
-    double vcm[5], vcp[5], sgm[5], sgp[5], fm[5], fp[5];
+    TC vcm[5], vcp[5], sgm[5], sgp[5], fm[5], fp[5];
 #pragma unroll
     for (int p = 0; p < 5; p++) {
       vcm[p] = sV[p][ym + HALO][xm + HALO];
@@ -289,33 +289,43 @@ __device__ __forceinline__ void tile_face(double (*sV)[LY][LX], int (*sID)[LX], 
     const int idm = sID[ym + HALO][xm + HALO], idp = sID[y + HALO][x + HALO];
     const bool per = g.bc[2 * DIR] == BC_PERIODIC;
     if (gpl == 0 && !per) {
-      nrev += face_state<EK, false>(vcp, sgp, idp, fp);
-      ghost_face<DIR>(g.bc[2 * DIR], fp, vcm, fm);
+      nrev += face_state<EK, false, TC>(vcp, sgp, idp, fp);
+      ghost_face<DIR, TC>(g.bc[2 * DIR], fp, vcm, fm);
     } else if (gpl == gn && !per) {
-      nrev += face_state<EK, true>(vcm, sgm, idm, fm);
-      ghost_face<DIR>(g.bc[2 * DIR + 1], fm, vcp, fp);
+      nrev += face_state<EK, true, TC>(vcm, sgm, idm, fm);
+      ghost_face<DIR, TC>(g.bc[2 * DIR + 1], fm, vcp, fp);
     } else {
-      nrev += face_state<EK, true>(vcm, sgm, idm, fm);
-      nrev += face_state<EK, false>(vcp, sgp, idp, fp);
+      nrev += face_state<EK, true, TC>(vcm, sgm, idm, fm);
+      nrev += face_state<EK, false, TC>(vcp, sgp, idp, fp);
     }
-    face_flux<FLUX, EK, DIR>(fm, fp, vcm, vcp, idm, idp, F);
+    face_flux<FLUX, EK, DIR, TC>(fm, fp, vcm, vcp, idm, idp, F);
   }
 #pragma unroll
   for (int p = 0; p < 5; p++) sF[p * NF + fi] = F[p];
 }
 
+// convert a fetched cell state to the compute type of the stencil
+template <typename TC> __device__ __forceinline__ void to_tc(const double *v, TC *o) {
+#pragma unroll
+  for (int p = 0; p < 5; p++) o[p] = TC(v[p]);
+}
+
 // MODE 0: residual only (Out = R). MODE 1: intermediate stage (Out = U). MODE 2: final stage
 // (Out = V, plus the CFL reduction for the next step).
-template <int IN_CONS, int MODE, int FLUX, int EK>
+// TC is the arithmetic type of reconstruction, limiters, face-state checks and fluxes: double for
+// the drop-in path; float for the mixed form, where the fetch (with ConservativeToPrimitive and
+// its p(e) inversion), the residual accumulation, the stage combination, ConservativeToPrimitive
+// of the result, clipping and the CFL reduction stay in double.
+template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double>
 __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
   const Grid &g = c_grid;
   if (!a.ignore_active && !a.clk->active) return;
 
-  __shared__ double sV[5][LY][LX];
+  __shared__ TC sV[5][LY][LX];
   __shared__ int sID[LY][LX];
-  __shared__ double sS[5 * NSL];              // slopes
-  __shared__ double sFx[5 * NFX];             // x-face fluxes
-  __shared__ double sFy[5 * NFY];             // y-face fluxes
+  __shared__ TC sS[5 * NSL];                  // slopes
+  __shared__ TC sFx[5 * NFX];                 // x-face fluxes
+  __shared__ TC sFy[5 * NFY];                 // y-face fluxes
   __shared__ unsigned sFlags;
 
   const int tid = threadIdx.x;
@@ -336,30 +346,33 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
 
   const double dt = a.clk->dt;
   const double cdt = a.c * dt;
+  constexpr bool EXACT = sizeof(TC) == sizeof(double);
 
   // ---- z prologue: sigma_z(kbeg), flux F_{kbeg-1/2}
-  double w0[5], w1[5], w2[5], sz0[5], sz1[5], Flo[5];
+  TC w0[5], w1[5], w2[5], sz0[5], sz1[5], Flo[5];
+  double w0d[5];   // own cell in double (drop-in path: the stencil value itself)
   int id0, id1, id2;
   int lo_iface = 0;
   {
-    double vm2[5], vm1[5];
+    double d[5];
+    TC vm2[5], vm1[5];
     int idm2, idm1;
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 2, vm2, idm2, flags);
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 1, vm1, idm1, flags);
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg, w0, id0, flags);
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 1, w1, id1, flags);
-    double szm[5];
-    cell_slopes<2>(vm2, vm1, w0, idm2, idm1, id0, szm);
-    cell_slopes<2>(vm1, w0, w1, idm1, id0, id1, sz0);
-    double fm[5], fp[5];
-    nrev += face_state<EK, false>(w0, sz0, id0, fp);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 2, d, idm2, flags); to_tc(d, vm2);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 1, d, idm1, flags); to_tc(d, vm1);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg, w0d, id0, flags); to_tc(w0d, w0);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 1, d, id1, flags); to_tc(d, w1);
+    TC szm[5];
+    cell_slopes<2, TC>(vm2, vm1, w0, idm2, idm1, id0, szm);
+    cell_slopes<2, TC>(vm1, w0, w1, idm1, id0, id1, sz0);
+    TC fm[5], fp[5];
+    nrev += face_state<EK, false, TC>(w0, sz0, id0, fp);
     if (kbeg == 0 && !zper) {
-      ghost_face<2>(g.bc[4], fp, vm1, fm);
+      ghost_face<2, TC>(g.bc[4], fp, vm1, fm);
     } else {
-      nrev += face_state<EK, true>(vm1, szm, idm1, fm);
+      nrev += face_state<EK, true, TC>(vm1, szm, idm1, fm);
     }
-    lo_iface = face_flux<FLUX, EK, 2>(fm, fp, vm1, w0, idm1, id0, Flo);
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 2, w2, id2, flags);
+    lo_iface = face_flux<FLUX, EK, 2, TC>(fm, fp, vm1, w0, idm1, id0, Flo);
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 2, d, id2, flags); to_tc(d, w2);
   }
 
   // halo cell owned by this thread (x halo: 4 columns x TY rows, then y halo: 4 rows x TX columns)
@@ -375,18 +388,26 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
     hx = c;
   }
   const int hi = min(i0 + hx, g.nx + 1), hj = min(j0 + hy, g.ny + 1);
-  double hv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  TC hv[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
   int hid = 0;
-  if (tid < NHALO) fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, kbeg, hv, hid, flags);
+  if (tid < NHALO) {
+    double d[5];
+    fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, kbeg, d, hid, flags);
+    to_tc(d, hv);
+  }
 
   for (int k = kbeg; k < kend; k++) {
     // ---- prefetch V(k+3) of the column and the halo cell of plane k+1 (consumed at the end)
-    double w3[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double hvn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double w3d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    TC hvn[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
     int id3 = 0, hidn = 0;
     if (k + 1 < kend) {
-      fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 3, w3, id3, flags);
-      if (tid < NHALO) fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, k + 1, hvn, hidn, flags);
+      fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 3, w3d, id3, flags);
+      if (tid < NHALO) {
+        double d[5];
+        fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, k + 1, d, hidn, flags);
+        to_tc(d, hvn);
+      }
     }
 
     // ---- plane k into LDS: own cells and the halo cell from registers
@@ -399,7 +420,7 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
       sID[hy + HALO][hx + HALO] = hid;
     }
     // ---- z slope of k+1 (registers)
-    cell_slopes<2>(w0, w1, w2, id0, id1, id2, sz1);
+    cell_slopes<2, TC>(w0, w1, w2, id0, id1, id2, sz1);
     __syncthreads();
 
     // ---- slopes in x and y, once per cell and direction
@@ -408,29 +429,29 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
       int it = tid + r * NTHR;
       if (it < NSX) {
         int x = it % (TX + 2) - 1, y = it / (TX + 2);
-        double vl[5], vc[5], vr[5], sg[5];
+        TC vl[5], vc[5], vr[5], sg[5];
 #pragma unroll
         for (int p = 0; p < 5; p++) {
           vl[p] = sV[p][y + HALO][x - 1 + HALO];
           vc[p] = sV[p][y + HALO][x + HALO];
           vr[p] = sV[p][y + HALO][x + 1 + HALO];
         }
-        cell_slopes<0>(vl, vc, vr, sID[y + HALO][x - 1 + HALO], sID[y + HALO][x + HALO],
-                       sID[y + HALO][x + 1 + HALO], sg);
+        cell_slopes<0, TC>(vl, vc, vr, sID[y + HALO][x - 1 + HALO], sID[y + HALO][x + HALO],
+                           sID[y + HALO][x + 1 + HALO], sg);
 #pragma unroll
         for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
       } else if (it < NSL) {
         int t = it - NSX;
         int x = t % TX, y = t / TX - 1;
-        double vl[5], vc[5], vr[5], sg[5];
+        TC vl[5], vc[5], vr[5], sg[5];
 #pragma unroll
         for (int p = 0; p < 5; p++) {
           vl[p] = sV[p][y - 1 + HALO][x + HALO];
           vc[p] = sV[p][y + HALO][x + HALO];
           vr[p] = sV[p][y + 1 + HALO][x + HALO];
         }
-        cell_slopes<1>(vl, vc, vr, sID[y - 1 + HALO][x + HALO], sID[y + HALO][x + HALO],
-                       sID[y + 1 + HALO][x + HALO], sg);
+        cell_slopes<1, TC>(vl, vc, vr, sID[y - 1 + HALO][x + HALO], sID[y + HALO][x + HALO],
+                           sID[y + 1 + HALO][x + HALO], sg);
 #pragma unroll
         for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
       }
@@ -439,33 +460,33 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
 
     // ---- x and y face fluxes: the west/south face of every column, then the east column of
     //      x faces and the north row of y faces
-    tile_face<FLUX, EK, 0>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
-    tile_face<FLUX, EK, 1>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
+    tile_face<FLUX, EK, 0, TC>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
+    tile_face<FLUX, EK, 1, TC>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
     {
       // the TY east x faces go to wave 0, the TX north y faces are spread over waves 1..3 so
       // that every wave evaluates one extra flux
       const int wave = tid >> 6, lane = tid & 63;
       constexpr int YPER = (TX + (NTHR / 64 - 1) - 1) / (NTHR / 64 - 1);
       if (wave == 0) {
-        if (lane < TY) tile_face<FLUX, EK, 0>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
+        if (lane < TY) tile_face<FLUX, EK, 0, TC>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
       } else {
         const int x = (wave - 1) * YPER + lane;
-        if (lane < YPER && x < TX) tile_face<FLUX, EK, 1>(sV, sID, sS, sFy, x, TY, TY * TX + x, i0, j0, nrev);
+        if (lane < YPER && x < TX) tile_face<FLUX, EK, 1, TC>(sV, sID, sS, sFy, x, TY, TY * TX + x, i0, j0, nrev);
       }
     }
 
     // ---- z face k+1/2 (registers)
-    double Fhi[5];
+    TC Fhi[5];
     int hi_iface;
     {
-      double fm[5], fp[5];
-      nrev += face_state<EK, true>(w0, sz0, id0, fm);
+      TC fm[5], fp[5];
+      nrev += face_state<EK, true, TC>(w0, sz0, id0, fm);
       if (k + 1 == g.nz && !zper) {
-        ghost_face<2>(g.bc[5], fm, w1, fp);
+        ghost_face<2, TC>(g.bc[5], fm, w1, fp);
       } else {
-        nrev += face_state<EK, false>(w1, sz1, id1, fp);
+        nrev += face_state<EK, false, TC>(w1, sz1, id1, fp);
       }
-      hi_iface = face_flux<FLUX, EK, 2>(fm, fp, w0, w1, id0, id1, Fhi);
+      hi_iface = face_flux<FLUX, EK, 2, TC>(fm, fp, w0, w1, id0, id1, Fhi);
     }
     __syncthreads();
 
@@ -476,12 +497,12 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
       const int fs = ty * TX + tx, fn = fs + TX;
 #pragma unroll
       for (int p = 0; p < 5; p++) {
-        Fw[p] = sFx[p * NFX + fw];
-        Fe[p] = sFx[p * NFX + fe];
-        Fs[p] = sFy[p * NFY + fs];
-        Fn[p] = sFy[p * NFY + fn];
-        Fb[p] = Flo[p];
-        Ft[p] = Fhi[p];
+        Fw[p] = double(sFx[p * NFX + fw]);
+        Fe[p] = double(sFx[p * NFX + fe]);
+        Fs[p] = double(sFy[p * NFY + fs]);
+        Fn[p] = double(sFy[p * NFY + fn]);
+        Fb[p] = double(Flo[p]);
+        Ft[p] = double(Fhi[p]);
       }
       if (EK < 0 && g.has_interfaces && a.Fi) {
         // interface faces: the plus cell takes localflux2 of its low face, the minus cell
@@ -542,9 +563,12 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
         if (IN_CONS) {
 #pragma unroll
           for (int p = 0; p < 5; p++) vn[p] = a.Vn[p * g.N + idx];
+        } else if constexpr (EXACT) {
+#pragma unroll
+          for (int p = 0; p < 5; p++) vn[p] = double(w0[p]);
         } else {
 #pragma unroll
-          for (int p = 0; p < 5; p++) vn[p] = w0[p];
+          for (int p = 0; p < 5; p++) vn[p] = w0d[p];
         }
         eos_p2c<double, EK>(m, vn, un);
         if (a.first) {
@@ -590,7 +614,16 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
       sz0[p] = sz1[p];
       w0[p] = w1[p];
       w1[p] = w2[p];
-      w2[p] = w3[p];
+      w2[p] = TC(w3d[p]);
+    }
+    if constexpr (!EXACT) {
+      if (!IN_CONS && MODE != 0 && k + 1 < kend) {
+      // mixed form: the own cell of the next plane again in double (the stage combination needs
+      // V_n exactly)
+        int dummy;
+        unsigned fl = 0;
+        fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 1, w0d, dummy, fl);
+      }
     }
     lo_iface = hi_iface;
     id0 = id1; id1 = id2; id2 = id3;

@@ -38,7 +38,7 @@ struct m2c_ctx {
   Clock *clk = nullptr;
   Status *st = nullptr;
   int kchunk = 1;
-  int variant = 1;             // 0 HLLC+SG, 1 HLLC generic EOS, 2 Roe, 3 LLF
+  int variant = 1;             // 0 HLLC+SG, 1 HLLC generic EOS, 2 Roe, 3 LLF; +4: mixed precision
   dim3 grid3;
   // two materials / level set
   bool iface = false;          // exact Riemann pass at material interfaces
@@ -231,7 +231,9 @@ static int upload_constants(m2c_ctx *c) {
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_eos), c->eos, sizeof(c->eos)));
   if ((c->iface || c->ls) && m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
   if (m2c_stage_upload_hllc_sg(c->grid, c->eos) || m2c_stage_upload_hllc_gen(c->grid, c->eos) ||
-      m2c_stage_upload_roe_gen(c->grid, c->eos) || m2c_stage_upload_llf_gen(c->grid, c->eos))
+      m2c_stage_upload_roe_gen(c->grid, c->eos) || m2c_stage_upload_llf_gen(c->grid, c->eos) ||
+      m2c_stage_upload_hllc_sg_f32(c->grid, c->eos) || m2c_stage_upload_hllc_gen_f32(c->grid, c->eos) ||
+      m2c_stage_upload_roe_gen_f32(c->grid, c->eos) || m2c_stage_upload_llf_gen_f32(c->grid, c->eos))
     return M2C_ERR_HIP;
   g_resident = c;
   return M2C_OK;
@@ -259,7 +261,11 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
   case 0: rc = m2c_stage_launch_hllc_sg(in_cons, mode, c->grid3, c->stream, s); break;
   case 1: rc = m2c_stage_launch_hllc_gen(in_cons, mode, c->grid3, c->stream, s); break;
   case 2: rc = m2c_stage_launch_roe_gen(in_cons, mode, c->grid3, c->stream, s); break;
-  default: rc = m2c_stage_launch_llf_gen(in_cons, mode, c->grid3, c->stream, s); break;
+  case 3: rc = m2c_stage_launch_llf_gen(in_cons, mode, c->grid3, c->stream, s); break;
+  case 4: rc = m2c_stage_launch_hllc_sg_f32(in_cons, mode, c->grid3, c->stream, s); break;
+  case 5: rc = m2c_stage_launch_hllc_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
+  case 6: rc = m2c_stage_launch_roe_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
+  default: rc = m2c_stage_launch_llf_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
   }
   if (rc == 1) return M2C_ERR_ARG;
   if (rc) return M2C_ERR_HIP;
@@ -528,7 +534,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   if (!cfg || !out) return M2C_ERR_ARG;
   if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || cfg->nmat < 1 || cfg->nmat > M2C_MAX_MAT) return M2C_ERR_ARG;
   if (cfg->flux == M2C_FLUX_GODUNOV) return M2C_ERR_UNSUPPORTED;
-  if (cfg->precision != M2C_PREC_FP64) return M2C_ERR_UNSUPPORTED;
+  if (cfg->precision != M2C_PREC_FP64 && cfg->precision != M2C_PREC_MIXED) return M2C_ERR_ARG;
   for (int s = 0; s < 6; s++) {
     int b = cfg->bc[s];
     if (b < M2C_BC_INLET || b > M2C_BC_PERIODIC) return M2C_ERR_ARG;
@@ -573,6 +579,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   for (int m = 0; m < cfg->nmat; m++) all_sg = all_sg && cfg->mat[m].eos == M2C_EOS_SG;
   if (cfg->flux == M2C_FLUX_HLLC) c->variant = (all_sg && cfg->inactive_id < 0 && !g.has_interfaces) ? 0 : 1;
   else c->variant = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
+  if (cfg->precision == M2C_PREC_MIXED) c->variant += 4;
   c->iface = g.has_interfaces && cfg->mp_flux != 2;
   c->ls = cfg->ls_matid >= 0;
   int bad = 0;

@@ -111,6 +111,15 @@ template <typename T, int EK> __device__ __forceinline__ T eos_c2(const DevEos &
   return eos_dpdrho<T, EK>(m, rho, e) + eos_p<T, EK>(m, rho, e) / rho * eos_G<T, EK>(m, rho);
 }
 
+// c^2 of a state whose pressure is known: in double the reference's form above (p recomputed
+// from e); in float the state's own p, which avoids the p_inf cancellation of p(e) in single
+// precision (stiffened gas: (gamma-1) rho e - gamma p_inf)
+template <typename T, int EK> __device__ __forceinline__ T eos_c2_st(const DevEos &m, T rho, T e, T p) {
+  if (sizeof(T) == sizeof(double)) return eos_c2<T, EK>(m, rho, e);
+  if (kind_of<EK>(m) == EOS_DUMMY) return T(2.2250738585072014e-308);
+  return eos_dpdrho<T, EK>(m, rho, e) + p / rho * eos_G<T, EK>(m, rho);
+}
+
 // H = e + |u|^2/2 + p/rho (VarFcnBase.h:250-254)
 template <typename T, int EK> __device__ __forceinline__ T eos_H(const DevEos &m, const T *V) {
   if (kind_of<EK>(m) == EOS_DUMMY) return T(m.d_e0);
@@ -267,8 +276,8 @@ __device__ __forceinline__ void flux_hllc(const DevEos &m, const T *Vm, const T 
   T G_hat = (g1 + g2) / denom;
   T c_hat2 = dpdrho_hat + G_hat * p_over_rho_hat;
   T c_hat = (c_hat2 <= T(0)) ? eps : dsqrt<T>(c_hat2);
-  T cm = dsqrt<T>(eos_c2<T, EK>(m, Vm[0], em));
-  T cp = dsqrt<T>(eos_c2<T, EK>(m, Vp[0], ep));
+  T cm = dsqrt<T>(eos_c2_st<T, EK>(m, Vm[0], em, Vm[4]));
+  T cp = dsqrt<T>(eos_c2_st<T, EK>(m, Vp[0], ep, Vp[4]));
   T u_hat = wavg<T>(sqrt_rhom, vm, sqrt_rhop, vp);
   T Sm = smin<T>(vm - cm, u_hat - c_hat);
   T Sp = smax<T>(vp + cp, u_hat + c_hat);
@@ -298,8 +307,8 @@ template <typename T, int EKM, int EKP, int DIR>
 __device__ __forceinline__ void flux_llf2(const DevEos &mm, const DevEos &mp, const T *Vm, const T *Vp, T *F) {
   T em = eos_e<T, EKM>(mm, Vm[0], Vm[4]);
   T ep = eos_e<T, EKP>(mp, Vp[0], Vp[4]);
-  T cm = dsqrt<T>(eos_c2<T, EKM>(mm, Vm[0], em));
-  T cp = dsqrt<T>(eos_c2<T, EKP>(mp, Vp[0], ep));
+  T cm = dsqrt<T>(eos_c2_st<T, EKM>(mm, Vm[0], em, Vm[4]));
+  T cp = dsqrt<T>(eos_c2_st<T, EKP>(mp, Vp[0], ep, Vp[4]));
   T a = smax<T>(dabs<T>(Vm[DIR + 1]) + cm, dabs<T>(Vp[DIR + 1]) + cp);
   T fm[5], fp[5], Um[5], Up[5];
   phys_flux<T, EKM, DIR>(mm, Vm, fm);

@@ -1,11 +1,16 @@
 // stage_inst.hip -- one (flux, equation-of-state) variant of the fused stage kernel.
-// Compiled with -DM2C_FLUX=<flux code> -DM2C_EK=<eos kind or -1> -DM2C_TAG=<name>.
+// Compiled with -DM2C_FLUX=<flux code> -DM2C_EK=<eos kind or -1> -DM2C_TAG=<name> and
+// -DM2C_TC=<double|float> (arithmetic type of reconstruction and fluxes).
 #include "stage_launch.hpp"
 
 #define M2C_CAT2(a, b) a##b
 #define M2C_CAT(a, b) M2C_CAT2(a, b)
 
 using namespace m2c;
+
+#ifndef M2C_TC
+#define M2C_TC double
+#endif
 
 int M2C_CAT(m2c_stage_upload_, M2C_TAG)(const Grid &g, const DevEos *eos) {
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_grid), &g, sizeof(Grid)) != hipSuccess) return 2;
@@ -15,11 +20,11 @@ int M2C_CAT(m2c_stage_upload_, M2C_TAG)(const Grid &g, const DevEos *eos) {
 
 int M2C_CAT(m2c_stage_launch_, M2C_TAG)(int in_cons, int mode, dim3 grid, hipStream_t s, const StageArgs &a) {
   dim3 blk(NTHR);
-  if (in_cons == 0 && mode == 0) hipLaunchKernelGGL((stage_kernel<0, 0, M2C_FLUX, M2C_EK>), grid, blk, 0, s, a);
-  else if (in_cons == 0 && mode == 1) hipLaunchKernelGGL((stage_kernel<0, 1, M2C_FLUX, M2C_EK>), grid, blk, 0, s, a);
-  else if (in_cons == 0 && mode == 2) hipLaunchKernelGGL((stage_kernel<0, 2, M2C_FLUX, M2C_EK>), grid, blk, 0, s, a);
-  else if (in_cons == 1 && mode == 1) hipLaunchKernelGGL((stage_kernel<1, 1, M2C_FLUX, M2C_EK>), grid, blk, 0, s, a);
-  else if (in_cons == 1 && mode == 2) hipLaunchKernelGGL((stage_kernel<1, 2, M2C_FLUX, M2C_EK>), grid, blk, 0, s, a);
+  if (in_cons == 0 && mode == 0) hipLaunchKernelGGL((stage_kernel<0, 0, M2C_FLUX, M2C_EK, M2C_TC>), grid, blk, 0, s, a);
+  else if (in_cons == 0 && mode == 1) hipLaunchKernelGGL((stage_kernel<0, 1, M2C_FLUX, M2C_EK, M2C_TC>), grid, blk, 0, s, a);
+  else if (in_cons == 0 && mode == 2) hipLaunchKernelGGL((stage_kernel<0, 2, M2C_FLUX, M2C_EK, M2C_TC>), grid, blk, 0, s, a);
+  else if (in_cons == 1 && mode == 1) hipLaunchKernelGGL((stage_kernel<1, 1, M2C_FLUX, M2C_EK, M2C_TC>), grid, blk, 0, s, a);
+  else if (in_cons == 1 && mode == 2) hipLaunchKernelGGL((stage_kernel<1, 2, M2C_FLUX, M2C_EK, M2C_TC>), grid, blk, 0, s, a);
   else return 1;
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }

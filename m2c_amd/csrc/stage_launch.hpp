@@ -15,3 +15,7 @@ M2C_DECLARE_VARIANT(hllc_sg)
 M2C_DECLARE_VARIANT(hllc_gen)
 M2C_DECLARE_VARIANT(roe_gen)
 M2C_DECLARE_VARIANT(llf_gen)
+M2C_DECLARE_VARIANT(hllc_sg_f32)
+M2C_DECLARE_VARIANT(hllc_gen_f32)
+M2C_DECLARE_VARIANT(roe_gen_f32)
+M2C_DECLARE_VARIANT(llf_gen_f32)
