@@ -146,6 +146,31 @@ int m2c_flux_batch(m2c_ctx *ctx, int dir, int n, const double *Vm, const double 
 int m2c_riemann_batch(m2c_ctx *ctx, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
                       const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status);
 
+/* ---- many independent instances (batched) ------------------------------------------------
+ * A batch holds n single-material instances (each its own grid, boundaries, state and
+ * CFL-adaptive time step; one material table, flux, precision and RK2/RK3 scheme for all). Every
+ * RK stage of every instance runs in one persistent launch draining per-XCD tile queues; a time
+ * step is captured once in a hipGraph and replayed. An instance evolves bit for bit as it would
+ * in its own context (no counterpart in the reference, which runs one problem per process). */
+typedef struct m2c_batch m2c_batch;
+typedef struct {
+  int kchunk;                       /* z planes per work item */
+  int blocks;                       /* resident persistent workgroups */
+  double ms_per_step;               /* best time per batch step among the candidates */
+  int candidates;
+} m2c_batch_tuning;
+int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out);
+int m2c_batch_destroy(m2c_batch *b);
+int m2c_batch_upload(m2c_batch *b, int i, const double *V, const int32_t *id);
+int m2c_batch_download(m2c_batch *b, int i, double *V, int32_t *id);
+/* nsteps steps of every instance; use_graph: replay the captured step graph; ms: elapsed (HIP events) */
+int m2c_batch_run(m2c_batch *b, int nsteps, int use_graph, double *ms);
+int m2c_batch_status(m2c_batch *b, int i, m2c_status *st);
+/* times (z-chunk depth, resident workgroups) candidates for trial_steps steps each on a copy of
+ * the state, keeps the fastest; the state, clocks and counters are restored */
+int m2c_batch_autotune(m2c_batch *b, int trial_steps, m2c_batch_tuning *out);
+int m2c_batch_set_tuning(m2c_batch *b, int kchunk, int blocks);
+
 /* Timing helper: runs `nsteps` CFL-adaptive time steps (no tmax) and reports the average
  * duration (ms) of one fused stage-kernel launch and of one whole step, measured with HIP events
  * recorded on the context's stream around every stage launch and around the whole loop. */

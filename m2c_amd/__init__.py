@@ -49,6 +49,10 @@ class Config(C.Structure):
                 ("reinit_maxits", C.c_int), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("kchunk", C.c_int), ("blocks", C.c_int), ("ms_per_step", C.c_double), ("candidates", C.c_int)]
+
+
 class Status(C.Structure):
     _fields_ = [("error", C.c_int), ("reverted_faces", C.c_longlong), ("clipped_cells", C.c_longlong),
                 ("riemann_errors", C.c_longlong), ("steps", C.c_longlong), ("t", C.c_double),
@@ -141,6 +145,15 @@ def lib():
         L.m2c_riemann_batch.argtypes = [P, C.c_int, I, D, I, D, I, D, I, D, D, I]
         L.m2c_upload_levelset.argtypes = [P, D]
         L.m2c_download_levelset.argtypes = [P, D]
+        Pb = C.c_void_p
+        L.m2c_batch_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(Pb)]
+        L.m2c_batch_destroy.argtypes = [Pb]
+        L.m2c_batch_upload.argtypes = [Pb, C.c_int, D, C.POINTER(C.c_int32)]
+        L.m2c_batch_download.argtypes = [Pb, C.c_int, D, C.POINTER(C.c_int32)]
+        L.m2c_batch_run.argtypes = [Pb, C.c_int, C.c_int, D]
+        L.m2c_batch_status.argtypes = [Pb, C.c_int, C.POINTER(Status)]
+        L.m2c_batch_autotune.argtypes = [Pb, C.c_int, C.POINTER(BatchTuning)]
+        L.m2c_batch_set_tuning.argtypes = [Pb, C.c_int, C.c_int]
         L.m2c_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -268,3 +281,60 @@ def stream_copy_GBs(solver, nbytes=1 << 30, reps=20):
 
 def version():
     return lib().m2c_version().decode()
+
+
+class Batch:
+    """Many independent single-material instances advanced together (one persistent launch per
+    RK stage, a hipGraph per step)."""
+
+    def __init__(self, cfgs):
+        self.L = lib()
+        arr = (Config * len(cfgs))(*cfgs)
+        self.h = C.c_void_p()
+        _chk(self.L.m2c_batch_create(arr, len(cfgs), C.byref(self.h)), "m2c_batch_create")
+        self.shapes = [(c.nz, c.ny, c.nx) for c in cfgs]
+
+    def close(self):
+        if self.h:
+            self.L.m2c_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, i, V, ID=None):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        assert V.shape == self.shapes[i] + (5,)
+        idp = None
+        if ID is not None:
+            ID = np.ascontiguousarray(ID, dtype=np.int32)
+            idp = ID.ctypes.data_as(C.POINTER(C.c_int32))
+        _chk(self.L.m2c_batch_upload(self.h, i, _dp(V), idp), "m2c_batch_upload")
+
+    def download(self, i):
+        V = np.empty(self.shapes[i] + (5,), dtype=np.float64)
+        ID = np.empty(self.shapes[i], dtype=np.int32)
+        _chk(self.L.m2c_batch_download(self.h, i, _dp(V), ID.ctypes.data_as(C.POINTER(C.c_int32))),
+             "m2c_batch_download")
+        return V, ID
+
+    def run(self, nsteps, graph=True):
+        ms = C.c_double()
+        _chk(self.L.m2c_batch_run(self.h, nsteps, int(graph), C.byref(ms)), "m2c_batch_run")
+        return ms.value
+
+    def status(self, i):
+        s = Status()
+        _chk(self.L.m2c_batch_status(self.h, i, C.byref(s)), "m2c_batch_status")
+        return s
+
+    def autotune(self, trial_steps=3):
+        t = BatchTuning()
+        _chk(self.L.m2c_batch_autotune(self.h, trial_steps, C.byref(t)), "m2c_batch_autotune")
+        return t
+
+    def set_tuning(self, kchunk, blocks):
+        _chk(self.L.m2c_batch_set_tuning(self.h, kchunk, blocks), "m2c_batch_set_tuning")

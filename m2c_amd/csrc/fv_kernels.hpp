@@ -86,9 +86,16 @@ struct StageArgs {
 };
 
 // Each translation unit that launches kernels holds its own copy of these tables (internal
-// linkage) and exports an upload function for it.
-static __constant__ Grid c_grid;
+// linkage) and exports an upload function for it. The batched translation unit (M2C_BATCHED)
+// reads the grid of the instance a block is working on from LDS instead.
 static __constant__ DevEos c_eos[8];
+#ifdef M2C_BATCHED
+static __shared__ Grid s_bgrid;
+#define M2C_G s_bgrid
+#else
+static __constant__ Grid c_grid;
+#define M2C_G c_grid
+#endif
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -103,7 +110,7 @@ __device__ __forceinline__ int image_idx(int i, int n, bool per) {
 template <int IN_CONS, int EK>
 __device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const int8_t *__restrict__ ID, int i,
                                            int j, int k, double *V, int &id, unsigned &flags) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   int oi = (i < 0) ? 0 : (i >= g.nx ? 1 : -1);
   int oj = (j < 0) ? 2 : (j >= g.ny ? 3 : -1);
   int ok = (k < 0) ? 4 : (k >= g.nz ? 5 : -1);
@@ -142,7 +149,7 @@ __device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const i
 }
 
 template <int DIM, typename TC> __device__ __forceinline__ TC limited_slope(TC dq0, TC dq1) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   switch (g.limiter) {
   case LIM_GENMINMOD: {
     TC A = TC(g.A[DIM]), B = TC(g.B[DIM]), al = TC(g.alpha);
@@ -165,7 +172,7 @@ template <int DIM, typename TC> __device__ __forceinline__ TC limited_slope(TC d
 template <int DIM, typename TC>
 __device__ __forceinline__ void cell_slopes(const TC *vl, const TC *vc, const TC *vr, int idl, int idc, int idr,
                                             TC *sig) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   if (!g.rec_linear || idc == g.inactive_id || (g.slope_zero && (idc != idl || idc != idr))) {
 #pragma unroll
     for (int p = 0; p < 5; p++) sig[p] = TC(0);
@@ -186,7 +193,7 @@ __device__ __forceinline__ int face_state(const TC *vc, const TC *sig, int id, T
 #pragma unroll
   for (int p = 0; p < 5; p++) out[p] = PLUS ? vc[p] + TC(0.5) * sig[p] : vc[p] - TC(0.5) * sig[p];
   const DevEos &m = c_eos[id];
-  if (id != c_grid.inactive_id && (eos_bad<TC, EK>(m, out) || eos_would_clip<TC, EK>(m, out))) {
+  if (id != M2C_G.inactive_id && (eos_bad<TC, EK>(m, out) || eos_would_clip<TC, EK>(m, out))) {
 #pragma unroll
     for (int p = 0; p < 5; p++) out[p] = vc[p];
     return 1;
@@ -213,7 +220,7 @@ template <int FLUX, int EK, int DIR, typename TC = double>
 __device__ __forceinline__ void numerical_flux(const TC *Vm, const TC *Vp, int id, TC *F) {
   const DevEos &m = c_eos[id];
   if (FLUX == FLUX_HLLC) flux_hllc<TC, EK, DIR>(m, Vm, Vp, F);
-  else if (FLUX == FLUX_ROE) flux_roe<TC, EK, DIR>(m, Vm, Vp, TC(c_grid.roe_delta), F);
+  else if (FLUX == FLUX_ROE) flux_roe<TC, EK, DIR>(m, Vm, Vp, TC(M2C_G.roe_delta), F);
   else flux_llf2<TC, EK, EK, DIR>(m, m, Vm, Vp, F);
 }
 
@@ -222,7 +229,7 @@ __device__ __forceinline__ void numerical_flux(const TC *Vm, const TC *Vp, int i
 template <int FLUX, int EK, int DIR, typename TC>
 __device__ __forceinline__ int face_flux(const TC *Vm, const TC *Vp, const TC *cm, const TC *cp, int idm, int idp,
                                          TC *F) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   if (idm == idp && idp != g.inactive_id) {
     numerical_flux<FLUX, EK, DIR, TC>(Vm, Vp, idp, F);
     return 0;
@@ -241,13 +248,13 @@ __device__ __forceinline__ int face_flux(const TC *Vm, const TC *Vp, const TC *c
 }
 
 __device__ __forceinline__ bool is_interface(int idm, int idp) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   return idm != idp && idm != g.inactive_id && idp != g.inactive_id && g.mp_flux != 2;
 }
 
 // face (i-1/2, j, k) of direction 0 (and likewise for y, z) is keyed by its plus cell (i,j,k)
 __device__ __forceinline__ long long face_key(int i, int j, int k) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   return ((long long)k * (g.ny + 1) + j) * (g.nx + 1) + i;
 }
 
@@ -266,7 +273,7 @@ __device__ __forceinline__ void warp_min_atomic(unsigned long long *dst, double 
 template <int FLUX, int EK, int DIR, typename TC>
 __device__ __forceinline__ void tile_face(TC (*sV)[LY][LX], int (*sID)[LX], const TC *sS, TC *sF, int x, int y, int fi,
                                           int i0, int j0, unsigned long long &nrev) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   constexpr int SOFF = DIR == 0 ? 0 : NSX;
   constexpr int NF = DIR == 0 ? NFX : NFY;
   const int xm = x - (DIR == 0), ym = y - (DIR == 1);
@@ -317,8 +324,8 @@ template <typename TC> __device__ __forceinline__ void to_tc(const double *v, TC
 // its p(e) inversion), the residual accumulation, the stage combination, ConservativeToPrimitive
 // of the result, clipping and the CFL reduction stay in double.
 template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double>
-__global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
-  const Grid &g = c_grid;
+__device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, const int by, const int bz) {
+  const Grid &g = M2C_G;
   if (!a.ignore_active && !a.clk->active) return;
 
   __shared__ TC sV[5][LY][LX];
@@ -330,8 +337,8 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
-  const int kbeg = blockIdx.z * a.kchunk;
+  const int i0 = bx * TX, j0 = by * TY;
+  const int kbeg = bz * a.kchunk;
   const int kend = min(kbeg + a.kchunk, g.nz);
   const int gi = i0 + tx, gj = j0 + ty;
   const bool own = gi < g.nx && gj < g.ny;
@@ -638,5 +645,12 @@ __global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
   __syncthreads();
   if (tid == 0 && sFlags) atomicOr(&a.st->bits, sFlags);
 }
+
+#ifndef M2C_BATCHED
+template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double>
+__global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
+  stage_body<IN_CONS, MODE, FLUX, EK, TC>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+#endif
 
 }  // namespace m2c

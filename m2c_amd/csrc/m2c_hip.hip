@@ -530,8 +530,9 @@ extern "C" {
 
 const char *m2c_version(void) { return "m2c_hip 0.1 (gfx950)"; }
 
-int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
-  if (!cfg || !out) return M2C_ERR_ARG;
+// Grid constants, material tables and kernel variant of a configuration (shared with the batched
+// translation unit). Returns an m2c_status code.
+int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || cfg->nmat < 1 || cfg->nmat > M2C_MAX_MAT) return M2C_ERR_ARG;
   if (cfg->flux == M2C_FLUX_GODUNOV) return M2C_ERR_UNSUPPORTED;
   if (cfg->precision != M2C_PREC_FP64 && cfg->precision != M2C_PREC_MIXED) return M2C_ERR_ARG;
@@ -541,14 +542,10 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
     if (b == M2C_BC_OVERSET) return M2C_ERR_UNSUPPORTED;
     if ((b == M2C_BC_PERIODIC) != (cfg->bc[s ^ 1] == M2C_BC_PERIODIC)) return M2C_ERR_ARG;
   }
-  HIPCHK(hipSetDevice(cfg->device));
-  m2c_ctx *c = new m2c_ctx();
-  c->cfg = *cfg;
-  Grid &g = c->grid;
+  Grid &g = *gp;
   memset(&g, 0, sizeof(g));
   g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
   g.N = (long long)cfg->nx * cfg->ny * cfg->nz;
-  c->N = g.N;
   for (int s = 0; s < 6; s++) g.bc[s] = cfg->bc[s];
   for (int p = 0; p < 5; p++) { g.inlet[p] = cfg->inlet[p]; g.outlet[p] = cfg->outlet[p]; }
   // uniform mesh, MeshGenerator.cpp:146-158; volume and areas as SpaceOperator.cpp:166-177
@@ -571,15 +568,34 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   g.mp_flux = cfg->mp_flux; g.mp_recon = cfg->mp_recon;
   int nactive = 0;
   for (int m = 0; m < cfg->nmat; m++) {
-    build_eos(cfg->mat[m], c->eos[m]);
+    build_eos(cfg->mat[m], eos[m]);
     if (m != cfg->inactive_id) nactive++;
   }
   g.has_interfaces = nactive > 1;
   bool all_sg = true;
   for (int m = 0; m < cfg->nmat; m++) all_sg = all_sg && cfg->mat[m].eos == M2C_EOS_SG;
-  if (cfg->flux == M2C_FLUX_HLLC) c->variant = (all_sg && cfg->inactive_id < 0 && !g.has_interfaces) ? 0 : 1;
-  else c->variant = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
-  if (cfg->precision == M2C_PREC_MIXED) c->variant += 4;
+  int v;
+  if (cfg->flux == M2C_FLUX_HLLC) v = (all_sg && cfg->inactive_id < 0 && !g.has_interfaces) ? 0 : 1;
+  else v = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
+  if (cfg->precision == M2C_PREC_MIXED) v += 4;
+  *variant = v;
+  return M2C_OK;
+}
+
+int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
+  if (!cfg || !out) return M2C_ERR_ARG;
+  Grid g0;
+  DevEos e0[8];
+  int v0 = 0;
+  if (int rc = m2c_fill_grid(cfg, &g0, e0, &v0)) return rc;
+  HIPCHK(hipSetDevice(cfg->device));
+  m2c_ctx *c = new m2c_ctx();
+  c->cfg = *cfg;
+  c->grid = g0;
+  memcpy(c->eos, e0, sizeof(e0));
+  c->variant = v0;
+  c->N = g0.N;
+  Grid &g = c->grid;
   c->iface = g.has_interfaces && cfg->mp_flux != 2;
   c->ls = cfg->ls_matid >= 0;
   int bad = 0;

@@ -13,6 +13,11 @@ m2c_amd.make_config, V the primitive state (nz, ny, nx, 5) and ID the material I
                     in the water at x=64 cells with post-shock p=1e9 Pa; exact two-material Riemann
                     solves at interface faces, level-set reinitialisation every 5 steps, HLLC+MUSCL
                     (van Albada), RK2, CFL 0.5. `scale` shrinks every length for small test runs.
+  batched_riemann   independent single-material Riemann-type instances: edge lengths drawn
+                    uniformly from lo..hi per axis, left/right states log-uniform (rho in
+                    [0.1, 10], p in [0.1, 100], u = 0) split by a random plane (normal uniform on
+                    the sphere, through a uniform point of the box); gamma = 1.4, HLLC + van
+                    Albada, RK2, CFL 0.5, slip walls
 The periodic boundary and the Fourier-mode initial state have no counterpart in the reference's
 input format (IoData.h:266-267 has no periodic code); they are the documented extension.
 """
@@ -111,6 +116,37 @@ def shock_bubble(nx=384, ny=192, nz=192, radius=24.0, cx=160.0, shock_x=64.0, p_
     for c in range(5):
         V[..., c] = np.where(shocked, post[c], V[..., c])
     return settings, V, ID, phi
+
+
+BATCH_SEED = 4242
+
+
+def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0), p_range=(0.1, 100.0)):
+    """Returns a list of (settings, V, ID), one per instance.
+
+    CFL is 0.4: the time step follows SpaceOperator.cpp:2287, dt = CFL * min over the three
+    directions of dx/lambda, so an oblique wave sees the sum of the three directional Courant
+    numbers; at CFL 0.5 MUSCL-RK2 turns unstable within 20 steps on oblique split planes (the
+    oracle and the GPU agree on the blow-up)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx, ny, nz = (int(v) for v in rng.integers(lo, hi + 1, size=3))
+        normal = rng.normal(size=3)
+        normal /= np.linalg.norm(normal)
+        point = rng.uniform(0.0, 1.0, size=3) * np.array([nx, ny, nz])
+        rho = np.exp(rng.uniform(np.log(rho_range[0]), np.log(rho_range[1]), size=2))
+        p = np.exp(rng.uniform(np.log(p_range[0]), np.log(p_range[1]), size=2))
+        settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, float(nx), 0.0, float(ny), 0.0, float(nz)),
+                        bc=[BC_SYMMETRY] * 6, mats=[material(EOS_SG, [1.4, 0.0])], flux=FLUX_HLLC,
+                        limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.4)
+        z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+        left = ((x - point[0]) * normal[0] + (y - point[1]) * normal[1] + (z - point[2]) * normal[2]) < 0.0
+        V = np.zeros((nz, ny, nx, 5))
+        V[..., 0] = np.where(left, rho[0], rho[1])
+        V[..., 4] = np.where(left, p[0], p[1])
+        out.append((settings, V, np.zeros((nz, ny, nx), dtype=np.int32)))
+    return out
 
 
 def config_from(settings, **over):
