@@ -46,11 +46,18 @@ def _compile(unit, verbose):
     return obj
 
 
-def build(verbose=False, jobs=None):
+def build(verbose=False, jobs=None, only=None):
+    """Compile every unit (or, with `only`, just the named units, reusing the other objects of an
+    earlier build) and link libm2c_hip.so."""
     os.makedirs(OBJ, exist_ok=True)
-    jobs = jobs or min(len(UNITS), max(1, (os.cpu_count() or 2)))
+    todo = [u for u in UNITS if only is None or u[0] in only]
+    jobs = jobs or min(len(todo), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda u: _compile(u, verbose), UNITS))
+        list(ex.map(lambda u: _compile(u, verbose), todo))
+    objs = [os.path.join(OBJ, u[0] + ".o") for u in UNITS]
+    missing = [o for o in objs if not os.path.exists(o)]
+    if missing:
+        raise RuntimeError("objects missing (run a full build first): " + ", ".join(missing))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
@@ -81,4 +88,7 @@ def build_host_demo(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    only = None
+    if "--only" in sys.argv:
+        only = set(sys.argv[sys.argv.index("--only") + 1].split(","))
+    print(build(verbose="-v" in sys.argv, only=only))

@@ -407,12 +407,17 @@ static int setup_multiphase(m2c_ctx *c) {
   L.stream = c->stream;
   L.N = N;
   L.Nf = Nf;
+  L.nx = g.nx;
+  L.ny = g.ny;
+  L.nz = g.nz;
   L.flux = cfg->flux;
   L.ID = c->ID;
   L.st = c->st;
   L.clk = c->clk;
   mp.ls_matid = cfg->ls_matid;
   for (int sd = 0; sd < 6; sd++) mp.ls_bc[sd] = cfg->ls_bc[sd];
+  mp.ls_neumann = 1;
+  for (int sd = 0; sd < 6; sd++) mp.ls_neumann &= (cfg->ls_bc[sd] == 1);
   mp.reinit_freq = cfg->reinit_freq;
   mp.reinit_maxits = cfg->reinit_maxits;
   mp.reinit_tol = cfg->reinit_tol;
@@ -520,6 +525,26 @@ static int setup_multiphase(m2c_ctx *c) {
       mp.fdr[d] = c->fd + ro_off[d];
     }
     mp.fdr1y = c->fd + r1y_off;
+    // uniform interior coefficients (bitwise equal over indices 2..n-3)
+    auto uniform4 = [&](size_t off, int nn, double *out) -> int {
+      if (nn < 5) return 0;
+      const double *w0 = fd.data() + off + 4 * 2;
+      for (int i = 3; i <= nn - 3; i++)
+        if (memcmp(fd.data() + off + 4 * i, w0, 4 * sizeof(double))) return 0;
+      memcpy(out, w0, 4 * sizeof(double));
+      return 1;
+    };
+    for (int d = 0; d < 3; d++) {
+      const double *x = xc.data() + xo[d];
+      const double hd = x[1] - x[0];
+      int e2, ok = std::frexp(hd, &e2) == 0.5;
+      for (int i = 1; ok && i <= n[d]; i++) ok = (x[i + 1] - x[i]) == hd;
+      mp.dx_pow2[d] = ok;
+      mp.inv_dx[d] = ok ? 1.0 / hd : 0.0;
+    }
+    for (int d = 0; d < 3; d++)
+      mp.fd_uni[d] = uniform4(lo_off[d], n[d], mp.fdl_u[d]) & uniform4(ro_off[d], n[d], mp.fdr_u[d]);
+    mp.fd_uni_r1y = uniform4(r1y_off, n[1], mp.fdr1y_u);
   }
   return M2C_OK;
 }

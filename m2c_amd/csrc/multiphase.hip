@@ -75,36 +75,104 @@ __device__ __forceinline__ int cell_id(const int8_t *ID, int i, int j, int k) {
 // 7 periodic)
 __device__ inline double phi_at(const double *phi, int i, int j, int k) {
   const Grid &g = c_grid;
-  int n[3] = {g.nx, g.ny, g.nz};
-  int q[3] = {i, j, k};
-  int dim = -1;
-  if (i < 0 || i >= g.nx) dim = 0;
-  else if (j < 0 || j >= g.ny) dim = 1;
-  else if (k < 0 || k >= g.nz) dim = 2;
-  if (dim < 0) return phi[cidx3(i, j, k)];
-  const int gq = q[dim];
-  const int hi = gq >= n[dim];
+  int dim, n, q;
+  long long stride;
+  if (i < 0 || i >= g.nx) { dim = 0; n = g.nx; q = i; stride = 1; }
+  else if (j < 0 || j >= g.ny) { dim = 1; n = g.ny; q = j; stride = g.nx; }
+  else if (k < 0 || k >= g.nz) { dim = 2; n = g.nz; q = k; stride = (long long)g.nx * g.ny; }
+  else return phi[cidx3(i, j, k)];
+  // (i,j,k) is a ghost along `dim` only; c is its (virtual) linear index
+  const long long c = cidx3(i, j, k);
+  const int hi = q >= n;
   const int bc = c_mp.ls_bc[2 * dim + hi];
-  int im[3] = {q[0], q[1], q[2]};
-  if (bc == BC_PERIODIC) im[dim] = hi ? 0 : n[dim] - 1;
-  else im[dim] = hi ? n[dim] - 1 : 0;
-  double v = phi[cidx3(im[0], im[1], im[2])];
+  int im;
+  if (bc == BC_PERIODIC) im = hi ? 0 : n - 1;
+  else im = hi ? n - 1 : 0;
+  double v = phi[c + (long long)(im - q) * stride];
   if (bc == 2 || bc == 3) {
-    if (n[dim] >= 2) {
+    if (n >= 2) {
       const int s = hi ? -1 : 1;
-      int a1[3] = {q[0], q[1], q[2]}, a2[3] = {q[0], q[1], q[2]};
-      a1[dim] = gq + s;
-      a2[dim] = gq + 2 * s;
-      double r = XC(dim, gq), r1 = XC(dim, gq + s), r2 = XC(dim, gq + 2 * s);
-      double f1 = phi[cidx3(a1[0], a1[1], a1[2])], f2 = phi[cidx3(a2[0], a2[1], a2[2])];
+      double r = XC(dim, q), r1 = XC(dim, q + s), r2 = XC(dim, q + 2 * s);
+      double f1 = phi[c + s * stride], f2 = phi[c + 2 * s * stride];
       v = f1 + (f2 - f1) / (r2 - r1) * (r - r1);
     }
     if (bc == 3) {
-      double d2wall = 0.5 * dabs(XC(dim, im[dim]) - XC(dim, gq));
+      double d2wall = 0.5 * dabs(XC(dim, im) - XC(dim, q));
       v = smax(-d2wall, v);
     }
   }
   return v;
+}
+
+// linear index of (i,j,k) with each coordinate clamped into the box
+__device__ __forceinline__ long long clamped_idx(int i, int j, int k) {
+  const Grid &g = c_grid;
+  return cidx3(clampi(i, 0, g.nx - 1), clampi(j, 0, g.ny - 1), clampi(k, 0, g.nz - 1));
+}
+
+// Cell sweeps over the box: blocks of 64 x 4 (i, j) columns, each thread walking SWEEP_KB planes
+// of k; no integer division in the index arithmetic.
+constexpr int SWEEP_BX = 64, SWEEP_BY = 4, SWEEP_KB = 4;
+
+static inline dim3 sweep_grid(int nx, int ny, int nz) {
+  return dim3((nx + SWEEP_BX - 1) / SWEEP_BX, (ny + SWEEP_BY - 1) / SWEEP_BY, (nz + SWEEP_KB - 1) / SWEEP_KB);
+}
+
+__device__ __forceinline__ bool first_thread() {
+  return blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+}
+
+// block-wide maximum of non-negative v; thread (0,0) of each block folds it into *dst
+__device__ __forceinline__ void block_max_to(unsigned long long *dst, double v, bool skip_zero) {
+  __shared__ double red[SWEEP_BY];
+  for (int off = 32; off > 0; off >>= 1) v = smax(v, __shfl_xor(v, off, 64));
+  if (threadIdx.x == 0) red[threadIdx.y] = v;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    double m = red[0];
+    for (int w = 1; w < SWEEP_BY; w++) m = smax(m, red[w]);
+    if (!skip_zero || m > 0.0) atomic_max_pos(dst, m);
+  }
+}
+
+// calls f(i, j, k, idx, inner) for this thread's cells; inner = all six face neighbours are in the box
+template <class F>
+__device__ __forceinline__ void sweep_cells(F &&f) {
+  const Grid &g = c_grid;
+  const int i = blockIdx.x * SWEEP_BX + threadIdx.x, j = blockIdx.y * SWEEP_BY + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const int k0 = blockIdx.z * SWEEP_KB, k1 = min(g.nz, k0 + SWEEP_KB);
+  const bool inij = i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1;
+  const long long plane = (long long)g.nx * g.ny;
+  const long long idx0 = ((long long)k0 * g.ny + j) * g.nx + i;
+  if (k1 - k0 == SWEEP_KB) {   // full tile: unrolled so that the planes' loads can overlap
+#pragma unroll
+    for (int q = 0; q < SWEEP_KB; q++) f(i, j, k0 + q, idx0 + q * plane, inij && k0 + q > 0 && k0 + q < g.nz - 1);
+  } else {
+    long long idx = idx0;
+    for (int k = k0; k < k1; k++, idx += plane) f(i, j, k, idx, inij && k > 0 && k < g.nz - 1);
+  }
+}
+
+// the six face neighbours of cell (i,j,k) = idx, ghosts by the level-set boundary rules
+struct Nb6 { double im, ip, jm, jp, km, kp; };
+__device__ __forceinline__ Nb6 phi_nb6(const double *phi, int i, int j, int k, long long idx, bool inner) {
+  Nb6 n;
+  if (c_mp.ls_neumann) {   // zero-Neumann ghosts are the clamped image cells
+    n.im = phi[clamped_idx(i - 1, j, k)]; n.ip = phi[clamped_idx(i + 1, j, k)];
+    n.jm = phi[clamped_idx(i, j - 1, k)]; n.jp = phi[clamped_idx(i, j + 1, k)];
+    n.km = phi[clamped_idx(i, j, k - 1)]; n.kp = phi[clamped_idx(i, j, k + 1)];
+  } else if (inner) {
+    const long long sy = c_grid.nx, sz = (long long)c_grid.nx * c_grid.ny;
+    n.im = phi[idx - 1]; n.ip = phi[idx + 1];
+    n.jm = phi[idx - sy]; n.jp = phi[idx + sy];
+    n.km = phi[idx - sz]; n.kp = phi[idx + sz];
+  } else {
+    n.im = phi_at(phi, i - 1, j, k); n.ip = phi_at(phi, i + 1, j, k);
+    n.jm = phi_at(phi, i, j - 1, k); n.jp = phi_at(phi, i, j + 1, k);
+    n.km = phi_at(phi, i, j, k - 1); n.kp = phi_at(phi, i, j, k + 1);
+  }
+  return n;
 }
 
 // velocity of a cell of the stage field S (primitive, or conservative -> ConservativeToPrimitive)
@@ -125,45 +193,65 @@ __device__ __forceinline__ void cell_velocity(const double *S, const int8_t *ID,
   }
 }
 
+struct W4 { double w0, w1, w2, w3; };
+__device__ __forceinline__ W4 fd_w(bool uni, const double *u, const double *tab) {
+  W4 w;
+  if (uni) { w.w0 = u[0]; w.w1 = u[1]; w.w2 = u[2]; w.w3 = u[3]; }
+  else { w.w0 = tab[0]; w.w1 = tab[1]; w.w2 = tab[2]; w.w3 = tab[3]; }
+  return w;
+}
+
 // upwinded third-order level-set residual -(u . grad phi) of interior cell (i,j,k)
-// (LevelSetOperator.cpp:1066-1141 with the FD3 stencils of GradientCalculatorFD3.cpp:54-171)
-__device__ inline double ls_residual(const double *phi, int i, int j, int k, const double *u) {
+// (LevelSetOperator.cpp:1066-1141 with the FD3 stencils of GradientCalculatorFD3.cpp:54-171).
+// NB: 0 = ghosts by the boundary rules, 1 = every index within two cells is in the box (inner2),
+// 2 = zero-Neumann boundaries (ghost = clamped image).
+template <int NB>
+__device__ inline double ls_residual(const double *phi, int i, int j, int k, long long idx, bool inner2,
+                                     const double *u) {
   const Grid &g = c_grid;
   const int NX = g.nx, NY = g.ny, NZ = g.nz;
-  const double p0 = phi[cidx3(i, j, k)];
+  const long long sy = NX, sz = (long long)NX * NY;
+  auto P = [&](int ii, int jj, int kk) -> double {
+    if (NB == 1) return phi[idx + (ii - i) + (jj - j) * sy + (kk - k) * sz];
+    if (NB == 2) return phi[clamped_idx(ii, jj, kk)];
+    return phi_at(phi, ii, jj, kk);
+  };
+  const double p0 = phi[idx];
   double a, b, cc, d, e, f;
   if (i - 2 >= -1) {
-    const double *w = c_mp.fdl[0] + 4 * i;
-    a = w[0] * phi_at(phi, i - 2, j, k) + w[1] * phi_at(phi, i - 1, j, k) + w[2] * p0 + w[3] * phi_at(phi, i + 1, j, k);
+    const W4 w = fd_w(inner2 && c_mp.fd_uni[0], c_mp.fdl_u[0], c_mp.fdl[0] + 4 * i);
+    a = w.w0 * P(i - 2, j, k) + w.w1 * P(i - 1, j, k) + w.w2 * p0 + w.w3 * P(i + 1, j, k);
   } else
-    a = (p0 - phi_at(phi, i - 1, j, k)) / (XC(0, i) - XC(0, i - 1));
+    a = (p0 - P(i - 1, j, k)) / (XC(0, i) - XC(0, i - 1));
   if (i + 2 <= NX) {
-    const double *w = c_mp.fdr[0] + 4 * i;
-    b = w[0] * phi_at(phi, i - 1, j, k) + w[1] * p0 + w[2] * phi_at(phi, i + 1, j, k) + w[3] * phi_at(phi, i + 2, j, k);
+    const W4 w = fd_w(inner2 && c_mp.fd_uni[0], c_mp.fdr_u[0], c_mp.fdr[0] + 4 * i);
+    b = w.w0 * P(i - 1, j, k) + w.w1 * p0 + w.w2 * P(i + 1, j, k) + w.w3 * P(i + 2, j, k);
   } else
-    b = (phi_at(phi, i + 1, j, k) - p0) / (XC(0, i + 1) - XC(0, i));
+    b = (P(i + 1, j, k) - p0) / (XC(0, i + 1) - XC(0, i));
   if (j - 2 >= -1) {
-    const double *w = c_mp.fdl[1] + 4 * j;
-    cc = w[0] * phi_at(phi, i, j - 2, k) + w[1] * phi_at(phi, i, j - 1, k) + w[2] * p0 + w[3] * phi_at(phi, i, j + 1, k);
+    const W4 w = fd_w(inner2 && c_mp.fd_uni[1], c_mp.fdl_u[1], c_mp.fdl[1] + 4 * j);
+    cc = w.w0 * P(i, j - 2, k) + w.w1 * P(i, j - 1, k) + w.w2 * p0 + w.w3 * P(i, j + 1, k);
   } else
-    cc = (p0 - phi_at(phi, i, j - 1, k)) / (XC(1, j) - XC(1, j - 1));
+    cc = (p0 - P(i, j - 1, k)) / (XC(1, j) - XC(1, j - 1));
   if (j + 2 <= NY) {
     // the right-biased y coefficients are third order only where the column index i < NY-1
     // (GradientCalculatorFD3.cpp:415 tests i, not j)
-    const double *w = (i < NY - 1 && j < NY - 1) ? c_mp.fdr[1] + 4 * j : c_mp.fdr1y + 4 * j;
-    d = w[0] * phi_at(phi, i, j - 1, k) + w[1] * p0 + w[2] * phi_at(phi, i, j + 1, k) + w[3] * phi_at(phi, i, j + 2, k);
+    const bool third = i < NY - 1 && j < NY - 1;
+    const W4 w = third ? fd_w(inner2 && c_mp.fd_uni[1], c_mp.fdr_u[1], c_mp.fdr[1] + 4 * j)
+                       : fd_w(inner2 && c_mp.fd_uni_r1y, c_mp.fdr1y_u, c_mp.fdr1y + 4 * j);
+    d = w.w0 * P(i, j - 1, k) + w.w1 * p0 + w.w2 * P(i, j + 1, k) + w.w3 * P(i, j + 2, k);
   } else
-    d = (phi_at(phi, i, j + 1, k) - p0) / (XC(1, j + 1) - XC(1, j));
+    d = (P(i, j + 1, k) - p0) / (XC(1, j + 1) - XC(1, j));
   if (k - 2 >= -1) {
-    const double *w = c_mp.fdl[2] + 4 * k;
-    e = w[0] * phi_at(phi, i, j, k - 2) + w[1] * phi_at(phi, i, j, k - 1) + w[2] * p0 + w[3] * phi_at(phi, i, j, k + 1);
+    const W4 w = fd_w(inner2 && c_mp.fd_uni[2], c_mp.fdl_u[2], c_mp.fdl[2] + 4 * k);
+    e = w.w0 * P(i, j, k - 2) + w.w1 * P(i, j, k - 1) + w.w2 * p0 + w.w3 * P(i, j, k + 1);
   } else
-    e = (p0 - phi_at(phi, i, j, k - 1)) / (XC(2, k) - XC(2, k - 1));
+    e = (p0 - P(i, j, k - 1)) / (XC(2, k) - XC(2, k - 1));
   if (k + 2 <= NZ) {
-    const double *w = c_mp.fdr[2] + 4 * k;
-    f = w[0] * phi_at(phi, i, j, k - 1) + w[1] * p0 + w[2] * phi_at(phi, i, j, k + 1) + w[3] * phi_at(phi, i, j, k + 2);
+    const W4 w = fd_w(inner2 && c_mp.fd_uni[2], c_mp.fdr_u[2], c_mp.fdr[2] + 4 * k);
+    f = w.w0 * P(i, j, k - 1) + w.w1 * p0 + w.w2 * P(i, j, k + 1) + w.w3 * P(i, j, k + 2);
   } else
-    f = (phi_at(phi, i, j, k + 1) - p0) / (XC(2, k + 1) - XC(2, k));
+    f = (P(i, j, k + 1) - p0) / (XC(2, k + 1) - XC(2, k));
   double r = (u[0] >= 0 ? a * u[0] : b * u[0]) + (u[1] >= 0 ? cc * u[1] : d * u[1]) + (u[2] >= 0 ? e * u[2] : f * u[2]);
   return r * -1.0;
 }
@@ -186,31 +274,36 @@ __global__ void k_mp_face_reset(MpState *ms) { ms->nface = 0; }
 
 // interface faces of the current IDs: (dir, plus cell) with different active materials on the two
 // sides; code = dir*Nf + face_key
+// One thread per (i,j) column of the box extended by one cell on the high sides, walking k; cell
+// (i,j,k) owns the three faces on its low sides.
 __global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *faces, MpState *ms, const Clock *clk,
                                                      int ignore_active) {
   if (!ignore_active && !step_active(clk)) return;
   const Grid &g = c_grid;
+  if (g.mp_flux == 2) return;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-  const long long total = 3 * Nf;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < total; base += (long long)gridDim.x * blockDim.x) {
-    long long code = base + threadIdx.x;
-    bool pred = false;
-    if (code < total) {
-      int dir = (int)(code / Nf);
-      long long key = code - dir * Nf;
-      int i = (int)(key % (g.nx + 1));
-      long long r = key / (g.nx + 1);
-      int j = (int)(r % (g.ny + 1));
-      int k = (int)(r / (g.ny + 1));
-      bool valid = (dir == 0) ? (j < g.ny && k < g.nz) : (dir == 1 ? (i < g.nx && k < g.nz) : (i < g.nx && j < g.ny));
+  const int i = blockIdx.x * SWEEP_BX + threadIdx.x, j = blockIdx.y * SWEEP_BY + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  auto id_at = [&](int a, int b, int c) -> int {
+    if (a >= 0 && a < g.nx && b >= 0 && b < g.ny && c >= 0 && c < g.nz) return ID[cidx3(a, b, c)];
+    return cell_id(ID, a, b, c);
+  };
+  const int k0 = blockIdx.z * SWEEP_KB, k1 = min(g.nz + 1, k0 + SWEEP_KB);
+  for (int k = k0; k < k1; k++) {
+    const long long key = ((long long)k * (g.ny + 1) + j) * (g.nx + 1) + i;
+    const int idp = id_at(i, j, k);
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+      const bool valid = (dir == 0) ? (j < g.ny && k < g.nz) : (dir == 1 ? (i < g.nx && k < g.nz) : (i < g.nx && j < g.ny));
+      bool pred = false;
       if (valid) {
-        int idp = cell_id(ID, i, j, k);
-        int idm = cell_id(ID, i - (dir == 0), j - (dir == 1), k - (dir == 2));
-        pred = idm != idp && idm != g.inactive_id && idp != g.inactive_id && g.mp_flux != 2;
+        const int idm = id_at(i - (dir == 0), j - (dir == 1), k - (dir == 2));
+        pred = idm != idp && idm != g.inactive_id && idp != g.inactive_id;
       }
+      // the lanes of a wave share j and the k range, so they reach the append together
+      unsigned slot = wave_append(&ms->nface, pred);
+      if (pred) faces[slot] = (unsigned)(dir * Nf + key);
     }
-    unsigned slot = wave_append(&ms->nface, pred);
-    if (pred) faces[slot] = (unsigned)code;
   }
 }
 
@@ -320,43 +413,39 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *S, const int8_t 
   if (!step_active(clk)) return;
   const Grid &g = c_grid;
   const double cdt = coef * clk->dt;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < g.N; base += (long long)gridDim.x * blockDim.x) {
-    const long long idx = base + threadIdx.x;
+  sweep_cells([&](int i, int j, int k, long long idx, bool) {
     bool swept_out = false;
-    if (idx < g.N) {
-      const int i = (int)(idx % g.nx);
-      const long long r = idx / g.nx;
-      const int j = (int)(r % g.ny), k = (int)(r / g.ny);
-      double u[3];
-      cell_velocity<IN_CONS>(S, IDn, idx, u);
-      const double R = ls_residual(phiS, i, j, k, u);
-      double t;
-      if (MODE == LS_FIRST) {
-        double x = X[idx];
-        t = (x == 0.0) ? 0.0 * out[idx] + 1.0 * x : x;   // 0*old + 1*x differs from x only in the sign of zero
-      } else if (MODE == LS_AXBY) {
-        t = a * X[idx] + b * Y[idx];
-      } else {
-        t = X[idx];
-      }
-      const double pn = FINAL ? phin[idx] : 0.0;
-      const double v = t + cdt * R;
-      out[idx] = v;
-      if (FINAL) {
-        int id = IDn[idx];
-        int nid = id;
-        if (id != g.inactive_id && !(v * pn > 0.0)) {
-          if (v < 0) nid = c_mp.ls_matid;
-          else swept_out = true;
-        }
-        IDnew[idx] = (int8_t)nid;
-      }
+    double u[3];
+    cell_velocity<IN_CONS>(S, IDn, idx, u);
+    const bool inner2 = i >= 2 && i < g.nx - 2 && j >= 2 && j < g.ny - 2 && k >= 2 && k < g.nz - 2;
+    double R;
+    if (c_mp.ls_neumann) R = ls_residual<2>(phiS, i, j, k, idx, inner2, u);
+    else R = inner2 ? ls_residual<1>(phiS, i, j, k, idx, true, u) : ls_residual<0>(phiS, i, j, k, idx, false, u);
+    double t;
+    if (MODE == LS_FIRST) {
+      double x = X[idx];
+      t = (x == 0.0) ? 0.0 * out[idx] + 1.0 * x : x;   // 0*old + 1*x differs from x only in the sign of zero
+    } else if (MODE == LS_AXBY) {
+      t = a * X[idx] + b * Y[idx];
+    } else {
+      t = X[idx];
     }
+    const double pn = FINAL ? phin[idx] : 0.0;
+    const double v = t + cdt * R;
+    out[idx] = v;
     if (FINAL) {
+      int id = IDn[idx];
+      int nid = id;
+      if (id != g.inactive_id && !(v * pn > 0.0)) {
+        if (v < 0) nid = c_mp.ls_matid;
+        else swept_out = true;
+      }
+      IDnew[idx] = (int8_t)nid;
+      // every lane of the wave runs the same planes, so the append is reached together
       unsigned slot = wave_append(&ms->nrem, swept_out);
       if (swept_out) rem[slot] = (unsigned)idx;
     }
-  }
+  });
 }
 
 // nodes swept out of the level-set subdomain take a neighbour's untracked ID; when the only
@@ -479,29 +568,25 @@ __global__ void k_reinit_begin(MpState *ms, const Clock *clk, int retry) {
 __global__ __launch_bounds__(256) void k_reinit_tag(const double *phi, uint8_t *tag, double *sign, double *phibk,
                                                      MpState *ms) {
   if (!ms->rs.active) return;
-  const Grid &g = c_grid;
   int det = 0;
-  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % g.nx);
-    const long long r = idx / g.nx;
-    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+  sweep_cells([&](int i, int j, int k, long long idx, bool inner) {
     const double p = phi[idx];
+    const Nb6 n = phi_nb6(phi, i, j, k, idx, inner);
     int t = 0;
-    if (p * phi_at(phi, i - 1, j, k) <= 0) t = 1;
-    else if (p * phi_at(phi, i + 1, j, k) <= 0) t = 1;
-    else if (p * phi_at(phi, i, j - 1, k) <= 0) t = 1;
-    else if (p * phi_at(phi, i, j + 1, k) <= 0) t = 1;
-    else if (p * phi_at(phi, i, j, k - 1) <= 0) t = 1;
-    else if (p * phi_at(phi, i, j, k + 1) <= 0) t = 1;
+    if (p * n.im <= 0) t = 1;
+    else if (p * n.ip <= 0) t = 1;
+    else if (p * n.jm <= 0) t = 1;
+    else if (p * n.jp <= 0) t = 1;
+    else if (p * n.km <= 0) t = 1;
+    else if (p * n.kp <= 0) t = 1;
     tag[idx] = (uint8_t)t;
     det |= t;
     const double f = c_mp.sign_factor;
     sign[idx] = p / dsqrt<double>(p * p + f * f);
     const double ob = phibk[idx];
     phibk[idx] = 0.0 * ob + 1.0 * p;
-  }
-  if (__any(det) && (threadIdx.x & 63) == 0) atomicOr(&ms->rs.detected, 1);
+  });
+  if (__any(det) && threadIdx.x == 0) atomicOr(&ms->rs.detected, 1);
 }
 
 __global__ void k_reinit_restore(double *phi, const double *phibk, const MpState *ms) {
@@ -512,18 +597,20 @@ __global__ void k_reinit_restore(double *phi, const double *phibk, const MpState
 }
 
 // pseudo-time residual of the reinitialisation equation (LevelSetReinitializer.cpp:953-1035)
-__device__ inline double reinit_rhs(const double *phi, const uint8_t *tag, const double *sign, int i, int j, int k,
-                                    long long idx, double cfl, double *lres) {
+__device__ __forceinline__ double reinit_rhs(const double *phi, const uint8_t *tag, const double *sign, int i, int j,
+                                             int k, long long idx, bool inner, double cfl, double *lres) {
   *lres = 0.0;
   if (tag[idx] != 0) return 0.0;
   const double dt = cfl * c_mp.min_dx;
   const double p = phi[idx];
-  double a = (p - phi_at(phi, i - 1, j, k)) / (XC(0, i) - XC(0, i - 1));
-  double b = (phi_at(phi, i + 1, j, k) - p) / (XC(0, i + 1) - XC(0, i));
-  double c = (p - phi_at(phi, i, j - 1, k)) / (XC(1, j) - XC(1, j - 1));
-  double d = (phi_at(phi, i, j + 1, k) - p) / (XC(1, j + 1) - XC(1, j));
-  double e = (p - phi_at(phi, i, j, k - 1)) / (XC(2, k) - XC(2, k - 1));
-  double f = (phi_at(phi, i, j, k + 1) - p) / (XC(2, k + 1) - XC(2, k));
+  const Nb6 n = phi_nb6(phi, i, j, k, idx, inner);
+  double a, b, c, d, e, f;
+  if (c_mp.dx_pow2[0]) { a = (p - n.im) * c_mp.inv_dx[0]; b = (n.ip - p) * c_mp.inv_dx[0]; }
+  else { a = (p - n.im) / (XC(0, i) - XC(0, i - 1)); b = (n.ip - p) / (XC(0, i + 1) - XC(0, i)); }
+  if (c_mp.dx_pow2[1]) { c = (p - n.jm) * c_mp.inv_dx[1]; d = (n.jp - p) * c_mp.inv_dx[1]; }
+  else { c = (p - n.jm) / (XC(1, j) - XC(1, j - 1)); d = (n.jp - p) / (XC(1, j + 1) - XC(1, j)); }
+  if (c_mp.dx_pow2[2]) { e = (p - n.km) * c_mp.inv_dx[2]; f = (n.kp - p) * c_mp.inv_dx[2]; }
+  else { e = (p - n.km) / (XC(2, k) - XC(2, k - 1)); f = (n.kp - p) / (XC(2, k + 1) - XC(2, k)); }
   double ap = smax(a, 0.0), am = smin(a, 0.0), bp = smax(b, 0.0), bm = smin(b, 0.0);
   double cp = smax(c, 0.0), cm = smin(c, 0.0), dp = smax(d, 0.0), dm = smin(d, 0.0);
   double ep = smax(e, 0.0), em = smin(e, 0.0), fp = smax(f, 0.0), fm = smin(f, 0.0);
@@ -538,74 +625,61 @@ __device__ inline double reinit_rhs(const double *phi, const uint8_t *tag, const
 }
 
 // RK3 stage 1 of reinitialisation iteration `it`: rphi1a = (0*rphi1a + 1*phi) + 1*R(phi)
-__global__ __launch_bounds__(256) void k_reinit_A(const double *phi, double *rphi1a, const uint8_t *tag,
-                                                   const double *sign, MpState *ms, int it) {
+__global__ __launch_bounds__(256) void k_reinit_A(const double *__restrict__ phi, double *__restrict__ rphi1a,
+                                                   const uint8_t *__restrict__ tag,
+                                                   const double *__restrict__ sign, MpState *ms, int it) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || Rs.done) return;
   if (it > 0 && bits_to_d(Rs.dphi_bits) < c_mp.reinit_tol) {   // converged after iteration it-1
-    if (blockIdx.x == 0 && threadIdx.x == 0) { Rs.done = 1; Rs.iter = it - 1; }
+    if (first_thread()) { Rs.done = 1; Rs.iter = it - 1; }
     return;
   }
-  const Grid &g = c_grid;
   const double cfl = Rs.cfl;
   double mx = 0.0;
-  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % g.nx);
-    const long long r = idx / g.nx;
-    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+  sweep_cells([&](int i, int j, int k, long long idx, bool inner) {
     double lres;
-    double R = reinit_rhs(phi, tag, sign, i, j, k, idx, cfl, &lres);
+    double R = reinit_rhs(phi, tag, sign, i, j, k, idx, inner, cfl, &lres);
     if (lres > mx) mx = lres;
     double x = phi[idx];
     double t = (x == 0.0) ? 0.0 * rphi1a[idx] + 1.0 * x : x;
     rphi1a[idx] = t + 1.0 * R;
-  }
-  for (int off = 32; off > 0; off >>= 1) mx = smax(mx, __shfl_xor(mx, off, 64));
-  if ((threadIdx.x & 63) == 0 && mx > 0.0) atomic_max_pos(&Rs.maxres_bits, mx);
+  });
+  block_max_to(&Rs.maxres_bits, mx, true);
 }
 
 // stage 2: stop if the residual of stage 1 is below tolerance, else
 // rphi1b = (0.25*rphi1a + 0.75*phi) + 0.25*R(rphi1a)
-__global__ __launch_bounds__(256) void k_reinit_B(const double *phi, const double *rphi1a, double *rphi1b,
-                                                   const uint8_t *tag, const double *sign, MpState *ms, int it) {
+__global__ __launch_bounds__(256) void k_reinit_B(const double *__restrict__ phi, const double *__restrict__ rphi1a,
+                                                   double *__restrict__ rphi1b, const uint8_t *__restrict__ tag,
+                                                   const double *__restrict__ sign, MpState *ms, int it) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || Rs.done) return;
   if (bits_to_d(Rs.maxres_bits) < c_mp.reinit_tol) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { Rs.done = 1; Rs.iter = it; }
+    if (first_thread()) { Rs.done = 1; Rs.iter = it; }
     return;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) Rs.dphi_bits = 0ull;
-  const Grid &g = c_grid;
+  if (first_thread()) Rs.dphi_bits = 0ull;
   const double cfl = Rs.cfl;
-  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % g.nx);
-    const long long r = idx / g.nx;
-    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+  sweep_cells([&](int i, int j, int k, long long idx, bool inner) {
     double lres;
-    double R = reinit_rhs(rphi1a, tag, sign, i, j, k, idx, cfl, &lres);
+    double R = reinit_rhs(rphi1a, tag, sign, i, j, k, idx, inner, cfl, &lres);
     double t = 0.25 * rphi1a[idx] + 0.75 * phi[idx];
     rphi1b[idx] = t + 0.25 * R;
-  }
+  });
 }
 
 // stage 3: phi = (1/3*phi + 2/3*rphi1b) + 2/3*R(rphi1b), and the maximum relative change
-__global__ __launch_bounds__(256) void k_reinit_C(double *phi, const double *rphi1b, const uint8_t *tag,
-                                                   const double *sign, MpState *ms) {
+__global__ __launch_bounds__(256) void k_reinit_C(double *__restrict__ phi, const double *__restrict__ rphi1b,
+                                                   const uint8_t *__restrict__ tag, const double *__restrict__ sign,
+                                                   MpState *ms) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || Rs.done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) Rs.maxres_bits = 0ull;
-  const Grid &g = c_grid;
+  if (first_thread()) Rs.maxres_bits = 0ull;
   const double cfl = Rs.cfl;
   double mx = 0.0;
-  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % g.nx);
-    const long long r = idx / g.nx;
-    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+  sweep_cells([&](int i, int j, int k, long long idx, bool inner) {
     double lres;
-    double R = reinit_rhs(rphi1b, tag, sign, i, j, k, idx, cfl, &lres);
+    double R = reinit_rhs(rphi1b, tag, sign, i, j, k, idx, inner, cfl, &lres);
     const double po = phi[idx];
     double t = 1.0 / 3.0 * po + 2.0 / 3.0 * rphi1b[idx];
     const double pn = t + 2.0 / 3.0 * R;
@@ -613,9 +687,8 @@ __global__ __launch_bounds__(256) void k_reinit_C(double *phi, const double *rph
     double le = dabs(pn - po);
     if (pn != 0) le /= dabs(pn);
     mx = smax(mx, le);
-  }
-  for (int off = 32; off > 0; off >>= 1) mx = smax(mx, __shfl_xor(mx, off, 64));
-  if ((threadIdx.x & 63) == 0) atomic_max_pos(&Rs.dphi_bits, mx);
+  });
+  block_max_to(&Rs.dphi_bits, mx, false);
 }
 
 // after the last iteration: convergence on the last relative change, else the failsafe of
@@ -1028,10 +1101,9 @@ static int nblocks(long long n, int bs, int cap) {
 }
 
 extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_cons, int store, int ignore_active) {
-  const long long total = 3 * L.Nf;
   hipLaunchKernelGGL(k_mp_face_reset, dim3(1), dim3(1), 0, L.stream, L.ms);
-  hipLaunchKernelGGL(k_iface_list, dim3(nblocks(total, 256, 8192)), dim3(256), 0, L.stream, L.ID, L.faces, L.ms,
-                     L.clk, ignore_active);
+  hipLaunchKernelGGL(k_iface_list, sweep_grid(L.nx + 1, L.ny + 1, L.nz + 1), dim3(SWEEP_BX, SWEEP_BY), 0, L.stream,
+                     L.ID, L.faces, L.ms, L.clk, ignore_active);
   dim3 gr(L.riemann_threads / 64), bl(64);
 #define M2C_RIEMANN(F)                                                                                          \
   if (in_cons)                                                                                                  \
@@ -1063,7 +1135,7 @@ extern "C" int m2c_mp_step_begin(const MpLaunch &L) {
 extern "C" int m2c_mp_launch_ls(const MpLaunch &L, int in_cons, int mode, int final_, const double *S,
                                 const double *phiS, const double *X, const double *Y, double *out, double a, double b,
                                 double coef, const double *phin) {
-  dim3 gr(nblocks(L.N, 256, 16384)), bl(256);
+  const dim3 gr = sweep_grid(L.nx, L.ny, L.nz), bl(SWEEP_BX, SWEEP_BY);
 #define M2C_LS(IC, MO, FI)                                                                                      \
   hipLaunchKernelGGL((k_ls_stage<IC, MO, FI>), gr, bl, 0, L.stream, S, L.ID, phiS, X, Y, out, a, b, coef, phin, \
                      L.IDnew, L.rem, L.ms, L.clk)
@@ -1095,13 +1167,14 @@ extern "C" int m2c_mp_launch_id_remaining(const MpLaunch &L, int single) {
 
 extern "C" int m2c_mp_launch_reinit(const MpLaunch &L, double *phi, int maxits, int retry) {
   dim3 gr(nblocks(L.N, 256, 8192)), bl(256);
+  const dim3 gs = sweep_grid(L.nx, L.ny, L.nz), bs(SWEEP_BX, SWEEP_BY);
   hipLaunchKernelGGL(k_reinit_begin, dim3(1), dim3(1), 0, L.stream, L.ms, L.clk, retry);
   if (retry) hipLaunchKernelGGL(k_reinit_restore, gr, bl, 0, L.stream, phi, L.phibk, L.ms);
-  else hipLaunchKernelGGL(k_reinit_tag, gr, bl, 0, L.stream, phi, L.tag, L.sign, L.phibk, L.ms);
+  else hipLaunchKernelGGL(k_reinit_tag, gs, bs, 0, L.stream, phi, L.tag, L.sign, L.phibk, L.ms);
   for (int it = 0; it < maxits; it++) {
-    hipLaunchKernelGGL(k_reinit_A, gr, bl, 0, L.stream, phi, L.rphi1a, L.tag, L.sign, L.ms, it);
-    hipLaunchKernelGGL(k_reinit_B, gr, bl, 0, L.stream, phi, L.rphi1a, L.rphi1b, L.tag, L.sign, L.ms, it);
-    hipLaunchKernelGGL(k_reinit_C, gr, bl, 0, L.stream, phi, L.rphi1b, L.tag, L.sign, L.ms);
+    hipLaunchKernelGGL(k_reinit_A, gs, bs, 0, L.stream, phi, L.rphi1a, L.tag, L.sign, L.ms, it);
+    hipLaunchKernelGGL(k_reinit_B, gs, bs, 0, L.stream, phi, L.rphi1a, L.rphi1b, L.tag, L.sign, L.ms, it);
+    hipLaunchKernelGGL(k_reinit_C, gs, bs, 0, L.stream, phi, L.rphi1b, L.tag, L.sign, L.ms);
   }
   hipLaunchKernelGGL(k_reinit_end, dim3(1), dim3(1), 0, L.stream, L.ms, L.st);
   return hchk(hipGetLastError());

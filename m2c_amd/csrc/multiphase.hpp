@@ -21,6 +21,15 @@ struct MpParams {
   double min_dx;               // min(dx, dy, dz): pseudo-time step = cfl * min_dx
   const double *xc[3];         // cell-centre coordinates, index -1..n stored at [i+1]
   const double *fdl[3], *fdr[3], *fdr1y;   // FD3 coefficients per index along each axis
+  // fd_uni[d]: the coefficients of fdl[d] and fdr[d] are the same for every index 2..n-3 (cells whose
+  // stencil stays in the box); they are then also held in fdl_u/fdr_u. Likewise fdr1y.
+  int fd_uni[3], fd_uni_r1y;
+  double fdl_u[3][4], fdr_u[3][4], fdr1y_u[4];
+  int ls_neumann;              // every level-set boundary is zero-Neumann: a ghost is its clamped image
+  // dx_pow2[d]: all cell-centre spacings along d (ghosts included) equal one power of two h, so
+  // x / h == x * inv_dx[d] exactly
+  int dx_pow2[3];
+  double inv_dx[3];
   RiemannCfg rcfg;
   int path_cap;                // rarefaction path entries per side and thread
   double failsafe_density[8];
@@ -45,6 +54,7 @@ struct MpState {
 struct MpLaunch {
   hipStream_t stream;
   long long N, Nf;
+  int nx, ny, nz;
   int flux, riemann_threads;
   int8_t *ID, *IDnew;
   unsigned *faces;

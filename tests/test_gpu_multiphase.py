@@ -83,6 +83,18 @@ def test_shock_bubble_linear_extrapolation_bc():
     compare(settings, V0, ID0, phi0, 12, ls_bc=(2, 2, 3, 3, 2, 2))
 
 
+def test_shock_bubble_non_power_of_two_spacing():
+    # h = 0.7: level-set differences divide by the coordinate spacing, FD3 weights come per index
+    settings, V0, ID0, phi0 = W.shock_bubble(scale=0.125, h=0.7)
+    compare(settings, V0, ID0, phi0, 12, reinit_freq=3)
+
+
+def test_shock_bubble_power_of_two_spacing():
+    # h = 0.5: divisions by the spacing become exact multiplications by 2
+    settings, V0, ID0, phi0 = W.shock_bubble(scale=0.125, h=0.5)
+    compare(settings, V0, ID0, phi0, 12, reinit_freq=3)
+
+
 def test_two_gas_interface_without_level_set():
     # a static two-material Riemann problem: the interface faces use the exact solver, IDs fixed
     n = 64
