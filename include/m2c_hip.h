@@ -97,6 +97,8 @@ typedef struct {
   double t, dt, cfl;
   long long reinit_iterations;      /* pseudo-time iterations of all reinitialisations */
   long long unresolved_cells;       /* cells updated by FixUnresolvedNodes */
+  long long surrogate_accepted;     /* interface solves taken from the learned solver */
+  long long surrogate_fallbacks;    /* ... and those that failed its check and ran the exact solver */
 } m2c_status;
 
 typedef struct m2c_ctx m2c_ctx;
@@ -170,6 +172,62 @@ int m2c_batch_status(m2c_batch *b, int i, m2c_status *st);
  * the state, keeps the fastest; the state, clocks and counters are restored */
 int m2c_batch_autotune(m2c_batch *b, int trial_steps, m2c_batch_tuning *out);
 int m2c_batch_set_tuning(m2c_batch *b, int kchunk, int blocks);
+
+/* Learned interface solver (no counterpart in the reference). A 10-256-256-256-256-3 SiLU network
+ * predicts the star state of a two-stiffened-gas Riemann problem along a mesh axis; the predicted
+ * p* is polished by at most `newton_max` Newton updates of the closed-form stiffened-gas wave curves
+ * and accepted only when the velocity-continuity residual |u*_L(p) - u*_R(p)| / max(c_L, c_R) is
+ * <= riemann.surrogate_residual_tol. Otherwise ExactRiemannSolverBase::ComputeRiemannSolution
+ * (ExactRiemannSolverBase.h:35-39) runs as in the plain path. Network inputs: ln(rho_R/rho_L)/7,
+ * ln(P_R/P_L)/9, (u_R-u_L)/(c_L+c_R), ln(c_R/c_L)/5, ln gam_L, ln gam_R, pinf_L/P_L, pinf_R/P_R,
+ * ln(Z_R/Z_L)/7 and ln((p_acoustic + min pinf)/sqrt(P_L P_R))/4 with P = p + pinf, Z = rho c;
+ * outputs ln((p* + min pinf)/sqrt(P_L P_R))/4, (u* - (u_L+u_R)/2)/(c_L+c_R), ln(rho*_L/rho_L)/4.
+ * Parameters: W0 [256][10], b0, W1..W3 [256][256], b1..b3, W4 [3][256], b4 (FP32, flat, 200963);
+ * the kernels compute in bf16 MFMA with FP32 accumulation. */
+typedef struct m2c_mlp m2c_mlp;
+typedef struct {
+  long long n_samples;              /* training samples from the exact solver (a power of two) */
+  long long n_holdout;              /* held-out samples (multiple of 64; 0 = none) */
+  unsigned long long seed;
+  int batch;                        /* samples per Adam step (multiple of 64, <= n_samples) */
+  int epochs;
+  double lr, lr_min;                /* Adam step size, cosine-decayed to lr_min over the run */
+  int newton_max;                   /* Newton updates allowed after a prediction (held-out check) */
+  double tol;                       /* continuity-residual acceptance (held-out check) */
+} m2c_train_config;
+typedef struct {
+  double data_ms;                   /* exact-solver sample generation */
+  double train_ms;                  /* training steps (HIP events) */
+  double samples_per_s;             /* training throughput */
+  long long steps, valid_samples;
+  double first_epoch_loss, last_epoch_loss, holdout_loss;   /* mean squared error per sample */
+  double holdout_accept_mlp;        /* share of held-out problems accepted from the network's p* */
+  double holdout_accept_acoustic;   /* ... from the linearised (acoustic) p*, same Newton budget */
+  double holdout_iters_mlp, holdout_iters_acoustic;   /* mean Newton updates to reach tol (cap 50) */
+} m2c_train_report;
+int m2c_mlp_create(int device, unsigned long long seed, m2c_mlp **out);
+int m2c_mlp_destroy(m2c_mlp *net);
+int m2c_mlp_nparams(void);
+int m2c_mlp_get_params(m2c_mlp *net, float *params);
+int m2c_mlp_set_params(m2c_mlp *net, const float *params);   /* also resets the Adam moments */
+/* Y[n][3] = net(X[n][10]) (host arrays) */
+int m2c_mlp_forward(m2c_mlp *net, int n, const float *X, float *Y);
+/* loss = sum (y - t)^2 / n over rows and outputs (rows with NaN targets excluded) and its gradient
+ * (flat parameter layout); n a multiple of 64 */
+int m2c_mlp_gradient(m2c_mlp *net, int n, const float *X, const float *T, double *loss, float *grad);
+/* n training samples generated by the exact solver: X [n][10], T [n][3] (NaN where it fails),
+ * raw [n][14] = rho,u,p,gam,pinf of the left and right states, p*, u*, rho*_L, rho*_R (or NULL) */
+int m2c_mlp_samples(m2c_mlp *net, long long n, unsigned long long seed, float *X, float *T, double *raw);
+int m2c_mlp_train(m2c_mlp *net, const m2c_train_config *cfg, m2c_train_report *report);
+/* average duration (ms) of one inference launch over n device-resident rows */
+int m2c_mlp_bench_infer(m2c_mlp *net, int n, int reps, double *ms_per_launch);
+/* Interface faces of `ctx` use the network (NULL: the exact solver only). The network must stay
+ * alive while attached. Requires two-material interfaces and MultiPhase.Flux = numerical. */
+int m2c_set_surrogate(m2c_ctx *ctx, const m2c_mlp *net, int newton_max);
+/* m2c_riemann_batch through the network with the exact fallback; path[t] = 1 network, 0 exact */
+int m2c_surrogate_riemann_batch(m2c_ctx *ctx, const m2c_mlp *net, int newton_max, int n, const int *dir,
+                                const double *Vm, const int *idm, const double *Vp, const int *idp, double *Vs,
+                                int *id, double *Vsm, double *Vsp, int *status, int *path);
 
 /* Timing helper: runs `nsteps` CFL-adaptive time steps (no tmax) and reports the average
  * duration (ms) of one fused stage-kernel launch and of one whole step, measured with HIP events

@@ -57,7 +57,25 @@ class Status(C.Structure):
     _fields_ = [("error", C.c_int), ("reverted_faces", C.c_longlong), ("clipped_cells", C.c_longlong),
                 ("riemann_errors", C.c_longlong), ("steps", C.c_longlong), ("t", C.c_double),
                 ("dt", C.c_double), ("cfl", C.c_double), ("reinit_iterations", C.c_longlong),
-                ("unresolved_cells", C.c_longlong)]
+                ("unresolved_cells", C.c_longlong), ("surrogate_accepted", C.c_longlong),
+                ("surrogate_fallbacks", C.c_longlong)]
+
+
+class TrainConfig(C.Structure):
+    _fields_ = [("n_samples", C.c_longlong), ("n_holdout", C.c_longlong), ("seed", C.c_ulonglong),
+                ("batch", C.c_int), ("epochs", C.c_int), ("lr", C.c_double), ("lr_min", C.c_double),
+                ("newton_max", C.c_int), ("tol", C.c_double)]
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("data_ms", C.c_double), ("train_ms", C.c_double), ("samples_per_s", C.c_double),
+                ("steps", C.c_longlong), ("valid_samples", C.c_longlong), ("first_epoch_loss", C.c_double),
+                ("last_epoch_loss", C.c_double), ("holdout_loss", C.c_double),
+                ("holdout_accept_mlp", C.c_double), ("holdout_accept_acoustic", C.c_double),
+                ("holdout_iters_mlp", C.c_double), ("holdout_iters_acoustic", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def material(kind, params, rhomin=0.0, pmin=-DBL_MAX, rhomax=DBL_MAX, pmax=DBL_MAX, failsafe_density=1e-10):
@@ -154,6 +172,18 @@ def lib():
         L.m2c_batch_status.argtypes = [Pb, C.c_int, C.POINTER(Status)]
         L.m2c_batch_autotune.argtypes = [Pb, C.c_int, C.POINTER(BatchTuning)]
         L.m2c_batch_set_tuning.argtypes = [Pb, C.c_int, C.c_int]
+        F = C.POINTER(C.c_float)
+        L.m2c_mlp_create.argtypes = [C.c_int, C.c_ulonglong, C.POINTER(P)]
+        L.m2c_mlp_destroy.argtypes = [P]
+        L.m2c_mlp_get_params.argtypes = [P, F]
+        L.m2c_mlp_set_params.argtypes = [P, F]
+        L.m2c_mlp_forward.argtypes = [P, C.c_int, F, F]
+        L.m2c_mlp_gradient.argtypes = [P, C.c_int, F, F, D, F]
+        L.m2c_mlp_samples.argtypes = [P, C.c_longlong, C.c_ulonglong, F, F, D]
+        L.m2c_mlp_train.argtypes = [P, C.POINTER(TrainConfig), C.POINTER(TrainReport)]
+        L.m2c_mlp_bench_infer.argtypes = [P, C.c_int, C.c_int, D]
+        L.m2c_set_surrogate.argtypes = [P, P, C.c_int]
+        L.m2c_surrogate_riemann_batch.argtypes = [P, P, C.c_int, C.c_int, I, D, I, D, I, D, I, D, D, I, I]
         L.m2c_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -233,6 +263,26 @@ class Solver:
         _chk(self.L.m2c_riemann_batch(self.h, n, cp(d), _dp(Vm), cp(im), _dp(Vp), cp(iq), _dp(Vs), cp(mid),
                                       _dp(Vsm), _dp(Vsp), cp(st)), "m2c_riemann_batch")
         return st, Vs, mid, Vsm, Vsp
+
+    def set_surrogate(self, net, newton_max=3):
+        """Interface faces use the learned solver `net` (an Mlp; None = exact solver only)."""
+        self._net = net   # kept alive while attached
+        _chk(self.L.m2c_set_surrogate(self.h, net.h if net is not None else None, newton_max), "m2c_set_surrogate")
+
+    def surrogate_riemann(self, net, dirs, Vm, idm, Vp, idp, newton_max=3):
+        """riemann() through the learned solver with the exact fallback: (status, Vs, id, Vsm, Vsp, path)."""
+        Vm = np.ascontiguousarray(np.atleast_2d(Vm), dtype=np.float64)
+        Vp = np.ascontiguousarray(np.atleast_2d(Vp), dtype=np.float64)
+        n = Vm.shape[0]
+        ip = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+        d, im, iq = ip(dirs), ip(idm), ip(idp)
+        Vs, Vsm, Vsp = np.empty_like(Vm), np.empty_like(Vm), np.empty_like(Vm)
+        mid, st, path = np.empty(n, np.int32), np.empty(n, np.int32), np.empty(n, np.int32)
+        cp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        _chk(self.L.m2c_surrogate_riemann_batch(self.h, net.h, newton_max, n, cp(d), _dp(Vm), cp(im), _dp(Vp),
+                                                cp(iq), _dp(Vs), cp(mid), _dp(Vsm), _dp(Vsp), cp(st), cp(path)),
+             "m2c_surrogate_riemann_batch")
+        return st, Vs, mid, Vsm, Vsp, path
 
     def residual(self):
         R = np.empty(self.shape + (5,), dtype=np.float64)
@@ -338,3 +388,75 @@ class Batch:
 
     def set_tuning(self, kchunk, blocks):
         _chk(self.L.m2c_batch_set_tuning(self.h, kchunk, blocks), "m2c_batch_set_tuning")
+
+
+NPARAM = 200963
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The learned interface solver's network (10-256x4-3 SiLU) on one GPU."""
+
+    def __init__(self, seed=1, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        _chk(self.L.m2c_mlp_create(device, seed, C.byref(self.h)), "m2c_mlp_create")
+
+    def close(self):
+        if self.h:
+            self.L.m2c_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.empty(NPARAM, np.float32)
+        _chk(self.L.m2c_mlp_get_params(self.h, _fp(p)), "m2c_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (NPARAM,)
+        _chk(self.L.m2c_mlp_set_params(self.h, _fp(p)), "m2c_mlp_set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], 3), np.float32)
+        _chk(self.L.m2c_mlp_forward(self.h, X.shape[0], _fp(X), _fp(Y)), "m2c_mlp_forward")
+        return Y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(NPARAM, np.float32)
+        loss = C.c_double()
+        _chk(self.L.m2c_mlp_gradient(self.h, X.shape[0], _fp(X), _fp(T), C.byref(loss), _fp(g)), "m2c_mlp_gradient")
+        return loss.value, g
+
+    def samples(self, n, seed, raw=True):
+        X = np.empty((n, 10), np.float32)
+        T = np.empty((n, 3), np.float32)
+        R = np.empty((n, 14), np.float64) if raw else None
+        _chk(self.L.m2c_mlp_samples(self.h, n, seed, _fp(X), _fp(T), _dp(R) if raw else None), "m2c_mlp_samples")
+        return X, T, R
+
+    def train(self, n_samples=1 << 20, n_holdout=1 << 16, seed=7, batch=8192, epochs=4, lr=2e-3, lr_min=1e-5,
+              newton_max=2, tol=1e-6):
+        c = TrainConfig()
+        c.n_samples, c.n_holdout, c.seed, c.batch, c.epochs = n_samples, n_holdout, seed, batch, epochs
+        c.lr, c.lr_min, c.newton_max, c.tol = lr, lr_min, newton_max, tol
+        r = TrainReport()
+        _chk(self.L.m2c_mlp_train(self.h, C.byref(c), C.byref(r)), "m2c_mlp_train")
+        return r
+
+    def bench_infer(self, n, reps=20):
+        ms = C.c_double()
+        _chk(self.L.m2c_mlp_bench_infer(self.h, n, reps, C.byref(ms)), "m2c_mlp_bench_infer")
+        return ms.value

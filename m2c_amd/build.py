@@ -28,7 +28,7 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
 VARIANTS = [("hllc_sg", 2, 0, "double"), ("hllc_gen", 2, -1, "double"), ("roe_gen", 0, -1, "double"),
             ("llf_gen", 1, -1, "double"), ("hllc_sg_f32", 2, 0, "float"), ("hllc_gen_f32", 2, -1, "float"),
             ("roe_gen_f32", 0, -1, "float"), ("llf_gen_f32", 1, -1, "float")]
-UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", [])] + [
+UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", []), ("mlp", "mlp.hip", [])] + [
     (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"])
     for t, f, e, tc in VARIANTS]
 

@@ -53,6 +53,8 @@ struct m2c_ctx {
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
 
+static int ensure_riemann_constants(m2c_ctx *c);
+
 // ---------------------------------------------------------------- small kernels
 
 __global__ void k_clock_init(Clock *clk, double t, double tmax, double cfl, int maxsteps) {
@@ -865,8 +867,89 @@ int m2c_get_status(m2c_ctx *c, m2c_status *out) {
     HIPCHK(hipMemcpy(&h, c->L.ms, sizeof(MpState), hipMemcpyDeviceToHost));
     out->reinit_iterations = h.reinit_iters;
     out->unresolved_cells = h.unresolved_total;
+    out->surrogate_accepted = (long long)h.sur_accepted;
+    out->surrogate_fallbacks = (long long)h.sur_fallback;
   }
   return M2C_OK;
+}
+
+int m2c_set_surrogate(m2c_ctx *c, const m2c_mlp *net, int newton_max) {
+  if (!c || newton_max < 0) return M2C_ERR_ARG;
+  if (!net) {
+    c->L.sur_on = 0;
+    return M2C_OK;
+  }
+  if (!c->iface) return M2C_ERR_ARG;                      // no material interfaces in this context
+  if (c->cfg.mp_flux != 0) return M2C_ERR_UNSUPPORTED;    // the network serves the numerical-flux form
+  MlpWeights w;
+  int dev = -1;
+  if (m2c_mlp_get_weights(net, &w, &dev)) return M2C_ERR_ARG;
+  if (dev != c->cfg.device) return M2C_ERR_ARG;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  if (!c->L.surX) {
+    const long long cap = std::min<long long>(3 * c->Nf, 1ll << 22);
+    int rc;
+    if ((rc = mp_alloc(c, &c->L.surX, (size_t)cap * MLP_IN, 0))) return rc;
+    if ((rc = mp_alloc(c, &c->L.surY, (size_t)cap * MLP_OUT, 0))) return rc;
+    c->L.sur_cap = (int)cap;
+  }
+  const double tol = c->cfg.riemann.surrogate_residual_tol;
+  c->L.sur_w = w;
+  c->L.sur_nmax = newton_max;
+  c->L.sur_tol = tol > 0.0 ? tol : 1e-6;
+  c->L.sur_on = 1;
+  return M2C_OK;
+}
+
+int m2c_surrogate_riemann_batch(m2c_ctx *c, const m2c_mlp *net, int newton_max, int n, const int *dir,
+                                const double *Vm, const int *idm, const double *Vp, const int *idp, double *Vs,
+                                int *id, double *Vsm, double *Vsp, int *status, int *path) {
+  if (!c || !net || newton_max < 0 || n < 0 ||
+      (n > 0 && (!dir || !Vm || !idm || !Vp || !idp || !Vs || !id || !Vsm || !Vsp || !status || !path)))
+    return M2C_ERR_ARG;
+  if (n == 0) return M2C_OK;
+  for (int t = 0; t < n; t++) {
+    if (dir[t] < 0 || dir[t] > 2) return M2C_ERR_ARG;
+    if (idm[t] < 0 || idm[t] >= c->cfg.nmat || idp[t] < 0 || idp[t] >= c->cfg.nmat) return M2C_ERR_ARG;
+  }
+  MlpWeights w;
+  int dev = -1;
+  if (m2c_mlp_get_weights(net, &w, &dev) || dev != c->cfg.device) return M2C_ERR_ARG;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  if (int rc0 = ensure_riemann_constants(c)) return rc0;
+  const double tol = c->cfg.riemann.surrogate_residual_tol > 0.0 ? c->cfg.riemann.surrogate_residual_tol : 1e-6;
+  const int nthreads = (int)std::min<long long>(4096, ((long long)n + 63) / 64 * 64);
+  const size_t b5 = sizeof(double) * 5 * (size_t)n, bi = sizeof(int) * (size_t)n;
+  const size_t bxy = sizeof(float) * (size_t)n * (MLP_IN + MLP_OUT);
+  const size_t sbytes = sizeof(double) * 6 * (size_t)c->mp.path_cap * nthreads;
+  char *d = nullptr;
+  HIPCHK(hipMalloc(&d, 5 * b5 + 6 * bi + sbytes + bxy + 64));
+  double *dVm = (double *)d, *dVp = dVm + 5 * (size_t)n, *dVs = dVp + 5 * (size_t)n, *dVsm = dVs + 5 * (size_t)n,
+         *dVsp = dVsm + 5 * (size_t)n;
+  double *scratch = dVsp + 5 * (size_t)n;
+  int *ddir = (int *)(scratch + 6 * (size_t)c->mp.path_cap * nthreads), *didm = ddir + n, *didp = didm + n,
+      *did = didp + n, *dst = did + n, *dpath = dst + n;
+  float *dX = (float *)(dpath + n), *dY = dX + (size_t)n * MLP_IN;
+  int rc = M2C_OK;
+  if (hipMemcpyAsync(dVm, Vm, b5, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(dVp, Vp, b5, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(ddir, dir, bi, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(didm, idm, bi, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(didp, idp, bi, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = M2C_ERR_HIP;
+  if (!rc && m2c_mp_launch_sur_batch(c->stream, n, ddir, dVm, didm, dVp, didp, dX, dY, w, newton_max, tol, dVs, did,
+                                     dVsm, dVsp, dst, dpath, scratch, nthreads))
+    rc = M2C_ERR_HIP;
+  if (!rc && (hipMemcpyAsync(Vs, dVs, b5, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(Vsm, dVsm, b5, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(Vsp, dVsp, b5, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(id, did, bi, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(status, dst, bi, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(path, dpath, bi, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = M2C_ERR_HIP;
+  hipFree(d);
+  return rc;
 }
 
 int m2c_upload_levelset(m2c_ctx *c, const double *phi) {
@@ -922,6 +1005,24 @@ int m2c_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const double *V
   return M2C_OK;
 }
 
+// the exact solver's constants live with the interface kernels: a context without interfaces
+// uploads them before a pointwise batch
+static int ensure_riemann_constants(m2c_ctx *c) {
+  if (c->iface || c->ls) return M2C_OK;
+  MpParams mp;
+  memset(&mp, 0, sizeof(mp));
+  const m2c_riemann_params &rp = c->cfg.riemann;
+  mp.rcfg.maxIts_main = rp.maxIts_main; mp.rcfg.maxIts_bracket = rp.maxIts_bracket;
+  mp.rcfg.maxIts_shock = rp.maxIts_shock; mp.rcfg.numSteps_rarefaction = rp.numSteps_rarefaction;
+  mp.rcfg.tol_main = rp.tol_main; mp.rcfg.tol_shock = rp.tol_shock; mp.rcfg.tol_rarefaction = rp.tol_rarefaction;
+  mp.rcfg.min_pressure = rp.min_pressure; mp.rcfg.failure_threshold = rp.failure_threshold;
+  mp.rcfg.pressure_at_failure = rp.pressure_at_failure;
+  mp.path_cap = 1024;
+  c->mp = mp;
+  if (m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
+  return M2C_OK;
+}
+
 int m2c_riemann_batch(m2c_ctx *c, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
                       const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status) {
   if (!c || n < 0 || (n > 0 && (!dir || !Vm || !idm || !Vp || !idp || !Vs || !id || !Vsm || !Vsp || !status)))
@@ -932,20 +1033,7 @@ int m2c_riemann_batch(m2c_ctx *c, int n, const int *dir, const double *Vm, const
     if (idm[t] < 0 || idm[t] >= c->cfg.nmat || idp[t] < 0 || idp[t] >= c->cfg.nmat) return M2C_ERR_ARG;
   }
   if (int rc0 = ensure_constants(c)) return rc0;
-  if (!c->iface && !c->ls) {
-    // the solver constants live with the interface kernels: upload them for this context
-    MpParams mp;
-    memset(&mp, 0, sizeof(mp));
-    const m2c_riemann_params &rp = c->cfg.riemann;
-    mp.rcfg.maxIts_main = rp.maxIts_main; mp.rcfg.maxIts_bracket = rp.maxIts_bracket;
-    mp.rcfg.maxIts_shock = rp.maxIts_shock; mp.rcfg.numSteps_rarefaction = rp.numSteps_rarefaction;
-    mp.rcfg.tol_main = rp.tol_main; mp.rcfg.tol_shock = rp.tol_shock; mp.rcfg.tol_rarefaction = rp.tol_rarefaction;
-    mp.rcfg.min_pressure = rp.min_pressure; mp.rcfg.failure_threshold = rp.failure_threshold;
-    mp.rcfg.pressure_at_failure = rp.pressure_at_failure;
-    mp.path_cap = 1024;
-    c->mp = mp;
-    if (m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
-  }
+  if (int rc0 = ensure_riemann_constants(c)) return rc0;
   const int nthreads = (int)std::min<long long>(4096, ((long long)n + 63) / 64 * 64);
   size_t b5 = sizeof(double) * 5 * (size_t)n, bi = sizeof(int) * (size_t)n;
   size_t sbytes = sizeof(double) * 6 * (size_t)c->mp.path_cap * nthreads;

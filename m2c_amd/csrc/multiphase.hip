@@ -29,6 +29,7 @@
 #include "fv_kernels.hpp"
 #include "multiphase.hpp"
 #include "riemann_dev.hpp"
+#include "sg_star.hpp"
 
 namespace m2c {
 
@@ -307,6 +308,89 @@ __global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *
   }
 }
 
+// ---------------------------------------------------------------- learned interface solver
+
+// the two sides of a mesh-axis face as stiffened gases (normal velocity along axis d); false unless
+// both materials are stiffened gases with admissible states
+__device__ __forceinline__ bool sur_sides(const double *vm, int idm, const double *vp, int idp, int d, SgSide &L,
+                                          SgSide &R) {
+  const DevEos &ml = c_eos[idm], &mr = c_eos[idp];
+  if (ml.type != EOS_SG || mr.type != EOS_SG) return false;
+  L.rho = vm[0]; L.u = vm[1 + d]; L.p = vm[4]; L.gam = ml.gam; L.pinf = ml.pstiff;
+  R.rho = vp[0]; R.u = vp[1 + d]; R.p = vp[4]; R.gam = mr.gam; R.pinf = mr.pstiff;
+  return sg_side_init(L) && sg_side_init(R);
+}
+
+// Star states of a face from the network's prediction y: the predicted p* is polished by at most
+// nmax Newton updates of the closed-form stiffened-gas relations and accepted when the velocity
+// continuity residual is <= tol. Outputs follow ComputeRiemannSolution's conventions
+// (ExactRiemannSolverBase.cpp:347-514): one-sided star states with the tangential velocity of their
+// side, the state at xi = 0 (closed-form sampling, Toro 4.5) and its material. 1 = accepted.
+__device__ inline int sur_face(int d, const double *vm, int idm, const double *vp, int idp, const float *y, int nmax,
+                               double tol, double *Vs, int *mid, double *Vsm, double *Vsp) {
+  SgSide L, R;
+  if (!sur_sides(vm, idm, vp, idp, d, L, R)) return 0;
+  float x[MLP_IN];
+  SgScales sc;
+  sg_features(L, R, x, &sc);
+  double ps, res;
+  int it;
+  if (!sg_newton(L, R, sg_decode_p(sc, y[0]), nmax, tol, &ps, &res, &it)) return 0;
+  double fl, dfl, fr, dfr;
+  sg_wave(L, ps, fl, dfl);
+  sg_wave(R, ps, fr, dfr);
+  const double us = 0.5 * (L.u + R.u) + 0.5 * (fr - fl);
+  const double rl = sg_star_rho(L, ps), rr = sg_star_rho(R, ps);
+  if (!(rl > 0.0) || !(rr > 0.0) || !__builtin_isfinite(us)) return 0;
+  double w[3];
+  sg_sample0(L, R, ps, us, rl, rr, w);
+  double utl[3] = {vm[1], vm[2], vm[3]}, utr[3] = {vp[1], vp[2], vp[3]};
+  utl[d] = 0.0;
+  utr[d] = 0.0;
+  *mid = us >= 0 ? idm : idp;
+  Vsm[0] = rl; Vsp[0] = rr; Vs[0] = w[0];
+  Vsm[4] = ps; Vsp[4] = ps; Vs[4] = w[2];
+  for (int q = 0; q < 3; q++) {
+    const double un = q == d ? 1.0 : 0.0;
+    Vsm[1 + q] = utl[q] + us * un;
+    Vsp[1 + q] = utr[q] + us * un;
+    const double ut = us > 0 ? utl[q] : (us < 0 ? utr[q] : 0.5 * (utl[q] + utr[q]));
+    Vs[1 + q] = w[1] * un + ut;
+  }
+  return 1;
+}
+
+// network inputs of the listed interface faces (the first `cap`); faces without two stiffened
+// gases get zeros and take the exact solver
+template <int IN_CONS>
+__global__ __launch_bounds__(256) void k_iface_feat(const double *S, const int8_t *ID, const unsigned *faces,
+                                                     const MpState *ms, float *X, int cap, const Clock *clk,
+                                                     int ignore_active) {
+  if (!ignore_active && !step_active(clk)) return;
+  const Grid &g = c_grid;
+  const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  const unsigned n = min(ms->nface, (unsigned)cap);
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const unsigned code = faces[e];
+    const int dir = (int)(code / Nf);
+    const long long key = code - (long long)dir * Nf;
+    const int i = (int)(key % (g.nx + 1));
+    const long long r = key / (g.nx + 1);
+    const int j = (int)(r % (g.ny + 1));
+    const int k = (int)(r / (g.ny + 1));
+    double vm[5], vp[5];
+    int idm, idp;
+    unsigned fl = 0;
+    fetch_cell<IN_CONS, -1>(S, ID, i - (dir == 0), j - (dir == 1), k - (dir == 2), vm, idm, fl);
+    fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp, idp, fl);
+    float x[MLP_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    SgSide L, R;
+    SgScales sc;
+    if (sur_sides(vm, idm, vp, idp, dir, L, R)) sg_features(L, R, x, &sc);
+    for (int q = 0; q < MLP_IN; q++) X[(size_t)e * MLP_IN + q] = x[q];
+  }
+}
+
 // ---------------------------------------------------------------- interface Riemann pass
 
 template <int FLUX, int DIR>
@@ -329,7 +413,8 @@ template <int FLUX, int IN_CONS>
 __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *ID, const unsigned *faces,
                                                  MpState *ms, double *Fi, double *rsV, unsigned long long *rsMeta,
                                                  int store, double *scratch, Status *st, const Clock *clk,
-                                                 int ignore_active) {
+                                                 int ignore_active, const float *pred, int sur_cap, int sur_nmax,
+                                                 double sur_tol) {
   if (!ignore_active && !step_active(clk)) return;
   const Grid &g = c_grid;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
@@ -350,7 +435,7 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
   S_.path3.stride = T;
   S_.path3.n = 0;
   unsigned flags = 0;
-  unsigned long long nerr = 0;
+  unsigned long long nerr = 0, nacc = 0, nfb = 0;
   const unsigned long long stamp = ms->epoch;
   for (unsigned e = tid; e < n; e += T) {
     const unsigned code = faces[e];
@@ -370,7 +455,18 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
     nrm[dir] = 1.0;
     double Vmid[5], Vsm[5], Vsp[5];
     int midid = 0;
-    int err = riemann_solve(S_, nrm, vm, idm, vp, idp, Vmid, &midid, Vsm, Vsp);
+    int err;
+    // a face with equal normal velocity and pressure on both sides is the exact solver's immediate
+    // return (ExactRiemannSolverBase.cpp:129), taken as it is
+    const bool trivial = vm[1 + dir] == vp[1 + dir] && vm[4] == vp[4];
+    if (pred && !trivial && e < (unsigned)sur_cap &&
+        sur_face(dir, vm, idm, vp, idp, pred + (size_t)e * MLP_OUT, sur_nmax, sur_tol, Vmid, &midid, Vsm, Vsp)) {
+      err = 0;
+      nacc++;
+    } else {
+      if (pred && !trivial) nfb++;
+      err = riemann_solve(S_, nrm, vm, idm, vp, idp, Vmid, &midid, Vsm, Vsp);
+    }
     if (err < 0) flags |= ST_BAD_STATE;
     if (err) nerr++;
     eos_clip<double, -1>(c_eos[idm], Vsm);
@@ -397,6 +493,8 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
   }
   if (S_.overflow) flags |= ST_RIEMANN_PATH;
   if (nerr) atomicAdd(&st->riemann_err, nerr);
+  if (nacc) atomicAdd(&ms->sur_accepted, nacc);
+  if (nfb) atomicAdd(&ms->sur_fallback, nfb);
   if (flags & (ST_BAD_STATE | ST_RIEMANN_PATH | ST_CLIP_INTERMEDIATE)) atomicOr(&st->bits, flags);
 }
 
@@ -1080,6 +1178,54 @@ __global__ __launch_bounds__(64) void k_riemann_batch(int n, const int *dir, con
   }
 }
 
+// pointwise batch of the learned solver: network inputs of every problem
+__global__ void k_sur_batch_feat(int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
+                                 const int *idp, float *X) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    float x[MLP_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    SgSide L, R;
+    SgScales sc;
+    if (sur_sides(Vm + 5 * t, idm[t], Vp + 5 * t, idp[t], dir[t], L, R)) sg_features(L, R, x, &sc);
+    for (int q = 0; q < MLP_IN; q++) X[(size_t)t * MLP_IN + q] = x[q];
+  }
+}
+
+// ... then the prediction, checked, with the exact solver as fallback; path[t] = 1 network, 0 exact
+__global__ __launch_bounds__(64) void k_sur_batch(int n, const int *dir, const double *Vm, const int *idm,
+                                                  const double *Vp, const int *idp, const float *Y, int nmax,
+                                                  double tol, double *Vs, int *id, double *Vsm, double *Vsp,
+                                                  int *status, int *path, double *scratch, int nthreads) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nthreads) return;
+  RSolver S_;
+  S_.mat = c_eos;
+  S_.cfg = c_mp.rcfg;
+  S_.overflow = 0;
+  const int cap = c_mp.path_cap;
+  S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
+  S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
+  S_.path3.n = 0;
+  for (int t = tid; t < n; t += nthreads) {
+    int mid = 0;
+    const double *vm = Vm + 5 * t, *vp = Vp + 5 * t;
+    const bool trivial = vm[1 + dir[t]] == vp[1 + dir[t]] && vm[4] == vp[4];   // the exact solver's immediate return
+    if (!trivial && sur_face(dir[t], vm, idm[t], vp, idp[t], Y + (size_t)t * MLP_OUT, nmax, tol, Vs + 5 * t, &mid,
+                             Vsm + 5 * t, Vsp + 5 * t)) {
+      id[t] = mid;
+      status[t] = 0;
+      path[t] = 1;
+      continue;
+    }
+    double nrm[3] = {0.0, 0.0, 0.0};
+    nrm[dir[t]] = 1.0;
+    S_.overflow = 0;
+    int r = riemann_solve(S_, nrm, Vm + 5 * t, idm[t], Vp + 5 * t, idp[t], Vs + 5 * t, &mid, Vsm + 5 * t, Vsp + 5 * t);
+    id[t] = mid;
+    status[t] = S_.overflow ? 2 : r;
+    path[t] = 0;
+  }
+}
+
 static int hchk(hipError_t e) { return e == hipSuccess ? 0 : 2; }
 
 }  // namespace m2c
@@ -1105,13 +1251,25 @@ extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_co
   hipLaunchKernelGGL(k_iface_list, sweep_grid(L.nx + 1, L.ny + 1, L.nz + 1), dim3(SWEEP_BX, SWEEP_BY), 0, L.stream,
                      L.ID, L.faces, L.ms, L.clk, ignore_active);
   dim3 gr(L.riemann_threads / 64), bl(64);
+  const float *pred = nullptr;
+  if (L.sur_on) {   // network inputs and predictions of the listed faces (count read on the device)
+    const dim3 gf(nblocks(L.sur_cap, 256, 4096));
+    if (in_cons)
+      hipLaunchKernelGGL((k_iface_feat<1>), gf, dim3(256), 0, L.stream, S, L.ID, L.faces, L.ms, L.surX, L.sur_cap,
+                         L.clk, ignore_active);
+    else
+      hipLaunchKernelGGL((k_iface_feat<0>), gf, dim3(256), 0, L.stream, S, L.ID, L.faces, L.ms, L.surX, L.sur_cap,
+                         L.clk, ignore_active);
+    if (m2c_mlp_launch_infer(L.sur_w, L.stream, L.surX, L.surY, L.sur_cap, &L.ms->nface)) return 2;
+    pred = L.surY;
+  }
 #define M2C_RIEMANN(F)                                                                                          \
   if (in_cons)                                                                                                  \
     hipLaunchKernelGGL((k_riemann<F, 1>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,   \
-                       store, L.scratch, L.st, L.clk, ignore_active);                                           \
+                       store, L.scratch, L.st, L.clk, ignore_active, pred, L.sur_cap, L.sur_nmax, L.sur_tol);   \
   else                                                                                                          \
     hipLaunchKernelGGL((k_riemann<F, 0>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,   \
-                       store, L.scratch, L.st, L.clk, ignore_active);
+                       store, L.scratch, L.st, L.clk, ignore_active, pred, L.sur_cap, L.sur_nmax, L.sur_tol);
   if (L.flux == FLUX_HLLC) { M2C_RIEMANN(FLUX_HLLC) }
   else if (L.flux == FLUX_ROE) { M2C_RIEMANN(FLUX_ROE) }
   else { M2C_RIEMANN(FLUX_LLF) }
@@ -1124,6 +1282,18 @@ extern "C" int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir,
                                            double *Vsp, int *status, double *scratch, int nthreads) {
   hipLaunchKernelGGL(k_riemann_batch, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Vs, id,
                      Vsm, Vsp, status, scratch, nthreads);
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_sur_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,
+                                       const double *Vp, const int *idp, float *X, float *Y, const MlpWeights &w,
+                                       int nmax, double tol, double *Vs, int *id, double *Vsm, double *Vsp,
+                                       int *status, int *path, double *scratch, int nthreads) {
+  hipLaunchKernelGGL(k_sur_batch_feat, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, dir, Vm, idm, Vp, idp, X);
+  if (hchk(hipGetLastError())) return 2;
+  if (m2c_mlp_launch_infer(w, s, X, Y, n, nullptr)) return 2;
+  hipLaunchKernelGGL(k_sur_batch, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Y, nmax, tol,
+                     Vs, id, Vsm, Vsp, status, path, scratch, nthreads);
   return hchk(hipGetLastError());
 }
 

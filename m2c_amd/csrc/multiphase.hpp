@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "fv_kernels.hpp"
+#include "mlp.hpp"
 #include "riemann_dev.hpp"
 
 namespace m2c {
@@ -48,6 +49,7 @@ struct MpState {
   int nseq, pad;
   ReinitState rs;
   long long reinit_iters, unresolved_total;
+  unsigned long long sur_accepted, sur_fallback;   // interface faces solved by the network / the exact solver
 };
 
 // device buffers and launch parameters of one context
@@ -72,11 +74,20 @@ struct MpLaunch {
   int8_t *cnt;
   double *Vnew;
   unsigned *unres;
+  // learned interface solver (sur_on): features and predictions of the first sur_cap listed faces
+  int sur_on, sur_cap, sur_nmax;
+  double sur_tol;
+  float *surX, *surY;
+  MlpWeights sur_w;
 };
 
 }  // namespace m2c
 
 extern "C" {
+int m2c_mp_launch_sur_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
+                            const int *idp, float *X, float *Y, const m2c::MlpWeights &w, int nmax, double tol,
+                            double *Vs, int *id, double *Vsm, double *Vsp, int *status, int *path, double *scratch,
+                            int nthreads);
 int m2c_mp_upload(const m2c::Grid &grid, const m2c::DevEos *eos, const m2c::MpParams &mp);
 int m2c_mp_step_begin(const m2c::MpLaunch &L);
 int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,
