@@ -8,7 +8,15 @@ and the device-side dt update.
 
 --workload shock_bubble: the 384x192x192 water/air shock-bubble (exact two-material Riemann
 solves at the interface faces, level-set advection, reinitialisation every 5 steps, material-ID
-and state update of swept cells), RK2, CFL 0.5, FP64; a step includes all of it.
+and state update of swept cells), RK2, CFL 0.5, FP64, 300 steps; a step includes all of it. The
+mixed-precision form, and both forms with the learned interface solver (parameters in
+m2c_amd/data/), are run beside it with their L1 differences.
+
+--workload batched_riemann: 64 independent instances (edges 48..160), mixed precision, one
+persistent launch per RK stage, hipGraph step replay, autotuned launch shape.
+
+--workload learned: the learned interface solver's kernels (inference roofline against the bf16
+MFMA peak, training steps) and its per-problem cost against the exact solver.
 
 Prints one JSON line with the throughput, the roofline of the fused stage kernel (algorithmic
 bytes per launch divided by its average duration, measured with HIP events on the library's own
@@ -55,11 +63,14 @@ def cpu_baseline(settings, V0, ID0, nsteps, phi0=None):
                       f"{el:.1f} s wall, C oracle -O3 with {threads} OpenMP threads"}
 
 
-def mixed_form(M, W, settings, V0, ID0, phi0, args, V64, value64):
-    """The mixed-precision form on the same workload and step count: throughput and the relative
-    L1 error per field of its final state against the FP64 run's."""
+def variant_form(M, W, settings, V0, ID0, phi0, args, V64, value64, precision, net=None):
+    """The same workload and step count in another form (mixed precision and/or the learned interface
+    solver): throughput and the relative L1 difference per field of its final state against the
+    FP64 run's."""
     cells = V0.shape[0] * V0.shape[1] * V0.shape[2]
-    s = M.Solver(W.config_from(settings, precision=M.PREC_MIXED))
+    s = M.Solver(W.config_from(settings, precision=precision))
+    if net is not None:
+        s.set_surrogate(net, newton_max=3)
     s.upload(V0, ID0)
     if phi0 is not None:
         s.upload_levelset(phi0)
@@ -70,20 +81,26 @@ def mixed_form(M, W, settings, V0, ID0, phi0, args, V64, value64):
     st = s.run(1e300, args.steps, sync_every=0)
     s.L.m2c_synchronize(s.h)
     el = time.perf_counter() - t0
-    V32, _ = s.download()
+    Vx, _ = s.download()
     ms_stage, _ = s.time_steps(max(5, min(args.steps, 20)))
     s.close()
     names = ["rho", "u", "v", "w", "p"]
     l1 = {}
     for f in range(5):
         den = float(np.abs(V64[..., f]).sum())
-        l1[names[f]] = float(np.abs(V32[..., f] - V64[..., f]).sum() / den) if den > 0 else 0.0
+        l1[names[f]] = float(np.abs(Vx[..., f] - V64[..., f]).sum() / den) if den > 0 else 0.0
     v = cells * args.steps / el
-    print(f"bench: mixed form {args.steps} steps in {el:.3f} s, L1(rho) {l1['rho']:.2e}", file=sys.stderr, flush=True)
-    return {"value": v, "unit": "cell-updates/s", "speedup_vs_fp64": v / value64, "status": st.error,
-            "l1_rel_err_vs_fp64": l1, "l1_bar": 1e-5, "avg_stage_launch_ms": ms_stage,
-            "form": "fp32 reconstruction/limiters/flux; fp64 fetch+C2P, accumulation, RK combination, CFL, "
-                    "exact Riemann"}
+    tag = ("mixed" if precision == M.PREC_MIXED else "fp64") + ("+surrogate" if net is not None else "")
+    print(f"bench: {tag} form {args.steps} steps in {el:.3f} s, L1(rho) {l1['rho']:.2e}", file=sys.stderr, flush=True)
+    out = {"value": v, "unit": "cell-updates/s", "ms_per_step": el / args.steps * 1e3, "speedup_vs_fp64": v / value64,
+           "status": st.error, "l1_rel_err_vs_fp64": l1, "l1_bar": 1e-5, "avg_stage_launch_ms": ms_stage}
+    if precision == M.PREC_MIXED:
+        out["form"] = ("fp32 reconstruction/limiters/flux; fp64 fetch+C2P, accumulation, RK combination, CFL, "
+                       "exact Riemann")
+    if net is not None:
+        out["network_solves"] = st.surrogate_accepted
+        out["exact_fallbacks"] = st.surrogate_fallbacks
+    return out
 
 
 def bench_batched(M, W, args):
@@ -133,17 +150,94 @@ def bench_batched(M, W, args):
     print(json.dumps(line), flush=True)
 
 
+MLP_FLOP_PER_SAMPLE = 2.0 * (10 * 256 + 3 * 256 * 256 + 256 * 3)   # forward, 399 872
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def bench_learned(M, W, args):
+    """The learned interface solver: fused bf16 MFMA inference over 2^20 device-resident rows
+    (roofline against the dense bf16 MFMA peak), training steps (batch 8192), and the per-problem
+    cost of the checked network path against the exact solver on water/air interface problems."""
+    net = M.load_surrogate()
+    n = 1 << 20
+    ms_inf = net.bench_infer(n, reps=max(args.steps // 10, 10))
+    inf_tflops = MLP_FLOP_PER_SAMPLE * n / (ms_inf * 1e-3) / 1e12
+    trainer = M.Mlp(seed=3)
+    B = 8192
+    ms_train = trainer.bench_train(B, steps=max(args.steps // 4, 20))
+    trainer.close()
+    train_tflops = 3.0 * MLP_FLOP_PER_SAMPLE * B / (ms_train * 1e-3) / 1e12
+    print(f"bench: inference {ms_inf:.3f} ms per 2^20 rows ({inf_tflops:.0f} TFLOP/s), training step "
+          f"{ms_train:.3f} ms (batch {B})", file=sys.stderr, flush=True)
+    settings, _, _, _ = W.shock_bubble(scale=0.125)
+    s = M.Solver(W.config_from(settings))
+    nb = 1 << 16
+    Vm, idm, Vp, idp = W.interface_problems(nb)
+    dirs = np.zeros(nb, np.int32)
+    s.riemann(dirs[:64], Vm[:64], idm[:64], Vp[:64], idp[:64])   # warm
+    t0 = time.perf_counter()
+    st_e = s.riemann(dirs, Vm, idm, Vp, idp)[0]
+    t1 = time.perf_counter()
+    st_s, _, _, _, _, path = s.surrogate_riemann(net, dirs, Vm, idm, Vp, idp, newton_max=3)
+    t2 = time.perf_counter()
+    # the problems the network accepts, through both paths (whole-batch times are set by the few
+    # slowest exact solves, which both paths run)
+    sel = np.nonzero(path == 1)[0]
+    t3 = time.perf_counter()
+    s.riemann(dirs[sel], Vm[sel], idm[sel], Vp[sel], idp[sel])
+    t4 = time.perf_counter()
+    s.surrogate_riemann(net, dirs[sel], Vm[sel], idm[sel], Vp[sel], idp[sel], newton_max=3)
+    t5 = time.perf_counter()
+    s.close()
+    net.close()
+    line = {"metric": "learned interface solver: inference samples/s (bf16 MFMA, FP32 accumulation)",
+            "value": n / (ms_inf * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_inf, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+            "config": {"workload": "mlp_10-256x4-3_silu_inference_2^20_rows", "rows": n,
+                       "flop_per_row": MLP_FLOP_PER_SAMPLE},
+            "roofline": {"bound": "mfma", "achieved": inf_tflops, "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": inf_tflops / BF16_MFMA_PEAK_TFLOPS, "traffic": None,
+                         "kernel": "m2c::k_mlp_fwd<0> (fused 5-layer MLP)", "avg_launch_ms": ms_inf},
+            "training": {"batch": B, "ms_per_step": ms_train, "samples_per_s": B / (ms_train * 1e-3),
+                         "tflops_3x_forward": train_tflops},
+            "interface_problems": {"n": nb, "exact_batch_ms": 1e3 * (t1 - t0), "network_batch_ms": 1e3 * (t2 - t1),
+                                   "accepted_only": {"n": int(sel.size), "exact_batch_ms": 1e3 * (t4 - t3),
+                                                     "network_batch_ms": 1e3 * (t5 - t4)},
+                                   "network_share": float((path == 1).mean()),
+                                   "exact_failures": int((st_e != 0).sum()), "checked_failures": int((st_s != 0).sum()),
+                                   "note": "host-inclusive pointwise batches (copies in and out)"}}
+    if not args.no_cpu:
+        from oracle import mlp_ref as R
+        p = M.load_surrogate()
+        params = p.params().astype(np.float64)
+        p.close()
+        X = np.random.default_rng(1).normal(size=(1 << 16, 10))
+        t0 = time.perf_counter()
+        R.forward(params, X, emulate=False)
+        el = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": X.shape[0] / el, "unit": "samples/s",
+                                "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                                "kind": "port", "sample": f"{X.shape[0]} rows through the numpy oracle's FP64 forward "
+                                                          f"({el:.2f} s)"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps (0: 200 for the periodic case, 300 for the shock-bubble, 100 batched)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0: 8 for the periodic case, 10 for the shock-bubble)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="periodic_modes", choices=["periodic_modes", "shock_bubble", "batched_riemann"])
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
+    ap.add_argument("--workload", default="periodic_modes",
+                    choices=["periodic_modes", "shock_bubble", "batched_riemann", "learned"])
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / surrogate comparison runs")
     args = ap.parse_args()
+    if args.steps <= 0:
+        args.steps = {"shock_bubble": 300, "batched_riemann": 100}.get(args.workload, 200)
     if args.cpu_steps <= 0:
         args.cpu_steps = 10 if args.workload == "shock_bubble" else 8
     if args.gpus != 1:
@@ -156,6 +250,8 @@ def main():
 
     if args.workload == "batched_riemann":
         return bench_batched(M, W, args)
+    if args.workload == "learned":
+        return bench_learned(M, W, args)
     mp = args.workload == "shock_bubble"
     if mp:
         settings, V0, ID0, phi0 = W.shock_bubble()
@@ -209,7 +305,12 @@ def main():
                        "measured_copy_GBs": copy_gbs, "copy_kernel_ms_per_GiB": copy_ms},
             "roofline": roof}
     if not args.no_mixed:
-        line["mixed"] = mixed_form(M, W, settings, V0, ID0, phi0 if mp else None, args, V64, value)
+        line["mixed"] = variant_form(M, W, settings, V0, ID0, phi0 if mp else None, args, V64, value, M.PREC_MIXED)
+        if mp and os.path.exists(M.SURROGATE_PARAMS):
+            net = M.load_surrogate()
+            line["fp64_surrogate"] = variant_form(M, W, settings, V0, ID0, phi0, args, V64, value, M.PREC_FP64, net)
+            line["mixed_surrogate"] = variant_form(M, W, settings, V0, ID0, phi0, args, V64, value, M.PREC_MIXED, net)
+            net.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(settings, V0, ID0, args.cpu_steps, phi0 if mp else None)
     print(json.dumps(line), flush=True)

@@ -194,6 +194,7 @@ typedef struct {
   double lr, lr_min;                /* Adam step size, cosine-decayed to lr_min over the run */
   int newton_max;                   /* Newton updates allowed after a prediction (held-out check) */
   double tol;                       /* continuity-residual acceptance (held-out check) */
+  int verbose;                      /* progress lines on stderr */
 } m2c_train_config;
 typedef struct {
   double data_ms;                   /* exact-solver sample generation */
@@ -221,6 +222,9 @@ int m2c_mlp_samples(m2c_mlp *net, long long n, unsigned long long seed, float *X
 int m2c_mlp_train(m2c_mlp *net, const m2c_train_config *cfg, m2c_train_report *report);
 /* average duration (ms) of one inference launch over n device-resident rows */
 int m2c_mlp_bench_infer(m2c_mlp *net, int n, int reps, double *ms_per_launch);
+/* average duration (ms) of one training step (forward, backward, weight gradients, Adam) with
+ * batch B on device-resident synthetic rows; the parameters change */
+int m2c_mlp_bench_train(m2c_mlp *net, int B, int steps, double *ms_per_step);
 /* Interface faces of `ctx` use the network (NULL: the exact solver only). The network must stay
  * alive while attached. Requires two-material interfaces and MultiPhase.Flux = numerical. */
 int m2c_set_surrogate(m2c_ctx *ctx, const m2c_mlp *net, int newton_max);

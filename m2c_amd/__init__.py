@@ -64,7 +64,7 @@ class Status(C.Structure):
 class TrainConfig(C.Structure):
     _fields_ = [("n_samples", C.c_longlong), ("n_holdout", C.c_longlong), ("seed", C.c_ulonglong),
                 ("batch", C.c_int), ("epochs", C.c_int), ("lr", C.c_double), ("lr_min", C.c_double),
-                ("newton_max", C.c_int), ("tol", C.c_double)]
+                ("newton_max", C.c_int), ("tol", C.c_double), ("verbose", C.c_int)]
 
 
 class TrainReport(C.Structure):
@@ -182,6 +182,7 @@ def lib():
         L.m2c_mlp_samples.argtypes = [P, C.c_longlong, C.c_ulonglong, F, F, D]
         L.m2c_mlp_train.argtypes = [P, C.POINTER(TrainConfig), C.POINTER(TrainReport)]
         L.m2c_mlp_bench_infer.argtypes = [P, C.c_int, C.c_int, D]
+        L.m2c_mlp_bench_train.argtypes = [P, C.c_int, C.c_int, D]
         L.m2c_set_surrogate.argtypes = [P, P, C.c_int]
         L.m2c_surrogate_riemann_batch.argtypes = [P, P, C.c_int, C.c_int, I, D, I, D, I, D, I, D, D, I, I]
         L.m2c_version.restype = C.c_char_p
@@ -448,15 +449,32 @@ class Mlp:
         return X, T, R
 
     def train(self, n_samples=1 << 20, n_holdout=1 << 16, seed=7, batch=8192, epochs=4, lr=2e-3, lr_min=1e-5,
-              newton_max=2, tol=1e-6):
+              newton_max=2, tol=1e-6, verbose=False):
         c = TrainConfig()
         c.n_samples, c.n_holdout, c.seed, c.batch, c.epochs = n_samples, n_holdout, seed, batch, epochs
-        c.lr, c.lr_min, c.newton_max, c.tol = lr, lr_min, newton_max, tol
+        c.lr, c.lr_min, c.newton_max, c.tol, c.verbose = lr, lr_min, newton_max, tol, int(verbose)
         r = TrainReport()
         _chk(self.L.m2c_mlp_train(self.h, C.byref(c), C.byref(r)), "m2c_mlp_train")
         return r
+
+    def bench_train(self, batch, steps=50):
+        ms = C.c_double()
+        _chk(self.L.m2c_mlp_bench_train(self.h, batch, steps, C.byref(ms)), "m2c_mlp_bench_train")
+        return ms.value
 
     def bench_infer(self, n, reps=20):
         ms = C.c_double()
         _chk(self.L.m2c_mlp_bench_infer(self.h, n, reps, C.byref(ms)), "m2c_mlp_bench_infer")
         return ms.value
+
+
+SURROGATE_PARAMS = os.path.join(HERE, "data", "sg_riemann_mlp.npy")
+
+
+def load_surrogate(path=SURROGATE_PARAMS, device=0):
+    """The learned interface solver with the parameters written by m2c_amd/train_surrogate.py."""
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: run m2c_amd/train_surrogate.py on the GPU")
+    net = Mlp(seed=0, device=device)
+    net.set_params(np.load(path, allow_pickle=False))
+    return net

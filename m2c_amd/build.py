@@ -28,8 +28,14 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
 VARIANTS = [("hllc_sg", 2, 0, "double"), ("hllc_gen", 2, -1, "double"), ("roe_gen", 0, -1, "double"),
             ("llf_gen", 1, -1, "double"), ("hllc_sg_f32", 2, 0, "float"), ("hllc_gen_f32", 2, -1, "float"),
             ("roe_gen_f32", 0, -1, "float"), ("llf_gen_f32", 1, -1, "float")]
-UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", []), ("mlp", "mlp.hip", [])] + [
-    (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"])
+# The mixed-form variants (FP32 reconstruction and fluxes) take the hardware's FP32 reciprocal and
+# square root instead of the correctly rounded sequences; FP64 code is not affected by this flag.
+# batch.hip holds the same FP32 stage bodies and takes it too (batched and single runs stay equal).
+FP32_FAST = ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", FP32_FAST),
+         ("mlp", "mlp.hip", [])] + [
+    (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"]
+     + (FP32_FAST if tc == "float" else []))
     for t, f, e, tc in VARIANTS]
 
 

@@ -648,7 +648,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
 
 #ifndef M2C_BATCHED
 template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double>
-__global__ __launch_bounds__(NTHR, 2) void stage_kernel(StageArgs a) {
+#ifndef M2C_STAGE_BLOCKS
+#define M2C_STAGE_BLOCKS 2   // resident 256-thread blocks per CU the register budget is sized for
+#endif
+__global__ __launch_bounds__(NTHR, M2C_STAGE_BLOCKS) void stage_kernel(StageArgs a) {
   stage_body<IN_CONS, MODE, FLUX, EK, TC>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 #endif

@@ -891,6 +891,7 @@ int m2c_set_surrogate(m2c_ctx *c, const m2c_mlp *net, int newton_max) {
     int rc;
     if ((rc = mp_alloc(c, &c->L.surX, (size_t)cap * MLP_IN, 0))) return rc;
     if ((rc = mp_alloc(c, &c->L.surY, (size_t)cap * MLP_OUT, 0))) return rc;
+    if ((rc = mp_alloc(c, &c->L.surFb, (size_t)cap, 0))) return rc;
     c->L.sur_cap = (int)cap;
   }
   const double tol = c->cfg.riemann.surrogate_residual_tol;
@@ -923,22 +924,25 @@ int m2c_surrogate_riemann_batch(m2c_ctx *c, const m2c_mlp *net, int newton_max, 
   const size_t bxy = sizeof(float) * (size_t)n * (MLP_IN + MLP_OUT);
   const size_t sbytes = sizeof(double) * 6 * (size_t)c->mp.path_cap * nthreads;
   char *d = nullptr;
-  HIPCHK(hipMalloc(&d, 5 * b5 + 6 * bi + sbytes + bxy + 64));
+  HIPCHK(hipMalloc(&d, 5 * b5 + 7 * bi + sbytes + bxy + 64));
   double *dVm = (double *)d, *dVp = dVm + 5 * (size_t)n, *dVs = dVp + 5 * (size_t)n, *dVsm = dVs + 5 * (size_t)n,
          *dVsp = dVsm + 5 * (size_t)n;
   double *scratch = dVsp + 5 * (size_t)n;
   int *ddir = (int *)(scratch + 6 * (size_t)c->mp.path_cap * nthreads), *didm = ddir + n, *didp = didm + n,
       *did = didp + n, *dst = did + n, *dpath = dst + n;
-  float *dX = (float *)(dpath + n), *dY = dX + (size_t)n * MLP_IN;
+  unsigned *dfb = (unsigned *)(dpath + n);
+  float *dX = (float *)(dfb + n), *dY = dX + (size_t)n * MLP_IN;
+  unsigned *dnfb = (unsigned *)(dY + (size_t)n * MLP_OUT);
   int rc = M2C_OK;
-  if (hipMemcpyAsync(dVm, Vm, b5, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+  if (hipMemsetAsync(dnfb, 0, sizeof(unsigned), c->stream) != hipSuccess) rc = M2C_ERR_HIP;
+  if (rc || hipMemcpyAsync(dVm, Vm, b5, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(dVp, Vp, b5, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(ddir, dir, bi, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(didm, idm, bi, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(didp, idp, bi, hipMemcpyHostToDevice, c->stream) != hipSuccess)
     rc = M2C_ERR_HIP;
   if (!rc && m2c_mp_launch_sur_batch(c->stream, n, ddir, dVm, didm, dVp, didp, dX, dY, w, newton_max, tol, dVs, did,
-                                     dVsm, dVsp, dst, dpath, scratch, nthreads))
+                                     dVsm, dVsp, dst, dpath, scratch, nthreads, dfb, dnfb))
     rc = M2C_ERR_HIP;
   if (!rc && (hipMemcpyAsync(Vs, dVs, b5, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
               hipMemcpyAsync(Vsm, dVsm, b5, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

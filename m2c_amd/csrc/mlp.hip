@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -352,6 +353,7 @@ struct SampleArgs {
   double *scratch;
   int cap, nthreads;
   unsigned long long *next;  // work counter (zero at launch): waves claim 64 samples at a time
+  long long s0;              // index of the first sample (the random stream is per global index)
 };
 
 // Waves claim 64 consecutive samples at a time (solve times vary by orders of magnitude). Sides are a gas (gamma 1.1-1.7, pinf 0) or a liquid
@@ -378,22 +380,23 @@ __global__ __launch_bounds__(64) void k_sg_samples(SampleArgs a) {
     const long long s = (long long)base + lane;
     if (s < a.n) {   // the claim above is reached by the whole wave
       const unsigned long long sd = a.seed;
-      const bool liqL = u01(sd, s, 0) < 0.5, liqR = u01(sd, s, 1) < 0.5;
+      const long long sg = a.s0 + s;   // global sample index: the random draws
+      const bool liqL = u01(sd, sg, 0) < 0.5, liqR = u01(sd, sg, 1) < 0.5;
       SgSide L, R;
-      L.gam = liqL ? 4.0 + 3.5 * u01(sd, s, 2) : 1.1 + 0.6 * u01(sd, s, 2);
-      R.gam = liqR ? 4.0 + 3.5 * u01(sd, s, 3) : 1.1 + 0.6 * u01(sd, s, 3);
-      L.pinf = liqL ? pow(10.0, 8.0 + u01(sd, s, 4)) : 0.0;
-      R.pinf = liqR ? pow(10.0, 8.0 + u01(sd, s, 5)) : 0.0;
-      L.p = pow(10.0, 4.0 + 5.0 * u01(sd, s, 6));
-      R.p = L.p * pow(10.0, 4.0 * (2.0 * u01(sd, s, 7) - 1.0));
-      L.rho = pow(10.0, -1.0 + 4.0 * u01(sd, s, 8));
-      R.rho = L.rho * pow(10.0, 3.0 * (2.0 * u01(sd, s, 9) - 1.0));
+      L.gam = liqL ? 4.0 + 3.5 * u01(sd, sg, 2) : 1.1 + 0.6 * u01(sd, sg, 2);
+      R.gam = liqR ? 4.0 + 3.5 * u01(sd, sg, 3) : 1.1 + 0.6 * u01(sd, sg, 3);
+      L.pinf = liqL ? pow(10.0, 8.0 + u01(sd, sg, 4)) : 0.0;
+      R.pinf = liqR ? pow(10.0, 8.0 + u01(sd, sg, 5)) : 0.0;
+      L.p = pow(10.0, 4.0 + 5.0 * u01(sd, sg, 6));
+      R.p = L.p * pow(10.0, 4.0 * (2.0 * u01(sd, sg, 7) - 1.0));
+      L.rho = pow(10.0, -1.0 + 4.0 * u01(sd, sg, 8));
+      R.rho = L.rho * pow(10.0, 3.0 * (2.0 * u01(sd, sg, 9) - 1.0));
       L.u = 0.0; R.u = 0.0;
       sg_side_init(L);
       sg_side_init(R);
       const double cs = L.c + R.c;
-      L.u = cs * (u01(sd, s, 11) - 0.5);
-      R.u = L.u + cs * (2.0 * u01(sd, s, 10) - 1.0);
+      L.u = cs * (u01(sd, sg, 11) - 0.5);
+      R.u = L.u + cs * (2.0 * u01(sd, sg, 10) - 1.0);
       sg_eos(mats[0], L.gam, L.pinf);
       sg_eos(mats[1], R.gam, R.pinf);
       const double Vm[5] = {L.rho, L.u, 0.0, 0.0, L.p}, Vp[5] = {R.rho, R.u, 0.0, 0.0, R.p};
@@ -750,19 +753,33 @@ static RiemannCfg default_rcfg() {
   return c;
 }
 
-static int gen_samples_dev(hipStream_t s, long long n, unsigned long long seed, float *X, float *T, double *raw) {
+// samples in launches of at most 2^19 (a progress line on stderr after each when verbose)
+static int gen_samples_dev(hipStream_t s, long long n, unsigned long long seed, float *X, float *T, double *raw,
+                           int verbose = 0) {
   // one wave per SIMD at this kernel's register use: 4 SIMDs x 256 CUs x 64 lanes
   const int cap = 1024, nthreads = 65536;
+  const long long chunk = 1ll << 19;
   double *scratch = nullptr;
   const size_t sb = sizeof(double) * 6 * (size_t)cap * nthreads;
   MCHK(hipMalloc(&scratch, sb + 256));
   unsigned long long *next = (unsigned long long *)((char *)scratch + sb);
-  MCHK(hipMemsetAsync(next, 0, sizeof(unsigned long long), s));
-  SampleArgs a;
-  a.n = n; a.seed = seed; a.X = X; a.T = T; a.raw = raw; a.cfg = default_rcfg(); a.scratch = scratch;
-  a.cap = cap; a.nthreads = nthreads; a.next = next;
-  hipLaunchKernelGGL(k_sg_samples, dim3(nthreads / 64), dim3(64), 0, s, a);
-  int rc = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess ? M2C_OK : M2C_ERR_HIP;
+  int rc = M2C_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  for (long long s0 = 0; s0 < n && !rc; s0 += chunk) {
+    const long long m = n - s0 < chunk ? n - s0 : chunk;
+    if (hipMemsetAsync(next, 0, sizeof(unsigned long long), s) != hipSuccess) { rc = M2C_ERR_HIP; break; }
+    SampleArgs a;
+    a.n = m; a.seed = seed; a.X = X + (size_t)s0 * MLP_IN; a.T = T + (size_t)s0 * MLP_OUT;
+    a.raw = raw ? raw + (size_t)s0 * RAW : nullptr; a.cfg = default_rcfg(); a.scratch = scratch;
+    a.cap = cap; a.nthreads = nthreads; a.next = next; a.s0 = s0;
+    hipLaunchKernelGGL(k_sg_samples, dim3(nthreads / 64), dim3(64), 0, s, a);
+    rc = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess ? M2C_OK : M2C_ERR_HIP;
+    if (verbose) {
+      const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      fprintf(stderr, "m2c_mlp: %lld / %lld exact-solver samples, %.1f s\n", s0 + m, n, el);
+      fflush(stderr);
+    }
+  }
   hipFree(scratch);
   return rc;
 }
@@ -807,10 +824,10 @@ int m2c_mlp_train(m2c_mlp *m, const m2c_train_config *cfg, m2c_train_report *rep
   unsigned long long *dcnt = (unsigned long long *)(d + bx + bt + br + by);
   int rc = M2C_OK;
   auto t0 = std::chrono::steady_clock::now();
-  rc = gen_samples_dev(m->stream, N, cfg->seed, dX, dT, nullptr);
+  rc = gen_samples_dev(m->stream, N, cfg->seed, dX, dT, nullptr, cfg->verbose);
   if (!rc && NH > 0)
     rc = gen_samples_dev(m->stream, NH, cfg->seed ^ 0x5eed5eed5eedull, dX + (size_t)N * MLP_IN,
-                         dT + (size_t)N * MLP_OUT, dR);
+                         dT + (size_t)N * MLP_OUT, dR, cfg->verbose);
   auto t1 = std::chrono::steady_clock::now();
   rep->data_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   // valid training samples
@@ -856,6 +873,10 @@ int m2c_mlp_train(m2c_mlp *m, const m2c_train_config *cfg, m2c_train_report *rep
     const double el = (double)hl / (double)(steps_per_epoch * B);
     if (ep == 0) rep->first_epoch_loss = el;
     rep->last_epoch_loss = el;
+    if (cfg->verbose) {
+      fprintf(stderr, "m2c_mlp: epoch %d loss %.6f (%.1f ms)\n", ep + 1, el, ms);
+      fflush(stderr);
+    }
   }
   rep->steps = step;
   rep->train_ms = step_ms_total;
@@ -919,6 +940,51 @@ int m2c_mlp_bench_infer(m2c_mlp *m, int n, int reps, double *ms_per_launch) {
               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
     rc = M2C_ERR_HIP;
   *ms_per_launch = ms / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(d);
+  return rc;
+}
+
+__global__ void k_fill_synthetic(float *X, float *T, long long n, unsigned long long seed) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    for (int j = 0; j < MLP_IN; j++) X[t * MLP_IN + j] = (float)(2.0 * u01(seed, t, j) - 1.0);
+    for (int j = 0; j < MLP_OUT; j++) T[t * MLP_OUT + j] = (float)(2.0 * u01(seed, t, 16 + j) - 1.0);
+  }
+}
+
+// average duration (ms) of one training step (forward, backward, weight gradients, reduction, Adam)
+// over `steps` steps with batch B on device-resident synthetic rows (the parameters change)
+int m2c_mlp_bench_train(m2c_mlp *m, int B, int steps, double *ms_per_step) {
+  if (!m || B <= 0 || B % MLP_TM || steps <= 0 || !ms_per_step) return M2C_ERR_ARG;
+  MCHK(hipSetDevice(m->device));
+  if (int rc = mlp_ensure_ws(m, B)) return rc;
+  const long long N = (long long)B * 8;
+  float *d = nullptr;
+  MCHK(hipMalloc(&d, sizeof(float) * (size_t)N * (MLP_IN + MLP_OUT)));
+  float *dT = d + (size_t)N * MLP_IN;
+  hipLaunchKernelGGL(k_fill_synthetic, dim3(1024), dim3(256), 0, m->stream, d, dT, N, 7ull);
+  const NetPtrs np = net_ptrs(m->net);
+  hipEvent_t e0, e1;
+  MCHK(hipEventCreate(&e0));
+  MCHK(hipEventCreate(&e1));
+  int rc = M2C_OK;
+  float ms = 0.f;
+  for (int k = -2; k < steps && !rc; k++) {   // two untimed steps first
+    if (k == 0 && hipEventRecord(e0, m->stream) != hipSuccess) rc = M2C_ERR_HIP;
+    const long long kk = k < 0 ? k + 2 : k;
+    const FwdArgs fa = train_args(m, d, dT, 1ull, 0ull, (unsigned long long)(N - 1), (kk % 8) * B, 2.0f / B, nullptr);
+    if (!rc) rc = mlp_grad_step(m, fa);
+    m->net.adam_t++;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)m->net.adam_t)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)m->net.adam_t)));
+    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, np, m->net.master,
+                       m->net.adam_m, m->net.adam_v, m->net.grad, 1e-4f, 0.9f, 0.999f, 1e-8f, c1, c2);
+  }
+  if (!rc && (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = M2C_ERR_HIP;
+  *ms_per_step = ms / steps;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipFree(d);

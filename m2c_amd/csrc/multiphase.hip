@@ -271,7 +271,10 @@ __global__ void k_mp_step_begin(MpState *ms, const Clock *clk) {
   ms->nseq = 0;
 }
 
-__global__ void k_mp_face_reset(MpState *ms) { ms->nface = 0; }
+__global__ void k_mp_face_reset(MpState *ms) {
+  ms->nface = 0;
+  ms->nfb = 0;
+}
 
 // interface faces of the current IDs: (dir, plus cell) with different active materials on the two
 // sides; code = dir*Nf + face_key
@@ -409,18 +412,22 @@ __device__ __forceinline__ void iface_fluxes(const double *vm, const double *Vsm
 
 // One thread per listed face (grid-stride). store != 0: keep the star states of this stage as
 // the step's Riemann solutions (first stage of a step).
-template <int FLUX, int IN_CONS>
+// PASS 0: the exact solver on every listed face. With the learned solver, PASS 1 takes the
+// network's checked solution where it is accepted (trivial faces and faces beyond the prediction
+// buffer go to the exact solver inline) and lists the rest in fbl; PASS 2 runs the exact solver on
+// that list, so waves of the expensive iterative solver hold only the faces that need it.
+template <int FLUX, int IN_CONS, int PASS>
 __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *ID, const unsigned *faces,
                                                  MpState *ms, double *Fi, double *rsV, unsigned long long *rsMeta,
                                                  int store, double *scratch, Status *st, const Clock *clk,
                                                  int ignore_active, const float *pred, int sur_cap, int sur_nmax,
-                                                 double sur_tol) {
+                                                 double sur_tol, unsigned *fbl) {
   if (!ignore_active && !step_active(clk)) return;
   const Grid &g = c_grid;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   const int T = gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned n = ms->nface;
+  const unsigned n = PASS == 2 ? ms->nfb : ms->nface;
   RSolver S_;
   S_.mat = c_eos;
   S_.cfg = c_mp.rcfg;
@@ -437,7 +444,8 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
   unsigned flags = 0;
   unsigned long long nerr = 0, nacc = 0, nfb = 0;
   const unsigned long long stamp = ms->epoch;
-  for (unsigned e = tid; e < n; e += T) {
+  for (unsigned t = tid; t < n; t += T) {
+    const unsigned e = PASS == 2 ? fbl[t] : t;
     const unsigned code = faces[e];
     const int dir = (int)(code / Nf);
     const long long key = code - (long long)dir * Nf;
@@ -455,16 +463,25 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
     nrm[dir] = 1.0;
     double Vmid[5], Vsm[5], Vsp[5];
     int midid = 0;
-    int err;
-    // a face with equal normal velocity and pressure on both sides is the exact solver's immediate
-    // return (ExactRiemannSolverBase.cpp:129), taken as it is
-    const bool trivial = vm[1 + dir] == vp[1 + dir] && vm[4] == vp[4];
-    if (pred && !trivial && e < (unsigned)sur_cap &&
-        sur_face(dir, vm, idm, vp, idp, pred + (size_t)e * MLP_OUT, sur_nmax, sur_tol, Vmid, &midid, Vsm, Vsp)) {
-      err = 0;
-      nacc++;
+    int err = 0;
+    if (PASS == 1) {
+      // a face with equal normal velocity and pressure on both sides is the exact solver's
+      // immediate return (ExactRiemannSolverBase.cpp:129), taken as it is
+      const bool trivial = vm[1 + dir] == vp[1 + dir] && vm[4] == vp[4];
+      const bool in_buf = e < (unsigned)sur_cap;
+      bool ok = false;
+      if (!trivial && in_buf)
+        ok = sur_face(dir, vm, idm, vp, idp, pred + (size_t)e * MLP_OUT, sur_nmax, sur_tol, Vmid, &midid, Vsm, Vsp);
+      const bool defer = !trivial && in_buf && !ok;
+      const unsigned slot = wave_append(&ms->nfb, defer);   // every active lane of the wave reaches it
+      if (defer) {
+        fbl[slot] = e;
+        continue;
+      }
+      if (ok) nacc++;
+      else err = riemann_solve(S_, nrm, vm, idm, vp, idp, Vmid, &midid, Vsm, Vsp);
     } else {
-      if (pred && !trivial) nfb++;
+      if (PASS == 2) nfb++;
       err = riemann_solve(S_, nrm, vm, idm, vp, idp, Vmid, &midid, Vsm, Vsp);
     }
     if (err < 0) flags |= ST_BAD_STATE;
@@ -1190,13 +1207,35 @@ __global__ void k_sur_batch_feat(int n, const int *dir, const double *Vm, const 
   }
 }
 
-// ... then the prediction, checked, with the exact solver as fallback; path[t] = 1 network, 0 exact
+// ... then the prediction, checked (PASS 1; failures listed in fbl), and the exact solver on the
+// listed problems (PASS 2); path[t] = 1 network, 0 exact
+template <int PASS>
 __global__ __launch_bounds__(64) void k_sur_batch(int n, const int *dir, const double *Vm, const int *idm,
                                                   const double *Vp, const int *idp, const float *Y, int nmax,
                                                   double tol, double *Vs, int *id, double *Vsm, double *Vsp,
-                                                  int *status, int *path, double *scratch, int nthreads) {
+                                                  int *status, int *path, double *scratch, int nthreads,
+                                                  unsigned *fbl, unsigned *nfb) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= nthreads) return;
+  if (PASS == 1) {
+    for (int t = tid; t < n; t += nthreads) {
+      int mid = 0;
+      const double *vm = Vm + 5 * t, *vp = Vp + 5 * t;
+      const bool trivial = vm[1 + dir[t]] == vp[1 + dir[t]] && vm[4] == vp[4];   // the exact solver's immediate return
+      const bool ok = !trivial && sur_face(dir[t], vm, idm[t], vp, idp[t], Y + (size_t)t * MLP_OUT, nmax, tol,
+                                           Vs + 5 * t, &mid, Vsm + 5 * t, Vsp + 5 * t);
+      const unsigned slot = wave_append(nfb, !ok);
+      if (ok) {
+        id[t] = mid;
+        status[t] = 0;
+        path[t] = 1;
+      } else {
+        fbl[slot] = (unsigned)t;
+      }
+    }
+    return;
+  }
+  const int nl = (int)*nfb;
   RSolver S_;
   S_.mat = c_eos;
   S_.cfg = c_mp.rcfg;
@@ -1205,17 +1244,9 @@ __global__ __launch_bounds__(64) void k_sur_batch(int n, const int *dir, const d
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
   S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
   S_.path3.n = 0;
-  for (int t = tid; t < n; t += nthreads) {
+  for (int q = tid; q < nl; q += nthreads) {
+    const int t = (int)fbl[q];
     int mid = 0;
-    const double *vm = Vm + 5 * t, *vp = Vp + 5 * t;
-    const bool trivial = vm[1 + dir[t]] == vp[1 + dir[t]] && vm[4] == vp[4];   // the exact solver's immediate return
-    if (!trivial && sur_face(dir[t], vm, idm[t], vp, idp[t], Y + (size_t)t * MLP_OUT, nmax, tol, Vs + 5 * t, &mid,
-                             Vsm + 5 * t, Vsp + 5 * t)) {
-      id[t] = mid;
-      status[t] = 0;
-      path[t] = 1;
-      continue;
-    }
     double nrm[3] = {0.0, 0.0, 0.0};
     nrm[dir[t]] = 1.0;
     S_.overflow = 0;
@@ -1263,17 +1294,24 @@ extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_co
     if (m2c_mlp_launch_infer(L.sur_w, L.stream, L.surX, L.surY, L.sur_cap, &L.ms->nface)) return 2;
     pred = L.surY;
   }
+#define M2C_RIEMANN_P(F, IC, P)                                                                                 \
+  hipLaunchKernelGGL((k_riemann<F, IC, P>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,    \
+                     store, L.scratch, L.st, L.clk, ignore_active, pred, L.sur_cap, L.sur_nmax, L.sur_tol, L.surFb)
+#define M2C_RIEMANN_IC(F, IC)                                                                                   \
+  if (pred) {                                                                                                   \
+    M2C_RIEMANN_P(F, IC, 1);                                                                                    \
+    M2C_RIEMANN_P(F, IC, 2);                                                                                    \
+  } else {                                                                                                      \
+    M2C_RIEMANN_P(F, IC, 0);                                                                                    \
+  }
 #define M2C_RIEMANN(F)                                                                                          \
-  if (in_cons)                                                                                                  \
-    hipLaunchKernelGGL((k_riemann<F, 1>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,   \
-                       store, L.scratch, L.st, L.clk, ignore_active, pred, L.sur_cap, L.sur_nmax, L.sur_tol);   \
-  else                                                                                                          \
-    hipLaunchKernelGGL((k_riemann<F, 0>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,   \
-                       store, L.scratch, L.st, L.clk, ignore_active, pred, L.sur_cap, L.sur_nmax, L.sur_tol);
+  if (in_cons) { M2C_RIEMANN_IC(F, 1) } else { M2C_RIEMANN_IC(F, 0) }
   if (L.flux == FLUX_HLLC) { M2C_RIEMANN(FLUX_HLLC) }
   else if (L.flux == FLUX_ROE) { M2C_RIEMANN(FLUX_ROE) }
   else { M2C_RIEMANN(FLUX_LLF) }
 #undef M2C_RIEMANN
+#undef M2C_RIEMANN_IC
+#undef M2C_RIEMANN_P
   return hchk(hipGetLastError());
 }
 
@@ -1288,12 +1326,15 @@ extern "C" int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir,
 extern "C" int m2c_mp_launch_sur_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,
                                        const double *Vp, const int *idp, float *X, float *Y, const MlpWeights &w,
                                        int nmax, double tol, double *Vs, int *id, double *Vsm, double *Vsp,
-                                       int *status, int *path, double *scratch, int nthreads) {
+                                       int *status, int *path, double *scratch, int nthreads, unsigned *fbl,
+                                       unsigned *nfb) {
   hipLaunchKernelGGL(k_sur_batch_feat, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, dir, Vm, idm, Vp, idp, X);
   if (hchk(hipGetLastError())) return 2;
   if (m2c_mlp_launch_infer(w, s, X, Y, n, nullptr)) return 2;
-  hipLaunchKernelGGL(k_sur_batch, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Y, nmax, tol,
-                     Vs, id, Vsm, Vsp, status, path, scratch, nthreads);
+  hipLaunchKernelGGL(k_sur_batch<1>, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Y, nmax,
+                     tol, Vs, id, Vsm, Vsp, status, path, scratch, nthreads, fbl, nfb);
+  hipLaunchKernelGGL(k_sur_batch<2>, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Y, nmax,
+                     tol, Vs, id, Vsm, Vsp, status, path, scratch, nthreads, fbl, nfb);
   return hchk(hipGetLastError());
 }
 

@@ -50,6 +50,7 @@ struct MpState {
   ReinitState rs;
   long long reinit_iters, unresolved_total;
   unsigned long long sur_accepted, sur_fallback;   // interface faces solved by the network / the exact solver
+  unsigned nfb, pad2;          // listed faces whose network solution failed its check (this stage)
 };
 
 // device buffers and launch parameters of one context
@@ -78,6 +79,7 @@ struct MpLaunch {
   int sur_on, sur_cap, sur_nmax;
   double sur_tol;
   float *surX, *surY;
+  unsigned *surFb;             // listed-face indices for the exact fallback pass
   MlpWeights sur_w;
 };
 
@@ -87,7 +89,7 @@ extern "C" {
 int m2c_mp_launch_sur_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
                             const int *idp, float *X, float *Y, const m2c::MlpWeights &w, int nmax, double tol,
                             double *Vs, int *id, double *Vsm, double *Vsp, int *status, int *path, double *scratch,
-                            int nthreads);
+                            int nthreads, unsigned *fbl, unsigned *nfb);
 int m2c_mp_upload(const m2c::Grid &grid, const m2c::DevEos *eos, const m2c::MpParams &mp);
 int m2c_mp_step_begin(const m2c::MpLaunch &L);
 int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,

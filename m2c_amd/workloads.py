@@ -154,3 +154,25 @@ def config_from(settings, **over):
     s = dict(settings)
     s.update(over)
     return make_config(s.pop("nx"), s.pop("ny"), s.pop("nz"), s.pop("box"), s.pop("bc"), s.pop("mats"), **s)
+
+
+def interface_problems(n, seed=8):
+    """Pointwise water/air interface problems of the shock-bubble's kind along x (water = material 0,
+    air = 1 of shock_bubble's settings; air on the left for half of them): (Vm, idm, Vp, idp)."""
+    rng = np.random.default_rng(seed)
+    Vm = np.zeros((n, 5))
+    Vp = np.zeros((n, 5))
+    Vm[:, 0] = rng.uniform(1000.0, 1250.0, n)
+    Vm[:, 4] = 10.0 ** rng.uniform(5.0, 9.0, n)
+    Vm[:, 1] = rng.uniform(-100.0, 450.0, n)
+    Vp[:, 0] = 1.2 * 10.0 ** rng.uniform(0.0, 2.5, n)
+    Vp[:, 4] = 10.0 ** rng.uniform(5.0, 9.0, n)
+    Vp[:, 1] = rng.uniform(-100.0, 100.0, n)
+    Vm[:, 2:4] = rng.uniform(-10.0, 10.0, (n, 2))
+    Vp[:, 2:4] = rng.uniform(-10.0, 10.0, (n, 2))
+    idm = np.zeros(n, np.int32)
+    idp = np.ones(n, np.int32)
+    swap = rng.random(n) < 0.5
+    Vm[swap], Vp[swap] = Vp[swap].copy(), Vm[swap].copy()
+    idm[swap], idp[swap] = 1, 0
+    return Vm, idm, Vp, idp

@@ -119,33 +119,12 @@ def test_training_gains_over_acoustic_start(trained):
     assert rep.holdout_iters_mlp <= rep.holdout_iters_acoustic - 0.5
 
 
-def water_air_problems(n, seed):
-    """Interface problems of the water/air shock-bubble kind (water = material 0, air = 1)."""
-    rng = np.random.default_rng(seed)
-    Vm = np.zeros((n, 5))
-    Vp = np.zeros((n, 5))
-    Vm[:, 0] = rng.uniform(1000.0, 1250.0, n)
-    Vm[:, 4] = 10.0 ** rng.uniform(5.0, 9.0, n)
-    Vm[:, 1] = rng.uniform(-100.0, 450.0, n)
-    Vp[:, 0] = 1.2 * 10.0 ** rng.uniform(0.0, 2.5, n)
-    Vp[:, 4] = 10.0 ** rng.uniform(5.0, 9.0, n)
-    Vp[:, 1] = rng.uniform(-100.0, 100.0, n)
-    Vm[:, 2:4] = rng.uniform(-10.0, 10.0, (n, 2))
-    Vp[:, 2:4] = rng.uniform(-10.0, 10.0, (n, 2))
-    idm = np.zeros(n, np.int32)
-    idp = np.ones(n, np.int32)
-    swap = rng.random(n) < 0.5   # air on the left for half of them
-    Vm[swap], Vp[swap] = Vp[swap].copy(), Vm[swap].copy()
-    idm[swap], idp[swap] = 1, 0
-    return Vm, idm, Vp, idp
-
-
 def test_surrogate_batch_checked_and_exact_fallback(trained):
     net, _ = trained
     settings, V0, ID0, phi0 = W.shock_bubble(scale=0.125)
     s = M.Solver(W.config_from(settings))
     n = 20000
-    Vm, idm, Vp, idp = water_air_problems(n, 8)
+    Vm, idm, Vp, idp = W.interface_problems(n, 8)
     dirs = np.zeros(n, np.int32)
     t0 = time.perf_counter()
     st_e, Vs_e, id_e, Vsm_e, Vsp_e = s.riemann(dirs, Vm, idm, Vp, idp)
@@ -155,6 +134,17 @@ def test_surrogate_batch_checked_and_exact_fallback(trained):
     acc = path == 1
     print(f"  accepted {acc.mean():.4f}; exact batch {1e3 * (t1 - t0):.1f} ms, surrogate batch {1e3 * (t2 - t1):.1f} ms")
     assert acc.mean() >= 0.9
+    # The gain. Whole-batch times are set by the slowest problems (a few near-vacuum expansions on
+    # which one exact solve runs for seconds; the network rejects them and they take the exact
+    # solver either way), so compare both paths on the problems the network accepts.
+    sel = np.nonzero(acc)[0]
+    t3 = time.perf_counter()
+    s.riemann(dirs[sel], Vm[sel], idm[sel], Vp[sel], idp[sel])
+    t4 = time.perf_counter()
+    s.surrogate_riemann(net, dirs[sel], Vm[sel], idm[sel], Vp[sel], idp[sel], newton_max=3)
+    t5 = time.perf_counter()
+    print(f"  accepted problems only: exact {1e3 * (t4 - t3):.1f} ms, network {1e3 * (t5 - t4):.1f} ms")
+    assert (t5 - t4) <= 0.5 * (t4 - t3)
     # fallbacks are the exact solver's results bit for bit
     fb = ~acc
     assert np.array_equal(Vsm_s[fb], Vsm_e[fb]) and np.array_equal(Vsp_s[fb], Vsp_e[fb]) and np.array_equal(st_s[fb], st_e[fb])
