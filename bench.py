@@ -42,6 +42,11 @@ BYTES_PER_CELL_UPDATE = 320.0
 # phi_n+1, ID read/write: 18 B) -- 260 B; reinitialisation adds 88 B per iteration every 5 steps
 BYTES_PER_CELL_UPDATE_MP = 260.0
 BYTES_PER_REINIT_ITER = 88.0
+# HBM bytes per launch of the fused stage kernel on the 256^3 periodic RK3 FP64 workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --steps 3 --warmup 1 --no-cpu
+# --no-mixed` (FETCH scaled by 2.0, WRITE by 1.0 after a copy-kernel calibration; summary in
+# profiles/stage_kernel_periodic256_rk3_fp64_traffic.md). Re-measure after changing the kernel.
+STAGE_TRAFFIC_PERIODIC256_FP64 = 3.430e9
 
 
 def cpu_baseline(settings, V0, ID0, nsteps, phi0=None):
@@ -63,10 +68,36 @@ def cpu_baseline(settings, V0, ID0, nsteps, phi0=None):
                       f"{el:.1f} s wall, C oracle -O3 with {threads} OpenMP threads"}
 
 
-def variant_form(M, W, settings, V0, ID0, phi0, args, V64, value64, precision, net=None):
+def l1_fields(Vx, Vref):
+    """Relative L1 difference per field; velocity components are measured against the L1 norm of
+    the whole velocity (a component that is zero in the reference would otherwise divide by ~0)."""
+    names = ["rho", "u", "v", "w", "p"]
+    vel = float(np.abs(Vref[..., 1:4]).sum())
+    out = {}
+    for f in range(5):
+        den = vel if 1 <= f <= 3 else float(np.abs(Vref[..., f]).sum())
+        out[names[f]] = float(np.abs(Vx[..., f] - Vref[..., f]).sum() / den) if den > 0 else 0.0
+    return out
+
+
+def run_untimed(M, W, settings, V0, ID0, phi0, nsteps, precision, net=None):
+    """State after nsteps steps from the initial data (accuracy checkpoint)."""
+    s = M.Solver(W.config_from(settings, precision=precision))
+    if net is not None:
+        s.set_surrogate(net, newton_max=3)
+    s.upload(V0, ID0)
+    if phi0 is not None:
+        s.upload_levelset(phi0)
+    s.run(1e300, nsteps, sync_every=0)
+    V, _ = s.download()
+    s.close()
+    return V
+
+
+def variant_form(M, W, settings, V0, ID0, phi0, args, V64, value64, precision, net=None, check=None):
     """The same workload and step count in another form (mixed precision and/or the learned interface
     solver): throughput and the relative L1 difference per field of its final state against the
-    FP64 run's."""
+    FP64 run's; with check = (steps, V_fp64_at_steps) also the difference after that many steps."""
     cells = V0.shape[0] * V0.shape[1] * V0.shape[2]
     s = M.Solver(W.config_from(settings, precision=precision))
     if net is not None:
@@ -84,11 +115,7 @@ def variant_form(M, W, settings, V0, ID0, phi0, args, V64, value64, precision, n
     Vx, _ = s.download()
     ms_stage, _ = s.time_steps(max(5, min(args.steps, 20)))
     s.close()
-    names = ["rho", "u", "v", "w", "p"]
-    l1 = {}
-    for f in range(5):
-        den = float(np.abs(V64[..., f]).sum())
-        l1[names[f]] = float(np.abs(Vx[..., f] - V64[..., f]).sum() / den) if den > 0 else 0.0
+    l1 = l1_fields(Vx, V64)
     v = cells * args.steps / el
     tag = ("mixed" if precision == M.PREC_MIXED else "fp64") + ("+surrogate" if net is not None else "")
     print(f"bench: {tag} form {args.steps} steps in {el:.3f} s, L1(rho) {l1['rho']:.2e}", file=sys.stderr, flush=True)
@@ -100,6 +127,10 @@ def variant_form(M, W, settings, V0, ID0, phi0, args, V64, value64, precision, n
     if net is not None:
         out["network_solves"] = st.surrogate_accepted
         out["exact_fallbacks"] = st.surrogate_fallbacks
+    if check is not None:
+        nchk, Vref = check
+        out[f"l1_rel_err_vs_fp64_at_step_{nchk}"] = l1_fields(
+            run_untimed(M, W, settings, V0, ID0, phi0, nchk, precision, net), Vref)
     return out
 
 
@@ -290,8 +321,9 @@ def main():
     total_steps = args.warmup + args.steps
     bpcu = BYTES_PER_CELL_UPDATE_MP + BYTES_PER_REINIT_ITER * reinit_its / max(total_steps, 1) if mp \
         else BYTES_PER_CELL_UPDATE
+    traffic = STAGE_TRAFFIC_PERIODIC256_FP64 if (not mp and args.n == 256) else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "m2c::stage_kernel (fused MUSCL+HLLC+RK stage)", "avg_launch_ms": ms_stage}
     line = {"metric": "cell-updates/s on 1 GPU: plain FP64 vs mixed form at L1 err<=1e-5; % HBM peak",
             "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -305,11 +337,17 @@ def main():
                        "measured_copy_GBs": copy_gbs, "copy_kernel_ms_per_GiB": copy_ms},
             "roofline": roof}
     if not args.no_mixed:
-        line["mixed"] = variant_form(M, W, settings, V0, ID0, phi0 if mp else None, args, V64, value, M.PREC_MIXED)
+        # the shock-bubble amplifies any perturbation once the shock strikes the bubble (~step 180),
+        # so the forms are also compared at step 150
+        check = (150, run_untimed(M, W, settings, V0, ID0, phi0, 150, M.PREC_FP64)) if mp else None
+        line["mixed"] = variant_form(M, W, settings, V0, ID0, phi0 if mp else None, args, V64, value, M.PREC_MIXED,
+                                     check=check)
         if mp and os.path.exists(M.SURROGATE_PARAMS):
             net = M.load_surrogate()
-            line["fp64_surrogate"] = variant_form(M, W, settings, V0, ID0, phi0, args, V64, value, M.PREC_FP64, net)
-            line["mixed_surrogate"] = variant_form(M, W, settings, V0, ID0, phi0, args, V64, value, M.PREC_MIXED, net)
+            line["fp64_surrogate"] = variant_form(M, W, settings, V0, ID0, phi0, args, V64, value, M.PREC_FP64, net,
+                                                  check=check)
+            line["mixed_surrogate"] = variant_form(M, W, settings, V0, ID0, phi0, args, V64, value, M.PREC_MIXED,
+                                                   net, check=check)
             net.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(settings, V0, ID0, args.cpu_steps, phi0 if mp else None)
