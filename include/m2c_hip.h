@@ -249,6 +249,43 @@ int m2c_device_state(m2c_ctx *ctx, double **V_soa);
 
 const char *m2c_version(void);
 
+/* ---- input decks ----------------------------------------------------------------------------
+ * Host-only calls (no device is touched). */
+
+/* IoData's defaults for the fields of m2c_config (the IoData.cpp constructors: MeshData :453,
+ * ReconstructionData :1049, SchemeData :1154, LevelSetReinitializationData :1205, LevelSetSchemeData
+ * :1245, ExactRiemannSolverData :1345, MultiPhaseData :1400, ExplicitData :1472, TsData :1493).
+ * No materials and no cells: nmat = 0, nx = ny = nz = -1, boundaries None (0). */
+int m2c_config_defaults(m2c_config *cfg);
+
+/* Run controls and the uniform initial state of a deck (TsData, IoData.cpp:1493-1528; the
+ * top-level DefaultInitialState, IoData.cpp:1695). */
+typedef struct {
+  int max_its;                      /* Time.MaxIts (INT_MAX by default) */
+  double max_time;                  /* Time.MaxTime (1e10) */
+  double time_step;                 /* Time.TimeStep: fixed dt when > 0 (-1: CFL-adaptive) */
+  double initial_state[5];          /* DefaultInitialState: rho, u, v, w, p (1e-6, 0, 0, 0, 0) */
+  int initial_material;             /* DefaultInitialState.MaterialID */
+} m2c_deck_run;
+
+/* Reads an M2C input deck (the text of the file main() passes to IoData::readCmdFile,
+ * IoData.cpp:3235, through the grammar of parser/Parser.y) into *cfg and, when run is not
+ * NULL, *run. Returns M2C_OK, M2C_ERR_ARG for a syntax error or a value the reference rejects, or
+ * M2C_ERR_UNSUPPORTED for a valid option this library does not provide. msg (msglen bytes, may
+ * be NULL) receives the error with its line, or on success the ignored options (output and other
+ * objects outside the finite-volume update), one per line. *cfg and *run are written only on
+ * success. */
+int m2c_config_from_deck(const char *text, m2c_config *cfg, m2c_deck_run *run, char *msg, int msglen);
+
+/* Initial state of a deck on its uniform mesh: SpaceOperator::SetInitialCondition
+ * (SpaceOperator.cpp:696-960) and, for the material of its level set, LevelSetOperator::
+ * SetInitialCondition (LevelSetOperator.cpp:291-556). DefaultInitialState everywhere, then the
+ * GeometricEntities Plane, CylinderAndCone, CylinderWithSphericalCaps, Sphere and Parallelepiped
+ * (each kind in index order, later ones overriding). V: nz*ny*nx*5 doubles, id: nz*ny*nx int32 (the
+ * layout of m2c_upload_state), phi: nz*ny*nx doubles or NULL (left alone when the deck has no level
+ * set). Spheroid, Point, ArbitraryEnclosure and UserDataFile give M2C_ERR_UNSUPPORTED. */
+int m2c_initial_condition_from_deck(const char *text, double *V, int32_t *id, double *phi, char *msg, int msglen);
+
 #ifdef __cplusplus
 }
 #endif

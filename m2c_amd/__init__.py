@@ -49,6 +49,11 @@ class Config(C.Structure):
                 ("reinit_maxits", C.c_int), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double)]
 
 
+class DeckRun(C.Structure):
+    _fields_ = [("max_its", C.c_int), ("max_time", C.c_double), ("time_step", C.c_double),
+                ("initial_state", C.c_double * 5), ("initial_material", C.c_int)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("kchunk", C.c_int), ("blocks", C.c_int), ("ms_per_step", C.c_double), ("candidates", C.c_int)]
 
@@ -186,6 +191,9 @@ def lib():
         L.m2c_set_surrogate.argtypes = [P, P, C.c_int]
         L.m2c_surrogate_riemann_batch.argtypes = [P, P, C.c_int, C.c_int, I, D, I, D, I, D, I, D, D, I, I]
         L.m2c_version.restype = C.c_char_p
+        L.m2c_config_defaults.argtypes = [C.POINTER(Config)]
+        L.m2c_config_from_deck.argtypes = [C.c_char_p, C.POINTER(Config), C.POINTER(DeckRun), C.c_char_p, C.c_int]
+        L.m2c_initial_condition_from_deck.argtypes = [C.c_char_p, D, C.POINTER(C.c_int32), D, C.c_char_p, C.c_int]
         _lib = L
     return _lib
 
@@ -332,6 +340,59 @@ def stream_copy_GBs(solver, nbytes=1 << 30, reps=20):
 
 def version():
     return lib().m2c_version().decode()
+
+
+def config_defaults():
+    """IoData's defaults for every field of Config (no materials, no cells, boundaries None)."""
+    c = Config()
+    _chk(lib().m2c_config_defaults(C.byref(c)), "m2c_config_defaults")
+    return c
+
+
+class DeckError(M2CError):
+    """An input deck the reader rejects: `code` is ERR_ARG (the reference rejects it too) or
+    ERR_UNSUPPORTED (valid for the reference, not provided here)."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+def config_from_deck(text, msglen=1 << 16):
+    """Read an M2C input deck (its text) into (Config, DeckRun, ignored) where `ignored` lists the
+    options outside the finite-volume update that the reader skipped. Raises DeckError."""
+    cfg, run = Config(), DeckRun()
+    buf = C.create_string_buffer(msglen)
+    rc = lib().m2c_config_from_deck(text.encode(), C.byref(cfg), C.byref(run), buf, msglen)
+    msg = buf.value.decode()
+    if rc != OK:
+        raise DeckError(rc, msg)
+    return cfg, run, [line for line in msg.splitlines() if line]
+
+
+def initial_condition_from_deck(text, msglen=1 << 16):
+    """Initial state of a deck on its mesh: (V (nz, ny, nx, 5), ID (nz, ny, nx) int32, phi) where phi
+    is the level set of the deck's level-set material, or None when it has none. Raises DeckError."""
+    cfg, _, _ = config_from_deck(text, msglen)
+    nz, ny, nx = cfg.nz, cfg.ny, cfg.nx
+    V = np.zeros((nz, ny, nx, 5))
+    ID = np.zeros((nz, ny, nx), dtype=np.int32)
+    phi = np.zeros((nz, ny, nx)) if cfg.ls_matid >= 0 else None
+    buf = C.create_string_buffer(msglen)
+    rc = lib().m2c_initial_condition_from_deck(text.encode(), _dp(V), ID.ctypes.data_as(C.POINTER(C.c_int32)),
+                                               None if phi is None else _dp(phi), buf, msglen)
+    if rc != OK:
+        raise DeckError(rc, buf.value.decode())
+    return V, ID, phi
+
+
+def read_deck(path):
+    """An input deck file: (Config, DeckRun, ignored options, V, ID, phi)."""
+    with open(path) as f:
+        text = f.read()
+    cfg, run, ignored = config_from_deck(text)
+    V, ID, phi = initial_condition_from_deck(text)
+    return cfg, run, ignored, V, ID, phi
 
 
 class Batch:

@@ -37,6 +37,9 @@ UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), (
     (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"]
      + (FP32_FAST if tc == "float" else []))
     for t, f, e, tc in VARIANTS]
+# host-only C++ (no device code), compiled with the host compiler
+HOST_UNITS = [("input_deck", "input_deck.cpp")]
+CXX = os.environ.get("CXX", "g++")
 
 
 def _compile(unit, verbose):
@@ -52,15 +55,31 @@ def _compile(unit, verbose):
     return obj
 
 
+def _compile_host(unit, verbose):
+    name, src = unit
+    obj = os.path.join(OBJ, name + ".o")
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"{CXX} failed on {src}")
+    return obj
+
+
 def build(verbose=False, jobs=None, only=None):
     """Compile every unit (or, with `only`, just the named units, reusing the other objects of an
     earlier build) and link libm2c_hip.so."""
     os.makedirs(OBJ, exist_ok=True)
     todo = [u for u in UNITS if only is None or u[0] in only]
-    jobs = jobs or min(len(todo), max(1, (os.cpu_count() or 2)))
+    jobs = jobs or max(1, min(len(todo), os.cpu_count() or 2))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda u: _compile(u, verbose), todo))
-    objs = [os.path.join(OBJ, u[0] + ".o") for u in UNITS]
+    for u in HOST_UNITS:
+        if only is None or u[0] in only:
+            _compile_host(u, verbose)
+    objs = [os.path.join(OBJ, u[0] + ".o") for u in UNITS + HOST_UNITS]
     missing = [o for o in objs if not os.path.exists(o)]
     if missing:
         raise RuntimeError("objects missing (run a full build first): " + ", ".join(missing))

@@ -3,8 +3,10 @@
 //
 //   m2c_demo sod <n> <tmax> <out.bin>                    Sod 400x1x1-type problem to tmax
 //   m2c_demo bubble <scale> <steps> <out.bin>            shock-bubble, RK2, reinit every 5 steps
+//   m2c_demo deck <input file> <out.bin>                 any input deck: its mesh, options, initial
+//                                                        state and Time.MaxTime / MaxIts / TimeStep
 //
-// Writes the primitive state (nz*ny*nx*5 doubles), then the IDs (int32) and, for the bubble, the
+// Writes the primitive state (nz*ny*nx*5 doubles), then the IDs (int32) and, with a level set, the
 // level set, to out.bin; prints steps and time.
 #include <cmath>
 #include <cstdio>
@@ -18,6 +20,7 @@ using namespace m2c_host;
 
 static int run_sod(int n, double tmax, const char *out) {
   m2c_config c = default_config(n, 1, 1, 0.0, 1.0, 0.0, 1.0 / n, 0.0, 1.0 / n);
+  c.limiter = M2C_LIM_VANALBADA;
   add_stiffened_gas(c, 1.4, 0.0);
   c.bc[0] = M2C_BC_INLET;
   c.bc[1] = M2C_BC_OUTLET;
@@ -51,6 +54,7 @@ static int run_bubble(double scale, int steps, const char *out) {
   int nx = (int)lround(384 * scale), ny = (int)lround(192 * scale), nz = (int)lround(192 * scale);
   double radius = 24.0 * scale, cx = 160.0 * scale, shock_x = 64.0 * scale;
   m2c_config c = default_config(nx, ny, nz, 0.0, nx, 0.0, ny, 0.0, nz);
+  c.limiter = M2C_LIM_VANALBADA;
   add_stiffened_gas(c, 6.12, 3.43e8);
   add_stiffened_gas(c, 1.4, 0.0);
   c.ls_matid = 1;
@@ -100,12 +104,40 @@ static int run_bubble(double scale, int steps, const char *out) {
   return 0;
 }
 
+// Main.cpp's flow: read the deck, set the initial condition, march to MaxTime or MaxIts
+static int run_deck(const char *path, const char *out) {
+  IoData iod(path);
+  for (const std::string &line : iod.ignored) fprintf(stderr, "m2c_demo: %s\n", line.c_str());
+  std::vector<double> V, phi;
+  std::vector<int32_t> ID;
+  iod.InitialCondition(V, ID, phi);
+  Context ctx(iod.config);
+  SpaceOperator spo(ctx);
+  LevelSetOperator lso(ctx);
+  TimeIntegrator ti(ctx);
+  spo.SetState(V.data(), ID.data());
+  if (iod.has_levelset()) lso.SetPhi(phi.data());
+  m2c_status s = iod.run.time_step > 0.0 ? ti.RunFixed(iod.run.time_step, iod.run.max_time, iod.run.max_its)
+                                         : ti.Run(iod.run.max_time, iod.run.max_its, 64);
+  spo.GetState(V.data(), ID.data());
+  if (iod.has_levelset()) lso.GetPhi(phi.data());
+  FILE *f = fopen(out, "wb");
+  if (!f) return 1;
+  fwrite(V.data(), sizeof(double), V.size(), f);
+  fwrite(ID.data(), sizeof(int32_t), ID.size(), f);
+  fwrite(phi.data(), sizeof(double), phi.size(), f);
+  fclose(f);
+  printf("deck %dx%dx%d steps=%lld t=%.17g\n", iod.config.nx, iod.config.ny, iod.config.nz, s.steps, s.t);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc < 5) {
-    fprintf(stderr, "usage: %s sod <n> <tmax> <out> | bubble <scale> <steps> <out>\n", argv[0]);
+  if (argc < 4 || (strcmp(argv[1], "deck") && argc < 5)) {
+    fprintf(stderr, "usage: %s sod <n> <tmax> <out> | bubble <scale> <steps> <out> | deck <input> <out>\n", argv[0]);
     return 2;
   }
   try {
+    if (!strcmp(argv[1], "deck")) return run_deck(argv[2], argv[3]);
     if (!strcmp(argv[1], "sod")) return run_sod(atoi(argv[2]), atof(argv[3]), argv[4]);
     if (!strcmp(argv[1], "bubble")) return run_bubble(atof(argv[2]), atoi(argv[3]), argv[4]);
   } catch (const Error &e) {

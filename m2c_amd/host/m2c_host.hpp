@@ -14,6 +14,8 @@
 #pragma once
 #include <cmath>
 #include <cstdint>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -44,8 +46,8 @@ inline m2c_config default_config(int nx, int ny, int nz, double x0, double x1, d
   c.inactive_id = -1;
   c.flux = M2C_FLUX_HLLC;
   c.roe_delta = 0.2;               // IoData.cpp:1158
-  c.rec_linear = 1;
-  c.limiter = M2C_LIM_VANALBADA;
+  c.rec_linear = 1;                // ReconstructionData: linear, no limiter (IoData.cpp:1049-1056)
+  c.limiter = M2C_LIM_NONE;
   c.gminmod_alpha = 1.2;
   c.ts = M2C_TS_RK2;               // TimeIntegration default: Runge-Kutta 2
   c.cfl = 0.5;
@@ -76,6 +78,54 @@ inline int add_stiffened_gas(m2c_config &c, double gamma, double pinf, double fa
   m.p[0] = gamma; m.p[1] = pinf;
   return c.nmat++;
 }
+
+// IoData(argc, argv) -> readCmdFile (IoData.cpp:3214-3250): an input deck read into the solver
+// options, the run controls of Time, the ignored options outside the finite-volume update, and the
+// initial state of DefaultInitialState and GeometricEntities on the deck's mesh.
+class IoData {
+ public:
+  explicit IoData(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) throw Error("cannot open input file " + path, M2C_ERR_ARG);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    text_ = ss.str();
+    parse();
+  }
+  static IoData FromText(const std::string &text) { return IoData(text, 0); }
+
+  m2c_config config;
+  m2c_deck_run run;
+  std::vector<std::string> ignored;
+
+  long long cells() const { return (long long)config.nx * config.ny * config.nz; }
+  bool has_levelset() const { return config.ls_matid >= 0; }
+  // SpaceOperator::SetInitialCondition and LevelSetOperator::SetInitialCondition; phi is left
+  // empty when the deck has no level set
+  void InitialCondition(std::vector<double> &V, std::vector<int32_t> &ID, std::vector<double> &phi) const {
+    V.assign(5 * (size_t)cells(), 0.0);
+    ID.assign((size_t)cells(), 0);
+    phi.assign(has_levelset() ? (size_t)cells() : 0, 0.0);
+    char msg[1024];
+    check_msg(m2c_initial_condition_from_deck(text_.c_str(), V.data(), ID.data(), has_levelset() ? phi.data() : nullptr,
+                                              msg, sizeof msg),
+              msg);
+  }
+
+ private:
+  IoData(const std::string &text, int) : text_(text) { parse(); }
+  void parse() {
+    std::vector<char> msg(1 << 16);
+    check_msg(m2c_config_from_deck(text_.c_str(), &config, &run, msg.data(), (int)msg.size()), msg.data());
+    std::stringstream ss(msg.data());
+    for (std::string line; std::getline(ss, line);)
+      if (!line.empty()) ignored.push_back(line);
+  }
+  static void check_msg(int rc, const char *msg) {
+    if (rc != M2C_OK) throw Error(std::string("input deck: ") + msg, rc);
+  }
+  std::string text_;
+};
 
 // One device context: the SpaceOperator / TimeIntegrator / LevelSetOperator of a subdomain.
 class Context {
@@ -133,6 +183,16 @@ class TimeIntegrator {
   m2c_status Run(double tmax, int maxsteps, int sync_every = 64) {
     check(m2c_run(c_.get(), tmax, maxsteps, sync_every), "m2c_run");
     return status_check();
+  }
+  // Main.cpp:514-621 with a prescribed Time.TimeStep (SpaceOperator.cpp:2219-2221): dt each step,
+  // shortened to land on tmax
+  m2c_status RunFixed(double dt, double tmax, int maxsteps) {
+    m2c_status s = Status();
+    for (long long n = 0; s.t < tmax && n < maxsteps; n++) {
+      AdvanceOneTimeStep(s.t + dt > tmax ? tmax - s.t : dt);
+      s = Status();
+    }
+    return s;
   }
   m2c_status Status() const {
     m2c_status s;

@@ -118,6 +118,111 @@ def shock_bubble(nx=384, ny=192, nz=192, radius=24.0, cx=160.0, shock_x=64.0, p_
     return settings, V, ID, phi
 
 
+def _num(x):
+    """A number as deck text that reads back to the same double (17 significant digits)."""
+    return repr(float(x))
+
+
+def _state(name, V, matid=None):
+    s = f"  under {name} {{\n"
+    if matid is not None:
+        s += f"    MaterialID = {matid};\n"
+    for key, v in zip(("Density", "VelocityX", "VelocityY", "VelocityZ", "Pressure"), V):
+        s += f"    {key} = {_num(v)};\n"
+    return s + "  }\n"
+
+
+def sod_deck(n=400):
+    """sod_1d as an M2C input deck: the same settings and (through a plane) the same initial state."""
+    settings, _, _ = sod_1d(n)
+    return f"""// Sod shock tube, {n} x 1 x 1 cells
+under Mesh {{
+  X0 = 0.0; Xmax = 1.0; Y0 = 0.0; Ymax = {_num(1.0 / n)}; Z0 = 0.0; Zmax = {_num(1.0 / n)};
+  NumberOfCellsX = {n}; NumberOfCellsY = 1; NumberOfCellsZ = 1;
+  BoundaryConditionX0 = Farfield0; BoundaryConditionXmax = Farfield1;
+  BoundaryConditionY0 = Symmetry; BoundaryConditionYmax = Symmetry;
+  BoundaryConditionZ0 = Symmetry; BoundaryConditionZmax = Symmetry;
+}}
+under Equations {{
+  under Material[0] {{
+    EquationOfState = StiffenedGas;
+    under StiffenedGasModel {{ SpecificHeatRatio = 1.4; PressureConstant = 0.0; }}
+  }}
+}}
+under BoundaryConditions {{
+{_state("Inlet", settings["inlet"])}{_state("Outlet", settings["outlet"])}}}
+under DefaultInitialState {{ Density = 0.125; Pressure = 0.1; }}
+under InitialCondition {{
+  under GeometricEntities {{
+    under Plane[0] {{
+      Point_x = 0.5; Normal_x = -1.0;
+{_state("InitialState", (1.0, 0.0, 0.0, 0.0, 1.0))}    }}
+  }}
+}}
+under Space {{
+  under NavierStokes {{
+    Flux = HLLC;
+    under Reconstruction {{ Type = Linear; Limiter = VanAlbada; }}
+  }}
+}}
+under Time {{
+  Type = Explicit; MaxTime = 0.1; CFL = 0.5;
+  under Explicit {{ Type = RungeKutta2; }}
+}}
+under Output {{ Prefix = "results/"; Frequency = 100; }}
+"""
+
+
+def shock_bubble_deck(scale=1.0, reinit_freq=5):
+    """shock_bubble as an M2C input deck (h = 1): water, a level-set sphere of air, a planar shock."""
+    settings, _, _, _ = shock_bubble(scale=scale, reinit_freq=reinit_freq)
+    nx, ny, nz = settings["nx"], settings["ny"], settings["nz"]
+    radius, cx, shock_x = 24.0, 160.0, 64.0
+    if scale != 1.0:
+        radius, cx, shock_x = radius * scale, cx * scale, shock_x * scale
+    return f"""// shock-bubble: water (stiffened gas) around an air bubble, planar shock at x = {_num(shock_x)}
+under Mesh {{
+  X0 = 0; Xmax = {nx}; Y0 = 0; Ymax = {ny}; Z0 = 0; Zmax = {nz};
+  NumberOfCellsX = {nx}; NumberOfCellsY = {ny}; NumberOfCellsZ = {nz};
+  BoundaryConditionX0 = Inlet; BoundaryConditionXmax = Outlet;
+  BoundaryConditionY0 = Symmetry; BoundaryConditionYmax = Symmetry;
+  BoundaryConditionZ0 = Symmetry; BoundaryConditionZmax = Symmetry;
+}}
+under Equations {{
+  Material[0].EquationOfState = StiffenedGas;
+  Material[0].StiffenedGasModel.SpecificHeatRatio = {_num(WATER[0])};
+  Material[0].StiffenedGasModel.PressureConstant = {_num(WATER[1])};
+  under Material[1] {{
+    EquationOfState = StiffenedGas;
+    under StiffenedGasModel {{ SpecificHeatRatio = {_num(AIR[0])}; PressureConstant = {_num(AIR[1])}; }}
+  }}
+}}
+under BoundaryConditions {{
+{_state("Inlet", settings["inlet"])}{_state("Outlet", settings["outlet"])}}}
+under DefaultInitialState {{ Density = 1000.0; Pressure = 1.0e5; }}
+under GeometricEntities {{
+  under Plane[0] {{
+    Point_x = {_num(shock_x)}; Normal_x = -1;
+{_state("InitialState", settings["inlet"], 0)}  }}
+  under Sphere[0] {{
+    Center_x = {_num(cx)}; Center_y = {_num(0.5 * ny)}; Center_z = {_num(0.5 * nz)}; Radius = {_num(radius)};
+{_state("InitialState", (1.2, 0.0, 0.0, 0.0, 1.0e5), 1)}  }}
+}}
+under Space {{
+  under NavierStokes {{
+    Flux = HLLC;
+    under Reconstruction {{ Type = Linear; Limiter = VanAlbada; }}
+  }}
+  under LevelSet[0] {{
+    MaterialID = 1;
+    under Reinitialization {{ Frequency = {reinit_freq}; }}
+  }}
+}}
+under MultiPhase {{ Flux = Numerical; ReconstructionAtInterface = Constant; }}
+under Time {{ Type = Explicit; CFL = 0.5; under Explicit {{ Type = RungeKutta2; }} }}
+"""
+
+
 BATCH_SEED = 4242
 
 

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from m2c_amd import build, workloads as W
+from oracle import pyoracle as po
 from tests import cases
 
 pytestmark = pytest.mark.gpu
@@ -43,3 +44,51 @@ def test_cpp_host_layer_shock_bubble(tmp_path):
     assert np.array_equal(ID, IDo)
     assert np.array_equal(V, Vo)
     assert np.array_equal(phi, phio)
+
+
+def deck_file(tmp_path, text):
+    path = tmp_path / "input.st"
+    path.write_text(text)
+    return str(path)
+
+
+def test_cpp_host_layer_runs_the_sod_deck(tmp_path):
+    # Main.cpp's flow from an input deck: IoData, initial condition, march to Time.MaxTime
+    raw, txt = demo(["deck", deck_file(tmp_path, W.sod_deck(200))], tmp_path)
+    settings, V0, ID0 = W.sod_1d(200)
+    ost, t, n, Vo, _ = cases.run_oracle(settings, V0, ID0, tmax=0.1, maxsteps=100000, vanalbada_pow=False)
+    V = np.frombuffer(raw[:200 * 5 * 8], dtype=np.float64).reshape(1, 1, 200, 5)
+    assert f"steps={n} " in txt
+    assert np.array_equal(V, Vo)
+
+
+def test_cpp_host_layer_runs_the_shock_bubble_deck(tmp_path):
+    text = W.shock_bubble_deck(0.125, reinit_freq=5) + "Time.MaxIts = 12;\n"
+    raw, txt = demo(["deck", deck_file(tmp_path, text)], tmp_path)
+    settings, V0, ID0, phi0 = W.shock_bubble(scale=0.125)
+    ost, t, n, Vo, o, IDo, phio = cases.run_oracle(settings, V0, ID0, maxsteps=12, phi0=phi0, vanalbada_pow=False)
+    N = V0.shape[0] * V0.shape[1] * V0.shape[2]
+    assert "steps=12 " in txt
+    V = np.frombuffer(raw[:N * 40], dtype=np.float64).reshape(V0.shape)
+    ID = np.frombuffer(raw[N * 40:N * 44], dtype=np.int32).reshape(ID0.shape)
+    phi = np.frombuffer(raw[N * 44:N * 52], dtype=np.float64).reshape(ID0.shape)
+    assert np.array_equal(ID, IDo) and np.array_equal(V, Vo) and np.array_equal(phi, phio)
+
+
+def test_cpp_host_layer_fixed_time_step_deck(tmp_path):
+    # Time.TimeStep > 0 (SpaceOperator.cpp:2219-2221): a prescribed dt, the last step shortened to
+    # land on MaxTime (Main.cpp:535-539)
+    text = W.sod_deck(100).replace("MaxTime = 0.1;", "MaxTime = 0.01; TimeStep = 0.0011;")
+    raw, txt = demo(["deck", deck_file(tmp_path, text)], tmp_path)
+    settings, V0, ID0 = W.sod_1d(100)
+    o = po.Oracle(cases.oracle_config(settings, vanalbada_pow=False))
+    V, ID = o.ghosted(V0, ID0)
+    t, n = 0.0, 0
+    while t < 0.01:
+        dt = 0.01 - t if t + 0.0011 > 0.01 else 0.0011
+        assert o.advance(V, ID, dt) == 0
+        t += dt
+        n += 1
+    assert f"steps={n} " in txt and n >= 10
+    Vg = np.frombuffer(raw[:100 * 5 * 8], dtype=np.float64).reshape(1, 1, 100, 5)
+    assert np.array_equal(Vg, V[1:-1, 1:-1, 1:-1])
