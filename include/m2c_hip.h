@@ -51,6 +51,9 @@ typedef struct {
   double p[8];                      /* SG: gamma, p_inf. NASG: gamma, pc, b, q.
                                        MG: rho0, c0, Gamma0, s, e0. JWL: omega, A1, A2, R1, R2, rho0.
                                        DUMMY: p0, rho0, e0. */
+  double tlaw[6];                   /* temperature law (output only): cv, T0, e0, rho0, cp, h0.
+                                       SG: all six (VarFcnSG.h:126-142); NASG: cv; MG: cv, T0, cp, h0
+                                       (its e0 is p[4]); JWL and DUMMY have none. */
 } m2c_material;
 
 typedef struct {
@@ -139,6 +142,12 @@ int m2c_synchronize(m2c_ctx *ctx);
 /* FluxFcnBase::ComputeNumericalFluxAtCellInterface (FluxFcnBase.h:65-69) for n face pairs
  * (Vm, Vp: n*5 doubles, F: n*5 doubles), evaluated by the device kernels. */
 int m2c_flux_batch(m2c_ctx *ctx, int dir, int n, const double *Vm, const double *Vp, int id, double *F);
+
+/* VarFcnBase::GetInternalEnergyPerUnitMass and GetTemperature (VarFcnBase.h, VarFcnSG.h:70-78,
+ * VarFcnNASG.h:61, VarFcnMG.h:172-179) for n points (rho, p, material id), evaluated on the device
+ * with the context's materials: e[t] always, T[t] when T is not NULL (NaN for a material without a
+ * temperature law: JWL, dummy). Used by the outputs. */
+int m2c_eos_derived(m2c_ctx *ctx, int n, const double *rho, const double *p, const int32_t *id, double *e, double *T);
 
 /* ExactRiemannSolverBase::ComputeRiemannSolution (ExactRiemannSolverBase.h:35-39) for n
  * independent problems along the unit normal of axis dir[t] (0 x, 1 y, 2 z), evaluated by the device
@@ -285,6 +294,27 @@ int m2c_config_from_deck(const char *text, m2c_config *cfg, m2c_deck_run *run, c
  * layout of m2c_upload_state), phi: nz*ny*nx doubles or NULL (left alone when the deck has no level
  * set). Spheroid, Point, ArbitraryEnclosure and UserDataFile give M2C_ERR_UNSUPPORTED. */
 int m2c_initial_condition_from_deck(const char *text, double *V, int32_t *id, double *phi, char *msg, int msglen);
+
+/* ---- outputs ---------------------------------------------------------------------------------
+ * The Output object of an input deck (Output::Output / InitializeOutput, Output.cpp:21-113):
+ * creates <Prefix><Solution>.pvd (an empty collection, Output.cpp:28-46), opens the probe files
+ * with their headers (ProbeOutput.cpp:13-175) and locates probe and line-plot points
+ * (ProbeOutput.cpp:226-339). Host-only. msg receives the error, or the requested outputs this
+ * library does not write (one per line). */
+typedef struct m2c_output m2c_output;
+int m2c_output_create(const char *text, m2c_output **out, char *msg, int msglen);
+
+/* Output::OutputSolutions (Output.cpp:117-143) with the state of one step: the snapshot
+ * (<Solution>_NNNN.vtr and its pvd line), the probes and the line plots whose isTimeToWrite
+ * (Utils.cpp:293-313) holds. V, ID, phi: the host layout of m2c_download_state /
+ * m2c_download_levelset (phi required when the deck has a level set). ctx evaluates internal
+ * energy and temperature on the device (m2c_eos_derived); it may be NULL when
+ * m2c_output_needs_device(o) is 0. written (may be NULL) gets bit 0 snapshot, bit 1 probes,
+ * bit 2 line plots. */
+int m2c_output_write(m2c_output *o, m2c_ctx *ctx, double time, double dt, int time_step, const double *V,
+                     const int32_t *ID, const double *phi, int force_write, int *written);
+int m2c_output_needs_device(const m2c_output *o);
+int m2c_output_destroy(m2c_output *o);
 
 #ifdef __cplusplus
 }

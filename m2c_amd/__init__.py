@@ -24,7 +24,8 @@ DBL_MAX = 1.7976931348623157e308
 
 class Material(C.Structure):
     _fields_ = [("eos", C.c_int), ("rhomin", C.c_double), ("pmin", C.c_double), ("rhomax", C.c_double),
-                ("pmax", C.c_double), ("failsafe_density", C.c_double), ("p", C.c_double * 8)]
+                ("pmax", C.c_double), ("failsafe_density", C.c_double), ("p", C.c_double * 8),
+                ("tlaw", C.c_double * 6)]
 
 
 class RiemannParams(C.Structure):
@@ -83,13 +84,17 @@ class TrainReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-def material(kind, params, rhomin=0.0, pmin=-DBL_MAX, rhomax=DBL_MAX, pmax=DBL_MAX, failsafe_density=1e-10):
-    """MaterialModelData (IoData.cpp:860-906): EOS kind + parameters + cut-offs."""
+def material(kind, params, rhomin=0.0, pmin=-DBL_MAX, rhomax=DBL_MAX, pmax=DBL_MAX, failsafe_density=1e-10,
+             tlaw=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)):
+    """MaterialModelData (IoData.cpp:860-906): EOS kind + parameters + cut-offs, and the temperature
+    law (cv, T0, e0, rho0, cp, h0) used only by outputs."""
     m = Material()
     m.eos = kind
     m.rhomin, m.pmin, m.rhomax, m.pmax, m.failsafe_density = rhomin, pmin, rhomax, pmax, failsafe_density
     for i, v in enumerate(params):
         m.p[i] = v
+    for i, v in enumerate(tlaw):
+        m.tlaw[i] = v
     return m
 
 
@@ -194,6 +199,12 @@ def lib():
         L.m2c_config_defaults.argtypes = [C.POINTER(Config)]
         L.m2c_config_from_deck.argtypes = [C.c_char_p, C.POINTER(Config), C.POINTER(DeckRun), C.c_char_p, C.c_int]
         L.m2c_initial_condition_from_deck.argtypes = [C.c_char_p, D, C.POINTER(C.c_int32), D, C.c_char_p, C.c_int]
+        L.m2c_eos_derived.argtypes = [P, C.c_int, D, D, C.POINTER(C.c_int32), D, D]
+        L.m2c_output_create.argtypes = [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_int]
+        L.m2c_output_write.argtypes = [P, P, C.c_double, C.c_double, C.c_int, D, C.POINTER(C.c_int32), D, C.c_int,
+                                       C.POINTER(C.c_int)]
+        L.m2c_output_needs_device.argtypes = [P]
+        L.m2c_output_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -325,6 +336,17 @@ class Solver:
         _chk(self.L.m2c_flux_batch(self.h, d, Vm.shape[0], _dp(Vm), _dp(Vp), mid, _dp(F)), "m2c_flux_batch")
         return F
 
+    def eos_derived(self, rho, p, ids):
+        """Internal energy per unit mass and temperature at n points (device EOS): (e, T)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+        p = np.ascontiguousarray(p, dtype=np.float64).ravel()
+        ids = np.asarray(ids, dtype=np.int32)
+        ids = np.ascontiguousarray(ids.ravel() if ids.size == rho.size else np.broadcast_to(ids, rho.shape))
+        e, T = np.empty_like(rho), np.empty_like(rho)
+        _chk(self.L.m2c_eos_derived(self.h, rho.size, _dp(rho), _dp(p), ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    _dp(e), _dp(T)), "m2c_eos_derived")
+        return e, T
+
     def time_steps(self, nsteps):
         a, b = C.c_double(), C.c_double()
         _chk(self.L.m2c_time_steps(self.h, nsteps, C.byref(a), C.byref(b)), "m2c_time_steps")
@@ -384,6 +406,46 @@ def initial_condition_from_deck(text, msglen=1 << 16):
     if rc != OK:
         raise DeckError(rc, buf.value.decode())
     return V, ID, phi
+
+
+class Output:
+    """The Output object of a deck (snapshots, probes, line plots): Output::OutputSolutions."""
+
+    def __init__(self, text, msglen=1 << 16):
+        self.L = lib()
+        self.h = C.c_void_p()
+        buf = C.create_string_buffer(msglen)
+        rc = self.L.m2c_output_create(text.encode(), C.byref(self.h), buf, msglen)
+        msg = buf.value.decode()
+        if rc != OK:
+            raise DeckError(rc, msg)
+        self.not_written = [line for line in msg.splitlines() if line]
+
+    @property
+    def needs_device(self):
+        return bool(self.L.m2c_output_needs_device(self.h))
+
+    def write(self, time, dt, step, V, ID, phi=None, solver=None, force=False):
+        """Returns the bit set of what was written (1 snapshot, 2 probes, 4 line plots)."""
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        ID = np.ascontiguousarray(ID, dtype=np.int32)
+        ph = None if phi is None else _dp(np.ascontiguousarray(phi, dtype=np.float64))
+        w = C.c_int()
+        _chk(self.L.m2c_output_write(self.h, solver.h if solver is not None else None, time, dt, step, _dp(V),
+                                     ID.ctypes.data_as(C.POINTER(C.c_int32)), ph, int(force), C.byref(w)),
+             "m2c_output_write")
+        return w.value
+
+    def close(self):
+        if self.h:
+            self.L.m2c_output_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def read_deck(path):

@@ -38,7 +38,7 @@ UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), (
      + (FP32_FAST if tc == "float" else []))
     for t, f, e, tc in VARIANTS]
 # host-only C++ (no device code), compiled with the host compiler
-HOST_UNITS = [("input_deck", "input_deck.cpp")]
+HOST_UNITS = [("input_deck", "input_deck.cpp"), ("output", "output.cpp")]
 CXX = os.environ.get("CXX", "g++")
 
 

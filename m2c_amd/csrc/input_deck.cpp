@@ -33,14 +33,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/m2c_hip.h"
+#include "deck.hpp"
 
-namespace {
-
-struct DeckError {
-  int code;
-  std::string msg;
-};
+namespace m2c_deck {
 
 [[noreturn]] void fail(int code, int line, const std::string &m) {
   throw DeckError{code, (line > 0 ? "line " + std::to_string(line) + ": " : std::string()) + m};
@@ -166,37 +161,6 @@ class Lexer {
 };
 
 // ------------------------------------------------------------------ values and the object tree
-
-struct Value {
-  enum Kind { INT, DBL, SYM, STR } kind = INT;
-  long long i = 0;
-  double d = 0.0;
-  std::string s;
-  int line = 0;
-};
-
-struct Node {
-  std::string type;   // name of the object (aliases resolved); "#" for an element of an array
-  std::map<std::string, Value> leaves;
-  std::map<std::string, std::unique_ptr<Node>> kids;   // members; array elements are "#<index>"
-  int line = 0;       // first mention
-
-  Node *kid(const std::string &n, int at) {
-    auto &p = kids[n];
-    if (!p) {
-      p.reset(new Node());
-      p->type = n[0] == '#' ? "#" : n;
-      p->line = at;
-    }
-    return p.get();
-  }
-  const Node *find(const char *n) const {
-    auto it = kids.find(n);
-    return it == kids.end() ? nullptr : it->second.get();
-  }
-  const Node *elem(long long i) const { return find(("#" + std::to_string(i)).c_str()); }
-  bool empty() const { return leaves.empty() && kids.empty(); }
-};
 
 // names bound to one object in IoData
 std::string canonical(const std::string &n) {
@@ -534,11 +498,18 @@ struct Mapper {
     getd(mn, "DensityUpperLimit", mm.rhomax);
     getd(mn, "PressureUpperLimit", mm.pmax);
     getd(mn, "DensityPrescribedAtFailure", mm.failsafe_density);
+    // temperature laws (tlaw = cv, T0, e0, rho0, cp, h0), read for outputs
     if (eos == M2C_EOS_SG) {                // StiffenedGasModelData defaults
       mm.p[0] = 1.4;
       const Node *e = mn->find("StiffenedGasModel");
       getd(e, "SpecificHeatRatio", mm.p[0]);
       getd(e, "PressureConstant", mm.p[1]);
+      getd(e, "SpecificHeatAtConstantVolume", mm.tlaw[0]);
+      getd(e, "ReferenceTemperature", mm.tlaw[1]);
+      getd(e, "ReferenceSpecificInternalEnergy", mm.tlaw[2]);
+      getd(e, "ReferenceDensity", mm.tlaw[3]);
+      getd(e, "SpecificHeatAtConstantPressure", mm.tlaw[4]);
+      getd(e, "ReferenceSpecificEnthalpy", mm.tlaw[5]);
     } else if (eos == M2C_EOS_NASG) {       // NobleAbelStiffenedGasModelData defaults
       mm.p[0] = 1.4;
       const Node *e = mn->find("NobleAbelStiffenedGasModel");
@@ -546,14 +517,20 @@ struct Mapper {
       getd(e, "PressureConstant", mm.p[1]);
       getd(e, "VolumeConstant", mm.p[2]);
       getd(e, "EnergyConstant", mm.p[3]);
+      getd(e, "SpecificHeatAtConstantVolume", mm.tlaw[0]);
     } else if (eos == M2C_EOS_MG) {         // MieGruneisenModelData defaults (copper, g-mm-s units)
       mm.p[0] = 8.96e-3; mm.p[1] = 3.933e6; mm.p[2] = 1.99; mm.p[3] = 1.5; mm.p[4] = 0.0;
+      mm.tlaw[0] = 3.90e8;
       const Node *e = mn->find("MieGruneisenModel");
       getd(e, "ReferenceDensity", mm.p[0]);
       getd(e, "BulkSpeedOfSound", mm.p[1]);
       getd(e, "ReferenceGamma", mm.p[2]);
       getd(e, "HugoniotSlope", mm.p[3]);
       getd(e, "ReferenceSpecificInternalEnergy", mm.p[4]);
+      getd(e, "SpecificHeatAtConstantVolume", mm.tlaw[0]);
+      getd(e, "ReferenceTemperature", mm.tlaw[1]);
+      getd(e, "SpecificHeatAtConstantPressure", mm.tlaw[4]);
+      getd(e, "ReferenceSpecificEnthalpy", mm.tlaw[5]);
     } else {                                // JonesWilkinsLeeModelData defaults
       mm.p[0] = 0.28; mm.p[1] = 3.712e11; mm.p[2] = 3.23e9; mm.p[3] = 4.15; mm.p[4] = 0.95; mm.p[5] = 1.63e-3;
       const Node *e = mn->find("JonesWilkinsLeeModel");
@@ -968,7 +945,9 @@ struct IcBuilder {
   }
 };
 
-}  // namespace
+}  // namespace m2c_deck
+
+using namespace m2c_deck;
 
 extern "C" {
 
@@ -1008,7 +987,7 @@ int m2c_config_defaults(m2c_config *c) {
 
 }  // extern "C"
 
-namespace {
+namespace m2c_deck {
 
 // parses and maps a deck and checks its shapes; returns the status and fills `report` (the error, or
 // the ignored options)
@@ -1021,8 +1000,7 @@ int read_deck(const char *text, Node &root, m2c_config &c, m2c_deck_run &r, IcBu
   r.initial_state[0] = 1.0e-6;      // StateVariable
   r.initial_material = 0;
   try {
-    Parser p(text, &root);
-    p.deck();
+    parse(text, root);
     Mapper m;
     m.map(&root, c, r);
     ic.read(&root, c.nmat);
@@ -1044,7 +1022,32 @@ void put_msg(const std::string &report, char *msg, int msglen) {
   }
 }
 
-}  // namespace
+void parse(const char *text, Node &root) {
+  Parser p(text, &root);
+  p.deck();
+}
+
+int read_config(const char *text, Node &root, m2c_config &c, m2c_deck_run &r, std::string &report) {
+  IcBuilder ic;
+  return read_deck(text, root, c, r, ic, report);
+}
+
+const Value *leaf(const Node *n, const char *name) { return Mapper::leaf(n, name); }
+void getd(const Node *n, const char *name, double &out) { Mapper::getd(n, name, out); }
+void geti(const Node *n, const char *name, int &out) { Mapper::geti(n, name, out); }
+void gets(const Node *n, const char *name, std::string &out) {
+  const Value *v = leaf(n, name);
+  if (!v) return;
+  if (v->kind != Value::STR) fail(M2C_ERR_ARG, v->line, std::string(name) + " takes a string");
+  out = v->s;
+}
+void gett(const Node *n, const char *name, int &out, std::initializer_list<std::pair<const char *, int>> opts,
+          std::initializer_list<const char *> unsupported) {
+  Mapper::gett(n, name, out, opts, unsupported);
+}
+std::vector<const Node *> elements(const Node *arr) { return IcBuilder::elements(arr); }
+
+}  // namespace m2c_deck
 
 extern "C" {
 

@@ -1009,6 +1009,79 @@ int m2c_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const double *V
   return M2C_OK;
 }
 
+// Temperature laws per material for outputs: 0 none (JWL, dummy), 1 T0 + (e - e0)/cv (SG, MG),
+// 2 T0 + (e + p/rho - h0)/cp (SG, MG with cp > 0 and cv <= 0), 3 the SG law with a reference density
+// (cv > 0 and rho0 > 0), 4 NASG (VarFcnSG.h:70-78, 126-142; VarFcnNASG.h:61, 112; VarFcnMG.h:113-121, 172-179).
+struct TLaw {
+  int method[M2C_MAX_MAT];
+  double invcv[M2C_MAX_MAT], T0[M2C_MAX_MAT], e0[M2C_MAX_MAT], rho0[M2C_MAX_MAT], invcp[M2C_MAX_MAT],
+      h0[M2C_MAX_MAT];
+};
+
+static TLaw build_tlaw(const m2c_config &cfg) {
+  TLaw t;
+  memset(&t, 0, sizeof(t));
+  for (int m = 0; m < cfg.nmat; m++) {
+    const m2c_material &mm = cfg.mat[m];
+    const double cv = mm.tlaw[0], cp = mm.tlaw[4];
+    t.invcv[m] = cv == 0.0 ? 0.0 : 1.0 / cv;
+    t.invcp[m] = cp == 0.0 ? 0.0 : 1.0 / cp;
+    t.T0[m] = mm.tlaw[1]; t.e0[m] = mm.tlaw[2]; t.rho0[m] = mm.tlaw[3]; t.h0[m] = mm.tlaw[5];
+    if (mm.eos == M2C_EOS_SG) t.method[m] = (cv > 0 && mm.tlaw[3] > 0) ? 3 : ((cp > 0 && cv <= 0.0) ? 2 : 1);
+    else if (mm.eos == M2C_EOS_MG) { t.method[m] = (cp > 0 && cv <= 0.0) ? 2 : 1; t.e0[m] = mm.p[4]; }
+    else if (mm.eos == M2C_EOS_NASG) t.method[m] = 4;
+  }
+  return t;
+}
+
+// e(rho, p) of each point's material and, with a temperature law, T(rho, e)
+__global__ void k_eos_derived(int n, const double *__restrict__ rho, const double *__restrict__ p,
+                              const int *__restrict__ id, TLaw t, double *__restrict__ e, double *__restrict__ T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = id[i];
+  const DevEos &m = c_eos[k];
+  const double r = rho[i];
+  const double ee = eos_e<double, -1>(m, r, p[i]);
+  e[i] = ee;
+  double TT = __builtin_nan("");
+  switch (t.method[k]) {
+  case 1: TT = t.T0[k] + t.invcv[k] * (ee - t.e0[k]); break;
+  case 2: TT = t.T0[k] + t.invcp[k] * (ee + eos_p<double, -1>(m, r, ee) / r - t.h0[k]); break;
+  case 3:
+    TT = t.invcv[k] * (ee + m.pstiff / r) +
+         pow(r / t.rho0[k], m.gam1) * (t.T0[k] - t.invcv[k] * (t.e0[k] + m.pstiff / t.rho0[k]));
+    break;
+  case 4: TT = t.invcv[k] * (ee - m.q - m.pc * (1.0 / r - m.b)); break;
+  default: break;
+  }
+  T[i] = TT;
+}
+
+int m2c_eos_derived(m2c_ctx *c, int n, const double *rho, const double *p, const int32_t *id, double *e, double *T) {
+  if (!c || n < 0 || (n > 0 && (!rho || !p || !id || !e))) return M2C_ERR_ARG;
+  for (int i = 0; i < n; i++)
+    if (id[i] < 0 || id[i] >= c->cfg.nmat) return M2C_ERR_ARG;
+  if (n == 0) return M2C_OK;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  const size_t b = sizeof(double) * (size_t)n;
+  char *d;
+  HIPCHK(hipMalloc(&d, 4 * b + sizeof(int) * (size_t)n));
+  double *dr = (double *)d, *dp = dr + n, *de = dp + n, *dT = de + n;
+  int *di = (int *)(dT + n);
+  HIPCHK(hipMemcpyAsync(dr, rho, b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dp, p, b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(di, id, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_eos_derived, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, dr, dp, di, build_tlaw(c->cfg),
+                     de, dT);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(e, de, b, hipMemcpyDeviceToHost, c->stream));
+  if (T) HIPCHK(hipMemcpyAsync(T, dT, b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d));
+  return M2C_OK;
+}
+
 // the exact solver's constants live with the interface kernels: a context without interfaces
 // uploads them before a pointwise batch
 static int ensure_riemann_constants(m2c_ctx *c) {

@@ -104,10 +104,13 @@ static int run_bubble(double scale, int steps, const char *out) {
   return 0;
 }
 
-// Main.cpp's flow: read the deck, set the initial condition, march to MaxTime or MaxIts
+// Main.cpp's flow: read the deck, set the initial condition, march to MaxTime or MaxIts; with an
+// Output object in the deck, the outputs of Main.cpp:444, :619 and :632 (forced at the start and
+// the end, by Output.Frequency / TimeInterval in between)
 static int run_deck(const char *path, const char *out) {
   IoData iod(path);
-  for (const std::string &line : iod.ignored) fprintf(stderr, "m2c_demo: %s\n", line.c_str());
+  for (const std::string &line : iod.ignored)
+    if (line.compare(0, 14, "ignored Output") != 0) fprintf(stderr, "m2c_demo: %s\n", line.c_str());
   std::vector<double> V, phi;
   std::vector<int32_t> ID;
   iod.InitialCondition(V, ID, phi);
@@ -117,8 +120,35 @@ static int run_deck(const char *path, const char *out) {
   TimeIntegrator ti(ctx);
   spo.SetState(V.data(), ID.data());
   if (iod.has_levelset()) lso.SetPhi(phi.data());
-  m2c_status s = iod.run.time_step > 0.0 ? ti.RunFixed(iod.run.time_step, iod.run.max_time, iod.run.max_its)
-                                         : ti.Run(iod.run.max_time, iod.run.max_its, 64);
+  const bool outputs = iod.text().find("Output") != std::string::npos;
+  m2c_status s;
+  if (!outputs) {
+    s = iod.run.time_step > 0.0 ? ti.RunFixed(iod.run.time_step, iod.run.max_time, iod.run.max_its)
+                                : ti.Run(iod.run.max_time, iod.run.max_its, 64);
+  } else {
+    Output o(iod);
+    for (const std::string &line : o.not_written) fprintf(stderr, "m2c_demo: %s\n", line.c_str());
+    double *ph = iod.has_levelset() ? phi.data() : nullptr;
+    s = ti.Status();
+    o.OutputSolutions(ctx.get(), s.t, 0.0, (int)s.steps, V.data(), ID.data(), ph, true);
+    double dt = 0.0;
+    while (s.t < iod.run.max_time && s.steps < iod.run.max_its) {
+      if (iod.run.time_step > 0.0) {
+        dt = s.t + iod.run.time_step > iod.run.max_time ? iod.run.max_time - s.t : iod.run.time_step;
+        ti.AdvanceOneTimeStep(dt);
+      } else {
+        double cfl;           // the step's dt, as m2c_run will take it (Main.cpp:524-539)
+        spo.ComputeTimeStepSize(dt, cfl);
+        if (s.t + dt > iod.run.max_time) dt = iod.run.max_time - s.t;
+        ti.Run(iod.run.max_time, 1, 1);
+      }
+      s = ti.Status();
+      spo.GetState(V.data(), ID.data());
+      if (ph) lso.GetPhi(ph);
+      o.OutputSolutions(ctx.get(), s.t, dt, (int)s.steps, V.data(), ID.data(), ph, false);
+    }
+    o.OutputSolutions(ctx.get(), s.t, dt, (int)s.steps, V.data(), ID.data(), ph, true);
+  }
   spo.GetState(V.data(), ID.data());
   if (iod.has_levelset()) lso.GetPhi(phi.data());
   FILE *f = fopen(out, "wb");

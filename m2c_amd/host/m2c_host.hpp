@@ -100,6 +100,7 @@ class IoData {
 
   long long cells() const { return (long long)config.nx * config.ny * config.nz; }
   bool has_levelset() const { return config.ls_matid >= 0; }
+  const std::string &text() const { return text_; }
   // SpaceOperator::SetInitialCondition and LevelSetOperator::SetInitialCondition; phi is left
   // empty when the deck has no level set
   void InitialCondition(std::vector<double> &V, std::vector<int32_t> &ID, std::vector<double> &phi) const {
@@ -125,6 +126,33 @@ class IoData {
     if (rc != M2C_OK) throw Error(std::string("input deck: ") + msg, rc);
   }
   std::string text_;
+};
+
+// Output (Output.h): snapshots, probes and line plots of a deck's Output object.
+class Output {
+ public:
+  explicit Output(const IoData &iod) {
+    std::vector<char> msg(1 << 16);
+    const int rc = m2c_output_create(iod.text().c_str(), &out_, msg.data(), (int)msg.size());
+    if (rc != M2C_OK) throw Error(std::string("output: ") + msg.data(), rc);
+    std::stringstream ss(msg.data());
+    for (std::string line; std::getline(ss, line);)
+      if (!line.empty()) not_written.push_back(line);
+  }
+  ~Output() { m2c_output_destroy(out_); }
+  Output(const Output &) = delete;
+  Output &operator=(const Output &) = delete;
+  // Output::OutputSolutions (Output.cpp:117-143); returns what was written (1 snapshot, 2 probes, 4 lines)
+  int OutputSolutions(m2c_ctx *ctx, double t, double dt, int time_step, const double *V, const int32_t *ID,
+                      const double *phi, bool force_write) {
+    int w = 0;
+    check(m2c_output_write(out_, ctx, t, dt, time_step, V, ID, phi, force_write ? 1 : 0, &w), "OutputSolutions");
+    return w;
+  }
+  std::vector<std::string> not_written;
+
+ private:
+  m2c_output *out_ = nullptr;
 };
 
 // One device context: the SpaceOperator / TimeIntegrator / LevelSetOperator of a subdomain.

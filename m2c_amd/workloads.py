@@ -169,7 +169,7 @@ under Time {{
   Type = Explicit; MaxTime = 0.1; CFL = 0.5;
   under Explicit {{ Type = RungeKutta2; }}
 }}
-under Output {{ Prefix = "results/"; Frequency = 100; }}
+under Output {{ Prefix = ""; Solution = "sod"; Frequency = 100; Density = On; Velocity = On; Pressure = On; }}
 """
 
 

@@ -18,7 +18,8 @@ def demo(args, tmp_path):
     exe = build.DEMO
     assert os.path.exists(exe), "m2c_amd/m2c_demo is built by m2c_amd/build.py"
     out = str(tmp_path / "out.bin")
-    r = subprocess.run([exe] + [str(a) for a in args] + [out], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe] + [str(a) for a in args] + [out], capture_output=True, text=True, timeout=120,
+                       cwd=str(tmp_path))
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     return np.fromfile(out, dtype=np.uint8).tobytes(), r.stdout
@@ -56,6 +57,8 @@ def test_cpp_host_layer_runs_the_sod_deck(tmp_path):
     # Main.cpp's flow from an input deck: IoData, initial condition, march to Time.MaxTime
     raw, txt = demo(["deck", deck_file(tmp_path, W.sod_deck(200))], tmp_path)
     settings, V0, ID0 = W.sod_1d(200)
+    # the deck's Output object: snapshots every 100 steps plus the forced first and last
+    assert (tmp_path / "sod.pvd").exists() and (tmp_path / "sod_0000.vtr").exists()
     ost, t, n, Vo, _ = cases.run_oracle(settings, V0, ID0, tmax=0.1, maxsteps=100000, vanalbada_pow=False)
     V = np.frombuffer(raw[:200 * 5 * 8], dtype=np.float64).reshape(1, 1, 200, 5)
     assert f"steps={n} " in txt

@@ -52,7 +52,7 @@ def test_sod_deck_reads_back_to_the_workload():
     settings, V0, ID0 = W.sod_1d(200)
     assert same_config(cfg, W.config_from(settings))
     assert run.max_time == 0.1 and run.max_its == 2**31 - 1 and run.time_step == -1.0
-    line = text.splitlines().index('under Output { Prefix = "results/"; Frequency = 100; }') + 1
+    line = [ln.startswith("under Output") for ln in text.splitlines()].index(True) + 1
     assert ignored == [f"ignored Output (line {line})"]
     V, ID, phi = M.initial_condition_from_deck(text)
     assert phi is None
