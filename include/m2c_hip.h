@@ -54,6 +54,11 @@ typedef struct {
   double tlaw[6];                   /* temperature law (output only): cv, T0, e0, rho0, cp, h0.
                                        SG: all six (VarFcnSG.h:126-142); NASG: cv; MG: cv, T0, cp, h0
                                        (its e0 is p[4]); JWL and DUMMY have none. */
+  int viscosity;                    /* ViscosityModel.Type (ViscoFcn.h): 0 none, 1 constant, 2 Sutherland */
+  double mu;                        /* DynamicViscosity (Sutherland: ReferenceDynamicViscosity) */
+  double bulk;                      /* BulkViscosity */
+  double su_T0, su_S;               /* Sutherland: ReferenceTemperature, SutherlandConstant */
+  double kappa;                     /* HeatDiffusionModel (Type Constant): Diffusivity; 0 = none */
 } m2c_material;
 
 typedef struct {

@@ -25,7 +25,8 @@ DBL_MAX = 1.7976931348623157e308
 class Material(C.Structure):
     _fields_ = [("eos", C.c_int), ("rhomin", C.c_double), ("pmin", C.c_double), ("rhomax", C.c_double),
                 ("pmax", C.c_double), ("failsafe_density", C.c_double), ("p", C.c_double * 8),
-                ("tlaw", C.c_double * 6)]
+                ("tlaw", C.c_double * 6), ("viscosity", C.c_int), ("mu", C.c_double), ("bulk", C.c_double),
+                ("su_T0", C.c_double), ("su_S", C.c_double), ("kappa", C.c_double)]
 
 
 class RiemannParams(C.Structure):
@@ -84,11 +85,17 @@ class TrainReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+VISC_NONE, VISC_CONSTANT, VISC_SUTHERLAND = 0, 1, 2
+
+
 def material(kind, params, rhomin=0.0, pmin=-DBL_MAX, rhomax=DBL_MAX, pmax=DBL_MAX, failsafe_density=1e-10,
-             tlaw=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)):
-    """MaterialModelData (IoData.cpp:860-906): EOS kind + parameters + cut-offs, and the temperature
-    law (cv, T0, e0, rho0, cp, h0) used only by outputs."""
+             tlaw=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0), viscosity=VISC_NONE, mu=0.0, bulk=0.0, su_T0=273.0, su_S=111.0,
+             kappa=0.0):
+    """MaterialModelData (IoData.cpp:860-906): EOS kind + parameters + cut-offs, the temperature law
+    (cv, T0, e0, rho0, cp, h0; outputs, Sutherland viscosity and heat diffusion), the viscosity model
+    (ViscoFcn.h) and the heat diffusivity (HeatDiffusionFcn.h)."""
     m = Material()
+    m.viscosity, m.mu, m.bulk, m.su_T0, m.su_S, m.kappa = viscosity, mu, bulk, su_T0, su_S, kappa
     m.eos = kind
     m.rhomin, m.pmin, m.rhomax, m.pmax, m.failsafe_density = rhomin, pmin, rhomax, pmax, failsafe_density
     for i, v in enumerate(params):

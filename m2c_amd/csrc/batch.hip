@@ -300,6 +300,11 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
       delete b;
       return M2C_ERR_UNSUPPORTED;
     }
+    for (int m = 0; m < c.nmat; m++)      // inviscid, adiabatic instances only
+      if (c.mat[m].viscosity != 0 || c.mat[m].kappa != 0.0) {
+        delete b;
+        return M2C_ERR_UNSUPPORTED;
+      }
     int variant = 0;
     DevEos eos[8];
     int rc = m2c_fill_grid(&c, &b->hinst[i].grid, eos, &variant);

@@ -83,6 +83,12 @@ struct StageArgs {
   int kchunk;
   int ignore_active;           // 1: run even if the clock is inactive (direct API calls)
   int no_cfl;                  // 1: the final stage does not feed the CFL minimum (it is taken later)
+  // viscous and heat-diffusion terms per face (null: none). Dv[d]: 10 components x faces of
+  // direction d (0-4 added to the plus cell, 5-9 subtracted from the minus cell); Dh[d]: the heat
+  // flux per face (energy only). Faces of direction d are indexed (k, j, i) with n_d + 1 planes
+  // along d.
+  const double *Dv[3] = {nullptr, nullptr, nullptr};
+  const double *Dh[3] = {nullptr, nullptr, nullptr};
 };
 
 // Each translation unit that launches kernels holds its own copy of these tables (internal
@@ -548,6 +554,15 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
         }
       }
       double R[5];
+      // viscous and heat-diffusion faces of this cell (SpaceOperator.cpp:3367-3371): added after the
+      // advective sum, own faces first, then the east, north and front faces (ViscosityOperator.cpp:
+      // 259-393, HeatDiffusionOperator.cpp:96-131)
+      const bool diff = a.Dv[0] != nullptr;
+      const long long fxw = ((long long)k * g.ny + gj) * (g.nx + 1) + gi;
+      const long long fys = ((long long)k * (g.ny + 1) + gj) * g.nx + gi;
+      const long long fzb = ((long long)k * g.ny + gj) * g.nx + gi;
+      const long long nfx = (long long)(g.nx + 1) * g.ny * g.nz, nfy = (long long)g.nx * (g.ny + 1) * g.nz,
+                      nfz = (long long)g.nx * g.ny * (g.nz + 1), nxy = (long long)g.nx * g.ny;
 #pragma unroll
       for (int p = 0; p < 5; p++) {
         double s = 0.0;
@@ -557,6 +572,22 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
         s += Fe[p] * g.ax;
         s += Fn[p] * g.ay;
         s += Ft[p] * g.az;
+        if (diff) {
+          s += a.Dv[0][p * nfx + fxw];
+          s += a.Dv[1][p * nfy + fys];
+          s += a.Dv[2][p * nfz + fzb];
+          s -= a.Dv[0][(5 + p) * nfx + fxw + 1];
+          s -= a.Dv[1][(5 + p) * nfy + fys + g.nx];
+          s -= a.Dv[2][(5 + p) * nfz + fzb + nxy];
+          if (p == 4) {
+            s += a.Dh[0][fxw];
+            s += a.Dh[1][fys];
+            s += a.Dh[2][fzb];
+            s -= a.Dh[0][fxw + 1];
+            s -= a.Dh[1][fys + g.nx];
+            s -= a.Dh[2][fzb + nxy];
+          }
+        }
         R[p] = s * g.inv_mvol;
         if (!dfinite(R[p])) flags |= ST_NAN_FLUX;
       }

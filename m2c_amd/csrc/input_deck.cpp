@@ -541,8 +541,24 @@ struct Mapper {
       getd(e, "R2", mm.p[4]);
       getd(e, "Rho0", mm.p[5]);
     }
-    reject_type(mn->find("ViscosityModel"), "Material.ViscosityModel");
-    reject_type(mn->find("HeatDiffusionModel"), "Material.HeatDiffusionModel");
+    // ViscosityModelData / HeatDiffusionModelData (IoData.cpp defaults: mu 1.716e-5, Sutherland
+    // T0 273, S 111, mu0 1.716e-5, bulk 0; diffusivity 0)
+    const Node *vm = mn->find("ViscosityModel");
+    gett(vm, "Type", mm.viscosity, {{"None", 0}, {"Constant", 1}, {"Sutherland", 2}}, {"Artificial"});
+    double dyn = 1.716e-5, mu0 = 1.716e-5;
+    mm.bulk = 0.0; mm.su_T0 = 273.0; mm.su_S = 111.0;
+    getd(vm, "DynamicViscosity", dyn);
+    getd(vm, "ReferenceDynamicViscosity", mu0);
+    getd(vm, "BulkViscosity", mm.bulk);
+    getd(vm, "ReferenceTemperature", mm.su_T0);
+    getd(vm, "SutherlandConstant", mm.su_S);
+    mm.mu = mm.viscosity == 2 ? mu0 : (mm.viscosity == 1 ? dyn : 0.0);
+    const Node *hd = mn->find("HeatDiffusionModel");
+    int htype = 0;
+    gett(hd, "Type", htype, {{"None", 0}, {"Constant", 1}});
+    double kap = 0.0;
+    getd(hd, "Diffusivity", kap);
+    mm.kappa = htype == 1 ? kap : 0.0;
     reject_type(mn->find("HyperelasticityModel"), "Material.HyperelasticityModel");
   }
 

@@ -25,6 +25,20 @@ using namespace m2c;
     }                                                                               \
   } while (0)
 
+struct DiffTab {
+  const double *x[3];          // coordinates, index -1..n stored at [q+1]
+  const double *wl[3], *wr[3]; // interpolation weights of face q-1/2, q = 0..n
+  const double *cc[3];         // central node-derivative coefficients, 3 per node q = 0..n-1
+  double clo[3], chi[3];       // one-sided coefficients at q = 0 and q = n-1
+  int visc, heat;
+};
+
+struct DiffMat {
+  int visc[M2C_MAX_MAT];
+  double mu[M2C_MAX_MAT], bulk[M2C_MAX_MAT], suT0[M2C_MAX_MAT], suS[M2C_MAX_MAT], kappa[M2C_MAX_MAT];
+  TLaw tl;
+};
+
 struct m2c_ctx {
   m2c_config cfg;
   Grid grid;
@@ -49,11 +63,18 @@ struct m2c_ctx {
   double *phiA = nullptr, *phiB = nullptr, *phiD = nullptr;
   double *xc = nullptr, *fd = nullptr;   // coordinates and FD3 tables (device)
   std::vector<void *> mp_bufs;
+  // viscous and heat-diffusion terms
+  bool diff = false;
+  double *Dv[3] = {nullptr, nullptr, nullptr}, *Dh[3] = {nullptr, nullptr, nullptr};
+  long long nf[3] = {0, 0, 0};
+  DiffTab dtab;
+  DiffMat dmat;
 };
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
 
 static int ensure_riemann_constants(m2c_ctx *c);
+static TLaw build_tlaw(const m2c_config &cfg);
 
 // ---------------------------------------------------------------- small kernels
 
@@ -248,9 +269,295 @@ static int ensure_constants(m2c_ctx *c) {
   return upload_constants(c);
 }
 
+// device buffers owned by the context (freed with it)
+template <typename T> static int mp_alloc(m2c_ctx *c, T **p, size_t count, int fill) {
+  HIPCHK(hipMalloc((void **)p, sizeof(T) * (count ? count : 1)));
+  HIPCHK(hipMemset(*p, fill, sizeof(T) * (count ? count : 1)));
+  c->mp_bufs.push_back((void *)*p);
+  return M2C_OK;
+}
+
+// ---------------------------------------------------------------- viscous and heat-diffusion terms
+//
+// ViscosityOperator::AddDiffusionFluxes (ViscosityOperator.cpp:200-404) and HeatDiffusionOperator::
+// AddDiffusionFluxes (HeatDiffusionOperator.cpp:64-139): one thread per face computes the face
+// velocity (InterpolatorLinear, Interpolator.cpp:32-181), the velocity gradient (central differences
+// across the face; along the face the node derivatives of GradientCalculatorCentral.cpp:22-207, with
+// its one-sided rows next to the ghost layer, interpolated to the face) and the stresses of
+// ViscoFcn.h, and the harmonic-mean heat flux. The stage kernel adds them after the advective sum.
+// Reproduced as the reference writes them: the k-1/2 face of two materials takes the upper cell's
+// viscosity function for both sides (ViscosityOperator.cpp:386); Sutherland's law evaluates the
+// temperature at (rho, p) (ViscoFcn.h:255); the j-1/2 and k-1/2 heat fluxes divide by the i-1/2
+// face's diffusivity sum, carried over from the last cell that had one on the north ghost row and
+// the top ghost plane (HeatDiffusionOperator.cpp:106-127).
+
+// material id of (possibly ghost) cell (i, j, k): the face ghosts take their image's
+__device__ __forceinline__ int id_at(const int8_t *__restrict__ ID, int i, int j, int k) {
+  const Grid &g = c_grid;
+  int ii = clampi(image_idx(i, g.nx, (i < 0 ? g.bc[0] : g.bc[1]) == BC_PERIODIC), 0, g.nx - 1);
+  int jj = clampi(image_idx(j, g.ny, (j < 0 ? g.bc[2] : g.bc[3]) == BC_PERIODIC), 0, g.ny - 1);
+  int kk = clampi(image_idx(k, g.nz, (k < 0 ? g.bc[4] : g.bc[5]) == BC_PERIODIC), 0, g.nz - 1);
+  return ID[((long long)kk * g.ny + jj) * g.nx + ii];
+}
+
+template <int IN_CONS>
+__device__ __forceinline__ void dcell(const double *__restrict__ S, const int8_t *__restrict__ ID, int i, int j,
+                                      int k, double *V, int &id) {
+  unsigned fl = 0;
+  fetch_cell<IN_CONS, -1>(S, ID, i, j, k, V, id, fl);
+}
+
+// d u_m / d x_d at node (i, j, k), m = 0..2 (velocity), given the node's own velocity uc
+template <int IN_CONS>
+__device__ void node_deriv(const DiffTab &t, const double *__restrict__ S, const int8_t *__restrict__ ID, int d,
+                           int i, int j, int k, const double *uc, double *out) {
+  const Grid &g = c_grid;
+  const int n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+  const int q = d == 0 ? i : (d == 1 ? j : k);
+  bool ghostrow;
+  if (d == 0) ghostrow = (j == -1 || j == g.ny || k == -1 || k == g.nz);
+  else if (d == 1) ghostrow = (i == -1 || i == g.nx || k == -1 || k == g.nz);
+  else ghostrow = (i == -1 || i == g.nx || j == -1 || j == g.ny);
+  const int di = d == 0, dj = d == 1, dk = d == 2;
+  double W[5];
+  int idd;
+  if (q == 0 && ghostrow) {
+    dcell<IN_CONS>(S, ID, i + di, j + dj, k + dk, W, idd);
+    const double coeff = t.clo[d], c1 = -coeff, c2 = coeff;
+    for (int m = 0; m < 3; m++) out[m] = c1 * uc[m] + c2 * W[1 + m];
+    return;
+  }
+  if (q == n - 1 && ghostrow) {
+    dcell<IN_CONS>(S, ID, i - di, j - dj, k - dk, W, idd);
+    const double coeff = t.chi[d], c0 = -coeff, c1 = coeff;
+    for (int m = 0; m < 3; m++) out[m] = c0 * W[1 + m] + c1 * uc[m];
+    return;
+  }
+  double P[5];
+  dcell<IN_CONS>(S, ID, i - di, j - dj, k - dk, W, idd);
+  dcell<IN_CONS>(S, ID, i + di, j + dj, k + dk, P, idd);
+  const double c0 = t.cc[d][3 * q], c1 = t.cc[d][3 * q + 1], c2 = t.cc[d][3 * q + 2];
+  for (int m = 0; m < 3; m++) out[m] = c0 * W[1 + m] + c1 * uc[m] + c2 * P[1 + m];
+}
+
+// EvaluateViscousFluxFunction_F/G/H (ViscoFcn.h:61-110), Constant and Sutherland stresses
+template <int DIR>
+__device__ void visc_flux(const DiffMat &dm, int mid, const double D[3][3], const double *V3, double rho, double p,
+                          double *F) {
+  if (dm.visc[mid] == 0) {
+    F[0] = F[1] = F[2] = F[3] = F[4] = 0.0;
+    return;
+  }
+  double dyna = dm.mu[mid];
+  if (dm.visc[mid] == 2) {
+    const double T = tlaw_temperature(dm.tl, c_eos[mid], mid, rho, p);
+    dyna = dm.mu[mid] * pow(T / dm.suT0[mid], 1.5) * (dm.suT0[mid] + dm.suS[mid]) / (T + dm.suS[mid]);
+  }
+  const double bulk = dm.bulk[mid];
+  const double *dVdx = D[0], *dVdy = D[1], *dVdz = D[2];
+  const double div = dVdx[0] + dVdy[1] + dVdz[2];
+  double t0, t1, t2;
+  if (DIR == 0) {
+    t0 = bulk * div + dyna * (2.0 * dVdx[0] - 2.0 / 3.0 * div);
+    t1 = dyna * (dVdx[1] + dVdy[0]);
+    t2 = dyna * (dVdx[2] + dVdz[0]);
+  } else if (DIR == 1) {
+    t0 = dyna * (dVdx[1] + dVdy[0]);
+    t1 = bulk * div + dyna * (2.0 * dVdy[1] - 2.0 / 3.0 * div);
+    t2 = dyna * (dVdy[2] + dVdz[1]);
+  } else {
+    t0 = dyna * (dVdx[2] + dVdz[0]);
+    t1 = dyna * (dVdy[2] + dVdz[1]);
+    t2 = bulk * div + dyna * (2.0 * dVdz[2] - 2.0 / 3.0 * div);
+  }
+  F[0] = 0.0;
+  F[1] = t0;
+  F[2] = t1;
+  F[3] = t2;
+  F[4] = V3[0] * t0 + V3[1] * t1 + V3[2] * t2;
+}
+
+template <int DIR, int IN_CONS>
+__global__ __launch_bounds__(256) void k_diff_faces(const double *__restrict__ S, const int8_t *__restrict__ ID,
+                                                    DiffTab t, DiffMat dm, double *__restrict__ Dv,
+                                                    double *__restrict__ Dh, long long nf) {
+  const Grid &g = c_grid;
+  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int nxf = g.nx + (DIR == 0), nyf = g.ny + (DIR == 1);
+  const int i = (int)(f % nxf), j = (int)((f / nxf) % nyf), k = (int)(f / ((long long)nxf * nyf));
+  const int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  double Vh[5], Vl[5];
+  int idh, idl;
+  dcell<IN_CONS>(S, ID, i, j, k, Vh, idh);
+  dcell<IN_CONS>(S, ID, i - di, j - dj, k - dk, Vl, idl);
+  const int q = DIR == 0 ? i : (DIR == 1 ? j : k);
+  const double *x = t.x[DIR] + 1;
+  const double area = DIR == 0 ? g.ax : (DIR == 1 ? g.ay : g.az);
+  const int inactive = g.inactive_id;
+  double out[10];
+#pragma unroll
+  for (int c = 0; c < 10; c++) out[c] = 0.0;
+  if (t.visc && idh != inactive) {
+    const double wl = t.wl[DIR][q], wr = t.wr[DIR][q], dd = x[q] - x[q - 1];
+    double V3[3], D[3][3];
+    for (int m = 0; m < 3; m++) {
+      V3[m] = wl * Vl[1 + m] + wr * Vh[1 + m];
+      D[DIR][m] = (Vh[1 + m] - Vl[1 + m]) / dd;
+    }
+    for (int ax = 0; ax < 3; ax++) {
+      if (ax == DIR) continue;
+      double Dl[3], Dhh[3];
+      node_deriv<IN_CONS>(t, S, ID, ax, i - di, j - dj, k - dk, Vl + 1, Dl);
+      node_deriv<IN_CONS>(t, S, ID, ax, i, j, k, Vh + 1, Dhh);
+      for (int m = 0; m < 3; m++) D[ax][m] = wl * Dl[m] + wr * Dhh[m];
+    }
+    double F[5];
+    if (idl == idh) {
+      visc_flux<DIR>(dm, idh, D, V3, 0.5 * (Vh[0] + Vl[0]), 0.5 * (Vh[4] + Vl[4]), F);
+      for (int c = 0; c < 5; c++) out[c] = out[5 + c] = F[c] * area;
+    } else {
+      visc_flux<DIR>(dm, idh, D, V3, Vh[0], Vh[4], F);
+      for (int c = 0; c < 5; c++) out[c] = area * F[c];
+      if (idl != inactive) {
+        visc_flux<DIR>(dm, DIR == 2 ? idh : idl, D, V3, Vl[0], Vl[4], F);
+        for (int c = 0; c < 5; c++) out[5 + c] = area * F[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 10; c++) Dv[c * nf + f] = out[c];
+  double hf = 0.0;
+  if (t.heat && idh != inactive) {
+    const double myk = dm.kappa[idh], neighk = dm.kappa[idl];
+    double denom;
+    if (DIR == 0) denom = myk + neighk;
+    else if ((DIR == 1 && j < g.ny) || (DIR == 2 && k < g.nz)) denom = myk + dm.kappa[id_at(ID, i - 1, j, k)];
+    else {
+      // the last i-1/2 face computed before this ghost row / plane: cell (nx, ny-1, k') with
+      // k' = k on the north ghost row, nz-1 on the top ghost plane
+      const int kk = DIR == 1 ? k : g.nz - 1;
+      denom = dm.kappa[id_at(ID, g.nx, g.ny - 1, kk)] + dm.kappa[id_at(ID, g.nx - 1, g.ny - 1, kk)];
+    }
+    const double eh = eos_e<double, -1>(c_eos[idh], Vh[0], Vh[4]);
+    const double el = eos_e<double, -1>(c_eos[idl], Vl[0], Vl[4]);
+    const double Th = tlaw_temperature(dm.tl, c_eos[idh], idh, Vh[0], eh);
+    const double Tl = tlaw_temperature(dm.tl, c_eos[idl], idl, Vl[0], el);
+    const double dT = (Th - Tl) / (x[q] - x[q - 1]);
+    double flux = (denom == 0) ? 0.0 : 2.0 * myk * neighk / denom * dT;
+    flux *= area;
+    hf = flux;
+  }
+  Dh[f] = hf;
+}
+
+static int setup_diffusion(m2c_ctx *c) {
+  const m2c_config &cfg = c->cfg;
+  bool visc = false, heat = false;
+  for (int m = 0; m < cfg.nmat; m++) {
+    visc |= cfg.mat[m].viscosity != 0;
+    heat |= cfg.mat[m].kappa != 0.0;
+  }
+  c->diff = visc || heat;
+  if (!c->diff) return M2C_OK;
+  const Grid &g = c->grid;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  c->nf[0] = (long long)(g.nx + 1) * g.ny * g.nz;
+  c->nf[1] = (long long)g.nx * (g.ny + 1) * g.nz;
+  c->nf[2] = (long long)g.nx * g.ny * (g.nz + 1);
+  int rc;
+  for (int d = 0; d < 3; d++) {
+    if ((rc = mp_alloc(c, &c->Dv[d], (size_t)(10 * c->nf[d]), 0))) return rc;
+    if ((rc = mp_alloc(c, &c->Dh[d], (size_t)c->nf[d], 0))) return rc;
+  }
+  // coordinates x[-1..n] (MeshGenerator.cpp:146-158, ghosts SpaceOperator.cpp:218-265), interpolation
+  // weights (Interpolator.cpp:150-181) and node-derivative coefficients (GradientCalculatorCentral.cpp:
+  // 249-424), evaluated as the reference evaluates them
+  const double lo[3] = {cfg.x0, cfg.y0, cfg.z0}, h[3] = {g.dx, g.dy, g.dz};
+  std::vector<double> tab;
+  size_t ox[3], owl[3], owr[3], occ[3];
+  DiffTab t;
+  for (int d = 0; d < 3; d++) {
+    std::vector<double> x(n[d] + 2);
+    for (int q = 0; q < n[d]; q++) x[q + 1] = lo[d] + 0.5 * h[d] + q * h[d];
+    x[0] = x[1] - h[d];
+    x[n[d] + 1] = x[n[d]] + h[d];
+    const double *X = x.data() + 1;
+    ox[d] = tab.size();
+    tab.insert(tab.end(), x.begin(), x.end());
+    owl[d] = tab.size();
+    for (int q = 0; q <= n[d]; q++) {
+      const double xm = X[q] - 0.5 * h[d], dd = X[q] - X[q - 1];
+      tab.push_back((X[q] - xm) / dd);
+    }
+    owr[d] = tab.size();
+    for (int q = 0; q <= n[d]; q++) {
+      const double xm = X[q] - 0.5 * h[d], dd = X[q] - X[q - 1];
+      tab.push_back((xm - X[q - 1]) / dd);
+    }
+    occ[d] = tab.size();
+    for (int q = 0; q < n[d]; q++) {
+      const double d0 = X[q] - X[q - 1], d1 = X[q + 1] - X[q], d2 = d0 + d1;
+      tab.push_back(-d1 / (d0 * d2));
+      tab.push_back(1.0 / d0 - 1.0 / d1);
+      tab.push_back(d0 / (d1 * d2));
+    }
+    t.clo[d] = 1.0 / (X[1] - X[0]);
+    t.chi[d] = 1.0 / (X[n[d] - 1] - X[n[d] - 2]);
+  }
+  double *dt;
+  if ((rc = mp_alloc(c, &dt, tab.size(), 0))) return rc;
+  HIPCHK(hipMemcpy(dt, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  for (int d = 0; d < 3; d++) {
+    t.x[d] = dt + ox[d];
+    t.wl[d] = dt + owl[d];
+    t.wr[d] = dt + owr[d];
+    t.cc[d] = dt + occ[d];
+  }
+  t.visc = visc;
+  t.heat = heat;
+  c->dtab = t;
+  DiffMat &dm = c->dmat;
+  memset(&dm, 0, sizeof(dm));
+  for (int m = 0; m < cfg.nmat; m++) {
+    dm.visc[m] = cfg.mat[m].viscosity;
+    dm.mu[m] = cfg.mat[m].mu;
+    dm.bulk[m] = cfg.mat[m].bulk;
+    dm.suT0[m] = cfg.mat[m].su_T0;
+    dm.suS[m] = cfg.mat[m].su_S;
+    dm.kappa[m] = cfg.mat[m].kappa;
+  }
+  dm.tl = build_tlaw(cfg);
+  return M2C_OK;
+}
+
+static int diffusion_pass(m2c_ctx *c, const double *S, int in_cons) {
+  for (int d = 0; d < 3; d++) {
+    dim3 gb((unsigned)((c->nf[d] + 255) / 256)), bs(256);
+    if (in_cons) {
+      if (d == 0) hipLaunchKernelGGL((k_diff_faces<0, 1>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[0], c->Dh[0], c->nf[0]);
+      else if (d == 1) hipLaunchKernelGGL((k_diff_faces<1, 1>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[1], c->Dh[1], c->nf[1]);
+      else hipLaunchKernelGGL((k_diff_faces<2, 1>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[2], c->Dh[2], c->nf[2]);
+    } else {
+      if (d == 0) hipLaunchKernelGGL((k_diff_faces<0, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[0], c->Dh[0], c->nf[0]);
+      else if (d == 1) hipLaunchKernelGGL((k_diff_faces<1, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[1], c->Dh[1], c->nf[1]);
+      else hipLaunchKernelGGL((k_diff_faces<2, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[2], c->Dh[2], c->nf[2]);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return M2C_OK;
+}
+
 static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, const double *Vn, const double *Us,
                         double *Out, double a, double b, double cc, int first, int ignore_active) {
   StageArgs s;
+  if (c->diff) {
+    if (int rc = diffusion_pass(c, S, in_cons)) return rc;
+    for (int d = 0; d < 3; d++) {
+      s.Dv[d] = c->Dv[d];
+      s.Dh[d] = c->Dh[d];
+    }
+  }
   s.no_cfl = c->ls ? 1 : 0;
   s.S = S; s.Vn = Vn; s.Us = Us; s.Out = Out; s.ID = c->ID;
   s.Fi = c->Fi; s.Nf = c->Nf;
@@ -385,12 +692,6 @@ static void launch_cfl(m2c_ctx *c) {
   hipLaunchKernelGGL(k_cfl_min, dim3(nb), dim3(256), 0, c->stream, c->V, c->ID, c->clk, c->st);
 }
 
-template <typename T> static int mp_alloc(m2c_ctx *c, T **p, size_t count, int fill) {
-  HIPCHK(hipMalloc((void **)p, sizeof(T) * (count ? count : 1)));
-  HIPCHK(hipMemset(*p, fill, sizeof(T) * (count ? count : 1)));
-  c->mp_bufs.push_back((void *)*p);
-  return M2C_OK;
-}
 
 // Buffers and tables of the interface pass and the level set (per-cell arrays without a ghost
 // frame; face arrays keyed by the face's plus cell).
@@ -640,6 +941,21 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
       if (m != cfg->ls_matid && m != cfg->inactive_id) { nuni++; only = m; }
     c->id_single = (nuni == 1 && only == 0);
   }
+  // viscous and heat-diffusion terms: two or more cells along every axis (with one the reference
+  // reads unpopulated edge ghosts); Sutherland viscosity and heat diffusion need a temperature law
+  for (int m = 0; m < cfg->nmat; m++) {
+    const m2c_material &mm = cfg->mat[m];
+    if (mm.viscosity < 0 || mm.viscosity > 2) bad = bad ? bad : M2C_ERR_ARG;
+    const bool d = mm.viscosity != 0 || mm.kappa != 0.0;
+    if (d && (g.nx < 2 || g.ny < 2 || g.nz < 2)) bad = bad ? bad : M2C_ERR_UNSUPPORTED;
+    if (d && cfg->inactive_id >= 0) bad = bad ? bad : M2C_ERR_UNSUPPORTED;
+  }
+  {
+    bool needT = false;
+    for (int m = 0; m < cfg->nmat; m++) needT |= cfg->mat[m].viscosity == 2 || cfg->mat[m].kappa != 0.0;
+    for (int m = 0; needT && m < cfg->nmat; m++)
+      if (cfg->mat[m].eos == M2C_EOS_JWL || cfg->mat[m].eos == M2C_EOS_DUMMY) bad = bad ? bad : M2C_ERR_ARG;
+  }
   if (bad) {
     delete c;
     return bad;
@@ -658,6 +974,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   HIPCHK(hipMalloc(&c->st, sizeof(Status)));
   HIPCHK(hipMemset(c->st, 0, sizeof(Status)));
   if (int rc = setup_multiphase(c)) return rc;
+  if (int rc = setup_diffusion(c)) return rc;
   if (upload_constants(c)) return M2C_ERR_HIP;
   hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(1), 0, c->stream, c->clk, 0.0, 0.0, cfg->cfl, 0);
 
@@ -1009,15 +1326,7 @@ int m2c_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const double *V
   return M2C_OK;
 }
 
-// Temperature laws per material for outputs: 0 none (JWL, dummy), 1 T0 + (e - e0)/cv (SG, MG),
-// 2 T0 + (e + p/rho - h0)/cp (SG, MG with cp > 0 and cv <= 0), 3 the SG law with a reference density
-// (cv > 0 and rho0 > 0), 4 NASG (VarFcnSG.h:70-78, 126-142; VarFcnNASG.h:61, 112; VarFcnMG.h:113-121, 172-179).
-struct TLaw {
-  int method[M2C_MAX_MAT];
-  double invcv[M2C_MAX_MAT], T0[M2C_MAX_MAT], e0[M2C_MAX_MAT], rho0[M2C_MAX_MAT], invcp[M2C_MAX_MAT],
-      h0[M2C_MAX_MAT];
-};
-
+// the temperature-law constants of the materials (TLaw, physics.hpp)
 static TLaw build_tlaw(const m2c_config &cfg) {
   TLaw t;
   memset(&t, 0, sizeof(t));
@@ -1044,18 +1353,7 @@ __global__ void k_eos_derived(int n, const double *__restrict__ rho, const doubl
   const double r = rho[i];
   const double ee = eos_e<double, -1>(m, r, p[i]);
   e[i] = ee;
-  double TT = __builtin_nan("");
-  switch (t.method[k]) {
-  case 1: TT = t.T0[k] + t.invcv[k] * (ee - t.e0[k]); break;
-  case 2: TT = t.T0[k] + t.invcp[k] * (ee + eos_p<double, -1>(m, r, ee) / r - t.h0[k]); break;
-  case 3:
-    TT = t.invcv[k] * (ee + m.pstiff / r) +
-         pow(r / t.rho0[k], m.gam1) * (t.T0[k] - t.invcv[k] * (t.e0[k] + m.pstiff / t.rho0[k]));
-    break;
-  case 4: TT = t.invcv[k] * (ee - m.q - m.pc * (1.0 / r - m.b)); break;
-  default: break;
-  }
-  T[i] = TT;
+  T[i] = tlaw_temperature(t, m, k, r, ee);
 }
 
 int m2c_eos_derived(m2c_ctx *c, int n, const double *rho, const double *p, const int32_t *id, double *e, double *T) {

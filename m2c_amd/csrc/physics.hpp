@@ -33,6 +33,14 @@ struct DevEos {
   double d_p0, d_rho0, d_e0;                             // dummy
 };
 
+// Temperature laws per material: 0 none (JWL, dummy), 1 T0 + (e - e0)/cv (SG, MG), 2 T0 + (e + p/rho
+// - h0)/cp (SG, MG with cp > 0 and cv <= 0), 3 the SG law with a reference density (cv > 0 and
+// rho0 > 0), 4 NASG (VarFcnSG.h:70-78, 126-142; VarFcnNASG.h:61, 112; VarFcnMG.h:113-121, 172-179).
+struct TLaw {
+  int method[8];
+  double invcv[8], T0[8], e0[8], rho0[8], invcp[8], h0[8];
+};
+
 template <typename T> __device__ __forceinline__ T dsqrt(T x) { return __builtin_sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
 template <typename T> __device__ __forceinline__ T dexp(T x) { return exp(x); }
@@ -401,6 +409,20 @@ __device__ __forceinline__ void flux_roe(const DevEos &m, const T *Vm, const T *
       f -= T(0.5) * dabs<T>(lam[p]) * a[p] * r;
     }
     F[i] = f;
+  }
+}
+
+
+// VarFcn*::GetTemperature(rho, e) of material k (NaN without a law)
+__device__ inline double tlaw_temperature(const TLaw &t, const DevEos &m, int k, double r, double ee) {
+  switch (t.method[k]) {
+  case 1: return t.T0[k] + t.invcv[k] * (ee - t.e0[k]);
+  case 2: return t.T0[k] + t.invcp[k] * (ee + eos_p<double, -1>(m, r, ee) / r - t.h0[k]);
+  case 3:
+    return t.invcv[k] * (ee + m.pstiff / r) +
+           pow(r / t.rho0[k], m.gam1) * (t.T0[k] - t.invcv[k] * (t.e0[k] + m.pstiff / t.rho0[k]));
+  case 4: return t.invcv[k] * (ee - m.q - m.pc * (1.0 / r - m.b));
+  default: return __builtin_nan("");
   }
 }
 

@@ -39,6 +39,13 @@ typedef struct {
   double omega, A1, A2, R1, R2, R1rho0, R2rho0, omega_over_R1rho0, omega_over_R2rho0;
   /* dummy */
   double d_p0, d_rho0, d_e0;
+  /* temperature law (VarFcnSG.h:126-142, VarFcnNASG.h:112, VarFcnMG.h:113-121): cv, T0, e0, rho0, cp, h0 */
+  double t_cv, t_T0, t_e0, t_rho0, t_cp, t_h0;
+  /* viscosity (ViscoFcn.h): 0 none, 1 constant (mu, bulk), 2 Sutherland (mu = mu0, su_T0, su_S, bulk) */
+  int visc;
+  double mu, bulk, su_T0, su_S;
+  /* heat diffusion (HeatDiffusionFcn.h): constant diffusivity, 0 = none */
+  double kappa;
 } orc_eos;
 
 /* Derived constants, computed as the reference constructors do
@@ -85,6 +92,34 @@ static inline double orc_internal_energy(const orc_eos *m, double rho, double p)
   }
   case ORC_EOS_JWL: return (p - orc_jwl_fun(m, rho)) / (m->omega * rho);
   default: return m->d_e0;
+  }
+}
+
+/* VarFcn*::GetTemperature(rho, e): SG VarFcnSG.h:70-78 (three laws chosen as VarFcnSG.h:141-142),
+ * NASG VarFcnNASG.h:61, MG VarFcnMG.h:172-179 (its e0 is the EOS reference energy); NaN for the
+ * others, which have no temperature law. */
+static inline double orc_temperature(const orc_eos *m, double rho, double e) {
+  const double invcv = m->t_cv == 0.0 ? 0.0 : 1.0 / m->t_cv, invcp = m->t_cp == 0.0 ? 0.0 : 1.0 / m->t_cp;
+  switch (m->type) {
+  case ORC_EOS_SG:
+    if (m->t_cv > 0 && m->t_rho0 > 0)
+      return invcv * (e + m->pstiff / rho) +
+             pow(rho / m->t_rho0, m->gam1) * (m->t_T0 - invcv * (m->t_e0 + m->pstiff / m->t_rho0));
+    if (m->t_cp > 0 && m->t_cv <= 0.0) {
+      double p = orc_pressure(m, rho, e);
+      return m->t_T0 + invcp * (e + p / rho - m->t_h0);
+    }
+    return m->t_T0 + invcv * (e - m->t_e0);
+  case ORC_EOS_NASG:
+    return invcv * (e - m->q - m->pc * (1.0 / rho - m->b));
+  case ORC_EOS_MG:
+    if (m->t_cp > 0 && m->t_cv <= 0.0) {
+      double p = orc_pressure(m, rho, e);
+      return m->t_T0 + invcp * (e + p / rho - m->t_h0);
+    }
+    return m->t_T0 + invcv * (e - m->e0);
+  default:
+    return NAN;
   }
 }
 

@@ -85,6 +85,7 @@ void orc_destroy(orc_ctx *c) {
   for (int f = 0; f < 6; f++) free(c->face[f]);
   free(c->Un); free(c->U1); free(c->R); free(c->V1); free(c->V2);
   free(c->Fx); free(c->Fy); free(c->Fz); free(c->ferr);
+  for (int d = 0; d < 3; d++) { free(c->Dv[d]); free(c->Dh[d]); }
   orc_ls_free(c);
   free(c);
 }
@@ -395,18 +396,33 @@ int orc_compute_residual(orc_ctx *c, const double *V, const int *ID, double *R) 
   c->n_riemann_err += rerr;
   orc_riemann_end_stage(c);
 
+  /* viscous and heat-diffusion terms (SpaceOperator.cpp:3367-3371), added after the advective sum */
+  const int diff = orc_has_diffusion(c);
+  if (diff) {
+    const long nf[3] = {(long)(nx + 1) * ny * nz, (long)nx * (ny + 1) * nz, (long)nx * ny * (nz + 1)};
+    for (int d = 0; d < 3; d++)
+      if (!c->Dv[d]) {
+        c->Dv[d] = (double *)calloc((size_t)nf[d] * 10, sizeof(double));
+        c->Dh[d] = (double *)calloc((size_t)nf[d], sizeof(double));
+      }
+    orc_diffusion(c, V, ID, c->Dv, c->Dh);
+  }
+
   const double ax = c->area[0], ay = c->area[1], az = c->area[2];
   const double inv = 1.0 / (-c->vol);
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) {
-        const double *Fw = c->Fx + 10 * (((long)k * ny + j) * (nx + 1) + i) + 5;
-        const double *Fe = c->Fx + 10 * (((long)k * ny + j) * (nx + 1) + i + 1);
-        const double *Fs = c->Fy + 10 * (((long)k * (ny + 1) + j) * nx + i) + 5;
-        const double *Fn = c->Fy + 10 * (((long)k * (ny + 1) + j + 1) * nx + i);
-        const double *Fb = c->Fz + 10 * (((long)k * ny + j) * nx + i) + 5;
-        const double *Ft = c->Fz + 10 * (((long)(k + 1) * ny + j) * nx + i);
+        const long fxw = ((long)k * ny + j) * (nx + 1) + i, fys = ((long)k * (ny + 1) + j) * nx + i,
+                   fzb = ((long)k * ny + j) * nx + i;
+        const long fxe = fxw + 1, fyn = fys + nx, fzt = fzb + (long)nx * ny;
+        const double *Fw = c->Fx + 10 * fxw + 5;
+        const double *Fe = c->Fx + 10 * fxe;
+        const double *Fs = c->Fy + 10 * fys + 5;
+        const double *Fn = c->Fy + 10 * fyn;
+        const double *Fb = c->Fz + 10 * fzb + 5;
+        const double *Ft = c->Fz + 10 * fzt;
         double *r = R + 5 * ORC_IDX(c, i, j, k);
         for (int p = 0; p < 5; p++) {
           double s = 0.0;
@@ -416,6 +432,24 @@ int orc_compute_residual(orc_ctx *c, const double *V, const int *ID, double *R) 
           s += Fe[p] * ax;
           s += Fn[p] * ay;
           s += Ft[p] * az;
+          if (diff) {
+            /* ViscosityOperator.cpp:259-393: own faces added, then the east, north, front faces
+             * (visited from the neighbouring cells) subtracted */
+            s += c->Dv[0][10 * fxw + p];
+            s += c->Dv[1][10 * fys + p];
+            s += c->Dv[2][10 * fzb + p];
+            s -= c->Dv[0][10 * fxe + 5 + p];
+            s -= c->Dv[1][10 * fyn + 5 + p];
+            s -= c->Dv[2][10 * fzt + 5 + p];
+            if (p == 4) {            /* HeatDiffusionOperator.cpp:96-131 */
+              s += c->Dh[0][fxw];
+              s += c->Dh[1][fys];
+              s += c->Dh[2][fzb];
+              s -= c->Dh[0][fxe];
+              s -= c->Dh[1][fyn];
+              s -= c->Dh[2][fzt];
+            }
+          }
           r[p] = s * inv;
         }
       }

@@ -29,6 +29,7 @@ struct orc_ctx {
   double *xc[3];            /* cell-centre coordinates with one ghost on each side */
   double *fd_l[3], *fd_r[3], *fd_r1y;   /* FD3 coefficients per index along each axis */
   double reinit_cfl_state;  /* LevelSetReinitializer::cfl (persists, divided by 1.5 on retries) */
+  double *Dv[3], *Dh[3];    /* viscous (plus[5], minus[5]) and heat fluxes per face (oracle/diffusion.c) */
   long n_reinit_iters, n_unresolved;
 };
 
@@ -50,6 +51,10 @@ int orc_interface_flux(orc_ctx *c, int dir, const double *vm, int idm, const dou
 void orc_flux_godunov(const orc_ctx *c, int dir, const double *Vm, const double *Vp, int id, double *F);
 void orc_riemann_begin_stage(orc_ctx *c);
 void orc_riemann_end_stage(orc_ctx *c);
+
+/* viscous and heat-diffusion face fluxes (oracle/diffusion.c) */
+int orc_has_diffusion(const orc_ctx *c);
+void orc_diffusion(orc_ctx *c, const double *V, const int *ID, double *Dv[3], double *Dh[3]);
 
 /* level set + end-of-step tasks (oracle/multiphase.c) */
 int orc_step_all(orc_ctx *c, double *V, int *ID, double *phi, double dt, int step);

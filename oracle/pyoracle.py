@@ -25,7 +25,9 @@ class Eos(C.Structure):
         "pc", "b", "q", "gam_pc",
         "rho0", "c0", "Gamma0", "s", "e0", "rho0_c0_c0", "Gamma0_over_2", "Gamma0_rho0",
         "omega", "A1", "A2", "R1", "R2", "R1rho0", "R2rho0", "omega_over_R1rho0", "omega_over_R2rho0",
-        "d_p0", "d_rho0", "d_e0")]
+        "d_p0", "d_rho0", "d_e0", "t_cv", "t_T0", "t_e0", "t_rho0", "t_cp", "t_h0")] + [
+        ("visc", C.c_int), ("mu", C.c_double), ("bulk", C.c_double), ("su_T0", C.c_double), ("su_S", C.c_double),
+        ("kappa", C.c_double)]
 
 
 class RiemannCfg(C.Structure):

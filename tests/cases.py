@@ -21,6 +21,8 @@ def oracle_eos(m):
         e.omega, e.A1, e.A2, e.R1, e.R2, e.rho0 = p[0], p[1], p[2], p[3], p[4], p[5]
     elif m.eos == M.EOS_DUMMY:
         e.d_p0, e.d_rho0, e.d_e0 = p[0], p[1], p[2]
+    e.t_cv, e.t_T0, e.t_e0, e.t_rho0, e.t_cp, e.t_h0 = list(m.tlaw)
+    e.visc, e.mu, e.bulk, e.su_T0, e.su_S, e.kappa = m.viscosity, m.mu, m.bulk, m.su_T0, m.su_S, m.kappa
     return e
 
 

@@ -251,8 +251,7 @@ def test_missing_mesh_and_materials():
     "Time.Type = Implicit;", "Time.LocalTimeStepping = On;", "Time.ConvergenceTolerance = 1e-6;",
     "Mesh.Type = Cylindrical;", "Mesh.ControlPointX[0].Coordinate = 0.5;", "Mesh.BoundaryConditionX0 = Overset;",
     "Equations.Material[0].EquationOfState = ANEOSBirchMurnaghanDebye;",
-    "Equations.Material[0].ViscosityModel.Type = Constant;",
-    "Equations.Material[0].HeatDiffusionModel.Type = Constant;",
+    "Equations.Material[0].ViscosityModel.Type = Artificial;",
     "Equations.MaterialTransition[0].FromMaterialID = 0;",
     "Space.NavierStokes.Reconstruction.VariableType = Conservative;",
     "Space.NavierStokes.Reconstruction.Fixes.Sphere[0].Radius = 1;",
@@ -443,3 +442,19 @@ def test_host_driver_reports_a_bad_deck_before_touching_the_device(tmp_path):
     r = subprocess.run([build.DEMO, "deck", str(path), str(tmp_path / "out.bin")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 3 and "status 4" in r.stderr, r.stderr
+
+
+def test_viscosity_and_heat_diffusion_models():
+    cfg, _, _ = read("""under Equations {
+  under Material[0] { under ViscosityModel { Type = Constant; DynamicViscosity = 2.5e-3; BulkViscosity = 1e-4; }
+                      under HeatDiffusionModel { Type = Constant; Diffusivity = 0.6; } }
+  under Material[1] { EquationOfState = StiffenedGas; under ViscosityModel { Type = Sutherland; } }
+  under Material[2] { EquationOfState = StiffenedGas; HeatDiffusionModel.Diffusivity = 3.0; }
+}
+""")
+    m0, m1, m2 = cfg.mat[0], cfg.mat[1], cfg.mat[2]
+    assert (m0.viscosity, m0.mu, m0.bulk, m0.kappa) == (M.VISC_CONSTANT, 2.5e-3, 1e-4, 0.6)
+    # ViscosityModelData defaults (Sutherland: mu0 1.716e-5, T0 273, S 111)
+    assert (m1.viscosity, m1.mu, m1.su_T0, m1.su_S, m1.bulk) == (M.VISC_SUTHERLAND, 1.716e-5, 273.0, 111.0, 0.0)
+    # a diffusivity without Type = Constant is not used (HeatDiffusionModelData type None)
+    assert m2.viscosity == M.VISC_NONE and m2.kappa == 0.0
