@@ -18,6 +18,10 @@ persistent launch per RK stage, hipGraph step replay, autotuned launch shape.
 --workload learned: the learned interface solver's kernels (inference roofline against the bf16
 MFMA peak, training steps) and its per-problem cost against the exact solver.
 
+--workload viscous: the 256^3 periodic case with viscous and heat-diffusion terms (constant
+viscosity and diffusivity), RK3, FP64: per stage a face pass (velocity gradients, stresses, heat
+flux per face) before the fused stage kernel, which adds the face terms after the advective sum.
+
 Prints one JSON line with the throughput, the roofline of the fused stage kernel (algorithmic
 bytes per launch divided by its average duration, measured with HIP events on the library's own
 stream) and a CPU baseline (the C oracle timed on the host's cores on a bounded sample).
@@ -42,6 +46,11 @@ BYTES_PER_CELL_UPDATE = 320.0
 # phi_n+1, ID read/write: 18 B) -- 260 B; reinitialisation adds 88 B per iteration every 5 steps
 BYTES_PER_CELL_UPDATE_MP = 260.0
 BYTES_PER_REINIT_ITER = 88.0
+# viscous + heat diffusion, per stage: the face pass reads the stage field (40 B/cell) and writes 3
+# faces x (10 viscous + 1 heat) doubles (264 B/cell); the stage kernel reads them back, 6 doubles on
+# each side of 3 faces per cell (288 B/cell)
+BYTES_DIFF_PASS = 304.0
+BYTES_DIFF_STAGE_READ = 288.0
 # HBM bytes per launch of the fused stage kernel on the 256^3 periodic RK3 FP64 workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --steps 3 --warmup 1 --no-cpu
 # --no-mixed` (FETCH scaled by 2.0, WRITE by 1.0 after a copy-kernel calibration; summary in
@@ -264,13 +273,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0: 8 for the periodic case, 10 for the shock-bubble)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="periodic_modes",
-                    choices=["periodic_modes", "shock_bubble", "batched_riemann", "learned"])
+                    choices=["periodic_modes", "shock_bubble", "batched_riemann", "learned", "viscous"])
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / surrogate comparison runs")
     args = ap.parse_args()
     if args.steps <= 0:
         args.steps = {"shock_bubble": 300, "batched_riemann": 100}.get(args.workload, 200)
     if args.cpu_steps <= 0:
-        args.cpu_steps = 10 if args.workload == "shock_bubble" else 8
+        args.cpu_steps = {"shock_bubble": 10, "viscous": 4}.get(args.workload, 8)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design, see the sharding section of the design notes", file=sys.stderr)
@@ -284,8 +293,11 @@ def main():
     if args.workload == "learned":
         return bench_learned(M, W, args)
     mp = args.workload == "shock_bubble"
+    visc = args.workload == "viscous"
     if mp:
         settings, V0, ID0, phi0 = W.shock_bubble()
+    elif visc:
+        settings, V0, ID0 = W.periodic_viscous(n=args.n)
     else:
         settings, V0, ID0 = W.periodic_modes(n=args.n)
     cells = V0.shape[0] * V0.shape[1] * V0.shape[2]
@@ -313,15 +325,16 @@ def main():
     # stage-kernel duration with HIP events on the library's stream
     nprof = max(5, min(args.steps, 50))
     ms_stage, ms_step_ev = s.time_steps(nprof)
+    ms_diff = s.diffusion_pass_ms()
     copy_gbs, copy_ms = M.stream_copy_GBs(s, 1 << 30, 20)
     s.close()
     nstage = 2.0 if mp else 3.0
-    bytes_stage = (200.0 if mp else BYTES_PER_CELL_UPDATE) / nstage
+    bytes_stage = (200.0 if mp else BYTES_PER_CELL_UPDATE) / nstage + (BYTES_DIFF_STAGE_READ if visc else 0.0)
     achieved = bytes_stage * cells / (ms_stage * 1e-3) / 1e9
     total_steps = args.warmup + args.steps
     bpcu = BYTES_PER_CELL_UPDATE_MP + BYTES_PER_REINIT_ITER * reinit_its / max(total_steps, 1) if mp \
-        else BYTES_PER_CELL_UPDATE
-    traffic = STAGE_TRAFFIC_PERIODIC256_FP64 if (not mp and args.n == 256) else None
+        else BYTES_PER_CELL_UPDATE + (3 * (BYTES_DIFF_PASS + BYTES_DIFF_STAGE_READ) if visc else 0.0)
+    traffic = STAGE_TRAFFIC_PERIODIC256_FP64 if (not mp and not visc and args.n == 256) else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "m2c::stage_kernel (fused MUSCL+HLLC+RK stage)", "avg_launch_ms": ms_stage}
@@ -331,11 +344,21 @@ def main():
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": (f"shock_bubble_{V0.shape[2]}x{V0.shape[1]}x{V0.shape[0]}_rk2_hllc_vanalbada_"
                                     "exact_riemann_levelset_reinit5_fp64") if mp
-                       else f"periodic_modes_{args.n}^3_rk3_hllc_vanalbada_fp64", "cells": cells,
+                       else (f"periodic_viscous_{args.n}^3_rk3_hllc_vanalbada_mu0.01_kappa0.02_fp64" if visc
+                             else f"periodic_modes_{args.n}^3_rk3_hllc_vanalbada_fp64"), "cells": cells,
                        "precision": "fp64", "bytes_per_cell_update": bpcu,
                        "hbm_frac_whole_step": value * bpcu / 1e9 / HBM_PEAK_GBS,
                        "measured_copy_GBs": copy_gbs, "copy_kernel_ms_per_GiB": copy_ms},
             "roofline": roof}
+    if visc:
+        # the face pass (three face kernels per stage) beside the stage kernel
+        ach = BYTES_DIFF_PASS * cells / (ms_diff * 1e-3) / 1e9
+        line["diffusion_pass"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "avg_pass_ms": ms_diff,
+                                  "kernels": "k_diff_faces<0..2> (one thread per face)",
+                                  "bytes_per_cell": BYTES_DIFF_PASS}
+        line["ms_per_step_parts"] = {"stage_kernels": 3 * ms_stage, "diffusion_passes": 3 * ms_diff,
+                                     "whole_step": ms_step_ev}
     if not args.no_mixed:
         # the shock-bubble amplifies any perturbation once the shock strikes the bubble (~step 180),
         # so the forms are also compared at step 150

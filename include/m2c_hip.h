@@ -252,6 +252,10 @@ int m2c_surrogate_riemann_batch(m2c_ctx *ctx, const m2c_mlp *net, int newton_max
  * recorded on the context's stream around every stage launch and around the whole loop. */
 int m2c_time_steps(m2c_ctx *ctx, int nsteps, double *ms_stage_kernel, double *ms_step);
 
+/* Average duration (ms) of one viscous / heat-diffusion face pass (its three face kernels) in the
+ * last m2c_time_steps call, which leaves it out of ms_stage_kernel; 0 without those terms. */
+int m2c_time_diffusion_pass(m2c_ctx *ctx, double *ms);
+
 /* Measured HBM copy bandwidth: a plain device copy kernel moves `nbytes` (read + write) `reps`
  * times between two buffers of that size; returns the average kernel duration (ms), timed with
  * HIP events on the context's stream. Also the known-size calibration kernel for traffic

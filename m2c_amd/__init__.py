@@ -207,6 +207,7 @@ def lib():
         L.m2c_config_from_deck.argtypes = [C.c_char_p, C.POINTER(Config), C.POINTER(DeckRun), C.c_char_p, C.c_int]
         L.m2c_initial_condition_from_deck.argtypes = [C.c_char_p, D, C.POINTER(C.c_int32), D, C.c_char_p, C.c_int]
         L.m2c_eos_derived.argtypes = [P, C.c_int, D, D, C.POINTER(C.c_int32), D, D]
+        L.m2c_time_diffusion_pass.argtypes = [P, D]
         L.m2c_output_create.argtypes = [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_int]
         L.m2c_output_write.argtypes = [P, P, C.c_double, C.c_double, C.c_int, D, C.POINTER(C.c_int32), D, C.c_int,
                                        C.POINTER(C.c_int)]
@@ -358,6 +359,12 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         _chk(self.L.m2c_time_steps(self.h, nsteps, C.byref(a), C.byref(b)), "m2c_time_steps")
         return a.value, b.value
+
+    def diffusion_pass_ms(self):
+        """Average viscous / heat-diffusion face pass (ms) of the last time_steps() call."""
+        ms = C.c_double()
+        _chk(self.L.m2c_time_diffusion_pass(self.h, C.byref(ms)), "m2c_time_diffusion_pass")
+        return ms.value
 
 
 def stream_copy_GBs(solver, nbytes=1 << 30, reps=20):

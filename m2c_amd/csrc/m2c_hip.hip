@@ -69,6 +69,10 @@ struct m2c_ctx {
   long long nf[3] = {0, 0, 0};
   DiffTab dtab;
   DiffMat dmat;
+  // profiling: events around each diffusion pass (m2c_time_steps), and its last average
+  std::vector<hipEvent_t> *diff_ev = nullptr;
+  size_t diff_ev_i = 0;
+  double diff_ms = 0.0;
 };
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
@@ -552,7 +556,10 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
                         double *Out, double a, double b, double cc, int first, int ignore_active) {
   StageArgs s;
   if (c->diff) {
+    const bool prof = c->diff_ev && c->diff_ev_i + 2 <= c->diff_ev->size();
+    if (prof) hipEventRecord((*c->diff_ev)[c->diff_ev_i++], c->stream);
     if (int rc = diffusion_pass(c, S, in_cons)) return rc;
+    if (prof) hipEventRecord((*c->diff_ev)[c->diff_ev_i++], c->stream);
     for (int d = 0; d < 3; d++) {
       s.Dv[d] = c->Dv[d];
       s.Dh[d] = c->Dh[d];
@@ -1136,27 +1143,50 @@ int m2c_time_steps(m2c_ctx *c, int nsteps, double *ms_stage, double *ms_step) {
     HIPCHK(hipStreamSynchronize(c->stream));
     step0 = h.steps;
   }
+  // the stage events bracket the diffusion pass too; its own events let it be taken out
+  std::vector<hipEvent_t> dev(c->diff ? 2 * ns * nsteps : 0);
+  for (auto &e : dev) HIPCHK(hipEventCreate(&e));
+  c->diff_ev = c->diff ? &dev : nullptr;
+  c->diff_ev_i = 0;
   for (int s = 0; s < nsteps; s++) {
     std::vector<hipEvent_t> sub(ev.begin() + 2 * ns * s, ev.begin() + 2 * ns * (s + 1));
     int rc = enqueue_step(c, 0, &sub, step0 + s + 1);
-    if (rc) return rc;
+    if (rc) {
+      c->diff_ev = nullptr;
+      return rc;
+    }
     hipLaunchKernelGGL(k_clock_finalize, dim3(1), dim3(1), 0, c->stream, c->clk, 1);
   }
+  c->diff_ev = nullptr;
   HIPCHK(hipEventRecord(b, c->stream));
   HIPCHK(hipEventSynchronize(b));
   float tot = 0.f;
   HIPCHK(hipEventElapsedTime(&tot, a, b));
-  double ks = 0.0;
+  double ks = 0.0, kd = 0.0;
   for (int i = 0; i < ns * nsteps; i++) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
     ks += t;
   }
+  for (size_t i = 0; i + 1 < c->diff_ev_i; i += 2) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, dev[i], dev[i + 1]));
+    kd += t;
+  }
+  ks -= kd;
+  c->diff_ms = c->diff_ev_i ? kd / (c->diff_ev_i / 2) : 0.0;
+  for (auto &e : dev) hipEventDestroy(e);
   for (auto &e : ev) hipEventDestroy(e);
   hipEventDestroy(a);
   hipEventDestroy(b);
   if (ms_stage) *ms_stage = ks / (ns * nsteps);
   if (ms_step) *ms_step = tot / nsteps;
+  return M2C_OK;
+}
+
+int m2c_time_diffusion_pass(m2c_ctx *c, double *ms) {
+  if (!c || !ms) return M2C_ERR_ARG;
+  *ms = c->diff_ms;
   return M2C_OK;
 }
 

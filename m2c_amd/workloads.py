@@ -13,6 +13,8 @@ m2c_amd.make_config, V the primitive state (nz, ny, nx, 5) and ID the material I
                     in the water at x=64 cells with post-shock p=1e9 Pa; exact two-material Riemann
                     solves at interface faces, level-set reinitialisation every 5 steps, HLLC+MUSCL
                     (van Albada), RK2, CFL 0.5. `scale` shrinks every length for small test runs.
+  periodic_viscous  periodic_modes with constant viscosity (mu 0.01, bulk 0.002) and heat diffusion
+                    (kappa 0.02, T = p / rho)
   batched_riemann   independent single-material Riemann-type instances: edge lengths drawn
                     uniformly from lo..hi per axis, left/right states log-uniform (rho in
                     [0.1, 10], p in [0.1, 100], u = 0) split by a random plane (normal uniform on
@@ -24,7 +26,7 @@ input format (IoData.h:266-267 has no periodic code); they are the documented ex
 import numpy as np
 
 from . import (BC_INLET, BC_OUTLET, BC_PERIODIC, BC_SYMMETRY, EOS_SG, FLUX_HLLC, LIM_VANALBADA, TS_RK2,
-               TS_RK3, material)
+               TS_RK3, VISC_CONSTANT, material)
 
 FOURIER_SEED = 2024
 
@@ -75,6 +77,16 @@ def periodic_modes(n=256, ny=None, nz=None, amp=1e-2, pulse=0.5, sigma=8.0, seed
     r2 = ((x * nx - cx) ** 2 + (y * ny - cy) ** 2 + (z * nz - cz) ** 2)
     V[..., 4] += pulse * np.exp(-r2 / (2.0 * sigma * sigma))
     return settings, V, np.zeros((nz, ny, nx), dtype=np.int32)
+
+
+def periodic_viscous(n=256, mu=0.01, bulk=0.002, kappa=0.02, **kw):
+    """periodic_modes with a viscous, heat-conducting gas: constant viscosity mu and bulk viscosity,
+    constant diffusivity kappa, temperature law T = e / cv with cv = 1 / (gamma - 1) (so T = p / rho).
+    The diffusive step limit h^2 rho / (6 mu) stays far above the CFL 0.4 step (h = 1)."""
+    settings, V, ID = periodic_modes(n=n, **kw)
+    gas = material(EOS_SG, [1.4, 0.0], tlaw=(2.5, 0.0, 0.0, 0.0, 0.0, 0.0), viscosity=VISC_CONSTANT, mu=mu, bulk=bulk,
+                   kappa=kappa)
+    return dict(settings, mats=[gas]), V, ID
 
 
 WATER = (6.12, 3.43e8)
