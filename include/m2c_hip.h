@@ -162,6 +162,28 @@ int m2c_eos_derived(m2c_ctx *ctx, int n, const double *rho, const double *p, con
 int m2c_riemann_batch(m2c_ctx *ctx, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
                       const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status);
 
+/* ExactRiemannSolverBase::ComputeOneSidedRiemannSolution (ExactRiemannSolverBase.h:45-49) for n
+ * independent problems: the fluid state Vm[t] (material idl[t]) against a wall moving with velocity
+ * Ustar[3t..3t+2], along the unit normal dir[3t..3t+2] that points from the fluid to the wall.
+ * Vs[t]: the state at the wall, id[t] its material (-1 and Vs zero when the wall recedes from the
+ * fluid); Vsm[t]: the fluid-side star state. status[t] as for m2c_riemann_batch. */
+int m2c_onesided_riemann_batch(m2c_ctx *ctx, int n, const double *dir, const double *Vm, const int *idl,
+                               const double *Ustar, double *Vs, int *id, double *Vsm, int *status);
+
+/* The flux on the fluid side of n mesh faces cut by an embedded wall, as SpaceOperator's advection
+ * loop computes it (SpaceOperator.cpp:2438-2484 for i-1/2 faces, :2569-2611 j-1/2, :2696-2737
+ * k-1/2): face t is normal to axis[t]; side[t] = +1 when the fluid cell (state V[t], material id[t])
+ * is on the minus side of the face, -1 on the plus side; nwall/vwall[3t..3t+2] are the wall's unit
+ * normal and velocity at the intersection. The one-sided problem runs along
+ * GetNormalForOneSidedRiemann's direction (SpaceOperator.cpp:3506-3533; normal_mode 0 the wall normal
+ * (EMBEDDED_SURFACE), 1 the mesh axis (MESH, the default), 2 their normalised sum (AVERAGE)); its
+ * star state is clipped (ClipDensityAndPressure) into Vstar[t] and F[t] is the context's numerical
+ * flux along the axis, F(V, V*) for side +1 and F(V*, V) for side -1. Finding the intersections
+ * (EmbeddedBoundaryDataSet) is the caller's. */
+int m2c_wall_face_flux_batch(m2c_ctx *ctx, int n, const int *axis, const int *side, int normal_mode,
+                             const double *nwall, const double *vwall, const double *V, const int *id, double *F,
+                             double *Vstar, int *status);
+
 /* ---- many independent instances (batched) ------------------------------------------------
  * A batch holds n single-material instances (each its own grid, boundaries, state and
  * CFL-adaptive time step; one material table, flux, precision and RK2/RK3 scheme for all). Every

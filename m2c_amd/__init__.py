@@ -178,6 +178,8 @@ def lib():
         L.m2c_stream_copy.argtypes = [P, C.c_longlong, C.c_int, D]
         I = C.POINTER(C.c_int)
         L.m2c_riemann_batch.argtypes = [P, C.c_int, I, D, I, D, I, D, I, D, D, I]
+        L.m2c_onesided_riemann_batch.argtypes = [P, C.c_int, D, D, I, D, D, I, D, I]
+        L.m2c_wall_face_flux_batch.argtypes = [P, C.c_int, I, I, C.c_int, D, D, D, I, D, D, I]
         L.m2c_upload_levelset.argtypes = [P, D]
         L.m2c_download_levelset.argtypes = [P, D]
         Pb = C.c_void_p
@@ -291,6 +293,37 @@ class Solver:
         _chk(self.L.m2c_riemann_batch(self.h, n, cp(d), _dp(Vm), cp(im), _dp(Vp), cp(iq), _dp(Vs), cp(mid),
                                       _dp(Vsm), _dp(Vsp), cp(st)), "m2c_riemann_batch")
         return st, Vs, mid, Vsm, Vsp
+
+    def onesided_riemann(self, dirs, Vm, idl, Ustar):
+        """One-sided (wall) Riemann solutions of n problems along unit normals dirs (n x 3) against walls
+        moving with velocities Ustar (n x 3): (status, Vs, id, Vsm); id = -1 and Vs = 0 where the wall
+        recedes."""
+        Vm = np.ascontiguousarray(np.atleast_2d(Vm), dtype=np.float64)
+        n = Vm.shape[0]
+        d3 = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n, 3)))
+        ip = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+        dd, us, im = d3(dirs), d3(Ustar), ip(idl)
+        Vs, Vsm = np.empty_like(Vm), np.empty_like(Vm)
+        mid, st = np.empty(n, np.int32), np.empty(n, np.int32)
+        cp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        _chk(self.L.m2c_onesided_riemann_batch(self.h, n, _dp(dd), _dp(Vm), cp(im), _dp(us), _dp(Vs), cp(mid),
+                                               _dp(Vsm), cp(st)), "m2c_onesided_riemann_batch")
+        return st, Vs, mid, Vsm
+
+    def wall_face_flux(self, axis, side, normal_mode, nwall, vwall, V, ids):
+        """Fluxes on the fluid side of n wall-cut faces: (status, F, Vstar). side +1: the fluid cell is
+        on the minus side of the face; normal_mode 0 wall normal, 1 mesh axis, 2 their average."""
+        V = np.ascontiguousarray(np.atleast_2d(V), dtype=np.float64)
+        n = V.shape[0]
+        d3 = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n, 3)))
+        ip = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+        ax, sd, im, nw, vw = ip(axis), ip(side), ip(ids), d3(nwall), d3(vwall)
+        F, Vst = np.empty_like(V), np.empty_like(V)
+        st = np.empty(n, np.int32)
+        cp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        _chk(self.L.m2c_wall_face_flux_batch(self.h, n, cp(ax), cp(sd), normal_mode, _dp(nw), _dp(vw), _dp(V),
+                                             cp(im), _dp(F), _dp(Vst), cp(st)), "m2c_wall_face_flux_batch")
+        return st, F, Vst
 
     def set_surrogate(self, net, newton_max=3):
         """Interface faces use the learned solver `net` (an Mlp; None = exact solver only)."""

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -1464,6 +1465,61 @@ int m2c_riemann_batch(m2c_ctx *c, int n, const int *dir, const double *Vm, const
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipFree(d));
   return M2C_OK;
+}
+
+// shared by the one-sided entry points: face = 0 the plain solver, 1 a wall-cut face
+static int onesided_batch(m2c_ctx *c, int face, int n, const double *dir, const double *Vm, const int *idl,
+                          const double *uw, const int *ax, const int *fb, int mode, double *Vs, int *id, double *Vsm,
+                          double *F, int *status) {
+  if (n == 0) return M2C_OK;
+  for (int t = 0; t < n; t++) {
+    if (idl[t] < 0 || idl[t] >= c->cfg.nmat) return M2C_ERR_ARG;
+    if (face && (ax[t] < 0 || ax[t] > 2 || (fb[t] != 1 && fb[t] != -1))) return M2C_ERR_ARG;
+  }
+  if (int rc0 = ensure_constants(c)) return rc0;
+  if (int rc0 = ensure_riemann_constants(c)) return rc0;
+  const int nthreads = (int)std::min<long long>(4096, ((long long)n + 63) / 64 * 64);
+  const size_t b5 = sizeof(double) * 5 * (size_t)n, b3 = sizeof(double) * 3 * (size_t)n, bi = sizeof(int) * (size_t)n;
+  const size_t sbytes = sizeof(double) * 6 * (size_t)c->mp.path_cap * nthreads;
+  char *d = nullptr;
+  HIPCHK(hipMalloc(&d, 4 * b5 + 2 * b3 + 5 * bi + 16 + sbytes));
+  double *dVm = (double *)d, *dVs = dVm + 5 * (size_t)n, *dVsm = dVs + 5 * (size_t)n, *dF = dVsm + 5 * (size_t)n;
+  double *ddir = dF + 5 * (size_t)n, *duw = ddir + 3 * (size_t)n;
+  int *didl = (int *)(duw + 3 * (size_t)n), *dax = didl + n, *dfb = dax + n, *did = dfb + n, *dst = did + n;
+  double *scratch = (double *)(((uintptr_t)(dst + n) + 15) & ~(uintptr_t)15);
+  HIPCHK(hipMemcpyAsync(dVm, Vm, b5, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(ddir, dir, b3, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(duw, uw, b3, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(didl, idl, bi, hipMemcpyHostToDevice, c->stream));
+  if (face) {
+    HIPCHK(hipMemcpyAsync(dax, ax, bi, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dfb, fb, bi, hipMemcpyHostToDevice, c->stream));
+  }
+  int rc = m2c_mp_launch_onesided_batch(c->stream, c->cfg.flux, face, n, ddir, dVm, didl, duw, dax, dfb, mode, dVs,
+                                        did, dVsm, dF, dst, scratch, nthreads);
+  if (rc) { hipFree(d); return M2C_ERR_HIP; }
+  if (Vs) HIPCHK(hipMemcpyAsync(Vs, dVs, b5, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(Vsm, dVsm, b5, hipMemcpyDeviceToHost, c->stream));
+  if (face) HIPCHK(hipMemcpyAsync(F, dF, b5, hipMemcpyDeviceToHost, c->stream));
+  if (id) HIPCHK(hipMemcpyAsync(id, did, bi, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(status, dst, bi, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d));
+  return M2C_OK;
+}
+
+int m2c_onesided_riemann_batch(m2c_ctx *c, int n, const double *dir, const double *Vm, const int *idl,
+                               const double *Ustar, double *Vs, int *id, double *Vsm, int *status) {
+  if (!c || n < 0 || (n > 0 && (!dir || !Vm || !idl || !Ustar || !Vs || !id || !Vsm || !status))) return M2C_ERR_ARG;
+  return onesided_batch(c, 0, n, dir, Vm, idl, Ustar, nullptr, nullptr, 1, Vs, id, Vsm, nullptr, status);
+}
+
+int m2c_wall_face_flux_batch(m2c_ctx *c, int n, const int *axis, const int *side, int normal_mode,
+                             const double *nwall, const double *vwall, const double *V, const int *id, double *F,
+                             double *Vstar, int *status) {
+  if (!c || n < 0 || normal_mode < 0 || normal_mode > 2) return M2C_ERR_ARG;
+  if (n > 0 && (!axis || !side || !nwall || !vwall || !V || !id || !F || !Vstar || !status)) return M2C_ERR_ARG;
+  return onesided_batch(c, 1, n, nwall, V, id, vwall, axis, side, normal_mode, nullptr, nullptr, Vstar, F, status);
 }
 
 int m2c_stream_copy(m2c_ctx *c, long long nbytes, int reps, double *ms_avg) {

@@ -1195,6 +1195,53 @@ __global__ __launch_bounds__(64) void k_riemann_batch(int n, const int *dir, con
   }
 }
 
+// ComputeOneSidedRiemannSolution for a batch of independent problems (FACE = 0): problem t has the
+// unit normal dir[3t..3t+2] and the wall velocity uw[3t..3t+2]. FACE = 1: the fluid side of a face
+// normal to axis ax[t] cut by an embedded wall (SpaceOperator.cpp:2438-2484, 2569-2611, 2696-2737):
+// the direction of GetNormalForOneSidedRiemann from the wall normal dir[3t..], fb[t] = +1 when the
+// fluid cell is on the minus side, the clipped star state in Vsm and the numerical flux along the
+// axis, F(V, V*) or F(V*, V), in F. Vs is zero when the wall recedes (id = -1).
+template <int FLUX, int FACE>
+__global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir, const double *Vm, const int *idl,
+                                                        const double *uw, const int *ax, const int *fb, int mode,
+                                                        double *Vs, int *id, double *Vsm, double *F, int *status,
+                                                        double *scratch, int nthreads) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nthreads) return;
+  RSolver S_;
+  S_.mat = c_eos;
+  S_.cfg = c_mp.rcfg;
+  S_.overflow = 0;
+  const int cap = c_mp.path_cap;
+  S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
+  S_.path3 = S_.path1;
+  for (int t = tid; t < n; t += nthreads) {
+    double nrm[3], vs[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, vsm[5];
+    const double *v = Vm + 5 * t;
+    const int m = idl[t];
+    if (FACE) onesided_normal(ax[t], fb[t], mode, dir + 3 * t, nrm);
+    else { nrm[0] = dir[3 * t]; nrm[1] = dir[3 * t + 1]; nrm[2] = dir[3 * t + 2]; }
+    int mid = 0;
+    S_.overflow = 0;
+    int r = onesided_solve(S_, nrm, v, m, uw + 3 * t, vs, &mid, vsm);
+    if (FACE) {   // an invalid input state (r < 0) leaves the flux at zero
+      double f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (r >= 0) {
+        eos_clip<double, -1>(c_eos[m], vsm);
+        const bool minus = fb[t] > 0;
+        const double *a = minus ? v : vsm, *b = minus ? vsm : v;
+        if (ax[t] == 0) numerical_flux<FLUX, -1, 0>(a, b, m, f);
+        else if (ax[t] == 1) numerical_flux<FLUX, -1, 1>(a, b, m, f);
+        else numerical_flux<FLUX, -1, 2>(a, b, m, f);
+      }
+      for (int q = 0; q < 5; q++) F[5 * t + q] = f[q];
+    }
+    for (int q = 0; q < 5; q++) { Vs[5 * t + q] = vs[q]; Vsm[5 * t + q] = vsm[q]; }
+    id[t] = mid;
+    status[t] = S_.overflow ? 2 : r;
+  }
+}
+
 // pointwise batch of the learned solver: network inputs of every problem
 __global__ void k_sur_batch_feat(int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
                                  const int *idp, float *X) {
@@ -1320,6 +1367,22 @@ extern "C" int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir,
                                            double *Vsp, int *status, double *scratch, int nthreads) {
   hipLaunchKernelGGL(k_riemann_batch, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Vs, id,
                      Vsm, Vsp, status, scratch, nthreads);
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_onesided_batch(hipStream_t s, int flux, int face, int n, const double *dir,
+                                            const double *Vm, const int *idl, const double *uw, const int *ax,
+                                            const int *fb, int mode, double *Vs, int *id, double *Vsm, double *F,
+                                            int *status, double *scratch, int nthreads) {
+  const dim3 gr((nthreads + 63) / 64), bl(64);
+#define M2C_OS(FL, FA)                                                                                           \
+  hipLaunchKernelGGL((k_onesided_batch<FL, FA>), gr, bl, 0, s, n, dir, Vm, idl, uw, ax, fb, mode, Vs, id, Vsm, F, \
+                     status, scratch, nthreads)
+  if (!face) M2C_OS(FLUX_HLLC, 0);
+  else if (flux == FLUX_HLLC) M2C_OS(FLUX_HLLC, 1);
+  else if (flux == FLUX_ROE) M2C_OS(FLUX_ROE, 1);
+  else M2C_OS(FLUX_LLF, 1);
+#undef M2C_OS
   return hchk(hipGetLastError());
 }
 

@@ -95,6 +95,9 @@ int m2c_mp_step_begin(const m2c::MpLaunch &L);
 int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,
                                 const double *Vp, const int *idp, double *Vs, int *id, double *Vsm, double *Vsp,
                                 int *status, double *scratch, int nthreads);
+int m2c_mp_launch_onesided_batch(hipStream_t s, int flux, int face, int n, const double *dir, const double *Vm,
+                                 const int *idl, const double *uw, const int *ax, const int *fb, int mode, double *Vs,
+                                 int *id, double *Vsm, double *F, int *status, double *scratch, int nthreads);
 int m2c_mp_launch_iface(const m2c::MpLaunch &L, const double *S, int in_cons, int store, int ignore_active);
 int m2c_mp_launch_ls(const m2c::MpLaunch &L, int in_cons, int mode, int final_, const double *S, const double *phiS,
                      const double *X, const double *Y, double *out, double a, double b, double coef,

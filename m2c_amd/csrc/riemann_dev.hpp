@@ -835,4 +835,339 @@ __device__ inline int riemann_solve(RSolver &S, const double *dir, const double 
   return r;
 }
 
+
+// ---------------------------------------------------------------- one-sided (wall) problem
+//
+// ExactRiemannSolverBase::ComputeOneSidedRiemannSolution (ExactRiemannSolverBase.cpp:1931-2198) and
+// its helpers: FinalizeOneSidedSolution :519-630, FindInitialIntervalOneSided :800-958,
+// FindInitialFeasiblePointsOneSidedByAcousticTheory :1119-1168, ComputeOneSidedRarefaction
+// :1586-1728, FindInitialFeasiblePointsOneSided :2946-3012. The fluid is on the left of the wall,
+// dir points from the fluid to the wall; only the 1-wave exists. M2C's solver object
+// (Main.cpp:142) is the adaptive one and the one-sided solver has no fixed-step retry, so every
+// wave connection here runs with nonadaptive = 0.
+
+// the state at the wall and the fluid-side star state; id = -1 (Vs untouched) when u* < 0.
+// -1 where the reference exits (rho* <= 0 or c*^2 < 0 behind a rarefaction)
+__device__ inline int os_finalize(RSolver &S, const double *dir, const double *Vm, double rhol, double ul, double pl,
+                                  int idl, double rhol2, double u2, double p2, int trans_rare, const double *Vrare,
+                                  double *Vs, int *id, double *Vsm) {
+  double utanl[3] = {Vm[1] - ul * dir[0], Vm[2] - ul * dir[1], Vm[3] - ul * dir[2]};
+  *id = (u2 >= 0) ? idl : -1;
+  if (*id != -1) {
+    Vs[0] = Vs[1] = Vs[2] = Vs[3] = Vs[4] = 0.0;
+    if (trans_rare) {
+      Vs[0] = Vrare[0];
+      Vs[1] = Vrare[1] * dir[0];
+      Vs[2] = Vrare[1] * dir[1];
+      Vs[3] = Vrare[1] * dir[2];
+      Vs[4] = Vrare[2];
+    } else {
+      int star = 0;
+      if (pl >= p2) {
+        const DevEos &m = S.mat[idl];
+        double el2 = rs_e(m, rhol2, p2);
+        double cl2 = rs_c2(m, rhol2, el2);
+        if (rhol2 <= 0 || cl2 < 0) return -1;
+        if (u2 - dsqrt<double>(cl2) <= 0) star = 1;
+      } else {
+        double us = (rhol2 * u2 - rhol * ul) / (rhol2 - rhol);
+        if (us <= 0) star = 1;
+      }
+      if (star) { Vs[0] = rhol2; Vs[1] = u2 * dir[0]; Vs[2] = u2 * dir[1]; Vs[3] = u2 * dir[2]; Vs[4] = p2; }
+      else { Vs[0] = rhol; Vs[1] = ul * dir[0]; Vs[2] = ul * dir[1]; Vs[3] = ul * dir[2]; Vs[4] = pl; }
+    }
+    for (int i = 1; i <= 3; i++) Vs[i] += utanl[i - 1];
+  }
+  Vsm[0] = rhol2;
+  Vsm[1] = utanl[0] + u2 * dir[0];
+  Vsm[2] = utanl[1] + u2 * dir[1];
+  Vsm[3] = utanl[2] + u2 * dir[2];
+  Vsm[4] = p2;
+  return 0;
+}
+
+// the 1-rarefaction integrated with the embedded RK step until its velocity reaches us; steps are
+// sized by velocity and cut back on overshoot
+__device__ inline int os_rarefaction(RSolver &S, double rho, double u, double p, double c, int id, double us,
+                                     double *rhos, double *ps, int *trans_rare, double *Vrare) {
+  const DevEos &m = S.mat[id];
+  const RiemannCfg &C = S.cfg;
+  *rhos = rho;
+  *ps = p;
+  double du_target = (us - u) / C.numSteps_rarefaction;
+  double du_max = du_target * 1.25;
+  double dp_op1 = dabs(du_target * c * rho);
+  double dp_op2 = (dabs(p) + 1.0e-6) / (C.numSteps_rarefaction * 2.5);
+  double dp = smin(dp_op1, dp_op2);
+  double rhos_0 = rho, us_0 = u, ps_0 = p, xi_0 = u - c;
+  double rhos_1 = rho, us_1 = u, ps_1 = p, xi_1 = 0.0;
+  double du;
+  double loc_tol = C.tol_rarefaction * smax(us - u, smax(dabs(us), dabs(u)));
+  double uErr = 0, rhoErr = 0;
+  for (int i = 0; i < C.numSteps_rarefaction * 5; i++) {
+    if (!rk_adaptive(m, 1, rhos_0, us_0, ps_0, dp, &rhos_1, &us_1, &ps_1, &xi_1, &uErr, &rhoErr)) {
+      dp /= 2.0;
+      continue;
+    }
+    du = us_1 - us_0;
+    if (du > du_max) {
+      dp = dp / du * du_target;
+      continue;
+    }
+    if (us_1 - us > loc_tol) {
+      if (du != 0) dp = dp / du * (us - us_0);
+      else dp /= 2.0;
+      continue;
+    }
+    if (xi_0 * xi_1 <= 0) {
+      *trans_rare = 1;
+      double w0 = dabs(xi_1), w1 = dabs(xi_0);
+      double ww = w0 + w1;
+      w0 /= ww;
+      w1 /= ww;
+      Vrare[0] = w0 * rhos_0 + w1 * rhos_1;
+      Vrare[1] = w0 * us_0 + w1 * us_1;
+      Vrare[2] = w0 * ps_0 + w1 * ps_1;
+    }
+    if (dabs(us - us_1) <= loc_tol) {
+      *rhos = rhos_1;
+      *ps = ps_1;
+      return eos_bad_rp(m, *rhos, *ps) ? 0 : 1;
+    }
+    dp = smin(dp / du * smin(du_target, us - us_1), dp * 5.0);
+    rhos_0 = rhos_1;
+    us_0 = us_1;
+    ps_0 = ps_1;
+    xi_0 = xi_1;
+  }
+  return 0;
+}
+
+struct OsBracket { double p0, rhol0, ul0, p1, rhol1, ul1; };
+
+__device__ inline int os_feasible_acoustic(RSolver &S, double rhol, double ul, double pl, double cl, int idl,
+                                           double ustar, OsBracket &B) {
+  double Cl = rhol * cl;
+  B.p0 = pl + Cl * (ul - ustar);
+  if (rho_u_star(S, 1, S.path1, rhol, ul, pl, B.p0, idl, rhol, (B.p0 > pl) ? rhol * 1.1 : rhol * 0.9, &B.rhol0,
+                 &B.ul0, nullptr, nullptr) <= 0) {
+    B.p0 = 1.5 * pl;
+    if (rho_u_star(S, 1, S.path1, rhol, ul, pl, B.p0, idl, rhol, (B.p0 > pl) ? rhol * 1.1 : rhol * 0.9, &B.rhol0,
+                   &B.ul0, nullptr, nullptr) <= 0)
+      return 0;
+  }
+  double Clbar = (B.ul0 == ul) ? Cl : dabs(B.p0 - pl) / dabs(B.ul0 - ul);
+  B.p1 = pl + Clbar * (ul - ustar);
+  double tmp = smax(dabs(B.p0), dabs(B.p1));
+  if (dabs(B.p1 - B.p0) / tmp < 1.0e-8) B.p1 = B.p0 + 1.0e-8 * tmp;
+  if (rho_u_star(S, 1, S.path1, rhol, ul, pl, B.p1, idl, rhol, B.rhol0, &B.rhol1, &B.ul1, nullptr, nullptr) <= 0)
+    return 1;
+  return 2;
+}
+
+__device__ inline int os_feasible(RSolver &S, double rhol, double ul, double pl, double cl, int idl, double ustar,
+                                  OsBracket &B) {
+  const RiemannCfg &C = S.cfg;
+  bool ok = true;
+  int found = os_feasible_acoustic(S, rhol, ul, pl, cl, idl, ustar, B);
+  if (found == 2) return 1;
+  if (found != 1) {
+    double dp = ul > ustar ? 0.5 * pl : -0.5 * pl;
+    for (int i = 0; i < C.maxIts_bracket; i++) {
+      B.p0 = pl + 0.01 * (i + 1) * (i + 1) * dp;
+      if (B.p0 < C.min_pressure) B.p0 = C.pressure_at_failure;
+      ok = rho_u_star(S, 1, S.path1, rhol, ul, pl, B.p0, idl, rhol, (B.p0 > pl) ? rhol * 1.1 : rhol * 0.9, &B.rhol0,
+                      &B.ul0, nullptr, nullptr) > 0;
+      if (ok) break;
+    }
+    if (!ok) return 0;
+  }
+  double dp = B.p0 - pl;
+  for (int i = 0; i < C.maxIts_bracket; i++) {
+    B.p1 = B.p0 + 0.01 * (i + 1) * (i + 1) * dp;
+    ok = rho_u_star(S, 1, S.path1, rhol, ul, pl, B.p1, idl, rhol, B.rhol0, &B.rhol1, &B.ul1, nullptr, nullptr) > 0;
+    if (ok) break;
+  }
+  if (!ok) return 0;
+  if (B.p0 > B.p1) { dswap(B.p0, B.p1); dswap(B.rhol0, B.rhol1); dswap(B.ul0, B.ul1); }
+  return 1;
+}
+
+__device__ inline int os_interval(RSolver &S, double rhol, double ul, double pl, double cl, int idl, double ustar,
+                                  OsBracket &B) {
+  const RiemannCfg &C = S.cfg;
+  if (!os_feasible(S, rhol, ul, pl, cl, idl, ustar, B)) {
+    B.p0 = B.p1 = C.pressure_at_failure;
+    return 0;
+  }
+  double p2 = 0, rhol2 = 0, ul2 = 0, f0, f1;
+  int i;
+  bool ok = true;
+  double fmin, p_fmin, rhol_fmin, ul_fmin;
+  if (dabs(B.ul0 - ustar) < dabs(B.ul1 - ustar)) {
+    fmin = dabs(B.ul0 - ustar); p_fmin = B.p0; rhol_fmin = B.rhol0; ul_fmin = B.ul0;
+  } else {
+    fmin = dabs(B.ul1 - ustar); p_fmin = B.p1; rhol_fmin = B.rhol1; ul_fmin = B.ul1;
+  }
+  for (i = 0; i < C.maxIts_bracket; i++) {
+    f0 = B.ul0 - ustar;
+    f1 = B.ul1 - ustar;
+    if (f0 * f1 <= 0.0) return 1;
+    if (dabs(f0 - f1) > 1e-9) {
+      p2 = B.p1 - f1 * (B.p1 - B.p0) / (f1 - f0);
+      if (p2 < B.p0) p2 -= 0.1 * (B.p1 - B.p0);
+      else p2 += 0.1 * (B.p1 - B.p0);
+    } else {
+      p2 = 1.1 * B.p1;
+    }
+    if (p2 < pl || i == (int)(C.maxIts_bracket / 2)) p2 = pl;
+    ok = rho_u_star(S, 1, S.path1, rhol, ul, pl, p2, idl, B.rhol0, B.rhol1, &rhol2, &ul2, nullptr, nullptr) > 0;
+    if (!ok) {
+      for (int j = 0; j < C.maxIts_bracket / 2; j++) {
+        if (p2 < B.p0) p2 = B.p0 - 0.5 * (B.p0 - p2);
+        else p2 = B.p1 + 0.5 * (p2 - B.p1);
+        if (p2 < pl) p2 = pl;
+        ok = rho_u_star(S, 1, S.path1, rhol, ul, pl, p2, idl, B.rhol0, B.rhol1, &rhol2, &ul2, nullptr, nullptr) > 0;
+        if (ok) break;
+      }
+    }
+    if (!ok) break;
+    if (dabs(ul2 - ustar) < fmin) { fmin = dabs(ul2 - ustar); p_fmin = p2; rhol_fmin = rhol2; ul_fmin = ul2; }
+    if (p2 < B.p0) {
+      B.p1 = B.p0; B.rhol1 = B.rhol0; B.ul1 = B.ul0;
+      B.p0 = p2; B.rhol0 = rhol2; B.ul0 = ul2;
+    } else {
+      B.p0 = B.p1; B.rhol0 = B.rhol1; B.ul0 = B.ul1;
+      B.p1 = p2; B.rhol1 = rhol2; B.ul1 = ul2;
+    }
+  }
+  if (!ok || i == C.maxIts_bracket) {
+    if (fmin < C.failure_threshold * dabs(ul - ustar)) {
+      B.p0 = B.p1 = p_fmin; B.rhol0 = B.rhol1 = rhol_fmin; B.ul0 = B.ul1 = ul_fmin;
+    } else {
+      p2 = C.pressure_at_failure;
+      ok = rho_u_star(S, 1, S.path1, rhol, ul, pl, p2, idl, B.rhol0, B.rhol1, &rhol2, &ul2, nullptr, nullptr) > 0;
+      if (ok) { B.p0 = B.p1 = p2; B.rhol0 = B.rhol1 = rhol2; B.ul0 = B.ul1 = ul2; }
+      else { B.p0 = B.p1 = p_fmin; B.rhol0 = B.rhol1 = rhol_fmin; B.ul0 = B.ul1 = ul_fmin; }
+    }
+    return 0;
+  }
+  return 1;
+}
+
+// the answer when the solver gives up: the input state moving with the wall
+__device__ inline void os_give_up(const double *dir, const double *Vm, int idl, double ul, double ustar, double *Vs,
+                                  int *id, double *Vsm) {
+  double utanl[3] = {Vm[1] - ul * dir[0], Vm[2] - ul * dir[1], Vm[3] - ul * dir[2]};
+  Vsm[0] = Vm[0];
+  Vsm[1] = utanl[0] + ustar * dir[0];
+  Vsm[2] = utanl[1] + ustar * dir[1];
+  Vsm[3] = utanl[2] + ustar * dir[2];
+  Vsm[4] = Vm[4];
+  if (ustar >= 0) {
+    *id = idl;
+    for (int i = 0; i < 5; i++) Vs[i] = Vsm[i];
+  } else {
+    *id = -1;
+  }
+}
+
+// 0 = success, 1 = failure (counted as a Riemann error), -1 = invalid state (the reference exits)
+__device__ inline int onesided_solve(RSolver &S, const double *dir, const double *Vm, int idl, const double *Ustar,
+                                     double *Vs, int *id, double *Vsm) {
+  const RiemannCfg &C = S.cfg;
+  S.nonadaptive = 0;
+  double rhol = Vm[0];
+  double ul = Vm[1] * dir[0] + Vm[2] * dir[1] + Vm[3] * dir[2];
+  double pl = Vm[4];
+  double ustar = Ustar[0] * dir[0] + Ustar[1] * dir[1] + Ustar[2] * dir[2];
+  const DevEos &ml = S.mat[idl];
+  double el = rs_e(ml, rhol, pl);
+  double cl = rs_c2(ml, rhol, el);
+  if (rhol <= 0 || cl < 0) return -1;
+  cl = dsqrt<double>(cl);
+  path_reset(S.path1, pl, rhol, ul);
+  OsBracket B;
+  const double dmin_ = 2.2250738585072014e-308;
+  B.p0 = dmin_; B.ul0 = 0.0; B.rhol0 = dmin_;
+  B.p1 = dmin_; B.ul1 = 0.0; B.rhol1 = dmin_;
+  double p2 = dmin_, ul2 = 0.0, rhol2 = dmin_;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  int trans_rare = 0;
+  double Vrare[3] = {0, 0, 0};
+  if (dabs(ul - ustar) < 1.0e-20)
+    return os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol, ul, pl, trans_rare, Vrare, Vs, id, Vsm);
+  if (ul < ustar) {
+    if (!os_rarefaction(S, rhol, ul, pl, cl, idl, ustar, &rhol2, &p2, &trans_rare, Vrare)) {
+      os_give_up(dir, Vm, idl, ul, ustar, Vs, id, Vsm);
+      return 1;
+    }
+    return os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol2, ustar, p2, trans_rare, Vrare, Vs, id, Vsm);
+  }
+  if (!os_interval(S, rhol, ul, pl, cl, idl, ustar, B)) {
+    if (rho_u_star(S, 1, S.path1, rhol, ul, pl, B.p1, idl, B.rhol0, B.rhol0 * 1.1, &rhol2, &ul2, &trans_rare,
+                   Vrare) <= 0) {
+      os_give_up(dir, Vm, idl, ul, ustar, Vs, id, Vsm);
+      return 1;
+    }
+    int r = os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol2, ustar, B.p1, trans_rare, Vrare, Vs, id, Vsm);
+    return r < 0 ? r : 1;
+  }
+  double p0 = B.p0, p1 = B.p1, rhol0 = B.rhol0, rhol1 = B.rhol1;
+  f0 = B.ul0 - ustar;
+  f1 = B.ul1 - ustar;
+  double denom = 0;
+  int iter = 0;
+  double err_p = 1.0, err_u = 1.0;
+  p2 = p1;
+  f2 = f1;
+  for (iter = 0; iter < C.maxIts_main; iter++) {
+    denom = f1 - f0;
+    if (denom == 0) {
+      p2 = 0.5 * (p0 + p1);
+    } else {
+      p2 = p2 - f2 * (p1 - p0) / denom;
+      if (p2 <= p0 || p2 >= p1) p2 = 0.5 * (p0 + p1);
+    }
+    bool broke = false;
+    for (;;) {   // the reference's try_again label
+      if (rho_u_star(S, 1, S.path1, rhol, ul, pl, p2, idl, rhol0, rhol1, &rhol2, &ul2, &trans_rare, Vrare) > 0) break;
+      p2 = 0.5 * (p2 + p0);
+      iter++;
+      if (iter < C.maxIts_main) continue;
+      broke = true;
+      break;
+    }
+    if (broke) break;
+    f2 = ul2 - ustar;
+    if (f0 * f2 < 0.0) { p1 = p2; f1 = f2; rhol1 = rhol2; }
+    else { p0 = p2; f0 = f2; rhol0 = rhol2; }
+    err_p = dabs(p1 - p0) / dabs(pl + 0.5 * rhol * ul * ul);
+    err_u = dabs(f2) / cl;
+    if ((err_p < C.tol_main && err_u < C.tol_main) || (err_p < C.tol_main * 1e-3) || (err_u < C.tol_main * 1e-3))
+      break;
+    trans_rare = 0;
+  }
+  int r = os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol2, ustar, p2, trans_rare, Vrare, Vs, id, Vsm);
+  if (r < 0) return r;
+  return iter == C.maxIts_main ? 1 : 0;
+}
+
+// GetNormalForOneSidedRiemann (SpaceOperator.cpp:3506-3533): mode 0 the wall's normal with its d
+// component turned along fb, 1 the mesh normal, 2 their normalised sum
+__device__ inline void onesided_normal(int d, int fb, int mode, const double *nwall, double *dir) {
+  dir[0] = dir[1] = dir[2] = 0.0;
+  if (mode == 1) {
+    dir[d] = (double)fb;
+    return;
+  }
+  dir[0] = nwall[0]; dir[1] = nwall[1]; dir[2] = nwall[2];
+  if (dir[d] * (double)fb < 0.0) dir[d] *= -1;
+  if (mode == 2) {
+    dir[d] += (double)fb;
+    double s = 1.0 / dsqrt<double>(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= s; dir[1] *= s; dir[2] *= s;
+  }
+}
+
 }  // namespace m2c

@@ -106,6 +106,18 @@ void orc_update_id(orc_ctx *c, const double *phi0, const double *phi, int *ID);
 int orc_riemann(const orc_ctx *c, const double *dir, const double *Vm, int idm, const double *Vp, int idp,
                 double *Vs, int *id, double *Vsm, double *Vsp);
 
+/* One-sided (wall) Riemann problem (ExactRiemannSolverBase::ComputeOneSidedRiemannSolution) of the
+ * fluid state Vm (material idl) against a wall moving with velocity Ustar along the unit normal dir
+ * from the fluid to the wall. id = -1 (Vs untouched) when the wall recedes. */
+int orc_riemann_onesided(const orc_ctx *c, const double *dir, const double *Vm, int idl, const double *Ustar,
+                         double *Vs, int *id, double *Vsm);
+
+/* GetNormalForOneSidedRiemann (SpaceOperator.cpp:3506-3533) and the flux on the fluid side of a
+ * wall-cut face (SpaceOperator.cpp:2438-2484); side +1: fluid cell on the minus side of the face. */
+void orc_onesided_normal(int d, int fb, int mode, const double *nwall, double *dir);
+int orc_wall_face_flux(const orc_ctx *c, int axis, int side, int mode, const double *nwall, const double *vwall,
+                       const double *V, int id, double *F, double *Vstar);
+
 /* Pointwise EOS of material id: PrimitiveToConservative / ConservativeToPrimitive
  * (VarFcnBase.h:169-197) and the squared sound speed at (rho, p) (VarFcnBase.h:215-219). */
 void orc_eos_p2c(const orc_ctx *c, int id, const double *V, double *U);

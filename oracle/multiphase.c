@@ -667,3 +667,8 @@ int orc_riemann(const orc_ctx *c, const double *dir, const double *Vm, int idm, 
                 double *Vs, int *id, double *Vsm, double *Vsp) {
   return orc_riemann_solve(c->cfg.mat, &c->cfg.riemann, dir, Vm, idm, Vp, idp, Vs, id, Vsm, Vsp);
 }
+
+int orc_riemann_onesided(const orc_ctx *c, const double *dir, const double *Vm, int idl, const double *Ustar,
+                         double *Vs, int *id, double *Vsm) {
+  return orc_riemann_onesided_solve(c->cfg.mat, &c->cfg.riemann, dir, Vm, idl, Ustar, Vs, id, Vsm);
+}

@@ -42,6 +42,8 @@ long orc_c2p_clip(orc_ctx *c, const double *U, const int *ID, double *V);
 void orc_rs_store(orc_ctx *c, int dir, long hi, const double *Vsm, int idm, const double *Vsp, int idp);
 int orc_riemann_solve(const orc_eos *mat, const orc_riemann_cfg *cfg, const double *dir, const double *Vm, int idl,
                       const double *Vp, int idr, double *Vs, int *id, double *Vsm, double *Vsp);
+int orc_riemann_onesided_solve(const orc_eos *mat, const orc_riemann_cfg *cfg, const double *dir, const double *Vm,
+                               int idl, const double *Ustar, double *Vs, int *id, double *Vsm);
 double orc_pow_quarter(double x);
 double orc_pow_fifth(double x);
 

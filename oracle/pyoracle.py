@@ -147,6 +147,9 @@ def lib():
         L.orc_ls_reinitialize.argtypes = [P, D]
         L.orc_update_id.argtypes = [P, D, D, I]
         L.orc_riemann.argtypes = [P, D, D, C.c_int, D, C.c_int, D, I, D, D]
+        L.orc_riemann_onesided.argtypes = [P, D, D, C.c_int, D, D, I, D]
+        L.orc_onesided_normal.argtypes = [C.c_int, C.c_int, C.c_int, D, D]
+        L.orc_wall_face_flux.argtypes = [P, C.c_int, C.c_int, C.c_int, D, D, D, C.c_int, D, D]
         L.orc_eos_p2c.argtypes = [P, C.c_int, D, D]
         L.orc_eos_c2p.argtypes = [P, C.c_int, D, D]
         L.orc_eos_c2.argtypes = [P, C.c_int, C.c_double, C.c_double]
@@ -252,6 +255,32 @@ class Oracle:
         st = self.L.orc_riemann(self.h, _dp(dd), _dp(Vm), idm, _dp(Vp), idp, _dp(Vs), C.byref(mid),
                                 _dp(Vsm), _dp(Vsp))
         return st, Vs, mid.value, Vsm, Vsp
+
+    def riemann_onesided(self, d, Vm, idl, Ustar):
+        """One-sided (wall) Riemann solution along unit normal d: (status, Vs, id, Vsm); Vs is zero
+        when id == -1 (the wall recedes)."""
+        dd = np.ascontiguousarray(d, dtype=np.float64)
+        Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+        Us = np.ascontiguousarray(Ustar, dtype=np.float64)
+        Vs, Vsm = np.zeros(5), np.zeros(5)
+        mid = C.c_int()
+        st = self.L.orc_riemann_onesided(self.h, _dp(dd), _dp(Vm), idl, _dp(Us), _dp(Vs), C.byref(mid), _dp(Vsm))
+        return st, Vs, mid.value, Vsm
+
+    def onesided_normal(self, d, fb, mode, nwall):
+        nw = np.ascontiguousarray(nwall, dtype=np.float64)
+        out = np.zeros(3)
+        self.L.orc_onesided_normal(d, fb, mode, _dp(nw), _dp(out))
+        return out
+
+    def wall_face_flux(self, axis, side, mode, nwall, vwall, V, mid=0):
+        """Flux on the fluid side of a wall-cut face: (status, F, Vstar)."""
+        nw = np.ascontiguousarray(nwall, dtype=np.float64)
+        vw = np.ascontiguousarray(vwall, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        F, Vs = np.zeros(5), np.zeros(5)
+        st = self.L.orc_wall_face_flux(self.h, axis, side, mode, _dp(nw), _dp(vw), _dp(V), mid, _dp(F), _dp(Vs))
+        return st, F, Vs
 
     def p2c(self, V, mid=0):
         V = np.ascontiguousarray(V, dtype=np.float64)

@@ -836,7 +836,333 @@ int orc_riemann_solve(const orc_eos *mat, const orc_riemann_cfg *cfg, const doub
   return r;
 }
 
-/* ------------------------------------------------------------ grid hooks ---------------- */
+/* ------------------------------------------------------------ one-sided (wall) problem -- */
+
+/* FinalizeOneSidedSolution (ExactRiemannSolverBase.cpp:519-630): the state at the wall (xi = 0) and
+ * the fluid-side star state. id = -1 when the wall recedes (u* < 0); Vs is then left as it was.
+ * Returns -1 where the reference exits (rho* <= 0 or c*^2 < 0 behind a rarefaction). */
+static int os_finalize(orc_rs *S, const double *dir, const double *Vm, double rhol, double ul, double pl, int idl,
+                       double rhol2, double u2, double p2, int trans_rare, const double *Vrare, double *Vs, int *id,
+                       double *Vsm) {
+  double utanl[3] = {Vm[1] - ul * dir[0], Vm[2] - ul * dir[1], Vm[3] - ul * dir[2]};
+  *id = (u2 >= 0) ? idl : -1;
+  if (*id != -1) {
+    Vs[0] = Vs[1] = Vs[2] = Vs[3] = Vs[4] = 0.0;
+    if (trans_rare) {
+      Vs[0] = Vrare[0];
+      Vs[1] = Vrare[1] * dir[0];
+      Vs[2] = Vrare[1] * dir[1];
+      Vs[3] = Vrare[1] * dir[2];
+      Vs[4] = Vrare[2];
+    } else {
+      int star = 0;
+      if (pl >= p2) {
+        const orc_eos *m = &S->mat[idl];
+        double el2 = orc_internal_energy(m, rhol2, p2);
+        double cl2 = orc_c2(m, rhol2, el2);
+        if (rhol2 <= 0 || cl2 < 0) return -1;
+        if (u2 - sqrt(cl2) <= 0) star = 1;
+      } else {
+        double us = (rhol2 * u2 - rhol * ul) / (rhol2 - rhol);
+        if (us <= 0) star = 1;
+      }
+      if (star) { Vs[0] = rhol2; Vs[1] = u2 * dir[0]; Vs[2] = u2 * dir[1]; Vs[3] = u2 * dir[2]; Vs[4] = p2; }
+      else { Vs[0] = rhol; Vs[1] = ul * dir[0]; Vs[2] = ul * dir[1]; Vs[3] = ul * dir[2]; Vs[4] = pl; }
+    }
+    for (int i = 1; i <= 3; i++) Vs[i] += utanl[i - 1];
+  }
+  Vsm[0] = rhol2;
+  Vsm[1] = utanl[0] + u2 * dir[0];
+  Vsm[2] = utanl[1] + u2 * dir[1];
+  Vsm[3] = utanl[2] + u2 * dir[2];
+  Vsm[4] = p2;
+  return 0;
+}
+
+/* ComputeOneSidedRarefaction (ExactRiemannSolverBase.cpp:1586-1728): the 1-rarefaction from
+ * (rho, u, p) integrated with the embedded RK step until its velocity reaches the wall's, us. The
+ * step is sized by velocity (du_target) and cut back when it overshoots. */
+static int os_rarefaction(orc_rs *S, double rho, double u, double p, double c, int id, double us, double *rhos,
+                          double *ps, int *trans_rare, double *Vrare) {
+  const orc_eos *m = &S->mat[id];
+  const orc_riemann_cfg *C = &S->cfg;
+  *rhos = rho;
+  *ps = p;
+  double du_target = (us - u) / C->numSteps_rarefaction;
+  double du_max = du_target * 1.25;
+  double dp_op1 = fabs(du_target * c * rho);
+  double dp_op2 = (fabs(p) + 1.0e-6) / (C->numSteps_rarefaction * 2.5);
+  double dp = dmin(dp_op1, dp_op2);
+  double rhos_0 = rho, us_0 = u, ps_0 = p, xi_0;
+  double rhos_1 = rho, us_1 = u, ps_1 = p, xi_1 = 0.0;
+  double du;
+  xi_0 = u - c;
+  double loc_tol = C->tol_rarefaction * dmax(us - u, dmax(fabs(us), fabs(u)));
+  double uErr = 0, rhoErr = 0;
+  for (int i = 0; i < C->numSteps_rarefaction * 5; i++) {
+    if (!rk_adaptive(m, 1, rhos_0, us_0, ps_0, dp, &rhos_1, &us_1, &ps_1, &xi_1, &uErr, &rhoErr)) {
+      dp /= 2.0;
+      continue;
+    }
+    du = us_1 - us_0;
+    if (du > du_max) {
+      dp = dp / du * du_target;
+      continue;
+    }
+    if (us_1 - us > loc_tol) {
+      if (du != 0) dp = dp / du * (us - us_0);
+      else dp /= 2.0;
+      continue;
+    }
+    if (xi_0 * xi_1 <= 0) {
+      *trans_rare = 1;
+      double w0 = fabs(xi_1), w1 = fabs(xi_0);
+      double ww = w0 + w1;
+      w0 /= ww;
+      w1 /= ww;
+      Vrare[0] = w0 * rhos_0 + w1 * rhos_1;
+      Vrare[1] = w0 * us_0 + w1 * us_1;
+      Vrare[2] = w0 * ps_0 + w1 * ps_1;
+    }
+    if (fabs(us - us_1) <= loc_tol) {
+      *rhos = rhos_1;
+      *ps = ps_1;
+      return orc_check_state_rp(m, *rhos, *ps) ? 0 : 1;
+    }
+    dp = dmin(dp / du * dmin(du_target, us - us_1), dp * 5.0);
+    rhos_0 = rhos_1;
+    us_0 = us_1;
+    ps_0 = ps_1;
+    xi_0 = xi_1;
+  }
+  return 0;
+}
+
+/* FindInitialFeasiblePointsOneSidedByAcousticTheory (ExactRiemannSolverBase.cpp:1119-1168):
+ * number of starting points found (0, 1 or 2) */
+static int os_feasible_acoustic(orc_rs *S, double rhol, double ul, double pl, double cl, int idl, double ustar,
+                                double *p0, double *rhol0, double *ul0, double *p1, double *rhol1, double *ul1) {
+  double Cl = rhol * cl;
+  *p0 = pl + Cl * (ul - ustar);
+  if (rho_u_star(S, 1, &S->path1, rhol, ul, pl, *p0, idl, rhol, (*p0 > pl) ? rhol * 1.1 : rhol * 0.9, rhol0, ul0,
+                 NULL, NULL) <= 0) {
+    *p0 = 1.5 * pl;
+    if (rho_u_star(S, 1, &S->path1, rhol, ul, pl, *p0, idl, rhol, (*p0 > pl) ? rhol * 1.1 : rhol * 0.9, rhol0, ul0,
+                   NULL, NULL) <= 0)
+      return 0;
+  }
+  double Clbar = (*ul0 == ul) ? Cl : fabs(*p0 - pl) / fabs(*ul0 - ul);
+  *p1 = pl + Clbar * (ul - ustar);
+  double tmp = dmax(fabs(*p0), fabs(*p1));
+  if (fabs(*p1 - *p0) / tmp < 1.0e-8) *p1 = *p0 + 1.0e-8 * tmp;
+  if (rho_u_star(S, 1, &S->path1, rhol, ul, pl, *p1, idl, rhol, *rhol0, rhol1, ul1, NULL, NULL) <= 0) return 1;
+  return 2;
+}
+
+/* FindInitialFeasiblePointsOneSided (ExactRiemannSolverBase.cpp:2946-3021) */
+static int os_feasible(orc_rs *S, double rhol, double ul, double pl, double cl, int idl, double ustar, double *p0,
+                       double *rhol0, double *ul0, double *p1, double *rhol1, double *ul1) {
+  const orc_riemann_cfg *C = &S->cfg;
+  int ok = 1;
+  int found = os_feasible_acoustic(S, rhol, ul, pl, cl, idl, ustar, p0, rhol0, ul0, p1, rhol1, ul1);
+  if (found == 2) return 1;
+  if (found != 1) {
+    double dp = ul > ustar ? 0.5 * pl : -0.5 * pl;
+    for (int i = 0; i < C->maxIts_bracket; i++) {
+      *p0 = pl + 0.01 * (i + 1) * (i + 1) * dp;
+      if (*p0 < C->min_pressure) *p0 = C->pressure_at_failure;
+      ok = rho_u_star(S, 1, &S->path1, rhol, ul, pl, *p0, idl, rhol, (*p0 > pl) ? rhol * 1.1 : rhol * 0.9, rhol0,
+                      ul0, NULL, NULL) > 0;
+      if (ok) break;
+    }
+    if (!ok) return 0;
+  }
+  double dp = *p0 - pl;
+  for (int i = 0; i < C->maxIts_bracket; i++) {
+    *p1 = *p0 + 0.01 * (i + 1) * (i + 1) * dp;
+    ok = rho_u_star(S, 1, &S->path1, rhol, ul, pl, *p1, idl, rhol, *rhol0, rhol1, ul1, NULL, NULL) > 0;
+    if (ok) break;
+  }
+  if (!ok) return 0;
+  if (*p0 > *p1) { SWAPD(*p0, *p1); SWAPD(*rhol0, *rhol1); SWAPD(*ul0, *ul1); }
+  return 1;
+}
+
+/* FindInitialIntervalOneSided (ExactRiemannSolverBase.cpp:800-958): secant steps outward from
+ * the starting points until f = u*(p) - ustar changes sign; on failure p0 = p1 = the best point
+ * found (or the prescribed failure pressure) and 0 is returned */
+static int os_interval(orc_rs *S, double rhol, double ul, double pl, double cl, int idl, double ustar, double *p0,
+                       double *rhol0, double *ul0, double *p1, double *rhol1, double *ul1) {
+  const orc_riemann_cfg *C = &S->cfg;
+  if (!os_feasible(S, rhol, ul, pl, cl, idl, ustar, p0, rhol0, ul0, p1, rhol1, ul1)) {
+    *p0 = *p1 = C->pressure_at_failure;
+    return 0;
+  }
+  double p2 = 0, rhol2 = 0, ul2 = 0, f0, f1;
+  int i, ok = 1;
+  double fmin, p_fmin, rhol_fmin, ul_fmin;
+  if (fabs(*ul0 - ustar) < fabs(*ul1 - ustar)) {
+    fmin = fabs(*ul0 - ustar); p_fmin = *p0; rhol_fmin = *rhol0; ul_fmin = *ul0;
+  } else {
+    fmin = fabs(*ul1 - ustar); p_fmin = *p1; rhol_fmin = *rhol1; ul_fmin = *ul1;
+  }
+  for (i = 0; i < C->maxIts_bracket; i++) {
+    f0 = *ul0 - ustar;
+    f1 = *ul1 - ustar;
+    if (f0 * f1 <= 0.0) return 1;
+    if (fabs(f0 - f1) > 1e-9) {
+      p2 = *p1 - f1 * (*p1 - *p0) / (f1 - f0);
+      if (p2 < *p0) p2 -= 0.1 * (*p1 - *p0);
+      else p2 += 0.1 * (*p1 - *p0);
+    } else {
+      p2 = 1.1 * *p1;
+    }
+    if (p2 < pl || i == (int)(C->maxIts_bracket / 2)) p2 = pl;
+    ok = rho_u_star(S, 1, &S->path1, rhol, ul, pl, p2, idl, *rhol0, *rhol1, &rhol2, &ul2, NULL, NULL) > 0;
+    if (!ok) {
+      for (int j = 0; j < C->maxIts_bracket / 2; j++) {
+        if (p2 < *p0) p2 = *p0 - 0.5 * (*p0 - p2);
+        else p2 = *p1 + 0.5 * (p2 - *p1);
+        if (p2 < pl) p2 = pl;
+        ok = rho_u_star(S, 1, &S->path1, rhol, ul, pl, p2, idl, *rhol0, *rhol1, &rhol2, &ul2, NULL, NULL) > 0;
+        if (ok) break;
+      }
+    }
+    if (!ok) break;
+    if (fabs(ul2 - ustar) < fmin) { fmin = fabs(ul2 - ustar); p_fmin = p2; rhol_fmin = rhol2; ul_fmin = ul2; }
+    if (p2 < *p0) {
+      *p1 = *p0; *rhol1 = *rhol0; *ul1 = *ul0;
+      *p0 = p2; *rhol0 = rhol2; *ul0 = ul2;
+    } else {
+      *p0 = *p1; *rhol0 = *rhol1; *ul0 = *ul1;
+      *p1 = p2; *rhol1 = rhol2; *ul1 = ul2;
+    }
+  }
+  if (!ok || i == C->maxIts_bracket) {
+    if (fmin < C->failure_threshold * fabs(ul - ustar)) {
+      *p0 = *p1 = p_fmin; *rhol0 = *rhol1 = rhol_fmin; *ul0 = *ul1 = ul_fmin;
+    } else {
+      p2 = C->pressure_at_failure;
+      ok = rho_u_star(S, 1, &S->path1, rhol, ul, pl, p2, idl, *rhol0, *rhol1, &rhol2, &ul2, NULL, NULL) > 0;
+      if (ok) { *p0 = *p1 = p2; *rhol0 = *rhol1 = rhol2; *ul0 = *ul1 = ul2; }
+      else { *p0 = *p1 = p_fmin; *rhol0 = *rhol1 = rhol_fmin; *ul0 = *ul1 = ul_fmin; }
+    }
+    return 0;
+  }
+  return 1;
+}
+
+/* the reference's answer when the one-sided solver gives up: the input state moving with the wall
+ * (ExactRiemannSolverBase.cpp:1990-2008, 2040-2056) */
+static void os_give_up(const double *dir, const double *Vm, int idl, double ul, double ustar, double *Vs, int *id,
+                       double *Vsm) {
+  double utanl[3] = {Vm[1] - ul * dir[0], Vm[2] - ul * dir[1], Vm[3] - ul * dir[2]};
+  Vsm[0] = Vm[0];
+  Vsm[1] = utanl[0] + ustar * dir[0];
+  Vsm[2] = utanl[1] + ustar * dir[1];
+  Vsm[3] = utanl[2] + ustar * dir[2];
+  Vsm[4] = Vm[4];
+  if (ustar >= 0) {
+    *id = idl;
+    for (int i = 0; i < 5; i++) Vs[i] = Vsm[i];
+  } else {
+    *id = -1;
+  }
+}
+
+/* ExactRiemannSolverBase::ComputeOneSidedRiemannSolution (ExactRiemannSolverBase.cpp:1931-2198):
+ * the half Riemann problem of the fluid state Vm (material idl) against a wall moving with velocity
+ * Ustar, along the unit normal dir pointing from the fluid to the wall. The solver object M2C
+ * creates (Main.cpp:142) is the adaptive one, so the 1-wave connection is the adaptive
+ * ComputeRhoUStar, and there is no fixed-step retry. 0 = success, 1 = failure (the reference counts
+ * a Riemann error), -1 = invalid state (the reference exits). */
+static int os_solve(orc_rs *S, const double *dir, const double *Vm, int idl, const double *Ustar, double *Vs,
+                    int *id, double *Vsm) {
+  const orc_riemann_cfg *C = &S->cfg;
+  double rhol = Vm[0];
+  double ul = Vm[1] * dir[0] + Vm[2] * dir[1] + Vm[3] * dir[2];
+  double pl = Vm[4];
+  double ustar = Ustar[0] * dir[0] + Ustar[1] * dir[1] + Ustar[2] * dir[2];
+  const orc_eos *ml = &S->mat[idl];
+  double el = orc_internal_energy(ml, rhol, pl);
+  double cl = orc_c2(ml, rhol, el);
+  if (rhol <= 0 || cl < 0) return -1;
+  cl = sqrt(cl);
+  path_reset(&S->path1, pl, rhol, ul);
+  double p0 = DBL_MIN, ul0 = 0.0, rhol0 = DBL_MIN;
+  double p1 = DBL_MIN, ul1 = 0.0, rhol1 = DBL_MIN;
+  double p2 = DBL_MIN, ul2 = 0.0, rhol2 = DBL_MIN;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  int trans_rare = 0;
+  double Vrare[3] = {0, 0, 0};
+  if (fabs(ul - ustar) < 1.0e-20)
+    return os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol, ul, pl, trans_rare, Vrare, Vs, id, Vsm);
+  if (ul < ustar) {
+    if (!os_rarefaction(S, rhol, ul, pl, cl, idl, ustar, &rhol2, &p2, &trans_rare, Vrare)) {
+      os_give_up(dir, Vm, idl, ul, ustar, Vs, id, Vsm);
+      return 1;
+    }
+    return os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol2, ustar, p2, trans_rare, Vrare, Vs, id, Vsm);
+  }
+  if (!os_interval(S, rhol, ul, pl, cl, idl, ustar, &p0, &rhol0, &ul0, &p1, &rhol1, &ul1)) {
+    if (rho_u_star(S, 1, &S->path1, rhol, ul, pl, p1, idl, rhol0, rhol0 * 1.1, &rhol2, &ul2, &trans_rare, Vrare) <= 0) {
+      os_give_up(dir, Vm, idl, ul, ustar, Vs, id, Vsm);
+      return 1;
+    }
+    int r = os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol2, ustar, p1, trans_rare, Vrare, Vs, id, Vsm);
+    return r < 0 ? r : 1;
+  }
+  f0 = ul0 - ustar;
+  f1 = ul1 - ustar;
+  double denom = 0;
+  int iter = 0;
+  double err_p = 1.0, err_u = 1.0;
+  p2 = p1;
+  f2 = f1;
+  for (iter = 0; iter < C->maxIts_main; iter++) {
+    denom = f1 - f0;
+    if (denom == 0) {
+      p2 = 0.5 * (p0 + p1);
+    } else {
+      p2 = p2 - f2 * (p1 - p0) / denom;
+      if (p2 <= p0 || p2 >= p1) p2 = 0.5 * (p0 + p1);
+    }
+    int broke = 0;
+    for (;;) {   /* the reference's try_again label */
+      if (rho_u_star(S, 1, &S->path1, rhol, ul, pl, p2, idl, rhol0, rhol1, &rhol2, &ul2, &trans_rare, Vrare) > 0)
+        break;
+      p2 = 0.5 * (p2 + p0);
+      iter++;
+      if (iter < C->maxIts_main) continue;
+      broke = 1;
+      break;
+    }
+    if (broke) break;
+    f2 = ul2 - ustar;
+    if (f0 * f2 < 0.0) { p1 = p2; f1 = f2; rhol1 = rhol2; }
+    else { p0 = p2; f0 = f2; rhol0 = rhol2; }
+    err_p = fabs(p1 - p0) / fabs(pl + 0.5 * rhol * ul * ul);
+    err_u = fabs(f2) / cl;
+    if ((err_p < C->tol_main && err_u < C->tol_main) || (err_p < C->tol_main * 1e-3) || (err_u < C->tol_main * 1e-3))
+      break;
+    trans_rare = 0;
+  }
+  int r = os_finalize(S, dir, Vm, rhol, ul, pl, idl, rhol2, ustar, p2, trans_rare, Vrare, Vs, id, Vsm);
+  if (r < 0) return r;
+  return iter == C->maxIts_main ? 1 : 0;
+}
+
+int orc_riemann_onesided_solve(const orc_eos *mat, const orc_riemann_cfg *cfg, const double *dir, const double *Vm,
+                               int idl, const double *Ustar, double *Vs, int *id, double *Vsm) {
+  orc_rs S;
+  memset(&S, 0, sizeof(S));
+  S.mat = mat;
+  S.cfg = *cfg;
+  S.nonadaptive = 0;
+  int r = os_solve(&S, dir, Vm, idl, Ustar, Vs, id, Vsm);
+  free(S.path1.p); free(S.path1.rho); free(S.path1.u);
+  return r;
+}
 
 void orc_riemann_begin_stage(orc_ctx *c) { (void)c; }
 void orc_riemann_end_stage(orc_ctx *c) { (void)c; }
@@ -872,4 +1198,45 @@ void orc_flux_godunov(const orc_ctx *c, int dir, const double *Vm, const double 
   int midid = id;
   orc_riemann_solve(c->cfg.mat, &c->cfg.riemann, n, Vm, id, Vp, id, Vmid, &midid, Vsm, Vsp);
   orc_physical_flux(&c->cfg.mat[id], dir, Vmid, F);
+}
+
+/* ------------------------------------------------------------ wall-cut faces ------------ */
+
+/* SpaceOperator::GetNormalForOneSidedRiemann (SpaceOperator.cpp:3506-3533): the direction of the
+ * one-sided problem on a face normal to axis d, fb = +1 when the wall lies ahead of the fluid cell
+ * (x increasing), -1 behind it. mode 0 = the wall's normal with its d component turned to point
+ * along fb (the other components are kept), 1 = the mesh normal, 2 = their normalised sum. */
+void orc_onesided_normal(int d, int fb, int mode, const double *nwall, double *dir) {
+  dir[0] = dir[1] = dir[2] = 0.0;
+  if (mode == 1) {
+    dir[d] = (double)fb;
+    return;
+  }
+  dir[0] = nwall[0]; dir[1] = nwall[1]; dir[2] = nwall[2];
+  if (dir[d] * (double)fb < 0.0) dir[d] *= -1;
+  if (mode == 2) {
+    dir[d] += (double)fb;
+    double s = 1.0 / sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= s; dir[1] *= s; dir[2] *= s;
+  }
+}
+
+/* The flux on the fluid side of a mesh face cut by an embedded wall (SpaceOperator.cpp:2438-2484 for
+ * i-1/2 faces, :2569-2611 for j-1/2, :2696-2737 for k-1/2): the one-sided problem along the
+ * GetNormalForOneSidedRiemann direction, the star state clipped (ClipDensityAndPressure), then the
+ * numerical flux along the axis between the fluid state and the star state -- F(V, V*) when the
+ * fluid cell is on the minus side (side = +1), F(V*, V) when it is on the plus side (side = -1).
+ * Vstar receives the clipped star state. Returns the solver's status. */
+int orc_wall_face_flux(const orc_ctx *c, int axis, int side, int mode, const double *nwall, const double *vwall,
+                       const double *V, int id, double *F, double *Vstar) {
+  const orc_config *g = &c->cfg;
+  double dir[3], Vs[5] = {0, 0, 0, 0, 0};
+  int sid = 0;
+  orc_onesided_normal(axis, side, mode, nwall, dir);
+  int err = orc_riemann_onesided_solve(g->mat, &g->riemann, dir, V, id, vwall, Vs, &sid, Vstar);
+  if (err < 0) return err;
+  orc_clip(&g->mat[id], Vstar);
+  if (side > 0) orc_numerical_flux(c, axis, V, Vstar, id, F);
+  else orc_numerical_flux(c, axis, Vstar, V, id, F);
+  return err;
 }
