@@ -35,7 +35,8 @@ enum m2c_eos_kind { M2C_EOS_SG = 0, M2C_EOS_NASG = 1, M2C_EOS_MG = 2, M2C_EOS_JW
 /* Boundary codes, IoData.h:266-267, plus PERIODIC (no reference counterpart) */
 enum m2c_bc { M2C_BC_INLET = 1, M2C_BC_OUTLET = 2, M2C_BC_SLIPWALL = 3, M2C_BC_STICKWALL = 4,
               M2C_BC_SYMMETRY = 5, M2C_BC_OVERSET = 6, M2C_BC_PERIODIC = 7 };
-/* NavierStokes flux, IoData.h (SchemeData::Flux): ROE=0, LLF=1, HLLC=2, GODUNOV=3 */
+/* NavierStokes flux, SchemeData::flux (IoData.cpp:1172): ROE=0, LLF=1, HLLC=2, GODUNOV=3. GODUNOV
+ * (FluxFcnGodunov.h) solves the exact Riemann problem of every face; FP64 contexts only. */
 enum m2c_flux { M2C_FLUX_ROE = 0, M2C_FLUX_LLF = 1, M2C_FLUX_HLLC = 2, M2C_FLUX_GODUNOV = 3 };
 enum m2c_limiter { M2C_LIM_NONE = 0, M2C_LIM_GENMINMOD = 1, M2C_LIM_VANALBADA = 2 };
 enum m2c_ts { M2C_TS_FE = 0, M2C_TS_RK2 = 1, M2C_TS_RK3 = 2 };

@@ -22,12 +22,11 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]
 
-# (tag, flux code, eos kind) -- flux codes: ROE 0, LLF 1, HLLC 2; eos kind -1 = run-time dispatch
-# (tag, flux code, eos kind, arithmetic type) -- flux codes: ROE 0, LLF 1, HLLC 2; eos kind -1 = run-time
+# (tag, flux code, eos kind, arithmetic type) -- flux codes: ROE 0, LLF 1, HLLC 2, GODUNOV 3; eos kind -1 = run-time
 # dispatch; float = the mixed-precision form
 VARIANTS = [("hllc_sg", 2, 0, "double"), ("hllc_gen", 2, -1, "double"), ("roe_gen", 0, -1, "double"),
             ("llf_gen", 1, -1, "double"), ("hllc_sg_f32", 2, 0, "float"), ("hllc_gen_f32", 2, -1, "float"),
-            ("roe_gen_f32", 0, -1, "float"), ("llf_gen_f32", 1, -1, "float")]
+            ("roe_gen_f32", 0, -1, "float"), ("llf_gen_f32", 1, -1, "float"), ("god_gen", 3, -1, "double")]
 # The mixed-form variants (FP32 reconstruction and fluxes) take the hardware's FP32 reciprocal and
 # square root instead of the correctly rounded sequences; FP64 code is not affected by this flag.
 # batch.hip holds the same FP32 stage bodies and takes it too (batched and single runs stay equal).

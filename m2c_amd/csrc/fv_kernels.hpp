@@ -89,6 +89,9 @@ struct StageArgs {
   // along d.
   const double *Dv[3] = {nullptr, nullptr, nullptr};
   const double *Dh[3] = {nullptr, nullptr, nullptr};
+  // Godunov fluxes of the faces between two cells of one active material (FLUX_GODUNOV only):
+  // G[d][p * n_d + f], faces indexed as Dv
+  const double *G[3] = {nullptr, nullptr, nullptr};
 };
 
 // Each translation unit that launches kernels holds its own copy of these tables (internal
@@ -227,7 +230,10 @@ __device__ __forceinline__ void numerical_flux(const TC *Vm, const TC *Vp, int i
   const DevEos &m = c_eos[id];
   if (FLUX == FLUX_HLLC) flux_hllc<TC, EK, DIR>(m, Vm, Vp, F);
   else if (FLUX == FLUX_ROE) flux_roe<TC, EK, DIR>(m, Vm, Vp, TC(M2C_G.roe_delta), F);
-  else flux_llf2<TC, EK, EK, DIR>(m, m, Vm, Vp, F);
+  else if (FLUX == FLUX_GODUNOV) {   // the Godunov pass supplies these faces (G arrays)
+#pragma unroll
+    for (int p = 0; p < 5; p++) F[p] = TC(0);
+  } else flux_llf2<TC, EK, EK, DIR>(m, m, Vm, Vp, F);
 }
 
 // Flux of one face between cells with IDs idm (minus) and idp (plus). Interface faces solved by
@@ -365,6 +371,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
   TC w0[5], w1[5], w2[5], sz0[5], sz1[5], Flo[5];
   double w0d[5];   // own cell in double (drop-in path: the stencil value itself)
   int id0, id1, id2;
+  int idlo;        // material of the cell below (k-1)
   int lo_iface = 0;
   {
     double d[5];
@@ -385,6 +392,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       nrev += face_state<EK, true, TC>(vm1, szm, idm1, fm);
     }
     lo_iface = face_flux<FLUX, EK, 2, TC>(fm, fp, vm1, w0, idm1, id0, Flo);
+    idlo = idm1;
     fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 2, d, id2, flags); to_tc(d, w2);
   }
 
@@ -553,6 +561,29 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
           for (int p = 0; p < 5; p++) Ft[p] = f[p * Nf];
         }
       }
+      if (FLUX == FLUX_GODUNOV) {
+        // faces between two cells of this cell's material: the Godunov pass's fluxes
+        const int me = sID[ty + HALO][tx + HALO];
+        if (me != g.inactive_id) {
+          const long long gxw = ((long long)k * g.ny + gj) * (g.nx + 1) + gi;
+          const long long gys = ((long long)k * (g.ny + 1) + gj) * g.nx + gi;
+          const long long gzb = ((long long)k * g.ny + gj) * g.nx + gi;
+          const long long mx = (long long)(g.nx + 1) * g.ny * g.nz, my = (long long)g.nx * (g.ny + 1) * g.nz,
+                          mz = (long long)g.nx * g.ny * (g.nz + 1), mxy = (long long)g.nx * g.ny;
+          const bool w = sID[ty + HALO][tx - 1 + HALO] == me, e = sID[ty + HALO][tx + 1 + HALO] == me;
+          const bool so = sID[ty - 1 + HALO][tx + HALO] == me, no = sID[ty + 1 + HALO][tx + HALO] == me;
+          const bool bo = idlo == me, to = id1 == me;
+#pragma unroll
+          for (int p = 0; p < 5; p++) {
+            if (w) Fw[p] = a.G[0][p * mx + gxw];
+            if (e) Fe[p] = a.G[0][p * mx + gxw + 1];
+            if (so) Fs[p] = a.G[1][p * my + gys];
+            if (no) Fn[p] = a.G[1][p * my + gys + g.nx];
+            if (bo) Fb[p] = a.G[2][p * mz + gzb];
+            if (to) Ft[p] = a.G[2][p * mz + gzb + mxy];
+          }
+        }
+      }
       double R[5];
       // viscous and heat-diffusion faces of this cell (SpaceOperator.cpp:3367-3371): added after the
       // advective sum, own faces first, then the east, north and front faces (ViscosityOperator.cpp:
@@ -664,6 +695,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       }
     }
     lo_iface = hi_iface;
+    idlo = id0;
     id0 = id1; id1 = id2; id2 = id3;
 #pragma unroll
     for (int p = 0; p < 5; p++) hv[p] = hvn[p];

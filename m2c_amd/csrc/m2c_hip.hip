@@ -53,7 +53,7 @@ struct m2c_ctx {
   Clock *clk = nullptr;
   Status *st = nullptr;
   int kchunk = 1;
-  int variant = 1;             // 0 HLLC+SG, 1 HLLC generic EOS, 2 Roe, 3 LLF; +4: mixed precision
+  int variant = 1;             // 0 HLLC+SG, 1 HLLC generic EOS, 2 Roe, 3 LLF; +4: mixed precision; 8 Godunov
   dim3 grid3;
   // two materials / level set
   bool iface = false;          // exact Riemann pass at material interfaces
@@ -74,11 +74,18 @@ struct m2c_ctx {
   std::vector<hipEvent_t> *diff_ev = nullptr;
   size_t diff_ev_i = 0;
   double diff_ms = 0.0;
+  // Godunov flux: per-direction face fluxes of the face pass and the exact solver's path cache
+  bool godunov = false;
+  double *G[3] = {nullptr, nullptr, nullptr};
+  long long gnf[3] = {0, 0, 0};
+  double *gscratch = nullptr;
+  int gthreads = 0;
 };
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
 
 static int ensure_riemann_constants(m2c_ctx *c);
+static int setup_godunov(m2c_ctx *c);
 static TLaw build_tlaw(const m2c_config &cfg);
 
 // ---------------------------------------------------------------- small kernels
@@ -257,8 +264,9 @@ static m2c_ctx *g_resident = nullptr;
 static int upload_constants(m2c_ctx *c) {
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_grid), &c->grid, sizeof(Grid)));
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_eos), c->eos, sizeof(c->eos)));
-  if ((c->iface || c->ls) && m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
+  if ((c->iface || c->ls || c->godunov) && m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
   if (m2c_stage_upload_hllc_sg(c->grid, c->eos) || m2c_stage_upload_hllc_gen(c->grid, c->eos) ||
+      m2c_stage_upload_god_gen(c->grid, c->eos) ||
       m2c_stage_upload_roe_gen(c->grid, c->eos) || m2c_stage_upload_llf_gen(c->grid, c->eos) ||
       m2c_stage_upload_hllc_sg_f32(c->grid, c->eos) || m2c_stage_upload_hllc_gen_f32(c->grid, c->eos) ||
       m2c_stage_upload_roe_gen_f32(c->grid, c->eos) || m2c_stage_upload_llf_gen_f32(c->grid, c->eos))
@@ -566,6 +574,12 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
       s.Dh[d] = c->Dh[d];
     }
   }
+  if (c->godunov) {   // the exact solve of every same-material face, before the stage
+    if (m2c_mp_launch_godunov_faces(c->stream, in_cons, S, c->ID, c->G, c->gnf, c->gscratch, c->gthreads, c->st,
+                                    c->clk, ignore_active))
+      return M2C_ERR_HIP;
+    for (int d = 0; d < 3; d++) s.G[d] = c->G[d];
+  }
   s.no_cfl = c->ls ? 1 : 0;
   s.S = S; s.Vn = Vn; s.Us = Us; s.Out = Out; s.ID = c->ID;
   s.Fi = c->Fi; s.Nf = c->Nf;
@@ -582,7 +596,8 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
   case 4: rc = m2c_stage_launch_hllc_sg_f32(in_cons, mode, c->grid3, c->stream, s); break;
   case 5: rc = m2c_stage_launch_hllc_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
   case 6: rc = m2c_stage_launch_roe_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
-  default: rc = m2c_stage_launch_llf_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
+  case 7: rc = m2c_stage_launch_llf_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
+  default: rc = m2c_stage_launch_god_gen(in_cons, mode, c->grid3, c->stream, s); break;
   }
   if (rc == 1) return M2C_ERR_ARG;
   if (rc) return M2C_ERR_HIP;
@@ -870,7 +885,7 @@ const char *m2c_version(void) { return "m2c_hip 0.1 (gfx950)"; }
 // translation unit). Returns an m2c_status code.
 int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || cfg->nmat < 1 || cfg->nmat > M2C_MAX_MAT) return M2C_ERR_ARG;
-  if (cfg->flux == M2C_FLUX_GODUNOV) return M2C_ERR_UNSUPPORTED;
+  if (cfg->flux == M2C_FLUX_GODUNOV && cfg->precision != M2C_PREC_FP64) return M2C_ERR_UNSUPPORTED;
   if (cfg->precision != M2C_PREC_FP64 && cfg->precision != M2C_PREC_MIXED) return M2C_ERR_ARG;
   for (int s = 0; s < 6; s++) {
     int b = cfg->bc[s];
@@ -914,6 +929,7 @@ int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   if (cfg->flux == M2C_FLUX_HLLC) v = (all_sg && cfg->inactive_id < 0 && !g.has_interfaces) ? 0 : 1;
   else v = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
   if (cfg->precision == M2C_PREC_MIXED) v += 4;
+  if (cfg->flux == M2C_FLUX_GODUNOV) v = 8;
   *variant = v;
   return M2C_OK;
 }
@@ -934,6 +950,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   Grid &g = c->grid;
   c->iface = g.has_interfaces && cfg->mp_flux != 2;
   c->ls = cfg->ls_matid >= 0;
+  c->godunov = cfg->flux == M2C_FLUX_GODUNOV;
   int bad = 0;
   if (c->iface && cfg->mp_recon != 0) bad = M2C_ERR_UNSUPPORTED;   // linear reconstruction at interfaces
   if (c->ls) {
@@ -983,6 +1000,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   HIPCHK(hipMemset(c->st, 0, sizeof(Status)));
   if (int rc = setup_multiphase(c)) return rc;
   if (int rc = setup_diffusion(c)) return rc;
+  if (int rc = setup_godunov(c)) return rc;
   if (upload_constants(c)) return M2C_ERR_HIP;
   hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(1), 0, c->stream, c->clk, 0.0, 0.0, cfg->cfl, 0);
 
@@ -1229,6 +1247,7 @@ int m2c_set_surrogate(m2c_ctx *c, const m2c_mlp *net, int newton_max) {
   }
   if (!c->iface) return M2C_ERR_ARG;                      // no material interfaces in this context
   if (c->cfg.mp_flux != 0) return M2C_ERR_UNSUPPORTED;    // the network serves the numerical-flux form
+  if (c->godunov) return M2C_ERR_UNSUPPORTED;             // ... of the approximate fluxes
   MlpWeights w;
   int dev = -1;
   if (m2c_mlp_get_weights(net, &w, &dev)) return M2C_ERR_ARG;
@@ -1347,7 +1366,17 @@ int m2c_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const double *V
   HIPCHK(hipMemcpyAsync(d, Vm, b, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d + 5 * (size_t)n, Vp, b, hipMemcpyHostToDevice, c->stream));
   dim3 gb((n + 255) / 256);
-  if (c->cfg.flux == M2C_FLUX_HLLC) launch_flux_batch<FLUX_HLLC>(dir, gb, c->stream, n, d, d + 5 * (size_t)n, id, d + 10 * (size_t)n);
+  if (c->cfg.flux == M2C_FLUX_GODUNOV) {   // an exact solve per pair: the solver's path cache in scratch
+    if (int rc0 = ensure_riemann_constants(c)) { hipFree(d); return rc0; }
+    const int nthreads = (int)std::min<long long>(4096, ((long long)n + 63) / 64 * 64);
+    double *scr = nullptr;
+    HIPCHK(hipMalloc(&scr, sizeof(double) * 6 * (size_t)c->mp.path_cap * nthreads));
+    int rc = m2c_mp_launch_godunov_batch(c->stream, dir, n, d, d + 5 * (size_t)n, id, d + 10 * (size_t)n, scr,
+                                         nthreads);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(scr));
+    if (rc) { hipFree(d); return M2C_ERR_HIP; }
+  } else if (c->cfg.flux == M2C_FLUX_HLLC) launch_flux_batch<FLUX_HLLC>(dir, gb, c->stream, n, d, d + 5 * (size_t)n, id, d + 10 * (size_t)n);
   else if (c->cfg.flux == M2C_FLUX_ROE) launch_flux_batch<FLUX_ROE>(dir, gb, c->stream, n, d, d + 5 * (size_t)n, id, d + 10 * (size_t)n);
   else launch_flux_batch<FLUX_LLF>(dir, gb, c->stream, n, d, d + 5 * (size_t)n, id, d + 10 * (size_t)n);
   HIPCHK(hipGetLastError());
@@ -1413,20 +1442,40 @@ int m2c_eos_derived(m2c_ctx *c, int n, const double *rho, const double *p, const
 
 // the exact solver's constants live with the interface kernels: a context without interfaces
 // uploads them before a pointwise batch
-static int ensure_riemann_constants(m2c_ctx *c) {
-  if (c->iface || c->ls) return M2C_OK;
-  MpParams mp;
+static void riemann_only_params(const m2c_config &cfg, MpParams &mp) {
   memset(&mp, 0, sizeof(mp));
-  const m2c_riemann_params &rp = c->cfg.riemann;
+  const m2c_riemann_params &rp = cfg.riemann;
   mp.rcfg.maxIts_main = rp.maxIts_main; mp.rcfg.maxIts_bracket = rp.maxIts_bracket;
   mp.rcfg.maxIts_shock = rp.maxIts_shock; mp.rcfg.numSteps_rarefaction = rp.numSteps_rarefaction;
   mp.rcfg.tol_main = rp.tol_main; mp.rcfg.tol_shock = rp.tol_shock; mp.rcfg.tol_rarefaction = rp.tol_rarefaction;
   mp.rcfg.min_pressure = rp.min_pressure; mp.rcfg.failure_threshold = rp.failure_threshold;
   mp.rcfg.pressure_at_failure = rp.pressure_at_failure;
   mp.path_cap = 1024;
-  c->mp = mp;
+}
+
+static int ensure_riemann_constants(m2c_ctx *c) {
+  if (c->iface || c->ls || c->godunov) return M2C_OK;   // uploaded with the context's constants
+  riemann_only_params(c->cfg, c->mp);
   if (m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
   return M2C_OK;
+}
+
+// Buffers of the Godunov face pass: five flux components per face of each direction (faces indexed
+// (k, j, i) with n_d + 1 planes along d) and the exact solver's rarefaction path cache, one slice
+// per thread of the pass.
+static int setup_godunov(m2c_ctx *c) {
+  if (!c->godunov) return M2C_OK;
+  const Grid &g = c->grid;
+  if (!c->iface && !c->ls) riemann_only_params(c->cfg, c->mp);
+  c->gnf[0] = (long long)(g.nx + 1) * g.ny * g.nz;
+  c->gnf[1] = (long long)g.nx * (g.ny + 1) * g.nz;
+  c->gnf[2] = (long long)g.nx * g.ny * (g.nz + 1);
+  int rc;
+  for (int d = 0; d < 3; d++)
+    if ((rc = mp_alloc(c, &c->G[d], (size_t)(5 * c->gnf[d]), 0))) return rc;
+  const long long mx = std::max(c->gnf[0], std::max(c->gnf[1], c->gnf[2]));
+  c->gthreads = (int)std::max<long long>(64, std::min<long long>(16384, (mx + 63) / 64 * 64));
+  return mp_alloc(c, &c->gscratch, (size_t)6 * c->mp.path_cap * c->gthreads, 0);
 }
 
 int m2c_riemann_batch(m2c_ctx *c, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,

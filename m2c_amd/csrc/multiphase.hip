@@ -397,13 +397,16 @@ __global__ __launch_bounds__(256) void k_iface_feat(const double *S, const int8_
 // ---------------------------------------------------------------- interface Riemann pass
 
 template <int FLUX, int DIR>
-__device__ __forceinline__ void iface_fluxes(const double *vm, const double *Vsm, int idm, const double *Vsp,
-                                             const double *vp, int idp, const double *Vmid, int midid, double *lf1,
-                                             double *lf2) {
+__device__ __forceinline__ void iface_fluxes(RSolver &S, const double *vm, const double *Vsm, int idm,
+                                             const double *Vsp, const double *vp, int idp, const double *Vmid,
+                                             int midid, double *lf1, double *lf2) {
   if (c_grid.mp_flux == 1) {   // EXACT: physical flux of the state at xi = 0 to both sides
     phys_flux<double, -1, DIR>(c_eos[midid], Vmid, lf1);
 #pragma unroll
     for (int p = 0; p < 5; p++) lf2[p] = lf1[p];
+  } else if (FLUX == FLUX_GODUNOV) {   // the numerical flux is itself an exact solve per side
+    godunov_flux<DIR>(S, vm, Vsm, idm, lf1);
+    godunov_flux<DIR>(S, Vsp, vp, idp, lf2);
   } else {
     numerical_flux<FLUX, -1, DIR>(vm, Vsm, idm, lf1);
     numerical_flux<FLUX, -1, DIR>(Vsp, vp, idp, lf2);
@@ -498,9 +501,9 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
                                           (unsigned long long)(idp & 0xff);
     }
     double lf1[5], lf2[5];
-    if (dir == 0) iface_fluxes<FLUX, 0>(vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
-    else if (dir == 1) iface_fluxes<FLUX, 1>(vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
-    else iface_fluxes<FLUX, 2>(vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
+    if (dir == 0) iface_fluxes<FLUX, 0>(S_, vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
+    else if (dir == 1) iface_fluxes<FLUX, 1>(S_, vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
+    else iface_fluxes<FLUX, 2>(S_, vm, Vsm, idm, Vsp, vp, idp, Vmid, midid, lf1, lf2);
     double *f = Fi + (long long)dir * 10 * Nf + key;
 #pragma unroll
     for (int p = 0; p < 5; p++) {
@@ -1214,7 +1217,8 @@ __global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir,
   S_.overflow = 0;
   const int cap = c_mp.path_cap;
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
-  S_.path3 = S_.path1;
+  S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
+  S_.path3.n = 0;
   for (int t = tid; t < n; t += nthreads) {
     double nrm[3], vs[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, vsm[5];
     const double *v = Vm + 5 * t;
@@ -1230,7 +1234,11 @@ __global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir,
         eos_clip<double, -1>(c_eos[m], vsm);
         const bool minus = fb[t] > 0;
         const double *a = minus ? v : vsm, *b = minus ? vsm : v;
-        if (ax[t] == 0) numerical_flux<FLUX, -1, 0>(a, b, m, f);
+        if (FLUX == FLUX_GODUNOV) {
+          if (ax[t] == 0) godunov_flux<0>(S_, a, b, m, f);
+          else if (ax[t] == 1) godunov_flux<1>(S_, a, b, m, f);
+          else godunov_flux<2>(S_, a, b, m, f);
+        } else if (ax[t] == 0) numerical_flux<FLUX, -1, 0>(a, b, m, f);
         else if (ax[t] == 1) numerical_flux<FLUX, -1, 1>(a, b, m, f);
         else numerical_flux<FLUX, -1, 2>(a, b, m, f);
       }
@@ -1239,6 +1247,98 @@ __global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir,
     for (int q = 0; q < 5; q++) { Vs[5 * t + q] = vs[q]; Vsm[5 * t + q] = vsm[q]; }
     id[t] = mid;
     status[t] = S_.overflow ? 2 : r;
+  }
+}
+
+// ---------------------------------------------------------------- Godunov flux on the grid
+
+// FluxFcnGodunov (FluxFcnGodunov.h:37-60) on every face of direction DIR between two cells of one
+// active material. The face states are formed as the stage kernel forms them: the slopes of the
+// face's two cells from the four cells along the axis, the revert rule, and at a domain face the
+// boundary rule on the inner state. Then the exact solver runs along the axis and G[p * nf + f] takes
+// the physical flux of the state at xi = 0. Other faces are left as they are (the stage kernel
+// does not read them). One thread per face (grid-stride) with the rarefaction path cache in the
+// thread's slice of the scratch buffer; faces are indexed (k, j, i) with n_DIR + 1 planes along DIR.
+template <int DIR, int IN_CONS>
+__global__ __launch_bounds__(64) void k_godunov_faces(const double *S, const int8_t *ID, double *G, long long nf,
+                                                       double *scratch, Status *st, const Clock *clk,
+                                                       int ignore_active) {
+  if (!ignore_active && !step_active(clk)) return;
+  const Grid &g = c_grid;
+  const int T = gridDim.x * blockDim.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  RSolver S_;
+  S_.mat = c_eos;
+  S_.cfg = c_mp.rcfg;
+  S_.overflow = 0;
+  const int cap = c_mp.path_cap;
+  S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = T; S_.path1.n = 0;
+  S_.path3.base = scratch + (long long)3 * cap * T + tid; S_.path3.cap = cap; S_.path3.stride = T; S_.path3.n = 0;
+  constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  const int fx = g.nx + di, fy = g.ny + dj;
+  const int gn = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+  const bool per = g.bc[2 * DIR] == BC_PERIODIC;
+  unsigned flags = 0;
+  for (long long f = tid; f < nf; f += T) {
+    const int i = (int)(f % fx);
+    const long long r = f / fx;
+    const int j = (int)(r % fy);
+    const int k = (int)(r / fy);
+    double vm1[5], vp0[5];
+    int idm1, idp0;
+    unsigned fl = 0;
+    fetch_cell<IN_CONS, -1>(S, ID, i - di, j - dj, k - dk, vm1, idm1, fl);
+    fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp0, idp0, fl);
+    if (idm1 != idp0 || idp0 == g.inactive_id) continue;
+    double vm2[5], vp1[5];
+    int idm2, idp1;
+    fetch_cell<IN_CONS, -1>(S, ID, i - 2 * di, j - 2 * dj, k - 2 * dk, vm2, idm2, fl);
+    fetch_cell<IN_CONS, -1>(S, ID, i + di, j + dj, k + dk, vp1, idp1, fl);
+    const int gpl = DIR == 0 ? i : (DIR == 1 ? j : k);
+    double sgm[5], sgp[5], fm[5], fp[5];
+    if (gpl == 0 && !per) {
+      cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+      face_state<-1, false, double>(vp0, sgp, idp0, fp);
+      ghost_face<DIR, double>(g.bc[2 * DIR], fp, vm1, fm);
+    } else if (gpl == gn && !per) {
+      cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
+      face_state<-1, true, double>(vm1, sgm, idm1, fm);
+      ghost_face<DIR, double>(g.bc[2 * DIR + 1], fm, vp0, fp);
+    } else {
+      cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
+      cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+      face_state<-1, true, double>(vm1, sgm, idm1, fm);
+      face_state<-1, false, double>(vp0, sgp, idp0, fp);
+    }
+    double F[5];
+    S_.overflow = 0;
+    if (godunov_flux<DIR>(S_, fm, fp, idp0, F) < 0) flags |= ST_BAD_STATE;
+    if (S_.overflow) flags |= ST_RIEMANN_PATH;
+#pragma unroll
+    for (int p = 0; p < 5; p++) G[p * nf + f] = F[p];
+  }
+  if (flags) atomicOr(&st->bits, flags);
+}
+
+// FluxFcnGodunov for n independent face-state pairs along axis dir (the pointwise flux batch)
+template <int DIR>
+__global__ __launch_bounds__(64) void k_godunov_batch(int n, const double *Vm, const double *Vp, int id, double *F,
+                                                       double *scratch, int nthreads) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nthreads) return;
+  RSolver S_;
+  S_.mat = c_eos;
+  S_.cfg = c_mp.rcfg;
+  S_.overflow = 0;
+  const int cap = c_mp.path_cap;
+  S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
+  S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
+  S_.path3.n = 0;
+  for (int t = tid; t < n; t += nthreads) {
+    double vm[5], vp[5], f[5];
+    for (int p = 0; p < 5; p++) { vm[p] = Vm[5 * (size_t)t + p]; vp[p] = Vp[5 * (size_t)t + p]; }
+    godunov_flux<DIR>(S_, vm, vp, id, f);
+    for (int p = 0; p < 5; p++) F[5 * (size_t)t + p] = f[p];
   }
 }
 
@@ -1355,6 +1455,9 @@ extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_co
   if (in_cons) { M2C_RIEMANN_IC(F, 1) } else { M2C_RIEMANN_IC(F, 0) }
   if (L.flux == FLUX_HLLC) { M2C_RIEMANN(FLUX_HLLC) }
   else if (L.flux == FLUX_ROE) { M2C_RIEMANN(FLUX_ROE) }
+  else if (L.flux == FLUX_GODUNOV) {   // no learned solver with this flux
+    if (in_cons) { M2C_RIEMANN_P(FLUX_GODUNOV, 1, 0); } else { M2C_RIEMANN_P(FLUX_GODUNOV, 0, 0); }
+  }
   else { M2C_RIEMANN(FLUX_LLF) }
 #undef M2C_RIEMANN
 #undef M2C_RIEMANN_IC
@@ -1381,8 +1484,30 @@ extern "C" int m2c_mp_launch_onesided_batch(hipStream_t s, int flux, int face, i
   if (!face) M2C_OS(FLUX_HLLC, 0);
   else if (flux == FLUX_HLLC) M2C_OS(FLUX_HLLC, 1);
   else if (flux == FLUX_ROE) M2C_OS(FLUX_ROE, 1);
+  else if (flux == FLUX_GODUNOV) M2C_OS(FLUX_GODUNOV, 1);
   else M2C_OS(FLUX_LLF, 1);
 #undef M2C_OS
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_godunov_faces(hipStream_t s, int in_cons, const double *S, const int8_t *ID,
+                                           double *const *G, const long long *nf, double *scratch, int nthreads,
+                                           Status *st, const Clock *clk, int ignore_active) {
+  const dim3 gr((nthreads + 63) / 64), bl(64);
+#define M2C_GF(D, IC)                                                                                           \
+  hipLaunchKernelGGL((k_godunov_faces<D, IC>), gr, bl, 0, s, S, ID, G[D], nf[D], scratch, st, clk, ignore_active)
+  if (in_cons) { M2C_GF(0, 1); M2C_GF(1, 1); M2C_GF(2, 1); }
+  else { M2C_GF(0, 0); M2C_GF(1, 0); M2C_GF(2, 0); }
+#undef M2C_GF
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_godunov_batch(hipStream_t s, int dir, int n, const double *Vm, const double *Vp, int id,
+                                           double *F, double *scratch, int nthreads) {
+  const dim3 gr((nthreads + 63) / 64), bl(64);
+  if (dir == 0) hipLaunchKernelGGL(k_godunov_batch<0>, gr, bl, 0, s, n, Vm, Vp, id, F, scratch, nthreads);
+  else if (dir == 1) hipLaunchKernelGGL(k_godunov_batch<1>, gr, bl, 0, s, n, Vm, Vp, id, F, scratch, nthreads);
+  else hipLaunchKernelGGL(k_godunov_batch<2>, gr, bl, 0, s, n, Vm, Vp, id, F, scratch, nthreads);
   return hchk(hipGetLastError());
 }
 

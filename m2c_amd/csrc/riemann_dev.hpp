@@ -836,6 +836,21 @@ __device__ inline int riemann_solve(RSolver &S, const double *dir, const double 
 }
 
 
+// FluxFcnGodunov::ComputeNumericalFluxAtCellInterface (FluxFcnGodunov.h:37-60): the exact problem
+// between Vm and Vp of one material along the axis, then the physical flux of the state at xi = 0 in
+// that material. The reference does not look at the solver's status; it is returned for the
+// invalid-state report (-1, where the reference exits).
+template <int DIR>
+__device__ inline int godunov_flux(RSolver &S, const double *Vm, const double *Vp, int id, double *F) {
+  double nrm[3] = {0.0, 0.0, 0.0};
+  nrm[DIR] = 1.0;
+  double Vmid[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Vsm[5], Vsp[5];
+  int midid = id;
+  const int r = riemann_solve(S, nrm, Vm, id, Vp, id, Vmid, &midid, Vsm, Vsp);
+  phys_flux<double, -1, DIR>(S.mat[id], Vmid, F);
+  return r;
+}
+
 // ---------------------------------------------------------------- one-sided (wall) problem
 //
 // ExactRiemannSolverBase::ComputeOneSidedRiemannSolution (ExactRiemannSolverBase.cpp:1931-2198) and

@@ -19,3 +19,4 @@ M2C_DECLARE_VARIANT(hllc_sg_f32)
 M2C_DECLARE_VARIANT(hllc_gen_f32)
 M2C_DECLARE_VARIANT(roe_gen_f32)
 M2C_DECLARE_VARIANT(llf_gen_f32)
+M2C_DECLARE_VARIANT(god_gen)

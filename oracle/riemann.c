@@ -1194,7 +1194,7 @@ int orc_interface_flux(orc_ctx *c, int dir, const double *vm, int idm, const dou
 void orc_flux_godunov(const orc_ctx *c, int dir, const double *Vm, const double *Vp, int id, double *F) {
   double n[3] = {0.0, 0.0, 0.0};
   n[dir] = 1.0;
-  double Vmid[5], Vsm[5], Vsp[5];
+  double Vmid[5] = {0, 0, 0, 0, 0}, Vsm[5], Vsp[5];   /* an invalid state (the reference exits) leaves Vmid */
   int midid = id;
   orc_riemann_solve(c->cfg.mat, &c->cfg.riemann, n, Vm, id, Vp, id, Vmid, &midid, Vsm, Vsp);
   orc_physical_flux(&c->cfg.mat[id], dir, Vmid, F);
