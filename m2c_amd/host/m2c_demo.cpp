@@ -5,6 +5,10 @@
 //   m2c_demo bubble <scale> <steps> <out.bin>            shock-bubble, RK2, reinit every 5 steps
 //   m2c_demo deck <input file> <out.bin>                 any input deck: its mesh, options, initial
 //                                                        state and Time.MaxTime / MaxIts / TimeStep
+//   m2c_demo onesided <problems.bin> <out.bin>           one-sided (wall) Riemann problems of air
+//                                                        (material 0) and water (material 1): 12
+//                                                        doubles per problem (dir, V, id, wall
+//                                                        velocity) in, 12 out (status, id, Vs, Vsm)
 //
 // Writes the primitive state (nz*ny*nx*5 doubles), then the IDs (int32) and, with a level set, the
 // level set, to out.bin; prints steps and time.
@@ -161,13 +165,51 @@ static int run_deck(const char *path, const char *out) {
   return 0;
 }
 
+static int run_onesided(const char *in, const char *out) {
+  std::vector<double> q;
+  {
+    FILE *f = fopen(in, "rb");
+    if (!f) return 1;
+    double v;
+    while (fread(&v, sizeof v, 1, f) == 1) q.push_back(v);
+    fclose(f);
+  }
+  if (q.size() % 12) return 1;
+  m2c_config c = default_config(4, 4, 4, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0);
+  add_stiffened_gas(c, 1.4, 0.0);
+  add_stiffened_gas(c, 6.12, 3.43e8);
+  Context ctx(c);
+  ExactRiemannSolver rs(ctx);
+  std::vector<double> r;
+  for (size_t t = 0; t < q.size(); t += 12) {
+    const double *p = &q[t];
+    double Vs[5] = {0, 0, 0, 0, 0}, Vsm[5];
+    int id = 0;
+    int st = rs.ComputeOneSidedRiemannSolution(p, p + 3, (int)p[8], p + 9, Vs, id, Vsm);
+    r.push_back(st);
+    r.push_back(id);
+    r.insert(r.end(), Vs, Vs + 5);
+    r.insert(r.end(), Vsm, Vsm + 5);
+  }
+  FILE *f = fopen(out, "wb");
+  if (!f) return 1;
+  fwrite(r.data(), sizeof(double), r.size(), f);
+  fclose(f);
+  printf("onesided problems=%zu\n", q.size() / 12);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc < 4 || (strcmp(argv[1], "deck") && argc < 5)) {
-    fprintf(stderr, "usage: %s sod <n> <tmax> <out> | bubble <scale> <steps> <out> | deck <input> <out>\n", argv[0]);
+  if (argc < 4 || (strcmp(argv[1], "deck") && strcmp(argv[1], "onesided") && argc < 5)) {
+    fprintf(stderr,
+            "usage: %s sod <n> <tmax> <out> | bubble <scale> <steps> <out> | deck <input> <out> | "
+            "onesided <in> <out>\n",
+            argv[0]);
     return 2;
   }
   try {
     if (!strcmp(argv[1], "deck")) return run_deck(argv[2], argv[3]);
+    if (!strcmp(argv[1], "onesided")) return run_onesided(argv[2], argv[3]);
     if (!strcmp(argv[1], "sod")) return run_sod(atoi(argv[2]), atof(argv[3]), argv[4]);
     if (!strcmp(argv[1], "bubble")) return run_bubble(atof(argv[2]), atoi(argv[3]), argv[4]);
   } catch (const Error &e) {

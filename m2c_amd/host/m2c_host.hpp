@@ -5,7 +5,8 @@
 //                           ConservativeToPrimitive-free state transfer)
 //   TimeIntegrator          TimeIntegrator.h:66-73 (AdvanceOneTimeStep) and the time loop of
 //                           Main.cpp:514-621
-//   ExactRiemannSolver      ExactRiemannSolverBase.h:35-39 (ComputeRiemannSolution)
+//   ExactRiemannSolver      ExactRiemannSolverBase.h:35-49 (ComputeRiemannSolution,
+//                           ComputeOneSidedRiemannSolution)
 //   LevelSetOperator        LevelSetOperator.h:86-90 (the level set Phi of one material)
 //
 // Arrays are the interior of SpaceVariable3D (k-j-i, dof innermost, no ghosts). Fatal conditions
@@ -252,6 +253,22 @@ class ExactRiemannSolver {
     int st = 0;
     check(m2c_riemann_batch(c_.get(), 1, &axis, Vm, &idm, Vp, &idp, Vs, &id, Vsm, Vsp, &st), "m2c_riemann_batch");
     if (st < 0) throw Error("ComputeRiemannSolution: invalid input state", M2C_ERR_STATE);
+    return st == 0 ? 0 : 1;
+  }
+
+  // ExactRiemannSolverBase::ComputeOneSidedRiemannSolution: the fluid state Vm (material idl)
+  // against a wall moving with velocity Ustar, along any unit normal dir from the fluid to the wall.
+  // id = -1 and Vs left as it was when the wall recedes from the fluid. Returns 0 on success, 1 on a
+  // solver failure (the reference counts it); throws on an invalid input state (the reference exits).
+  int ComputeOneSidedRiemannSolution(const double *dir, const double *Vm, int idl, const double *Ustar, double *Vs,
+                                     int &id, double *Vsm) {
+    double vs[5];
+    int st = 0;
+    check(m2c_onesided_riemann_batch(c_.get(), 1, dir, Vm, &idl, Ustar, vs, &id, Vsm, &st),
+          "m2c_onesided_riemann_batch");
+    if (st < 0) throw Error("ComputeOneSidedRiemannSolution: invalid input state", M2C_ERR_STATE);
+    if (id != -1)
+      for (int p = 0; p < 5; p++) Vs[p] = vs[p];
     return st == 0 ? 0 : 1;
   }
 

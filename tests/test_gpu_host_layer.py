@@ -1,6 +1,6 @@
 """The C++ host layer (m2c_amd/host/m2c_host.hpp, reference-shaped SpaceOperator / TimeIntegrator /
-LevelSetOperator classes over the C ABI) driving the GPU from a C++ program, checked bit for bit
-against the oracle on the same inputs."""
+LevelSetOperator / ExactRiemannSolver classes over the C ABI) driving the GPU from a C++ program,
+checked bit for bit against the oracle on the same inputs."""
 import os
 import subprocess
 
@@ -95,3 +95,29 @@ def test_cpp_host_layer_fixed_time_step_deck(tmp_path):
     assert f"steps={n} " in txt and n >= 10
     Vg = np.frombuffer(raw[:100 * 5 * 8], dtype=np.float64).reshape(1, 1, 100, 5)
     assert np.array_equal(Vg, V[1:-1, 1:-1, 1:-1])
+
+
+def test_cpp_host_layer_onesided_riemann(tmp_path):
+    # ExactRiemannSolver::ComputeOneSidedRiemannSolution of the host layer, air and water
+    rng = np.random.default_rng(13)
+    n = 300
+    q = np.zeros((n, 12))
+    d = rng.normal(size=(n, 3))
+    q[:, 0:3] = d / np.linalg.norm(d, axis=1)[:, None]
+    ids = rng.integers(0, 2, n)
+    q[:, 3] = np.where(ids == 0, rng.uniform(0.5, 2.0, n), rng.uniform(990.0, 1050.0, n))
+    q[:, 4:7] = rng.uniform(-80.0, 80.0, (n, 3))
+    q[:, 7] = np.where(ids == 0, rng.uniform(5e4, 2e5, n), rng.uniform(1e5, 1e7, n))
+    q[:, 8] = ids
+    q[:, 9:12] = rng.uniform(-80.0, 80.0, (n, 3))
+    src = tmp_path / "problems.bin"
+    q.tofile(str(src))
+    raw, txt = demo(["onesided", str(src)], tmp_path)
+    r = np.frombuffer(raw, dtype=np.float64).reshape(n, 12)
+    assert f"problems={n}" in txt
+    o = po.Oracle(po.make_config(4, 4, 4, (0, 1, 0, 1, 0, 1), [po.BC_SYMMETRY] * 6,
+                                 [po.sg(1.4, 0.0), po.sg(6.12, 3.43e8)]))
+    for t in range(n):
+        st, Vs, mid, Vsm = o.riemann_onesided(q[t, 0:3], q[t, 3:8], int(ids[t]), q[t, 9:12])
+        assert r[t, 0] == (0 if st == 0 else 1) and r[t, 1] == mid
+        assert np.array_equal(r[t, 2:7], Vs) and np.array_equal(r[t, 7:12], Vsm), t

@@ -206,6 +206,14 @@ under Time { under Explicit { Type = RungeKutta3; } MaxIts = 12; }
     assert cfg.cfl == 0.4 and cfg.ts == M.TS_RK3
 
 
+def test_navier_stokes_flux_keywords():
+    # SchemeData::flux: Roe 0, LocalLaxFriedrichs 1, HLLC 2, Godunov 3 (IoData.cpp:1170-1172)
+    for kw, code in (("Roe", M.FLUX_ROE), ("LocalLaxFriedrichs", M.FLUX_LLF), ("HLLC", M.FLUX_HLLC),
+                     ("Godunov", M.FLUX_GODUNOV)):
+        cfg, _, _ = read(f"Space.NavierStokes.Flux = {kw};")
+        assert cfg.flux == code, kw
+
+
 def test_multiphase_flux_keywords_map_to_the_library_numbering():
     # MultiPhaseData numbers Exact 0, Numerical 1, LocalLaxFriedrichs 2 (IoData.cpp:1425-1428)
     for kw, code in (("Exact", 1), ("Numerical", 0), ("LocalLaxFriedrichs", 2)):
