@@ -273,13 +273,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0: 8 for the periodic case, 10 for the shock-bubble)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="periodic_modes",
-                    choices=["periodic_modes", "shock_bubble", "batched_riemann", "learned", "viscous"])
+                    choices=["periodic_modes", "shock_bubble", "batched_riemann", "learned", "viscous", "godunov"])
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision / surrogate comparison runs")
     args = ap.parse_args()
     if args.steps <= 0:
-        args.steps = {"shock_bubble": 300, "batched_riemann": 100}.get(args.workload, 200)
+        args.steps = {"shock_bubble": 300, "batched_riemann": 100, "godunov": 20}.get(args.workload, 200)
     if args.cpu_steps <= 0:
-        args.cpu_steps = {"shock_bubble": 10, "viscous": 4}.get(args.workload, 8)
+        args.cpu_steps = {"shock_bubble": 10, "viscous": 4, "godunov": 2}.get(args.workload, 8)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design, see the sharding section of the design notes", file=sys.stderr)
@@ -294,12 +294,17 @@ def main():
         return bench_learned(M, W, args)
     mp = args.workload == "shock_bubble"
     visc = args.workload == "viscous"
+    god = args.workload == "godunov"
+    if god and args.n == 256:
+        args.n = 128   # an exact Riemann solve per face: 128^3 keeps the CPU sample near 20 s
     if mp:
         settings, V0, ID0, phi0 = W.shock_bubble()
     elif visc:
         settings, V0, ID0 = W.periodic_viscous(n=args.n)
     else:
         settings, V0, ID0 = W.periodic_modes(n=args.n)
+        if god:
+            settings = dict(settings, flux=M.FLUX_GODUNOV)
     cells = V0.shape[0] * V0.shape[1] * V0.shape[2]
     s = M.Solver(W.config_from(settings))
     s.upload(V0, ID0)
@@ -334,10 +339,11 @@ def main():
     total_steps = args.warmup + args.steps
     bpcu = BYTES_PER_CELL_UPDATE_MP + BYTES_PER_REINIT_ITER * reinit_its / max(total_steps, 1) if mp \
         else BYTES_PER_CELL_UPDATE + (3 * (BYTES_DIFF_PASS + BYTES_DIFF_STAGE_READ) if visc else 0.0)
-    traffic = STAGE_TRAFFIC_PERIODIC256_FP64 if (not mp and not visc and args.n == 256) else None
+    traffic = STAGE_TRAFFIC_PERIODIC256_FP64 if (not mp and not visc and not god and args.n == 256) else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "m2c::stage_kernel (fused MUSCL+HLLC+RK stage)", "avg_launch_ms": ms_stage}
+            "kernel": "m2c::stage_kernel (fused MUSCL+" + ("Godunov-flux read" if god else "HLLC") + "+RK stage)",
+            "avg_launch_ms": ms_stage}
     line = {"metric": "cell-updates/s on 1 GPU: plain FP64 vs mixed form at L1 err<=1e-5; % HBM peak",
             "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -345,7 +351,8 @@ def main():
             "config": {"workload": (f"shock_bubble_{V0.shape[2]}x{V0.shape[1]}x{V0.shape[0]}_rk2_hllc_vanalbada_"
                                     "exact_riemann_levelset_reinit5_fp64") if mp
                        else (f"periodic_viscous_{args.n}^3_rk3_hllc_vanalbada_mu0.01_kappa0.02_fp64" if visc
-                             else f"periodic_modes_{args.n}^3_rk3_hllc_vanalbada_fp64"), "cells": cells,
+                             else f"periodic_modes_{args.n}^3_rk3_{'godunov' if god else 'hllc'}_vanalbada_fp64"),
+                       "cells": cells,
                        "precision": "fp64", "bytes_per_cell_update": bpcu,
                        "hbm_frac_whole_step": value * bpcu / 1e9 / HBM_PEAK_GBS,
                        "measured_copy_GBs": copy_gbs, "copy_kernel_ms_per_GiB": copy_ms},
@@ -359,7 +366,14 @@ def main():
                                   "bytes_per_cell": BYTES_DIFF_PASS}
         line["ms_per_step_parts"] = {"stage_kernels": 3 * ms_stage, "diffusion_passes": 3 * ms_diff,
                                      "whole_step": ms_step_ev}
-    if not args.no_mixed:
+    if god:
+        # the Godunov face pass: an exact Riemann solve per face, three face kernels per stage
+        line["godunov_pass"] = {"bound": "latency (iterative exact solver per face)", "avg_pass_ms": ms_diff,
+                                "faces_per_pass": 3 * cells, "faces_per_s": 3 * cells / (ms_diff * 1e-3),
+                                "kernels": "k_godunov_faces<0..2> (one thread per face, grid-stride)"}
+        line["ms_per_step_parts"] = {"stage_kernels": 3 * ms_stage, "godunov_passes": 3 * ms_diff,
+                                     "whole_step": ms_step_ev}
+    if not args.no_mixed and not god:
         # the shock-bubble amplifies any perturbation once the shock strikes the bubble (~step 180),
         # so the forms are also compared at step 150
         check = (150, run_untimed(M, W, settings, V0, ID0, phi0, 150, M.PREC_FP64)) if mp else None

@@ -275,8 +275,9 @@ int m2c_surrogate_riemann_batch(m2c_ctx *ctx, const m2c_mlp *net, int newton_max
  * recorded on the context's stream around every stage launch and around the whole loop. */
 int m2c_time_steps(m2c_ctx *ctx, int nsteps, double *ms_stage_kernel, double *ms_step);
 
-/* Average duration (ms) of one viscous / heat-diffusion face pass (its three face kernels) in the
- * last m2c_time_steps call, which leaves it out of ms_stage_kernel; 0 without those terms. */
+/* Average duration (ms) per stage of the face passes that precede the stage kernel (viscous /
+ * heat-diffusion terms, Godunov fluxes; three face kernels each) in the last m2c_time_steps call,
+ * which leaves them out of ms_stage_kernel; 0 without such a pass. */
 int m2c_time_diffusion_pass(m2c_ctx *ctx, double *ms);
 
 /* Measured HBM copy bandwidth: a plain device copy kernel moves `nbytes` (read + write) `reps`

@@ -394,7 +394,8 @@ class Solver:
         return a.value, b.value
 
     def diffusion_pass_ms(self):
-        """Average viscous / heat-diffusion face pass (ms) of the last time_steps() call."""
+        """Average face pass before each stage (viscous / heat terms, Godunov fluxes; ms) of the last
+        time_steps() call."""
         ms = C.c_double()
         _chk(self.L.m2c_time_diffusion_pass(self.h, C.byref(ms)), "m2c_time_diffusion_pass")
         return ms.value

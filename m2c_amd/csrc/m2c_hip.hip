@@ -575,9 +575,12 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
     }
   }
   if (c->godunov) {   // the exact solve of every same-material face, before the stage
+    const bool prof = c->diff_ev && c->diff_ev_i + 2 <= c->diff_ev->size();
+    if (prof) hipEventRecord((*c->diff_ev)[c->diff_ev_i++], c->stream);
     if (m2c_mp_launch_godunov_faces(c->stream, in_cons, S, c->ID, c->G, c->gnf, c->gscratch, c->gthreads, c->st,
                                     c->clk, ignore_active))
       return M2C_ERR_HIP;
+    if (prof) hipEventRecord((*c->diff_ev)[c->diff_ev_i++], c->stream);
     for (int d = 0; d < 3; d++) s.G[d] = c->G[d];
   }
   s.no_cfl = c->ls ? 1 : 0;
@@ -1162,10 +1165,12 @@ int m2c_time_steps(m2c_ctx *c, int nsteps, double *ms_stage, double *ms_step) {
     HIPCHK(hipStreamSynchronize(c->stream));
     step0 = h.steps;
   }
-  // the stage events bracket the diffusion pass too; its own events let it be taken out
-  std::vector<hipEvent_t> dev(c->diff ? 2 * ns * nsteps : 0);
+  // the stage events bracket the face pass before the stage (viscous / heat terms, Godunov fluxes)
+  // too; its own events let it be taken out
+  const bool pre = c->diff || c->godunov;
+  std::vector<hipEvent_t> dev(pre ? 2 * ns * nsteps * (1 + (c->diff && c->godunov)) : 0);
   for (auto &e : dev) HIPCHK(hipEventCreate(&e));
-  c->diff_ev = c->diff ? &dev : nullptr;
+  c->diff_ev = pre ? &dev : nullptr;
   c->diff_ev_i = 0;
   for (int s = 0; s < nsteps; s++) {
     std::vector<hipEvent_t> sub(ev.begin() + 2 * ns * s, ev.begin() + 2 * ns * (s + 1));
@@ -1193,7 +1198,7 @@ int m2c_time_steps(m2c_ctx *c, int nsteps, double *ms_stage, double *ms_step) {
     kd += t;
   }
   ks -= kd;
-  c->diff_ms = c->diff_ev_i ? kd / (c->diff_ev_i / 2) : 0.0;
+  c->diff_ms = c->diff_ev_i ? kd / (ns * nsteps) : 0.0;
   for (auto &e : dev) hipEventDestroy(e);
   for (auto &e : ev) hipEventDestroy(e);
   hipEventDestroy(a);
@@ -1474,7 +1479,9 @@ static int setup_godunov(m2c_ctx *c) {
   for (int d = 0; d < 3; d++)
     if ((rc = mp_alloc(c, &c->G[d], (size_t)(5 * c->gnf[d]), 0))) return rc;
   const long long mx = std::max(c->gnf[0], std::max(c->gnf[1], c->gnf[2]));
-  c->gthreads = (int)std::max<long long>(64, std::min<long long>(16384, (mx + 63) / 64 * 64));
+  // every face is an iterative solve: one wave per SIMD of the 256 CUs (65 536 threads, 48 KB of
+  // path cache each); three waves per SIMD (the pass's 168 VGPRs allow it) measured no faster
+  c->gthreads = (int)std::max<long long>(64, std::min<long long>(65536, (mx + 63) / 64 * 64));
   return mp_alloc(c, &c->gscratch, (size_t)6 * c->mp.path_cap * c->gthreads, 0);
 }
 
