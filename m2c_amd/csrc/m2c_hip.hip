@@ -67,6 +67,7 @@ struct m2c_ctx {
   // viscous and heat-diffusion terms
   bool diff = false;
   double *Dv[3] = {nullptr, nullptr, nullptr}, *Dh[3] = {nullptr, nullptr, nullptr};
+  double *Vprim = nullptr;     // primitive copy of a conservative stage field for the face pass
   long long nf[3] = {0, 0, 0};
   DiffTab dtab;
   DiffMat dmat;
@@ -483,6 +484,7 @@ static int setup_diffusion(m2c_ctx *c) {
     if ((rc = mp_alloc(c, &c->Dv[d], (size_t)(10 * c->nf[d]), 0))) return rc;
     if ((rc = mp_alloc(c, &c->Dh[d], (size_t)c->nf[d], 0))) return rc;
   }
+  if ((rc = mp_alloc(c, &c->Vprim, (size_t)(5 * c->N), 0))) return rc;
   // coordinates x[-1..n] (MeshGenerator.cpp:146-158, ghosts SpaceOperator.cpp:218-265), interpolation
   // weights (Interpolator.cpp:150-181) and node-derivative coefficients (GradientCalculatorCentral.cpp:
   // 249-424), evaluated as the reference evaluates them
@@ -544,18 +546,34 @@ static int setup_diffusion(m2c_ctx *c) {
   return M2C_OK;
 }
 
+// ConservativeToPrimitive and the clip of every cell of a conservative stage field, once, as the
+// stencil fetch does them (the ghost rules then act on the primitive values, as they do there)
+__global__ __launch_bounds__(256) void k_prim_field(const double *__restrict__ U, const int8_t *__restrict__ ID,
+                                                    double *__restrict__ V) {
+  const Grid &g = c_grid;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  double u[5], v[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) u[p] = U[p * g.N + c];
+  const DevEos &m = c_eos[ID[c]];
+  eos_c2p<-1>(m, u, v);
+  eos_clip<double, -1>(m, v);
+#pragma unroll
+  for (int p = 0; p < 5; p++) V[p * g.N + c] = v[p];
+}
+
 static int diffusion_pass(m2c_ctx *c, const double *S, int in_cons) {
+  if (in_cons) {   // the face kernels fetch ~10 cells per face: convert each cell once instead
+    hipLaunchKernelGGL(k_prim_field, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, S, c->ID,
+                       c->Vprim);
+    S = c->Vprim;
+  }
   for (int d = 0; d < 3; d++) {
     dim3 gb((unsigned)((c->nf[d] + 255) / 256)), bs(256);
-    if (in_cons) {
-      if (d == 0) hipLaunchKernelGGL((k_diff_faces<0, 1>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[0], c->Dh[0], c->nf[0]);
-      else if (d == 1) hipLaunchKernelGGL((k_diff_faces<1, 1>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[1], c->Dh[1], c->nf[1]);
-      else hipLaunchKernelGGL((k_diff_faces<2, 1>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[2], c->Dh[2], c->nf[2]);
-    } else {
-      if (d == 0) hipLaunchKernelGGL((k_diff_faces<0, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[0], c->Dh[0], c->nf[0]);
-      else if (d == 1) hipLaunchKernelGGL((k_diff_faces<1, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[1], c->Dh[1], c->nf[1]);
-      else hipLaunchKernelGGL((k_diff_faces<2, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[2], c->Dh[2], c->nf[2]);
-    }
+    if (d == 0) hipLaunchKernelGGL((k_diff_faces<0, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[0], c->Dh[0], c->nf[0]);
+    else if (d == 1) hipLaunchKernelGGL((k_diff_faces<1, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[1], c->Dh[1], c->nf[1]);
+    else hipLaunchKernelGGL((k_diff_faces<2, 0>), gb, bs, 0, c->stream, S, c->ID, c->dtab, c->dmat, c->Dv[2], c->Dh[2], c->nf[2]);
   }
   HIPCHK(hipGetLastError());
   return M2C_OK;
