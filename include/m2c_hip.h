@@ -95,6 +95,12 @@ typedef struct {
   int reinit_freq;                  /* reinitialise every reinit_freq steps (<= 0: never) */
   int reinit_maxits;
   double reinit_cfl, reinit_tol;
+  /* MultiPhase.RiemannNormal (MultiPhaseData, IoData.cpp:1445-1448): direction of the exact Riemann
+   * problem at a material interface. 1 Mesh (the face's axis, the default); 0 LevelSet (the
+   * level-set gradient at the face, SpaceOperator::CalculateGradPhiAtCellInterface); 2 Average
+   * (their normalised sum). LevelSet and Average need the level set and interfaces only between
+   * material 0 and the level set's material. */
+  int mp_riemann_normal;
 } m2c_config;
 
 typedef struct {

@@ -641,7 +641,7 @@ struct Mapper {
     gett(mp, "Flux", c.mp_flux, {{"Exact", 1}, {"Numerical", 0}, {"LocalLaxFriedrichs", 2}});
     gett(mp, "ReconstructionAtInterface", c.mp_recon, {{"Constant", 0}, {"Linear", 1}});
     int v = 0;
-    gett(mp, "RiemannNormal", v, {{"Mesh", 1}}, {"LevelSet", "Average"});
+    gett(mp, "RiemannNormal", c.mp_riemann_normal, {{"LevelSet", 0}, {"Mesh", 1}, {"Average", 2}});
     gett(mp, "PhaseChange", v, {{"RiemannSolution", 0}}, {"Extrapolation"});
     gett(mp, "PhaseChangeDirection", v, {{"Upwind", 1}}, {"All"});
     gett(mp, "LatentHeatTransfer", v, {{"Off", 0}}, {"On"});
@@ -998,6 +998,7 @@ int m2c_config_defaults(m2c_config *c) {
   c->reinit_maxits = 30;
   c->reinit_cfl = 0.8;
   c->reinit_tol = 2.0e-4;
+  c->mp_riemann_normal = 1;         // MultiPhaseData::riemann_normal = MESH (IoData.cpp:1408)
   return M2C_OK;
 }
 

@@ -777,6 +777,7 @@ static int setup_multiphase(m2c_ctx *c) {
   mp.rcfg.min_pressure = rp.min_pressure; mp.rcfg.failure_threshold = rp.failure_threshold;
   mp.rcfg.pressure_at_failure = rp.pressure_at_failure;
   mp.path_cap = 1024;
+  mp.riemann_normal = cfg->mp_riemann_normal;
   for (int m = 0; m < cfg->nmat; m++) mp.failsafe_density[m] = cfg->mat[m].failsafe_density;
   int rc;
   if ((rc = mp_alloc(c, &L.ms, 1, 0))) return rc;
@@ -799,6 +800,7 @@ static int setup_multiphase(m2c_ctx *c) {
   if (c->ls) {
     if ((rc = mp_alloc(c, &c->phiA, (size_t)N, 0))) return rc;
     if ((rc = mp_alloc(c, &c->phiB, (size_t)N, 0))) return rc;
+    L.phin = c->phiA;   // Phi(n) throughout a step's fluid stages
     if (cfg->ts == M2C_TS_RK3 && (rc = mp_alloc(c, &c->phiD, (size_t)N, 0))) return rc;
     if ((rc = mp_alloc(c, &L.IDnew, (size_t)N, 0))) return rc;
     if ((rc = mp_alloc(c, &L.rem, (size_t)N, 0))) return rc;
@@ -974,6 +976,14 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   c->godunov = cfg->flux == M2C_FLUX_GODUNOV;
   int bad = 0;
   if (c->iface && cfg->mp_recon != 0) bad = M2C_ERR_UNSUPPORTED;   // linear reconstruction at interfaces
+  // RiemannNormal LevelSet / Average: the reference takes the gradient of the level set of the
+  // face's non-zero material (CalculateGradPhiAtCellInterface asserts that it has one)
+  if (cfg->mp_riemann_normal < 0 || cfg->mp_riemann_normal > 2) bad = M2C_ERR_ARG;
+  if (c->iface && cfg->mp_riemann_normal != 1) {
+    if (!c->ls) bad = bad ? bad : M2C_ERR_ARG;
+    for (int m = 0; m < cfg->nmat; m++)
+      if (m != 0 && m != cfg->ls_matid && m != cfg->inactive_id) bad = bad ? bad : M2C_ERR_UNSUPPORTED;
+  }
   if (c->ls) {
     if (cfg->ls_matid >= cfg->nmat || cfg->ls_matid == cfg->inactive_id) bad = M2C_ERR_ARG;
     for (int sd = 0; sd < 6; sd++) {
@@ -1271,6 +1281,7 @@ int m2c_set_surrogate(m2c_ctx *c, const m2c_mlp *net, int newton_max) {
   if (!c->iface) return M2C_ERR_ARG;                      // no material interfaces in this context
   if (c->cfg.mp_flux != 0) return M2C_ERR_UNSUPPORTED;    // the network serves the numerical-flux form
   if (c->godunov) return M2C_ERR_UNSUPPORTED;             // ... of the approximate fluxes
+  if (c->cfg.mp_riemann_normal != 1) return M2C_ERR_UNSUPPORTED;   // it solves along the mesh axes
   MlpWeights w;
   int dev = -1;
   if (m2c_mlp_get_weights(net, &w, &dev)) return M2C_ERR_ARG;
@@ -1474,6 +1485,7 @@ static void riemann_only_params(const m2c_config &cfg, MpParams &mp) {
   mp.rcfg.min_pressure = rp.min_pressure; mp.rcfg.failure_threshold = rp.failure_threshold;
   mp.rcfg.pressure_at_failure = rp.pressure_at_failure;
   mp.path_cap = 1024;
+  mp.riemann_normal = 1;
 }
 
 static int ensure_riemann_constants(m2c_ctx *c) {

@@ -105,6 +105,99 @@ __device__ inline double phi_at(const double *phi, int i, int j, int k) {
   return v;
 }
 
+// ---------------------------------------------------------------- interface normal from the level set
+
+// phi at (i,j,k) for the interface normal: periodic axes are wrapped first, then at most one index
+// is outside the box and takes the face-ghost rule of phi_at
+__device__ inline double phi_nrm(const double *phi, int i, int j, int k) {
+  const Grid &g = c_grid;
+  if (i < 0 && c_mp.ls_bc[0] == BC_PERIODIC) i += g.nx;
+  else if (i >= g.nx && c_mp.ls_bc[1] == BC_PERIODIC) i -= g.nx;
+  if (j < 0 && c_mp.ls_bc[2] == BC_PERIODIC) j += g.ny;
+  else if (j >= g.ny && c_mp.ls_bc[3] == BC_PERIODIC) j -= g.ny;
+  if (k < 0 && c_mp.ls_bc[4] == BC_PERIODIC) k += g.nz;
+  else if (k >= g.nz && c_mp.ls_bc[5] == BC_PERIODIC) k -= g.nz;
+  return phi_at(phi, i, j, k);
+}
+
+// CentralDifferenceLocal (SpaceOperator.h:218-223)
+__device__ __forceinline__ double cdiff_local(double p0, double p1, double p2, double x0, double x1, double x2) {
+  const double c0 = -(x2 - x1) / ((x1 - x0) * (x2 - x0));
+  const double c1 = 1.0 / (x1 - x0) - 1.0 / (x2 - x1);
+  const double c2 = (x1 - x0) / ((x2 - x0) * (x2 - x1));
+  return c0 * p0 + c1 * p1 + c2 * p2;
+}
+
+// SpaceOperator::CalculateGradientAtCellInterface (SpaceOperator.cpp:3052-3190): the normalised
+// gradient of phi at the D-face of cell q = (i,j,k) (between q - e_D and q). The normal component
+// is the difference across the face; each transverse one is the central difference (one-sided at
+// the ghost layer) in the two cells, interpolated to the face.
+template <int D>
+__device__ void grad_at_face(const double *phi, const int q[3], double *dir) {
+  const Grid &g = c_grid;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const double h[3] = {g.dx, g.dy, g.dz};
+  int m[3] = {q[0], q[1], q[2]};
+  m[D] -= 1;
+  auto P = [&](const int *c, int t, int s) {
+    int r[3] = {c[0], c[1], c[2]};
+    r[t] += s;
+    return phi_nrm(phi, r[0], r[1], r[2]);
+  };
+  dir[D] = (P(q, D, 0) - P(m, D, 0)) / (XC(D, q[D]) - XC(D, m[D]));
+  const double x_half = XC(D, q[D]) - 0.5 * h[D];
+  const double dd = XC(D, q[D]) - XC(D, m[D]);
+  const double c1 = (XC(D, q[D]) - x_half) / dd;
+  const double c2 = (x_half - XC(D, m[D])) / dd;
+#pragma unroll
+  for (int a = 1; a <= 2; a++) {
+    const int tt = a == 1 ? (D == 0 ? 1 : 0) : (D == 2 ? 1 : 2);   // the transverse axes in order
+    const int qt = q[tt];
+    double d1, d2;
+    if (qt < 0) {
+      d1 = (P(m, tt, 1) - P(m, tt, 0)) / (XC(tt, qt + 1) - XC(tt, qt));
+      d2 = (P(q, tt, 1) - P(q, tt, 0)) / (XC(tt, qt + 1) - XC(tt, qt));
+    } else if (qt >= n[tt]) {
+      d1 = (P(m, tt, 0) - P(m, tt, -1)) / (XC(tt, qt) - XC(tt, qt - 1));
+      d2 = (P(q, tt, 0) - P(q, tt, -1)) / (XC(tt, qt) - XC(tt, qt - 1));
+    } else {
+      d1 = cdiff_local(P(m, tt, -1), P(m, tt, 0), P(m, tt, 1), XC(tt, qt - 1), XC(tt, qt), XC(tt, qt + 1));
+      d2 = cdiff_local(P(q, tt, -1), P(q, tt, 0), P(q, tt, 1), XC(tt, qt - 1), XC(tt, qt), XC(tt, qt + 1));
+    }
+    dir[tt] = c1 * d1 + c2 * d2;
+  }
+  const double nrm = dsqrt<double>(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  if (nrm != 0.0) {
+    const double s = 1.0 / nrm;
+    dir[0] *= s; dir[1] *= s; dir[2] *= s;
+  }
+}
+
+// GetNormalForBimaterialRiemann with CalculateGradPhiAtCellInterface (SpaceOperator.cpp:2966-3048)
+// for one level set: the level-set gradient at the face, turned when the minus cell is material 0,
+// the mesh normal if it vanishes, its axis component made non-negative; AVERAGE adds the mesh normal
+// and normalises. myid: the plus cell's material, nbid: the minus cell's.
+__device__ void interface_normal(int D, int i, int j, int k, int myid, int nbid, const double *phi, double *dir) {
+  dir[0] = dir[1] = dir[2] = 0.0;
+  const int mode = c_mp.riemann_normal;
+  if (mode == 1 || !phi) {
+    dir[D] = 1.0;
+    return;
+  }
+  const int q[3] = {i, j, k};
+  if (D == 0) grad_at_face<0>(phi, q, dir);
+  else if (D == 1) grad_at_face<1>(phi, q, dir);
+  else grad_at_face<2>(phi, q, dir);
+  if (nbid == 0) { dir[0] *= -1.0; dir[1] *= -1.0; dir[2] *= -1.0; }
+  if (dsqrt<double>(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[D] = 1.0;
+  if (dir[D] < 0.0) dir[D] *= -1.0;
+  if (mode == 2) {
+    dir[D] += 1.0;
+    const double s = 1.0 / dsqrt<double>(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= s; dir[1] *= s; dir[2] *= s;
+  }
+}
+
 // linear index of (i,j,k) with each coordinate clamped into the box
 __device__ __forceinline__ long long clamped_idx(int i, int j, int k) {
   const Grid &g = c_grid;
@@ -424,7 +517,7 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
                                                  MpState *ms, double *Fi, double *rsV, unsigned long long *rsMeta,
                                                  int store, double *scratch, Status *st, const Clock *clk,
                                                  int ignore_active, const float *pred, int sur_cap, int sur_nmax,
-                                                 double sur_tol, unsigned *fbl) {
+                                                 double sur_tol, unsigned *fbl, const double *phin) {
   if (!ignore_active && !step_active(clk)) return;
   const Grid &g = c_grid;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
@@ -462,8 +555,8 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
     fetch_cell<IN_CONS, -1>(S, ID, i - (dir == 0), j - (dir == 1), k - (dir == 2), vm, idm, fl);
     fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp, idp, fl);
     flags |= fl;
-    double nrm[3] = {0.0, 0.0, 0.0};
-    nrm[dir] = 1.0;
+    double nrm[3];
+    interface_normal(dir, i, j, k, idp, idm, phin, nrm);
     double Vmid[5], Vsm[5], Vsp[5];
     int midid = 0;
     int err = 0;
@@ -1443,7 +1536,8 @@ extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_co
   }
 #define M2C_RIEMANN_P(F, IC, P)                                                                                 \
   hipLaunchKernelGGL((k_riemann<F, IC, P>), gr, bl, 0, L.stream, S, L.ID, L.faces, L.ms, L.Fi, L.rsV, L.rsMeta,    \
-                     store, L.scratch, L.st, L.clk, ignore_active, pred, L.sur_cap, L.sur_nmax, L.sur_tol, L.surFb)
+                     store, L.scratch, L.st, L.clk, ignore_active, pred, L.sur_cap, L.sur_nmax, L.sur_tol, L.surFb, \
+                     L.phin)
 #define M2C_RIEMANN_IC(F, IC)                                                                                   \
   if (pred) {                                                                                                   \
     M2C_RIEMANN_P(F, IC, 1);                                                                                    \

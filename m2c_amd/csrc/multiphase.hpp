@@ -34,6 +34,7 @@ struct MpParams {
   RiemannCfg rcfg;
   int path_cap;                // rarefaction path entries per side and thread
   double failsafe_density[8];
+  int riemann_normal;          // MultiPhase.RiemannNormal: 0 level set, 1 mesh, 2 average
 };
 
 struct ReinitState {
@@ -63,6 +64,7 @@ struct MpLaunch {
   unsigned *faces;
   MpState *ms;
   double *Fi, *rsV;
+  const double *phin;          // the level set at t_n (the interface normal of RiemannNormal LevelSet/Average)
   unsigned long long *rsMeta;
   double *scratch;
   Status *st;

@@ -57,6 +57,7 @@ typedef struct {
   int ls_bc[6];             /* 0 none, 1 zero-Neumann, 2 linear extrapolation, 3 non-negative, 7 periodic */
   int reinit_freq, reinit_maxits;
   double reinit_cfl, reinit_tol;
+  int mp_normal;            /* MultiPhase.RiemannNormal: 0 LevelSet, 1 Mesh, 2 Average */
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;
@@ -77,6 +78,16 @@ void orc_apply_bc_id(const orc_ctx *c, int *ID);
  * R is ghosted (only interior entries are meaningful). Returns the number of Riemann errors,
  * or -1 if a reconstructed state violates hyperbolicity (the reference exits). */
 int orc_compute_residual(orc_ctx *c, const double *V, const int *ID, double *R);
+
+/* The (ghosted) level set that orc_compute_residual's interface faces take their normal from when
+ * RiemannNormal is LevelSet or Average (NULL: the mesh normal). orc_run sets Phi(n) itself. */
+void orc_set_levelset(orc_ctx *c, const double *phi);
+
+/* Direction of the interface Riemann problem at the d-face of cell (i,j,k) (between (i,j,k) - e_d,
+ * material nbid, and (i,j,k), material myid): GetNormalForBimaterialRiemann with
+ * CalculateGradPhiAtCellInterface (SpaceOperator.cpp:2966-3048) on the ghosted level set phi. */
+void orc_interface_normal(const orc_ctx *c, const double *phi, int d, int i, int j, int k, int myid, int nbid,
+                          double *dir);
 
 /* SpaceOperator::ComputeTimeStepSize (SpaceOperator.cpp:2202-2227). Returns 0 or -1 (c^2<0). */
 int orc_time_step(const orc_ctx *c, const double *V, const int *ID, double *dt);

@@ -545,6 +545,8 @@ static void update_states(orc_ctx *c, const int *IDn, const int *ID, double *V) 
 
 /* ------------------------------------------------------------ time step ---------------- */
 
+void orc_set_levelset(orc_ctx *c, const double *phi) { c->phi_n = phi; }
+
 int orc_advance(orc_ctx *c, double *V, const int *ID, double dt) {
   return orc_step_all(c, V, (int *)ID, NULL, dt, 0);
 }
@@ -563,12 +565,23 @@ static int ls_stage(orc_ctx *c, const double *Vres, double *phi_io, double *phi_
   return 0;
 }
 
+static int step_body(orc_ctx *c, double *V, int *ID, double *phi, double dt, int step, int ls);
+
 int orc_step_all(orc_ctx *c, double *V, int *ID, double *phi, double dt, int step) {
   long n = orc_ncells_ghosted(c);
-  const orc_config *g = &c->cfg;
-  int ls = g->ls_on && phi;
-  double *R = c->R, *Un = c->Un, *U1 = c->U1, *V1 = c->V1, *V2 = c->V2;
+  int ls = c->cfg.ls_on && phi;
   if (ls) memcpy(c->phi_tmp, phi, (size_t)n * sizeof(double));
+  const double *phi_saved = c->phi_n;
+  c->phi_n = ls ? phi : NULL;   /* every stage's interfaces use Phi(n) (TimeIntegrator.cpp:375, 581, 637) */
+  int rc = step_body(c, V, ID, phi, dt, step, ls);
+  c->phi_n = phi_saved;
+  return rc;
+}
+
+static int step_body(orc_ctx *c, double *V, int *ID, double *phi, double dt, int step, int ls) {
+  long n = orc_ncells_ghosted(c);
+  const orc_config *g = &c->cfg;
+  double *R = c->R, *Un = c->Un, *U1 = c->U1, *V1 = c->V1, *V2 = c->V2;
   c->rs_stamp = step;
   c->store_riemann = 1;
   int st = orc_compute_residual(c, V, ID, R);

@@ -30,6 +30,7 @@ struct orc_ctx {
   double *fd_l[3], *fd_r[3], *fd_r1y;   /* FD3 coefficients per index along each axis */
   double reinit_cfl_state;  /* LevelSetReinitializer::cfl (persists, divided by 1.5 on retries) */
   double *Dv[3], *Dh[3];    /* viscous (plus[5], minus[5]) and heat fluxes per face (oracle/diffusion.c) */
+  const double *phi_n;      /* level set at t_n for the interface normal (RiemannNormal LevelSet/Average) */
   long n_reinit_iters, n_unresolved;
 };
 

@@ -51,7 +51,7 @@ class Config(C.Structure):
                 ("riemann", RiemannCfg), ("nthreads", C.c_int),
                 ("ls_on", C.c_int), ("ls_matid", C.c_int), ("ls_bc", C.c_int * 6),
                 ("reinit_freq", C.c_int), ("reinit_maxits", C.c_int),
-                ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double)]
+                ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double), ("mp_normal", C.c_int)]
 
 
 def default_riemann():
@@ -89,7 +89,7 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
                 flux=FLUX_HLLC, limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True,
                 vanalbada_pow=True, inactive_id=-1, nthreads=0, roe_delta=0.2, gminmod_alpha=1.2,
                 slope_zero=False, mp_flux=0, mp_recon=0, ls_matid=None, ls_bc=(1, 1, 1, 1, 1, 1),
-                reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8, reinit_tol=2.0e-4):
+                reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8, reinit_tol=2.0e-4, mp_normal=1):
     c = Config()
     c.nx, c.ny, c.nz = nx, ny, nz
     c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
@@ -117,6 +117,7 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
         c.ls_bc[i] = ls_bc[i]
     c.reinit_freq, c.reinit_maxits = reinit_freq, reinit_maxits
     c.reinit_cfl, c.reinit_tol = reinit_cfl, reinit_tol
+    c.mp_normal = mp_normal
     return c
 
 
@@ -139,6 +140,8 @@ def lib():
         L.orc_apply_bc.argtypes = [P, D]
         L.orc_apply_bc_id.argtypes = [P, I]
         L.orc_compute_residual.argtypes = [P, D, I, D]
+        L.orc_set_levelset.argtypes = [P, D]
+        L.orc_interface_normal.argtypes = [P, D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D]
         L.orc_time_step.argtypes = [P, D, I, D]
         L.orc_advance.argtypes = [P, D, I, C.c_double]
         L.orc_run.argtypes = [P, D, I, D, C.c_double, C.c_int, D, I]
@@ -205,9 +208,15 @@ class Oracle:
         self.L.orc_numerical_flux(self.h, d, _dp(Vm), _dp(Vp), mid, _dp(F))
         return F
 
-    def residual(self, V, ID):
+    def residual(self, V, ID, phi=None):
+        """Residual of the ghosted state; phi (ghosted) gives interface faces their level-set normal."""
         R = np.zeros_like(V)
+        if phi is not None:
+            phi = np.ascontiguousarray(phi, dtype=np.float64)
+            self.L.orc_set_levelset(self.h, _dp(phi))
         st = self.L.orc_compute_residual(self.h, _dp(V), _ip(ID), _dp(R))
+        if phi is not None:
+            self.L.orc_set_levelset(self.h, None)
         return st, R
 
     def time_step(self, V, ID):
@@ -281,6 +290,13 @@ class Oracle:
         F, Vs = np.zeros(5), np.zeros(5)
         st = self.L.orc_wall_face_flux(self.h, axis, side, mode, _dp(nw), _dp(vw), _dp(V), mid, _dp(F), _dp(Vs))
         return st, F, Vs
+
+    def interface_normal(self, phi, d, i, j, k, myid, nbid):
+        """Interface Riemann direction at the d-face of (i,j,k) from the ghosted level set phi."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        out = np.zeros(3)
+        self.L.orc_interface_normal(self.h, _dp(phi), d, i, j, k, myid, nbid, _dp(out))
+        return out
 
     def p2c(self, V, mid=0):
         V = np.ascontiguousarray(V, dtype=np.float64)

@@ -1167,12 +1167,103 @@ int orc_riemann_onesided_solve(const orc_eos *mat, const orc_riemann_cfg *cfg, c
 void orc_riemann_begin_stage(orc_ctx *c) { (void)c; }
 void orc_riemann_end_stage(orc_ctx *c) { (void)c; }
 
-/* SpaceOperator.cpp:2486-2542 with the mesh normal (MultiPhase.RiemannNormal = Mesh). */
+/* ------------------------------------------------------------ interface normal ---------- */
+
+/* phi at (i,j,k): periodic axes wrapped first, then the ghosted array (its face ghosts hold the
+ * level-set boundary rule) */
+static double phi_get(const orc_ctx *c, const double *phi, int i, int j, int k) {
+  const int *b = c->cfg.ls_bc;
+  if (i < 0 && b[0] == 7) i += c->nx; else if (i >= c->nx && b[1] == 7) i -= c->nx;
+  if (j < 0 && b[2] == 7) j += c->ny; else if (j >= c->ny && b[3] == 7) j -= c->ny;
+  if (k < 0 && b[4] == 7) k += c->nz; else if (k >= c->nz && b[5] == 7) k -= c->nz;
+  return phi[ORC_IDX(c, i, j, k)];
+}
+
+#define XCO(c, d, q) ((c)->xc[d][(q) + 1])
+
+/* CentralDifferenceLocal (SpaceOperator.h:218-223) */
+static double cdiff_local(double p0, double p1, double p2, double x0, double x1, double x2) {
+  double c0 = -(x2 - x1) / ((x1 - x0) * (x2 - x0));
+  double c1 = 1.0 / (x1 - x0) - 1.0 / (x2 - x1);
+  double c2 = (x1 - x0) / ((x2 - x0) * (x2 - x1));
+  return c0 * p0 + c1 * p1 + c2 * p2;
+}
+
+/* SpaceOperator::CalculateGradientAtCellInterface (SpaceOperator.cpp:3052-3190), written out for
+ * the three face directions as there */
+static void grad_at_face(const orc_ctx *c, const double *phi, int d, int i, int j, int k, double *dir) {
+  int q[3] = {i, j, k}, m[3] = {i, j, k}, N[3] = {c->nx, c->ny, c->nz};
+  double h[3] = {c->dx, c->dy, c->dz};
+  m[d] -= 1;
+  dir[0] = dir[1] = dir[2] = 0.0;
+  dir[d] = (phi_get(c, phi, q[0], q[1], q[2]) - phi_get(c, phi, m[0], m[1], m[2])) / (XCO(c, d, q[d]) - XCO(c, d, m[d]));
+  int ta[2];
+  if (d == 0) { ta[0] = 1; ta[1] = 2; } else if (d == 1) { ta[0] = 0; ta[1] = 2; } else { ta[0] = 0; ta[1] = 1; }
+  double t1[2], t2[2];
+  for (int a = 0; a < 2; a++) {
+    int t = ta[a], qt = q[t];
+    int mp[3] = {m[0], m[1], m[2]}, mm[3] = {m[0], m[1], m[2]}, qp[3] = {q[0], q[1], q[2]}, qm[3] = {q[0], q[1], q[2]};
+    mp[t] += 1; mm[t] -= 1; qp[t] += 1; qm[t] -= 1;
+    double fm = phi_get(c, phi, m[0], m[1], m[2]), fq = phi_get(c, phi, q[0], q[1], q[2]);
+    if (qt < 0) {
+      t1[a] = (phi_get(c, phi, mp[0], mp[1], mp[2]) - fm) / (XCO(c, t, qt + 1) - XCO(c, t, qt));
+      t2[a] = (phi_get(c, phi, qp[0], qp[1], qp[2]) - fq) / (XCO(c, t, qt + 1) - XCO(c, t, qt));
+    } else if (qt >= N[t]) {
+      t1[a] = (fm - phi_get(c, phi, mm[0], mm[1], mm[2])) / (XCO(c, t, qt) - XCO(c, t, qt - 1));
+      t2[a] = (fq - phi_get(c, phi, qm[0], qm[1], qm[2])) / (XCO(c, t, qt) - XCO(c, t, qt - 1));
+    } else {
+      t1[a] = cdiff_local(phi_get(c, phi, mm[0], mm[1], mm[2]), fm, phi_get(c, phi, mp[0], mp[1], mp[2]),
+                          XCO(c, t, qt - 1), XCO(c, t, qt), XCO(c, t, qt + 1));
+      t2[a] = cdiff_local(phi_get(c, phi, qm[0], qm[1], qm[2]), fq, phi_get(c, phi, qp[0], qp[1], qp[2]),
+                          XCO(c, t, qt - 1), XCO(c, t, qt), XCO(c, t, qt + 1));
+    }
+  }
+  double x_half = XCO(c, d, q[d]) - 0.5 * h[d];
+  double dd = XCO(c, d, q[d]) - XCO(c, d, m[d]);
+  double c1 = (XCO(c, d, q[d]) - x_half) / dd;
+  double c2 = (x_half - XCO(c, d, m[d])) / dd;
+  dir[ta[0]] = c1 * t1[0] + c2 * t2[0];
+  dir[ta[1]] = c1 * t1[1] + c2 * t2[1];
+  double nrm = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  if (nrm != 0.0) {
+    double s = 1.0 / nrm;
+    dir[0] *= s; dir[1] *= s; dir[2] *= s;
+  }
+}
+
+/* SpaceOperator::GetNormalForBimaterialRiemann + CalculateGradPhiAtCellInterface
+ * (SpaceOperator.cpp:2966-3048) for one level set; myid: plus cell, nbid: minus cell */
+void orc_interface_normal(const orc_ctx *c, const double *phi, int d, int i, int j, int k, int myid, int nbid,
+                          double *dir) {
+  dir[0] = dir[1] = dir[2] = 0.0;
+  int mode = c->cfg.mp_normal;
+  if (mode == 1 || !phi) { dir[d] = 1.0; return; }
+  (void)myid;
+  grad_at_face(c, phi, d, i, j, k, dir);
+  if (nbid == 0) { dir[0] *= -1.0; dir[1] *= -1.0; dir[2] *= -1.0; }
+  if (sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[d] = 1.0;
+  if (dir[d] < 0.0) dir[d] *= -1.0;
+  if (mode == 2) {
+    dir[d] += 1.0;
+    double s = 1.0 / sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= s; dir[1] *= s; dir[2] *= s;
+  }
+}
+
+/* SpaceOperator.cpp:2486-2542: the direction of GetNormalForBimaterialRiemann, the solve, clip,
+ * store, fluxes. */
 int orc_interface_flux(orc_ctx *c, int dir, const double *vm, int idm, const double *vp, int idp, long lo, long hi,
                        double *lf1, double *lf2) {
   const orc_config *g = &c->cfg;
-  double n[3] = {0.0, 0.0, 0.0};
-  n[dir] = 1.0;
+  double n[3];
+  {
+    long r = hi;
+    int ii = (int)(r % c->gx) - 1;
+    r /= c->gx;
+    int jj = (int)(r % c->gy) - 1;
+    int kk = (int)(r / c->gy) - 1;
+    orc_interface_normal(c, c->phi_n, dir, ii, jj, kk, idp, idm, n);
+  }
   double Vmid[5], Vsm[5], Vsp[5];
   int midid = 0;
   int err = orc_riemann_solve(g->mat, &g->riemann, n, vm, idm, vp, idp, Vmid, &midid, Vsm, Vsp);

@@ -206,6 +206,14 @@ under Time { under Explicit { Type = RungeKutta3; } MaxIts = 12; }
     assert cfg.cfl == 0.4 and cfg.ts == M.TS_RK3
 
 
+def test_riemann_normal_keywords():
+    # MultiPhaseData::riemann_normal: LevelSet 0, Mesh 1 (the default), Average 2 (IoData.cpp:1445-1448)
+    assert read("")[0].mp_riemann_normal == 1
+    for kw, code in (("LevelSet", 0), ("Mesh", 1), ("Average", 2)):
+        cfg, _, _ = read(f"MultiPhase.RiemannNormal = {kw};")
+        assert cfg.mp_riemann_normal == code, kw
+
+
 def test_navier_stokes_flux_keywords():
     # SchemeData::flux: Roe 0, LocalLaxFriedrichs 1, HLLC 2, Godunov 3 (IoData.cpp:1170-1172)
     for kw, code in (("Roe", M.FLUX_ROE), ("LocalLaxFriedrichs", M.FLUX_LLF), ("HLLC", M.FLUX_HLLC),
@@ -267,7 +275,7 @@ def test_missing_mesh_and_materials():
     "Space.LevelSet[0].Bandwidth = 6;", "Space.LevelSet[0].Solver = FiniteVolume;",
     "Space.LevelSet[0].Reinitialization.FirstLayerTreatment = ConstrainedMethod1;",
     "Space.LevelSet[0].MaterialID = 0; Space.LevelSet[1].MaterialID = 0;",
-    "MultiPhase.RiemannNormal = LevelSet;", "MultiPhase.PhaseChange = Extrapolation;",
+    "MultiPhase.PhaseChange = Extrapolation;",
     "MultiPhase.LatentHeatTransfer = On;", "MultiPhase.LevelSetCorrectionFrequency = 5;",
     "EmbeddedBoundaryMethod.EmbeddedSurfaces.Surface[0].MeshFile = \"a.top\";", "Laser.SourcePower = 1;",
     "GeometricEntities.Spheroid[0].Length = 1;", "InitialCondition.UserDataFile = \"ic.txt\";",
