@@ -975,7 +975,6 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   c->ls = cfg->ls_matid >= 0;
   c->godunov = cfg->flux == M2C_FLUX_GODUNOV;
   int bad = 0;
-  if (c->iface && cfg->mp_recon != 0) bad = M2C_ERR_UNSUPPORTED;   // linear reconstruction at interfaces
   // RiemannNormal LevelSet / Average: the reference takes the gradient of the level set of the
   // face's non-zero material (CalculateGradPhiAtCellInterface asserts that it has one)
   if (cfg->mp_riemann_normal < 0 || cfg->mp_riemann_normal > 2) bad = M2C_ERR_ARG;
@@ -1282,6 +1281,7 @@ int m2c_set_surrogate(m2c_ctx *c, const m2c_mlp *net, int newton_max) {
   if (c->cfg.mp_flux != 0) return M2C_ERR_UNSUPPORTED;    // the network serves the numerical-flux form
   if (c->godunov) return M2C_ERR_UNSUPPORTED;             // ... of the approximate fluxes
   if (c->cfg.mp_riemann_normal != 1) return M2C_ERR_UNSUPPORTED;   // it solves along the mesh axes
+  if (c->cfg.mp_recon != 0) return M2C_ERR_UNSUPPORTED;            // ... between the cell states
   MlpWeights w;
   int dev = -1;
   if (m2c_mlp_get_weights(net, &w, &dev)) return M2C_ERR_ARG;

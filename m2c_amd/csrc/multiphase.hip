@@ -487,6 +487,42 @@ __global__ __launch_bounds__(256) void k_iface_feat(const double *S, const int8_
   }
 }
 
+// ---------------------------------------------------------------- reconstructed face states
+
+// The two reconstructed states of the DIR-face of cell (i,j,k) (between (i,j,k) - e_DIR, state vm1
+// of material idm1, and (i,j,k), state vp0 of material idp0), formed as the stage kernel forms them:
+// the slopes of the two cells from the four cells along the axis, the revert rule, and at a domain
+// face the boundary rule on the inner state.
+template <int DIR, int IN_CONS>
+__device__ void face_states(const double *S, const int8_t *ID, int i, int j, int k, const double *vm1, int idm1,
+                            const double *vp0, int idp0, double *fm, double *fp) {
+  const Grid &g = c_grid;
+  constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  const int gn = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+  const bool per = g.bc[2 * DIR] == BC_PERIODIC;
+  double vm2[5], vp1[5];
+  int idm2, idp1;
+  unsigned fl = 0;
+  fetch_cell<IN_CONS, -1>(S, ID, i - 2 * di, j - 2 * dj, k - 2 * dk, vm2, idm2, fl);
+  fetch_cell<IN_CONS, -1>(S, ID, i + di, j + dj, k + dk, vp1, idp1, fl);
+  const int gpl = DIR == 0 ? i : (DIR == 1 ? j : k);
+  double sgm[5], sgp[5];
+  if (gpl == 0 && !per) {
+    cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+    face_state<-1, false, double>(vp0, sgp, idp0, fp);
+    ghost_face<DIR, double>(g.bc[2 * DIR], fp, vm1, fm);
+  } else if (gpl == gn && !per) {
+    cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
+    face_state<-1, true, double>(vm1, sgm, idm1, fm);
+    ghost_face<DIR, double>(g.bc[2 * DIR + 1], fm, vp0, fp);
+  } else {
+    cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
+    cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+    face_state<-1, true, double>(vm1, sgm, idm1, fm);
+    face_state<-1, false, double>(vp0, sgp, idp0, fp);
+  }
+}
+
 // ---------------------------------------------------------------- interface Riemann pass
 
 template <int FLUX, int DIR>
@@ -555,6 +591,14 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
     fetch_cell<IN_CONS, -1>(S, ID, i - (dir == 0), j - (dir == 1), k - (dir == 2), vm, idm, fl);
     fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp, idp, fl);
     flags |= fl;
+    if (c_grid.mp_recon) {   // linear reconstruction at the interface: vr of the minus cell, vl of the plus cell
+      double fm[5], fp[5];
+      if (dir == 0) face_states<0, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp);
+      else if (dir == 1) face_states<1, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp);
+      else face_states<2, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp);
+#pragma unroll
+      for (int p = 0; p < 5; p++) { vm[p] = fm[p]; vp[p] = fp[p]; }
+    }
     double nrm[3];
     interface_normal(dir, i, j, k, idp, idm, phin, nrm);
     double Vmid[5], Vsm[5], Vsp[5];
@@ -1369,8 +1413,6 @@ __global__ __launch_bounds__(64) void k_godunov_faces(const double *S, const int
   S_.path3.base = scratch + (long long)3 * cap * T + tid; S_.path3.cap = cap; S_.path3.stride = T; S_.path3.n = 0;
   constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
   const int fx = g.nx + di, fy = g.ny + dj;
-  const int gn = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
-  const bool per = g.bc[2 * DIR] == BC_PERIODIC;
   unsigned flags = 0;
   for (long long f = tid; f < nf; f += T) {
     const int i = (int)(f % fx);
@@ -1383,26 +1425,8 @@ __global__ __launch_bounds__(64) void k_godunov_faces(const double *S, const int
     fetch_cell<IN_CONS, -1>(S, ID, i - di, j - dj, k - dk, vm1, idm1, fl);
     fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp0, idp0, fl);
     if (idm1 != idp0 || idp0 == g.inactive_id) continue;
-    double vm2[5], vp1[5];
-    int idm2, idp1;
-    fetch_cell<IN_CONS, -1>(S, ID, i - 2 * di, j - 2 * dj, k - 2 * dk, vm2, idm2, fl);
-    fetch_cell<IN_CONS, -1>(S, ID, i + di, j + dj, k + dk, vp1, idp1, fl);
-    const int gpl = DIR == 0 ? i : (DIR == 1 ? j : k);
-    double sgm[5], sgp[5], fm[5], fp[5];
-    if (gpl == 0 && !per) {
-      cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
-      face_state<-1, false, double>(vp0, sgp, idp0, fp);
-      ghost_face<DIR, double>(g.bc[2 * DIR], fp, vm1, fm);
-    } else if (gpl == gn && !per) {
-      cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
-      face_state<-1, true, double>(vm1, sgm, idm1, fm);
-      ghost_face<DIR, double>(g.bc[2 * DIR + 1], fm, vp0, fp);
-    } else {
-      cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
-      cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
-      face_state<-1, true, double>(vm1, sgm, idm1, fm);
-      face_state<-1, false, double>(vp0, sgp, idp0, fp);
-    }
+    double fm[5], fp[5];
+    face_states<DIR, IN_CONS>(S, ID, i, j, k, vm1, idm1, vp0, idp0, fm, fp);
     double F[5];
     S_.overflow = 0;
     if (godunov_flux<DIR>(S_, fm, fp, idp0, F) < 0) flags |= ST_BAD_STATE;
