@@ -127,3 +127,19 @@ def test_rejections():
     with pytest.raises(M.M2CError):
         s.wall_face_flux(0, 1, 3, [1.0, 0, 0], [0, 0, 0], [1.0, 0, 0, 0, 1.0], 0)
     s.close()
+
+
+def test_empty_batches():
+    # zero problems: every pointwise entry point returns at once with empty results
+    s, o = setup()
+    st, Vs, mid, Vsm = s.onesided_riemann(np.zeros((0, 3)), np.zeros((0, 5)), np.zeros(0, np.int32),
+                                          np.zeros((0, 3)))
+    assert st.shape == (0,) and Vs.shape == (0, 5)
+    st, F, Vst = s.wall_face_flux(np.zeros(0, np.int32), np.zeros(0, np.int32), 1, np.zeros((0, 3)),
+                                  np.zeros((0, 3)), np.zeros((0, 5)), np.zeros(0, np.int32))
+    assert F.shape == (0, 5)
+    st, Vs, mid, Vsm, Vsp = s.riemann(np.zeros(0, np.int32), np.zeros((0, 5)), np.zeros(0, np.int32),
+                                      np.zeros((0, 5)), np.zeros(0, np.int32))
+    assert Vs.shape == (0, 5)
+    assert s.flux(0, np.zeros((0, 5)), np.zeros((0, 5)), 0).shape == (0, 5)
+    s.close()
