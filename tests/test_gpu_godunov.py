@@ -166,3 +166,21 @@ def test_rejections():
         M.Solver(W.config_from(settings, flux=M.FLUX_GODUNOV, precision=M.PREC_MIXED))
     with pytest.raises(M.M2CError):
         M.Batch([W.config_from(settings, flux=M.FLUX_GODUNOV)])
+
+
+def test_with_viscous_and_heat_terms_bitwise():
+    # the Godunov face pass and the viscous / heat face pass both precede each stage
+    nx, ny, nz = 14, 9, 7
+    V0 = smooth_state(nx, ny, nz, seed=8, amp=0.05)
+    gas = M.material(M.EOS_SG, [1.4, 0.0], tlaw=(2.5, 0.0, 0.0, 0.0, 0.0, 0.0), viscosity=M.VISC_CONSTANT, mu=0.004,
+                     bulk=0.001, kappa=0.006)
+    settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, 1.0, 0.0, 0.6, 0.0, 0.4),
+                    bc=[M.BC_INLET, M.BC_OUTLET, M.BC_SLIPWALL, M.BC_STICKWALL, M.BC_SYMMETRY, M.BC_SLIPWALL],
+                    mats=[gas], inlet=(1.2, 0.3, 0.0, 0.0, 1.1), outlet=(0.9, 0.1, 0.0, 0.0, 0.95),
+                    flux=M.FLUX_GODUNOV, limiter=M.LIM_VANALBADA, ts=M.TS_RK3, cfl=0.4)
+    ID0 = np.zeros((nz, ny, nx), dtype=np.int32)
+    print("case: Godunov with viscous and heat terms, RK3, 5 steps", flush=True)
+    st, V, _, _ = gpu_run(settings, V0, ID0, maxsteps=5)
+    ost, t, n, Vo, _ = run_oracle(settings, V0, ID0, maxsteps=5, vanalbada_pow=False)
+    assert st.error == 0 and ost == 0 and n == st.steps == 5
+    assert np.array_equal(V, Vo), f"max diff {np.max(np.abs(V - Vo))}"
