@@ -101,6 +101,10 @@ typedef struct {
    * (their normalised sum). LevelSet and Average need the level set and interfaces only between
    * material 0 and the level set's material. */
   int mp_riemann_normal;
+  /* MultiPhase.ConstantReconstructionDepth (MultiPhaseData::conRec_depth): cells with
+   * |phi(t_n)| below it use constant reconstruction (SpaceOperator::TagNodesOutsideConRecDepth);
+   * 0 = none (the default) */
+  double mp_conrec_depth;
 } m2c_config;
 
 typedef struct {

@@ -49,7 +49,7 @@ class Config(C.Structure):
                 ("precision", C.c_int), ("device", C.c_int),
                 ("ls_matid", C.c_int), ("ls_bc", C.c_int * 6), ("reinit_freq", C.c_int),
                 ("reinit_maxits", C.c_int), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
-                ("mp_riemann_normal", C.c_int)]
+                ("mp_riemann_normal", C.c_int), ("mp_conrec_depth", C.c_double)]
 
 
 class DeckRun(C.Structure):
@@ -120,10 +120,10 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
                 limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True, inactive_id=-1, roe_delta=0.2,
                 gminmod_alpha=1.2, slope_zero=False, mp_flux=0, mp_recon=0, precision=PREC_FP64, device=0,
                 ls_matid=None, ls_bc=(1, 1, 1, 1, 1, 1), reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8,
-                reinit_tol=2.0e-4, mp_normal=1):
+                reinit_tol=2.0e-4, mp_normal=1, conrec_depth=0.0):
     """Solver options; the level-set ones follow LevelSetSchemeData / LevelSetReinitializationData
     (IoData.cpp:1205-1266): ls_matid=None disables the level set. mp_normal: MultiPhase.RiemannNormal
-    (1 Mesh, 0 LevelSet, 2 Average)."""
+    (1 Mesh, 0 LevelSet, 2 Average); conrec_depth: MultiPhase.ConstantReconstructionDepth."""
     c = Config()
     c.nx, c.ny, c.nz = nx, ny, nz
     c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
@@ -149,6 +149,7 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
     c.reinit_freq, c.reinit_maxits = reinit_freq, reinit_maxits
     c.reinit_cfl, c.reinit_tol = reinit_cfl, reinit_tol
     c.mp_riemann_normal = mp_normal
+    c.mp_conrec_depth = conrec_depth
     return c
 
 

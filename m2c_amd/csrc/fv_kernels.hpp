@@ -51,6 +51,7 @@ struct Grid {
   int K[3];
   int flux, limiter, rec_linear, slope_zero;
   double alpha, roe_delta;
+  double conrec_depth;         // |phi(t_n)| below this: constant reconstruction (0: none)
   int inactive_id, nmat, mp_flux, mp_recon;
   int has_interfaces;          // more than one active material
 };
@@ -92,6 +93,8 @@ struct StageArgs {
   // Godunov fluxes of the faces between two cells of one active material (FLUX_GODUNOV only):
   // G[d][p * n_d + f], faces indexed as Dv
   const double *G[3] = {nullptr, nullptr, nullptr};
+  // the level set at t_n when a constant-reconstruction depth is set (null otherwise)
+  const double *phin = nullptr;
 };
 
 // Each translation unit that launches kernels holds its own copy of these tables (internal
@@ -175,6 +178,23 @@ template <int DIM, typename TC> __device__ __forceinline__ TC limited_slope(TC d
   }
   default: return TC(0.5) * (dq0 + dq1);
   }
+}
+
+// SpaceOperator::TagNodesOutsideConRecDepth (SpaceOperator.cpp:2861-2920): the (possibly ghost)
+// cell (i,j,k), taken at its image as the stencil fetch takes it, reconstructs constantly when
+// |phi(t_n)| there is below the depth.
+__device__ __forceinline__ bool conrec_cell(const double *phin, int i, int j, int k) {
+  const Grid &g = M2C_G;
+  if (!phin) return false;
+  const int ii = clampi(image_idx(i, g.nx, (i < 0 ? g.bc[0] : g.bc[1]) == BC_PERIODIC), 0, g.nx - 1);
+  const int jj = clampi(image_idx(j, g.ny, (j < 0 ? g.bc[2] : g.bc[3]) == BC_PERIODIC), 0, g.ny - 1);
+  const int kk = clampi(image_idx(k, g.nz, (k < 0 ? g.bc[4] : g.bc[5]) == BC_PERIODIC), 0, g.nz - 1);
+  return __builtin_fabs(phin[((long long)kk * g.ny + jj) * g.nx + ii]) < g.conrec_depth;
+}
+
+template <typename TC> __device__ __forceinline__ void zero5(TC *s) {
+#pragma unroll
+  for (int p = 0; p < 5; p++) s[p] = TC(0);
 }
 
 // Slopes of one cell in direction DIM from its two neighbours.
@@ -384,6 +404,12 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     TC szm[5];
     cell_slopes<2, TC>(vm2, vm1, w0, idm2, idm1, id0, szm);
     cell_slopes<2, TC>(vm1, w0, w1, idm1, id0, id1, sz0);
+    if constexpr (EK < 0) {   // a level set means two materials: the generic-EOS variants only
+      if (a.phin) {
+        if (conrec_cell(a.phin, ci, cj, kbeg - 1)) zero5(szm);
+        if (conrec_cell(a.phin, ci, cj, kbeg)) zero5(sz0);
+      }
+    }
     TC fm[5], fp[5];
     nrev += face_state<EK, false, TC>(w0, sz0, id0, fp);
     if (kbeg == 0 && !zper) {
@@ -442,6 +468,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     }
     // ---- z slope of k+1 (registers)
     cell_slopes<2, TC>(w0, w1, w2, id0, id1, id2, sz1);
+    if constexpr (EK < 0) {
+      if (a.phin && conrec_cell(a.phin, ci, cj, k + 1)) zero5(sz1);
+    }
     __syncthreads();
 
     // ---- slopes in x and y, once per cell and direction
@@ -459,6 +488,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
         }
         cell_slopes<0, TC>(vl, vc, vr, sID[y + HALO][x - 1 + HALO], sID[y + HALO][x + HALO],
                            sID[y + HALO][x + 1 + HALO], sg);
+        if constexpr (EK < 0) {
+          if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
+        }
 #pragma unroll
         for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
       } else if (it < NSL) {
@@ -473,6 +505,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
         }
         cell_slopes<1, TC>(vl, vc, vr, sID[y - 1 + HALO][x + HALO], sID[y + HALO][x + HALO],
                            sID[y + 1 + HALO][x + HALO], sg);
+        if constexpr (EK < 0) {
+          if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
+        }
 #pragma unroll
         for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
       }

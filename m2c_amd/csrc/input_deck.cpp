@@ -649,9 +649,7 @@ struct Mapper {
     int lcf = -1;
     geti(mp, "LevelSetCorrectionFrequency", lcf);
     if (lcf > 0) fail(M2C_ERR_UNSUPPORTED, 0, "MultiPhase.LevelSetCorrectionFrequency is not provided");
-    double depth = 0.0;
-    getd(mp, "ConstantReconstructionDepth", depth);
-    if (depth > 0.0) fail(M2C_ERR_UNSUPPORTED, 0, "MultiPhase.ConstantReconstructionDepth is not provided");
+    getd(mp, "ConstantReconstructionDepth", c.mp_conrec_depth);
   }
 
   void riemann(const Node *er, m2c_riemann_params &r) {
@@ -999,6 +997,7 @@ int m2c_config_defaults(m2c_config *c) {
   c->reinit_cfl = 0.8;
   c->reinit_tol = 2.0e-4;
   c->mp_riemann_normal = 1;         // MultiPhaseData::riemann_normal = MESH (IoData.cpp:1408)
+  c->mp_conrec_depth = 0.0;         // MultiPhaseData::conRec_depth (IoData.cpp:1416)
   return M2C_OK;
 }
 

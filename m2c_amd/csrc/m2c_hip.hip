@@ -579,6 +579,11 @@ static int diffusion_pass(m2c_ctx *c, const double *S, int in_cons) {
   return M2C_OK;
 }
 
+// Phi(n) for the constant-reconstruction depth (MultiPhase.ConstantReconstructionDepth), else null
+static const double *conrec_phi(const m2c_ctx *c) {
+  return (c->ls && c->cfg.mp_conrec_depth > 0.0) ? c->phiA : nullptr;
+}
+
 static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, const double *Vn, const double *Us,
                         double *Out, double a, double b, double cc, int first, int ignore_active) {
   StageArgs s;
@@ -596,11 +601,12 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
     const bool prof = c->diff_ev && c->diff_ev_i + 2 <= c->diff_ev->size();
     if (prof) hipEventRecord((*c->diff_ev)[c->diff_ev_i++], c->stream);
     if (m2c_mp_launch_godunov_faces(c->stream, in_cons, S, c->ID, c->G, c->gnf, c->gscratch, c->gthreads, c->st,
-                                    c->clk, ignore_active))
+                                    c->clk, ignore_active, conrec_phi(c)))
       return M2C_ERR_HIP;
     if (prof) hipEventRecord((*c->diff_ev)[c->diff_ev_i++], c->stream);
     for (int d = 0; d < 3; d++) s.G[d] = c->G[d];
   }
+  s.phin = conrec_phi(c);
   s.no_cfl = c->ls ? 1 : 0;
   s.S = S; s.Vn = Vn; s.Us = Us; s.Out = Out; s.ID = c->ID;
   s.Fi = c->Fi; s.Nf = c->Nf;
@@ -938,6 +944,7 @@ int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   g.flux = cfg->flux; g.limiter = cfg->limiter; g.rec_linear = cfg->rec_linear;
   g.slope_zero = cfg->slope_zero_near_interface;
   g.alpha = cfg->gminmod_alpha; g.roe_delta = cfg->roe_delta;
+  g.conrec_depth = cfg->mp_conrec_depth;
   g.inactive_id = cfg->inactive_id; g.nmat = cfg->nmat;
   g.mp_flux = cfg->mp_flux; g.mp_recon = cfg->mp_recon;
   int nactive = 0;

@@ -495,7 +495,7 @@ __global__ __launch_bounds__(256) void k_iface_feat(const double *S, const int8_
 // face the boundary rule on the inner state.
 template <int DIR, int IN_CONS>
 __device__ void face_states(const double *S, const int8_t *ID, int i, int j, int k, const double *vm1, int idm1,
-                            const double *vp0, int idp0, double *fm, double *fp) {
+                            const double *vp0, int idp0, double *fm, double *fp, const double *crphi) {
   const Grid &g = c_grid;
   constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
   const int gn = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
@@ -507,17 +507,23 @@ __device__ void face_states(const double *S, const int8_t *ID, int i, int j, int
   fetch_cell<IN_CONS, -1>(S, ID, i + di, j + dj, k + dk, vp1, idp1, fl);
   const int gpl = DIR == 0 ? i : (DIR == 1 ? j : k);
   double sgm[5], sgp[5];
+  // crphi: the level set at t_n when a constant-reconstruction depth is set
+  const bool cm = conrec_cell(crphi, i - di, j - dj, k - dk), cp = conrec_cell(crphi, i, j, k);
   if (gpl == 0 && !per) {
     cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+    if (cp) zero5(sgp);
     face_state<-1, false, double>(vp0, sgp, idp0, fp);
     ghost_face<DIR, double>(g.bc[2 * DIR], fp, vm1, fm);
   } else if (gpl == gn && !per) {
     cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
+    if (cm) zero5(sgm);
     face_state<-1, true, double>(vm1, sgm, idm1, fm);
     ghost_face<DIR, double>(g.bc[2 * DIR + 1], fm, vp0, fp);
   } else {
     cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
     cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+    if (cm) zero5(sgm);
+    if (cp) zero5(sgp);
     face_state<-1, true, double>(vm1, sgm, idm1, fm);
     face_state<-1, false, double>(vp0, sgp, idp0, fp);
   }
@@ -593,9 +599,10 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
     flags |= fl;
     if (c_grid.mp_recon) {   // linear reconstruction at the interface: vr of the minus cell, vl of the plus cell
       double fm[5], fp[5];
-      if (dir == 0) face_states<0, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp);
-      else if (dir == 1) face_states<1, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp);
-      else face_states<2, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp);
+      const double *crphi = c_grid.conrec_depth > 0.0 ? phin : nullptr;
+      if (dir == 0) face_states<0, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, crphi);
+      else if (dir == 1) face_states<1, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, crphi);
+      else face_states<2, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, crphi);
 #pragma unroll
       for (int p = 0; p < 5; p++) { vm[p] = fm[p]; vp[p] = fp[p]; }
     }
@@ -1399,7 +1406,7 @@ __global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir,
 template <int DIR, int IN_CONS>
 __global__ __launch_bounds__(64) void k_godunov_faces(const double *S, const int8_t *ID, double *G, long long nf,
                                                        double *scratch, Status *st, const Clock *clk,
-                                                       int ignore_active) {
+                                                       int ignore_active, const double *crphi) {
   if (!ignore_active && !step_active(clk)) return;
   const Grid &g = c_grid;
   const int T = gridDim.x * blockDim.x;
@@ -1426,7 +1433,7 @@ __global__ __launch_bounds__(64) void k_godunov_faces(const double *S, const int
     fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp0, idp0, fl);
     if (idm1 != idp0 || idp0 == g.inactive_id) continue;
     double fm[5], fp[5];
-    face_states<DIR, IN_CONS>(S, ID, i, j, k, vm1, idm1, vp0, idp0, fm, fp);
+    face_states<DIR, IN_CONS>(S, ID, i, j, k, vm1, idm1, vp0, idp0, fm, fp, crphi);
     double F[5];
     S_.overflow = 0;
     if (godunov_flux<DIR>(S_, fm, fp, idp0, F) < 0) flags |= ST_BAD_STATE;
@@ -1610,10 +1617,11 @@ extern "C" int m2c_mp_launch_onesided_batch(hipStream_t s, int flux, int face, i
 
 extern "C" int m2c_mp_launch_godunov_faces(hipStream_t s, int in_cons, const double *S, const int8_t *ID,
                                            double *const *G, const long long *nf, double *scratch, int nthreads,
-                                           Status *st, const Clock *clk, int ignore_active) {
+                                           Status *st, const Clock *clk, int ignore_active, const double *crphi) {
   const dim3 gr((nthreads + 63) / 64), bl(64);
 #define M2C_GF(D, IC)                                                                                           \
-  hipLaunchKernelGGL((k_godunov_faces<D, IC>), gr, bl, 0, s, S, ID, G[D], nf[D], scratch, st, clk, ignore_active)
+  hipLaunchKernelGGL((k_godunov_faces<D, IC>), gr, bl, 0, s, S, ID, G[D], nf[D], scratch, st, clk, ignore_active, \
+                     crphi)
   if (in_cons) { M2C_GF(0, 1); M2C_GF(1, 1); M2C_GF(2, 1); }
   else { M2C_GF(0, 0); M2C_GF(1, 0); M2C_GF(2, 0); }
 #undef M2C_GF

@@ -67,6 +67,7 @@ inline m2c_config default_config(int nx, int ny, int nz, double x0, double x1, d
   c.reinit_cfl = 0.8;
   c.reinit_tol = 2.0e-4;
   c.mp_riemann_normal = 1;         // MultiPhaseData::riemann_normal = MESH (IoData.cpp:1408)
+  c.mp_conrec_depth = 0.0;         // MultiPhaseData::conRec_depth (IoData.cpp:1416)
   return c;
 }
 

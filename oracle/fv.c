@@ -208,7 +208,10 @@ static void recon_cell(orc_ctx *c, const double *V, const int *ID, int i, int j,
   double *out[6];
   for (int f = 0; f < 6; f++) out[f] = c->face[f] + 5 * me;
   int myid = ID[me];
-  if (g->rec_type == ORC_REC_CONSTANT || myid == g->inactive_id) {
+  /* SpaceOperator::TagNodesOutsideConRecDepth (SpaceOperator.cpp:2861-2920): |phi(t_n)| below the
+   * depth selects constant reconstruction (Reconstructor.cpp:434-447) */
+  const int conrec = g->conrec_depth > 0 && c->phi_n && fabs(c->phi_n[me]) < g->conrec_depth;
+  if (g->rec_type == ORC_REC_CONSTANT || myid == g->inactive_id || conrec) {
     for (int f = 0; f < 6; f++) memcpy(out[f], v, 5 * sizeof(double));
     return;
   }

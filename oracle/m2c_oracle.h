@@ -58,6 +58,7 @@ typedef struct {
   int reinit_freq, reinit_maxits;
   double reinit_cfl, reinit_tol;
   int mp_normal;            /* MultiPhase.RiemannNormal: 0 LevelSet, 1 Mesh, 2 Average */
+  double conrec_depth;      /* MultiPhase.ConstantReconstructionDepth (0: none) */
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;

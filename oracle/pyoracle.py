@@ -51,7 +51,8 @@ class Config(C.Structure):
                 ("riemann", RiemannCfg), ("nthreads", C.c_int),
                 ("ls_on", C.c_int), ("ls_matid", C.c_int), ("ls_bc", C.c_int * 6),
                 ("reinit_freq", C.c_int), ("reinit_maxits", C.c_int),
-                ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double), ("mp_normal", C.c_int)]
+                ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double), ("mp_normal", C.c_int),
+                ("conrec_depth", C.c_double)]
 
 
 def default_riemann():
@@ -89,7 +90,8 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
                 flux=FLUX_HLLC, limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True,
                 vanalbada_pow=True, inactive_id=-1, nthreads=0, roe_delta=0.2, gminmod_alpha=1.2,
                 slope_zero=False, mp_flux=0, mp_recon=0, ls_matid=None, ls_bc=(1, 1, 1, 1, 1, 1),
-                reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8, reinit_tol=2.0e-4, mp_normal=1):
+                reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8, reinit_tol=2.0e-4, mp_normal=1,
+                conrec_depth=0.0):
     c = Config()
     c.nx, c.ny, c.nz = nx, ny, nz
     c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
@@ -118,6 +120,7 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
     c.reinit_freq, c.reinit_maxits = reinit_freq, reinit_maxits
     c.reinit_cfl, c.reinit_tol = reinit_cfl, reinit_tol
     c.mp_normal = mp_normal
+    c.conrec_depth = conrec_depth
     return c
 
 

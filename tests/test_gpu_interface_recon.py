@@ -63,3 +63,23 @@ def test_learned_solver_rejected():
         s.set_surrogate(net)
     net.close()
     s.close()
+
+
+@pytest.mark.parametrize("mp_recon,flux", [(0, M.FLUX_HLLC), (1, M.FLUX_HLLC), (0, M.FLUX_GODUNOV)])
+def test_constant_reconstruction_depth_bitwise(mp_recon, flux):
+    # MultiPhase.ConstantReconstructionDepth: cells within 2.5 cells of the interface (|phi(t_n)|)
+    # reconstruct constantly, in the stage kernel, the interface pass and the Godunov pass alike
+    settings, V0, ID0, phi0 = W.shock_bubble(scale=0.0625, reinit_freq=3)
+    V0 = V0.copy()
+    nz, ny, nx = ID0.shape
+    x = (np.arange(nx) + 0.5) / nx
+    V0[..., 0] *= 1.0 + 0.05 * np.sin(2 * np.pi * x)
+    V0[..., 1] += 0.5 * np.cos(2 * np.pi * x)
+    h = settings["box"][1] / settings["nx"]
+    settings = dict(settings, conrec_depth=2.5 * h, mp_recon=mp_recon, flux=flux)
+    print(f"case: shock-bubble constant-reconstruction depth, interface recon {mp_recon}, flux {flux}", flush=True)
+    st, V, ID, phi = gpu_run(settings, V0, ID0, phi0, 6)
+    ost, t, n, Vo, _, IDo, phio = run_oracle(settings, V0, ID0, maxsteps=6, phi0=phi0, vanalbada_pow=False)
+    assert st.error == 0 and ost == 0 and n == st.steps
+    assert np.array_equal(ID, IDo) and np.array_equal(phi, phio)
+    assert np.array_equal(V, Vo), f"max diff {np.max(np.abs(V - Vo))}"

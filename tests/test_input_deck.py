@@ -214,6 +214,11 @@ def test_riemann_normal_keywords():
         assert cfg.mp_riemann_normal == code, kw
 
 
+def test_constant_reconstruction_depth():
+    assert read("")[0].mp_conrec_depth == 0.0
+    assert read("MultiPhase.ConstantReconstructionDepth = 0.25;")[0].mp_conrec_depth == 0.25
+
+
 def test_navier_stokes_flux_keywords():
     # SchemeData::flux: Roe 0, LocalLaxFriedrichs 1, HLLC 2, Godunov 3 (IoData.cpp:1170-1172)
     for kw, code in (("Roe", M.FLUX_ROE), ("LocalLaxFriedrichs", M.FLUX_LLF), ("HLLC", M.FLUX_HLLC),
