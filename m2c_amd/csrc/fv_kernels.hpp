@@ -519,16 +519,16 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     tile_face<FLUX, EK, 0, TC>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
     tile_face<FLUX, EK, 1, TC>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
     {
-      // the TY east x faces go to wave 0, the TX north y faces are spread over waves 1..3 so
-      // that every wave evaluates one extra flux
+      // the TY east x faces and the TX north y faces each go to one wave, two waves apart, and
+      // the pair rotates with the block: the two other waves of the block skip the extra flux
+      // and their SIMDs run the co-resident block's waves meanwhile
+      static_assert(NTHR == 256 && TX <= 64 && TY <= 64, "one wave per edge row");
       const int wave = tid >> 6, lane = tid & 63;
-      constexpr int YPER = (TX + (NTHR / 64 - 1) - 1) / (NTHR / 64 - 1);
-      if (wave == 0) {
-        if (lane < TY) tile_face<FLUX, EK, 0, TC>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
-      } else {
-        const int x = (wave - 1) * YPER + lane;
-        if (lane < YPER && x < TX) tile_face<FLUX, EK, 1, TC>(sV, sID, sS, sFy, x, TY, TY * TX + x, i0, j0, nrev);
-      }
+      const int rot = (bx + by + bz) & 3;
+      if (wave == rot && lane < TY)
+        tile_face<FLUX, EK, 0, TC>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
+      if (wave == ((rot + 2) & 3) && lane < TX)
+        tile_face<FLUX, EK, 1, TC>(sV, sID, sS, sFy, lane, TY, TY * TX + lane, i0, j0, nrev);
     }
 
     // ---- z face k+1/2 (registers)
