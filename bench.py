@@ -38,6 +38,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# FP64 flops the fused stage kernel issues per cell and stage on the 256^3 periodic HLLC workload:
+# 64 x SQ_INSTS_VALU_FLOPS_FP64 per launch / cells, from a counter pass over the three RK3 stage
+# variants (full-wave count: masked lanes included)
+ISSUED_FP64_FLOP_PER_CELL_STAGE = 2640.0
 # algorithmic bytes per cell-update, FP64 single material RK3: stage 1 reads V_n and writes U_1
 # (80 B), stages 2 and 3 read U_s and V_n and write their output (120 B each)
 BYTES_PER_CELL_UPDATE = 320.0
@@ -344,6 +349,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "m2c::stage_kernel (fused MUSCL+" + ("Godunov-flux read" if god else "HLLC") + "+RK stage)",
             "avg_launch_ms": ms_stage}
+    if traffic is not None:
+        # the bound the kernel actually meets: FP64 issue and latency, against the vector FP64 peak
+        tf = ISSUED_FP64_FLOP_PER_CELL_STAGE * cells / (ms_stage * 1e-3) / 1e12
+        roof["fp64_valu"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": tf / FP64_PEAK_TFLOPS,
+                             "flop_per_cell_stage": ISSUED_FP64_FLOP_PER_CELL_STAGE,
+                             "counted": "issued FP64 flops (64 x SQ_INSTS_VALU_FLOPS_FP64), masked lanes included"}
     line = {"metric": "cell-updates/s on 1 GPU: plain FP64 vs mixed form at L1 err<=1e-5; % HBM peak",
             "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
