@@ -197,7 +197,8 @@ int m2c_wall_face_flux_batch(m2c_ctx *ctx, int n, const int *axis, const int *si
 
 /* ---- many independent instances (batched) ------------------------------------------------
  * A batch holds n single-material instances (each its own grid, boundaries, state and
- * CFL-adaptive time step; one material table, flux, precision and RK2/RK3 scheme for all). Every
+ * CFL-adaptive time step; one material table, flux, precision and RK2/RK3 scheme for all; the
+ * flux HLLC, or Roe or LLF in FP64, else M2C_ERR_UNSUPPORTED). Every
  * RK stage of every instance runs in one persistent launch draining per-XCD tile queues; a time
  * step is captured once in a hipGraph and replayed. An instance evolves bit for bit as it would
  * in its own context (no counterpart in the reference, which runs one problem per process). */

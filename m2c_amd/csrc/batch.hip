@@ -186,7 +186,7 @@ extern "C" int m2c_fill_grid(const m2c_config *cfg, Grid *g, DevEos *eos, int *v
 struct m2c_batch {
   std::vector<m2c_config> cfg;
   std::vector<BatchInst> hinst;
-  int n = 0, ts = M2C_TS_RK2, variant = 0, nstage = 2;
+  int n = 0, ts = M2C_TS_RK2, variant = 0, nstage = 2, flux = M2C_FLUX_HLLC;
   DevEos eos[8];
   hipStream_t stream = nullptr;
   long long cells = 0;
@@ -256,9 +256,12 @@ static void launch_stage_t(m2c_batch *b, int in_cons, int mode, int stage) {
 }
 
 static void launch_stage(m2c_batch *b, int in_cons, int mode, int stage) {
+  // the single-instance context's variants: Roe and LLF run the generic-EOS kernels
   switch (b->variant) {
   case 0: launch_stage_t<FLUX_HLLC, 0, double>(b, in_cons, mode, stage); break;
   case 1: launch_stage_t<FLUX_HLLC, -1, double>(b, in_cons, mode, stage); break;
+  case 2: launch_stage_t<FLUX_ROE, -1, double>(b, in_cons, mode, stage); break;
+  case 3: launch_stage_t<FLUX_LLF, -1, double>(b, in_cons, mode, stage); break;
   case 4: launch_stage_t<FLUX_HLLC, 0, float>(b, in_cons, mode, stage); break;
   default: launch_stage_t<FLUX_HLLC, -1, float>(b, in_cons, mode, stage); break;
   }
@@ -289,13 +292,14 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
   b->n = n;
   b->cfg.assign(cfgs, cfgs + n);
   b->ts = cfgs[0].ts;
+  b->flux = cfgs[0].flux;
   if (b->ts != M2C_TS_RK2 && b->ts != M2C_TS_RK3) { delete b; return M2C_ERR_UNSUPPORTED; }
   b->nstage = b->ts == M2C_TS_RK2 ? 2 : 3;
   b->hinst.resize(n);
   for (int i = 0; i < n; i++) {
     const m2c_config &c = cfgs[i];
     // one kernel variant and one material table for the whole batch
-    if (c.ts != b->ts || c.flux != M2C_FLUX_HLLC || c.precision != cfgs[0].precision || c.nmat != cfgs[0].nmat ||
+    if (c.ts != b->ts || c.flux != b->flux || c.precision != cfgs[0].precision || c.nmat != cfgs[0].nmat ||
         c.device != cfgs[0].device || memcmp(c.mat, cfgs[0].mat, sizeof(c.mat)) != 0 || c.ls_matid >= 0) {
       delete b;
       return M2C_ERR_UNSUPPORTED;
@@ -315,7 +319,8 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
     b->hinst[i].cell0 = b->cells;
     b->cells += b->hinst[i].grid.N;
   }
-  if (b->variant != 0 && b->variant != 1 && b->variant != 4 && b->variant != 5) { delete b; return M2C_ERR_UNSUPPORTED; }
+  // HLLC (0, 1), Roe (2) and LLF (3) in FP64, HLLC in the mixed form (4, 5)
+  if (b->variant < 0 || b->variant > 5) { delete b; return M2C_ERR_UNSUPPORTED; }
   BHIP(hipSetDevice(cfgs[0].device));
   BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   size_t bytes = sizeof(double) * 5 * (size_t)b->cells;

@@ -1,6 +1,7 @@
 """Batched instances (one persistent launch per RK stage over per-XCD tile queues, a hipGraph per
 step, autotuned launch shape): every instance must evolve bit for bit as it does in its own
-context, with and without the graph, before and after autotuning, in FP64 and mixed precision."""
+context, with and without the graph, before and after autotuning, in FP64 and mixed precision, and
+with the Roe and LLF fluxes in FP64."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,25 @@ def test_batch_matches_single_instances(prec, ts):
         bs = b.status(i)
         assert bs.error == 0 and bs.steps == st.steps == 9
         assert bs.t == st.t
+        assert np.array_equal(Vb, V), i
+    b.close()
+
+
+@pytest.mark.parametrize("flux", [M.FLUX_ROE, M.FLUX_LLF])
+def test_batch_roe_and_llf_match_single_instances(flux):
+    insts = W.batched_riemann(n=4, lo=6, hi=18, seed=5)
+    cfgs = [W.config_from(s, flux=flux, ts=M.TS_RK3) for s, _, _ in insts]
+    print(f"batch of {len(insts)} (flux {flux})", flush=True)
+    b = M.Batch(cfgs)
+    for i, (s, V0, ID0) in enumerate(insts):
+        b.upload(i, V0, ID0)
+    b.run(3, graph=False)
+    b.run(3, graph=True)
+    for i, (s, V0, ID0) in enumerate(insts):
+        st, V = solo(s, V0, ID0, 6, flux=flux, ts=M.TS_RK3)
+        Vb, _ = b.download(i)
+        bs = b.status(i)
+        assert bs.error == 0 and bs.steps == st.steps == 6 and bs.t == st.t
         assert np.array_equal(Vb, V), i
     b.close()
 
