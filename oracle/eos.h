@@ -14,9 +14,12 @@
  *   dummy (inactive)        VarFcnDummy.h:28-55
  *   base-class formulas     VarFcnBase.h:169-197 (C2P/P2C), :215-219 (c^2), :250-254 (H),
  *                           :113-142 (generic CheckState), :259-294 (clipping)
- * Parity pins: closed forms (tests/test_oracle_eos.py): p(rho,e(rho,p)) round trip, c^2 of an
- * ideal gas = gamma p / rho, NASG with b=q=0 equals SG, MG at eta=0 is linear in e,
- * JWL with A1=A2=0 is an ideal gas with gamma-1 = omega.
+ * Parity pins (tests/test_oracle_eos.py, tests/test_oracle_fv.py): NASG with b=q=0 equals SG, JWL
+ * with A1=A2=0 is an ideal gas with gamma-1 = omega, MG at rho=rho0 is linear in e, MG on its
+ * principal Hugoniot gives rho0 Us up, JWL against its textbook relative-volume form, the NASG sound
+ * speed of Le Metayer & Saurel, dp/drho|e, Gamma and c^2 against central differences of p(rho,e)
+ * (c^2 along the isentrope de = p/rho^2 drho), p(rho,e(rho,p)) = p, SG c^2 = gamma (p+p_inf)/rho,
+ * CheckState on the edges of the valid region.
  */
 #ifndef M2C_ORACLE_EOS_H
 #define M2C_ORACLE_EOS_H

@@ -538,3 +538,16 @@ double orc_eos_c2(const orc_ctx *c, int id, double rho, double p) {
   const orc_eos *m = &c->cfg.mat[id];
   return orc_c2(m, rho, orc_internal_energy(m, rho, p));
 }
+
+/* Pointwise EOS of material id at (rho, e) and (rho, p): out[0] p(rho,e), out[1] e(rho,p),
+ * out[2] dp/drho|e (rho,e), out[3] Gamma(rho,e), out[4] c^2(rho,e), out[5] CheckState(rho,p)
+ * (VarFcnBase.h:62-164 through the per-EOS closed forms of eos.h). */
+void orc_eos_point(const orc_ctx *c, int id, double rho, double e, double p, double *out) {
+  const orc_eos *m = &c->cfg.mat[id];
+  out[0] = orc_pressure(m, rho, e);
+  out[1] = orc_internal_energy(m, rho, p);
+  out[2] = orc_dpdrho(m, rho, e);
+  out[3] = orc_big_gamma(m, rho, e);
+  out[4] = orc_c2(m, rho, e);
+  out[5] = (double)orc_check_state_rp(m, rho, p);
+}

@@ -136,6 +136,23 @@ void orc_eos_p2c(const orc_ctx *c, int id, const double *V, double *U);
 void orc_eos_c2p(const orc_ctx *c, int id, const double *U, double *V);
 double orc_eos_c2(const orc_ctx *c, int id, double rho, double p);
 
+/* Pointwise EOS of material id: out[0] p(rho,e), out[1] e(rho,p), out[2] dp/drho at constant e,
+ * out[3] Gamma = (1/rho) dp/de, out[4] c^2(rho,e), out[5] CheckState(rho,p) (1 = invalid). */
+void orc_eos_point(const orc_ctx *c, int id, double rho, double e, double p, double *out);
+
+/* The interface Riemann solutions of V as the first stage of time step `stamp` stores them
+ * (SpaceOperator.cpp:2518-2523); returns the residual's status. */
+int orc_store_riemann(orc_ctx *c, const double *V, const int *ID, int stamp);
+/* UpdateStateVariablesByRiemannSolutions + FixUnresolvedNodes (MultiPhaseOperator.cpp:857-911,
+ * 1156-1320) on the ghosted V from the stored solutions; IDn before, ID after the interface moved.
+ * Returns the number of unresolved cells. */
+long orc_update_states(orc_ctx *c, const int *IDn, const int *ID, double *V);
+
+/* Multiphase counters since creation: reinitialisation iterations (retries included), unresolved
+ * cells of the state update, and the reinitialisation CFL number in force (divided by 1.5 on each
+ * retry, LevelSetReinitializer.cpp:155-168). */
+void orc_mp_counters(const orc_ctx *c, long *reinit_iters, long *unresolved, double *reinit_cfl);
+
 /* Deterministic log/exp used by the Riemann solver's step control (oracle/riemann.c). */
 double orc_det_log(double x);
 double orc_det_exp(double y);

@@ -543,6 +543,34 @@ static void update_states(orc_ctx *c, const int *IDn, const int *ID, double *V) 
   free(un);
 }
 
+/* The interface Riemann solutions of V stored as the first stage of time step `stamp` stores them
+ * (SpaceOperator.cpp:2518-2523): the residual of V is computed with the store on. Returns the
+ * residual's status. */
+int orc_store_riemann(orc_ctx *c, const double *V, const int *ID, int stamp) {
+  c->rs_stamp = stamp;
+  c->store_riemann = 1;
+  int st = orc_compute_residual(c, V, ID, c->R);
+  c->store_riemann = 0;
+  return st;
+}
+
+/* UpdateStateVariablesByRiemannSolutions + FixUnresolvedNodes (MultiPhaseOperator.cpp:857-911,
+ * 1156-1320) from the solutions of the last orc_store_riemann call, for the material IDs IDn (before)
+ * and ID (after). Returns the number of unresolved cells. */
+long orc_update_states(orc_ctx *c, const int *IDn, const int *ID, double *V) {
+  long before = c->n_unresolved;
+  update_states(c, IDn, ID, V);
+  return c->n_unresolved - before;
+}
+
+/* Multiphase counters since creation: reinitialisation pseudo-time iterations (retries included),
+ * unresolved cells of the state update, and the reinitialisation CFL number now in force. */
+void orc_mp_counters(const orc_ctx *c, long *reinit_iters, long *unresolved, double *reinit_cfl) {
+  *reinit_iters = c->n_reinit_iters;
+  *unresolved = c->n_unresolved;
+  *reinit_cfl = c->reinit_cfl_state;
+}
+
 /* ------------------------------------------------------------ time step ---------------- */
 
 void orc_set_levelset(orc_ctx *c, const double *phi) { c->phi_n = phi; }
@@ -661,19 +689,24 @@ static int step_body(orc_ctx *c, double *V, int *ID, double *phi, double dt, int
 }
 
 int orc_run(orc_ctx *c, double *V, int *ID, double *phi, double tmax, int maxsteps, double *t_out, int *steps_out) {
-  double t = 0.0;
-  int step = 0;
-  while (t < tmax && step < maxsteps) {
-    step++;
+  /* a later call continues the time and the step numbering (the reinitialisation schedule and the
+   * Riemann-solution stamps count from the first step of the first call) */
+  double t = c->t_done;
+  int n = 0, rc = 0;
+  while (t < tmax && n < maxsteps) {
+    n++;
+    const int step = c->steps_done + n;
     double dt;
-    if (orc_time_step(c, V, ID, &dt) < 0) { *t_out = t; *steps_out = step; return -1; }
+    if (orc_time_step(c, V, ID, &dt) < 0) { rc = -1; break; }
     if (t + dt > tmax) dt = tmax - t;
     t += dt;
-    if (orc_step_all(c, V, ID, phi, dt, step) < 0) { *t_out = t; *steps_out = step; return -1; }
+    if (orc_step_all(c, V, ID, phi, dt, step) < 0) { rc = -1; break; }
   }
+  c->steps_done += n;
+  c->t_done = t;
   *t_out = t;
-  *steps_out = step;
-  return 0;
+  *steps_out = n;
+  return rc;
 }
 
 int orc_riemann(const orc_ctx *c, const double *dir, const double *Vm, int idm, const double *Vp, int idp,

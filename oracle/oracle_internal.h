@@ -32,6 +32,8 @@ struct orc_ctx {
   double *Dv[3], *Dh[3];    /* viscous (plus[5], minus[5]) and heat fluxes per face (oracle/diffusion.c) */
   const double *phi_n;      /* level set at t_n for the interface normal (RiemannNormal LevelSet/Average) */
   long n_reinit_iters, n_unresolved;
+  int steps_done;           /* time steps of earlier orc_run calls: the numbering continues */
+  double t_done;            /* and their time */
 };
 
 #define ORC_IDX(c, i, j, k) ((((long)(k) + 1) * (c)->gy + ((j) + 1)) * (long)(c)->gx + ((i) + 1))

@@ -160,6 +160,11 @@ def lib():
         L.orc_eos_c2p.argtypes = [P, C.c_int, D, D]
         L.orc_eos_c2.argtypes = [P, C.c_int, C.c_double, C.c_double]
         L.orc_eos_c2.restype = C.c_double
+        L.orc_eos_point.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_double, D]
+        L.orc_store_riemann.argtypes = [P, D, I, C.c_int]
+        L.orc_update_states.argtypes = [P, I, I, D]
+        L.orc_update_states.restype = C.c_long
+        L.orc_mp_counters.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long), D]
         for f in ("orc_det_log", "orc_det_exp", "orc_pow_quarter", "orc_pow_fifth"):
             getattr(L, f).argtypes = [C.c_double]
             getattr(L, f).restype = C.c_double
@@ -315,6 +320,27 @@ class Oracle:
 
     def c2(self, rho, p, mid=0):
         return self.L.orc_eos_c2(self.h, mid, rho, p)
+
+    def eos_point(self, rho, e, p, mid=0):
+        """(p(rho,e), e(rho,p), dp/drho|e, Gamma, c^2(rho,e), CheckState(rho,p)) of material mid."""
+        out = np.zeros(6)
+        self.L.orc_eos_point(self.h, mid, rho, e, p, _dp(out))
+        return out
+
+    def store_riemann(self, V, ID, stamp=1):
+        """Store the interface Riemann solutions of the ghosted V as a step's first stage does."""
+        return self.L.orc_store_riemann(self.h, _dp(V), _ip(ID), stamp)
+
+    def update_states(self, IDn, ID, V):
+        """State update after the interface moved (IDn before, ID after); V in place. Returns the
+        number of unresolved cells."""
+        return self.L.orc_update_states(self.h, _ip(IDn), _ip(ID), _dp(V))
+
+    def mp_counters(self):
+        """(reinitialisation iterations, unresolved cells, reinitialisation CFL in force)."""
+        a, b, c = C.c_long(), C.c_long(), C.c_double()
+        self.L.orc_mp_counters(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
 
     def counters(self):
         a, b, c = C.c_long(), C.c_long(), C.c_long()
