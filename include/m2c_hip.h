@@ -165,13 +165,41 @@ int m2c_flux_batch(m2c_ctx *ctx, int dir, int n, const double *Vm, const double 
  * temperature law: JWL, dummy). Used by the outputs. */
 int m2c_eos_derived(m2c_ctx *ctx, int n, const double *rho, const double *p, const int32_t *id, double *e, double *T);
 
-/* ExactRiemannSolverBase::ComputeRiemannSolution (ExactRiemannSolverBase.h:35-39) for n
- * independent problems along the unit normal of axis dir[t] (0 x, 1 y, 2 z), evaluated by the device
- * solver of the interface faces. Vm, Vp, Vs, Vsm, Vsp: n*5 doubles; idm, idp, id, status: n ints.
- * status[t]: 0 success, 1 solver failure (counted as a Riemann error by the reference),
- * -1 invalid input state (the reference exits), 2 rarefaction path longer than the device cache. */
-int m2c_riemann_batch(m2c_ctx *ctx, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
+/* ExactRiemannSolverBase::ComputeRiemannSolution(double *dir, double *Vm, int idm, double *Vp, int idp,
+ * double *Vs, int &id, double *Vsm, double *Vsp) (ExactRiemannSolverBase.h:35-39) for n independent
+ * problems, evaluated by the device solver of the interface faces. dir: n*3 doubles, the unit normal
+ * of problem t at dir[3t..3t+2] (the mesh axis, or a level-set normal as
+ * SpaceOperator::GetNormalForBimaterialRiemann gives it); Vm, Vp, Vs, Vsm, Vsp: n*5 doubles; idm, idp,
+ * id, status: n ints. status[t]: 0 success, 1 solver failure (counted as a Riemann error by the
+ * reference), -1 invalid input state or material index (the reference exits), 2 rarefaction path
+ * longer than the device cache. */
+int m2c_riemann_batch(m2c_ctx *ctx, int n, const double *dir, const double *Vm, const int *idm, const double *Vp,
                       const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status);
+
+/* VarFcnBase's pointwise set (VarFcnBase.h:62-294) for n points, evaluated on the device with the
+ * context's materials: point t uses material id[t]. The operation fixes the strides of in / out:
+ *   (rho, e) -> 1: PRESSURE (GetPressure), DPDRHO (GetDpdrho), BIG_GAMMA (GetBigGamma),
+ *                  SOUND_SPEED_SQUARE (ComputeSoundSpeedSquare), SOUND_SPEED (ComputeSoundSpeed);
+ *   (rho, p) -> 1: INTERNAL_ENERGY (GetInternalEnergyPerUnitMass), ENTHALPY (ComputeEnthalpyPerUnitMass);
+ *   V[5] -> 1:     TOTAL_ENTHALPY (ComputeTotalEnthalpyPerUnitMass), MACH (ComputeMachNumber),
+ *                  CHECK_STATE (CheckState(V): 1 invalid, 0 valid);
+ *   U[5] -> V[5]:  C2P (ConservativeToPrimitive); V[5] -> U[5]: P2C (PrimitiveToConservative);
+ *   V[5] -> V[5]:  CLIP (ClipDensityAndPressure).
+ * flag (n ints, may be NULL): CHECK_STATE's result, CLIP's "clipped", and for SOUND_SPEED and MACH 1
+ * where c^2 <= 0 (resp. < 0) makes the reference exit (the output is NaN there). */
+enum m2c_varfcn_op {
+  M2C_VF_PRESSURE = 0, M2C_VF_INTERNAL_ENERGY = 1, M2C_VF_DPDRHO = 2, M2C_VF_BIG_GAMMA = 3,
+  M2C_VF_SOUND_SPEED_SQUARE = 4, M2C_VF_SOUND_SPEED = 5, M2C_VF_ENTHALPY = 6, M2C_VF_TOTAL_ENTHALPY = 7,
+  M2C_VF_MACH = 8, M2C_VF_C2P = 9, M2C_VF_P2C = 10, M2C_VF_CHECK_STATE = 11, M2C_VF_CLIP = 12
+};
+int m2c_varfcn_batch(m2c_ctx *ctx, int op, int n, const int32_t *id, const double *in, double *out, int32_t *flag);
+
+/* FluxFcnBase::ComputeNumericalFluxAtMaterialInterface(int dir, double *Vm, int idm, double *Vp, int idp,
+ * double *F) (FluxFcnBase.h:72-76), which the reference provides for the local Lax-Friedrichs flux
+ * (FluxFcnLLF.h:112-185) and SpaceOperator calls between two materials under MultiPhase.Flux = LLF
+ * (SpaceOperator.cpp:2492-2496), for n face pairs along axis dir: Vm, Vp, F n*5 doubles. */
+int m2c_interface_flux_batch(m2c_ctx *ctx, int dir, int n, const double *Vm, const int32_t *idm, const double *Vp,
+                             const int32_t *idp, double *F);
 
 /* ExactRiemannSolverBase::ComputeOneSidedRiemannSolution (ExactRiemannSolverBase.h:45-49) for n
  * independent problems: the fluid state Vm[t] (material idl[t]) against a wall moving with velocity

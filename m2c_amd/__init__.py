@@ -18,6 +18,9 @@ FLUX_ROE, FLUX_LLF, FLUX_HLLC, FLUX_GODUNOV = 0, 1, 2, 3
 LIM_NONE, LIM_GENMINMOD, LIM_VANALBADA = 0, 1, 2
 TS_FE, TS_RK2, TS_RK3 = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+# VarFcnBase pointwise operations of m2c_varfcn_batch (m2c_varfcn_op)
+(VF_PRESSURE, VF_INTERNAL_ENERGY, VF_DPDRHO, VF_BIG_GAMMA, VF_SOUND_SPEED_SQUARE, VF_SOUND_SPEED, VF_ENTHALPY,
+ VF_TOTAL_ENTHALPY, VF_MACH, VF_C2P, VF_P2C, VF_CHECK_STATE, VF_CLIP) = range(13)
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 DBL_MAX = 1.7976931348623157e308
 
@@ -181,7 +184,10 @@ def lib():
         L.m2c_device_state.argtypes = [P, C.POINTER(D)]
         L.m2c_stream_copy.argtypes = [P, C.c_longlong, C.c_int, D]
         I = C.POINTER(C.c_int)
-        L.m2c_riemann_batch.argtypes = [P, C.c_int, I, D, I, D, I, D, I, D, D, I]
+        L.m2c_riemann_batch.argtypes = [P, C.c_int, D, D, I, D, I, D, I, D, D, I]
+        L.m2c_varfcn_batch.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_int32), D, D, C.POINTER(C.c_int32)]
+        L.m2c_interface_flux_batch.argtypes = [P, C.c_int, C.c_int, D, C.POINTER(C.c_int32), D,
+                                               C.POINTER(C.c_int32), D]
         L.m2c_onesided_riemann_batch.argtypes = [P, C.c_int, D, D, I, D, D, I, D, I]
         L.m2c_wall_face_flux_batch.argtypes = [P, C.c_int, I, I, C.c_int, D, D, D, I, D, D, I]
         L.m2c_upload_levelset.argtypes = [P, D]
@@ -230,6 +236,17 @@ class M2CError(RuntimeError):
 def _chk(rc, what):
     if rc != OK:
         raise M2CError(f"{what} failed with status {rc}")
+
+
+def axis_normals(dirs, n):
+    """(n, 3) unit normals from mesh axes (n ints) or from normals already given as (n, 3)."""
+    a = np.asarray(dirs)
+    if a.ndim >= 1 and a.shape[-1] == 3 and a.dtype.kind == "f":
+        return np.ascontiguousarray(np.broadcast_to(a.astype(np.float64), (n, 3)))
+    ax = np.broadcast_to(a.astype(np.int64), (n,))
+    if ((ax < 0) | (ax > 2)).any():
+        raise ValueError("mesh axis outside 0..2")
+    return np.ascontiguousarray(np.eye(3)[ax])
 
 
 def _dp(a):
@@ -285,18 +302,45 @@ class Solver:
         return phi
 
     def riemann(self, dirs, Vm, idm, Vp, idp):
-        """Exact two-material Riemann solutions of n problems: (status, Vs, id, Vsm, Vsp)."""
+        """Exact two-material Riemann solutions of n problems: (status, Vs, id, Vsm, Vsp). dirs: unit
+        normals (n x 3), or mesh axes (n ints: 0 x, 1 y, 2 z) turned into unit normals here."""
         Vm = np.ascontiguousarray(np.atleast_2d(Vm), dtype=np.float64)
         Vp = np.ascontiguousarray(np.atleast_2d(Vp), dtype=np.float64)
         n = Vm.shape[0]
         ip = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
-        d, im, iq = ip(dirs), ip(idm), ip(idp)
+        dd = axis_normals(dirs, n)
+        im, iq = ip(idm), ip(idp)
         Vs, Vsm, Vsp = np.empty_like(Vm), np.empty_like(Vm), np.empty_like(Vm)
         mid, st = np.empty(n, np.int32), np.empty(n, np.int32)
         cp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
-        _chk(self.L.m2c_riemann_batch(self.h, n, cp(d), _dp(Vm), cp(im), _dp(Vp), cp(iq), _dp(Vs), cp(mid),
+        _chk(self.L.m2c_riemann_batch(self.h, n, _dp(dd), _dp(Vm), cp(im), _dp(Vp), cp(iq), _dp(Vs), cp(mid),
                                       _dp(Vsm), _dp(Vsp), cp(st)), "m2c_riemann_batch")
         return st, Vs, mid, Vsm, Vsp
+
+    def varfcn(self, op, ids, x):
+        """VarFcnBase pointwise operation `op` (VF_*) of materials ids on points x ((n, 2) for the
+        (rho, e) / (rho, p) operations, (n, 5) for state vectors): (out, flag)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        n = x.shape[0]
+        ids = np.ascontiguousarray(np.broadcast_to(np.asarray(ids, dtype=np.int32), (n,)))
+        out = np.empty((n, 5) if op in (VF_C2P, VF_P2C, VF_CLIP) else n, dtype=np.float64)
+        flag = np.empty(n, np.int32)
+        ci = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        _chk(self.L.m2c_varfcn_batch(self.h, op, n, ci(ids), _dp(x), _dp(out), ci(flag)), "m2c_varfcn_batch")
+        return out, flag
+
+    def interface_flux(self, d, Vm, idm, Vp, idp):
+        """Local Lax-Friedrichs flux between two materials along axis d for n face pairs (n x 5)."""
+        Vm = np.ascontiguousarray(np.atleast_2d(Vm), dtype=np.float64)
+        Vp = np.ascontiguousarray(np.atleast_2d(Vp), dtype=np.float64)
+        n = Vm.shape[0]
+        ip = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+        im, iq = ip(idm), ip(idp)
+        F = np.empty_like(Vm)
+        ci = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        _chk(self.L.m2c_interface_flux_batch(self.h, d, n, _dp(Vm), ci(im), _dp(Vp), ci(iq), _dp(F)),
+             "m2c_interface_flux_batch")
+        return F
 
     def onesided_riemann(self, dirs, Vm, idl, Ustar):
         """One-sided (wall) Riemann solutions of n problems along unit normals dirs (n x 3) against walls

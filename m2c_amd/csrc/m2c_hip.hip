@@ -86,6 +86,7 @@ struct m2c_ctx {
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
 
 static int ensure_riemann_constants(m2c_ctx *c);
+static bool riemann_batch_args_ok(const m2c_ctx *c, int n, const int *idm, const int *idp);
 static int setup_godunov(m2c_ctx *c);
 static TLaw build_tlaw(const m2c_config &cfg);
 
@@ -1427,6 +1428,160 @@ int m2c_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const double *V
   return M2C_OK;
 }
 
+// ---------------------------------------------------------------- pointwise VarFcnBase / interface flux
+
+// VarFcnBase's pointwise set (VarFcnBase.h:62-294) of material id[t] at point t; the operation and
+// the in/out strides are described with m2c_varfcn_op. CheckState and clipping also report a flag.
+__global__ void k_varfcn(int op, int n, const int *__restrict__ id, const double *__restrict__ in,
+                         double *__restrict__ out, int *__restrict__ flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const DevEos &m = c_eos[id[t]];
+  switch (op) {
+  case M2C_VF_PRESSURE: out[t] = eos_p<double, -1>(m, in[2 * t], in[2 * t + 1]); break;
+  case M2C_VF_INTERNAL_ENERGY: out[t] = eos_e<double, -1>(m, in[2 * t], in[2 * t + 1]); break;
+  case M2C_VF_DPDRHO: out[t] = eos_dpdrho<double, -1>(m, in[2 * t], in[2 * t + 1]); break;
+  case M2C_VF_BIG_GAMMA: out[t] = eos_G<double, -1>(m, in[2 * t]); break;
+  case M2C_VF_SOUND_SPEED_SQUARE:
+    // the dummy material's GetDpdrho/GetBigGamma return 0 and its c^2 is DBL_MIN (VarFcnDummy.h:43)
+    out[t] = eos_c2<double, -1>(m, in[2 * t], in[2 * t + 1]);
+    break;
+  case M2C_VF_SOUND_SPEED: {   // the reference exits for c^2 <= 0 (VarFcnBase.h:203-209): NaN and flag 1
+    const double c2 = eos_c2<double, -1>(m, in[2 * t], in[2 * t + 1]);
+    out[t] = c2 <= 0 ? __builtin_nan("") : __builtin_sqrt(c2);
+    if (flag) flag[t] = c2 <= 0;
+    break;
+  }
+  case M2C_VF_ENTHALPY: {      // h = e + p/rho (VarFcnBase.h:240-244)
+    const double r = in[2 * t], pp = in[2 * t + 1];
+    out[t] = eos_e<double, -1>(m, r, pp) + pp / r;
+    break;
+  }
+  case M2C_VF_TOTAL_ENTHALPY: out[t] = eos_H<double, -1>(m, in + 5 * t); break;
+  case M2C_VF_MACH: {          // VarFcnBase.h:224-236; c^2 < 0 exits in the reference: NaN and flag 1
+    const double *V = in + 5 * t;
+    double c = eos_c2<double, -1>(m, V[0], eos_e<double, -1>(m, V[0], V[4]));
+    const bool bad = c < 0;
+    c = bad ? __builtin_nan("") : __builtin_sqrt(c);
+    out[t] = __builtin_sqrt(V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) / c;
+    if (flag) flag[t] = bad;
+    break;
+  }
+  case M2C_VF_C2P: {
+    double V[5];
+    eos_c2p<-1>(m, in + 5 * t, V);
+#pragma unroll
+    for (int q = 0; q < 5; q++) out[5 * t + q] = V[q];
+    break;
+  }
+  case M2C_VF_P2C: {
+    double U[5];
+    eos_p2c<double, -1>(m, in + 5 * t, U);
+#pragma unroll
+    for (int q = 0; q < 5; q++) out[5 * t + q] = U[q];
+    break;
+  }
+  case M2C_VF_CHECK_STATE: {
+    const int bad = eos_bad<double, -1>(m, in + 5 * t) ? 1 : 0;
+    out[t] = bad;
+    if (flag) flag[t] = bad;
+    break;
+  }
+  case M2C_VF_CLIP: {          // ClipDensityAndPressure(V) (VarFcnBase.h:259-294); the dummy resets V
+    double V[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) V[q] = in[5 * t + q];
+    const int cl = eos_clip<double, -1>(m, V) ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) out[5 * t + q] = V[q];
+    if (flag) flag[t] = cl;
+    break;
+  }
+  default: break;
+  }
+}
+
+static int varfcn_strides(int op, int *sin, int *sout) {
+  switch (op) {
+  case M2C_VF_PRESSURE: case M2C_VF_INTERNAL_ENERGY: case M2C_VF_DPDRHO: case M2C_VF_BIG_GAMMA:
+  case M2C_VF_SOUND_SPEED_SQUARE: case M2C_VF_SOUND_SPEED: case M2C_VF_ENTHALPY:
+    *sin = 2; *sout = 1; return 0;
+  case M2C_VF_TOTAL_ENTHALPY: case M2C_VF_MACH: case M2C_VF_CHECK_STATE:
+    *sin = 5; *sout = 1; return 0;
+  case M2C_VF_C2P: case M2C_VF_P2C: case M2C_VF_CLIP:
+    *sin = 5; *sout = 5; return 0;
+  default: return -1;
+  }
+}
+
+int m2c_varfcn_batch(m2c_ctx *c, int op, int n, const int32_t *id, const double *in, double *out, int32_t *flag) {
+  int sin, sout;
+  if (!c || n < 0 || varfcn_strides(op, &sin, &sout)) return M2C_ERR_ARG;
+  if (n > 0 && (!id || !in || !out)) return M2C_ERR_ARG;
+  for (int t = 0; t < n; t++)
+    if (id[t] < 0 || id[t] >= c->cfg.nmat) return M2C_ERR_ARG;
+  if (n == 0) return M2C_OK;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  const size_t bin = sizeof(double) * sin * (size_t)n, bout = sizeof(double) * sout * (size_t)n,
+               bi = sizeof(int) * (size_t)n;
+  char *d = nullptr;
+  HIPCHK(hipMalloc(&d, bin + bout + 2 * bi));
+  double *din = (double *)d, *dout = din + sin * (size_t)n;
+  int *did = (int *)(dout + sout * (size_t)n), *dfl = did + n;
+  HIPCHK(hipMemcpyAsync(din, in, bin, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(did, id, bi, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(dfl, 0, bi, c->stream));
+  hipLaunchKernelGGL(k_varfcn, dim3((n + 255) / 256), dim3(256), 0, c->stream, op, n, did, din, dout, dfl);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, c->stream));
+  if (flag) HIPCHK(hipMemcpyAsync(flag, dfl, bi, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d));
+  return M2C_OK;
+}
+
+// FluxFcnLLF::ComputeNumericalFluxAtMaterialInterface (FluxFcnLLF.h:112-185), the flux the
+// reference's SpaceOperator takes between two materials with MultiPhase.Flux = LLF
+__global__ void k_iface_flux_batch(int dir, int n, const double *__restrict__ Vm, const int *__restrict__ idm,
+                                   const double *__restrict__ Vp, const int *__restrict__ idp, double *__restrict__ F) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double a[5], b[5], f[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) { a[q] = Vm[5 * t + q]; b[q] = Vp[5 * t + q]; }
+  const DevEos &mm = c_eos[idm[t]], &mp = c_eos[idp[t]];
+  if (dir == 0) flux_llf2<double, -1, -1, 0>(mm, mp, a, b, f);
+  else if (dir == 1) flux_llf2<double, -1, -1, 1>(mm, mp, a, b, f);
+  else flux_llf2<double, -1, -1, 2>(mm, mp, a, b, f);
+#pragma unroll
+  for (int q = 0; q < 5; q++) F[5 * t + q] = f[q];
+}
+
+int m2c_interface_flux_batch(m2c_ctx *c, int dir, int n, const double *Vm, const int32_t *idm, const double *Vp,
+                             const int32_t *idp, double *F) {
+  if (!c || n < 0 || dir < 0 || dir > 2) return M2C_ERR_ARG;
+  if (n > 0 && (!Vm || !idm || !Vp || !idp || !F)) return M2C_ERR_ARG;
+  if (!riemann_batch_args_ok(c, n, idm, idp)) return M2C_ERR_ARG;
+  if (n == 0) return M2C_OK;
+  if (int rc0 = ensure_constants(c)) return rc0;
+  const size_t b = sizeof(double) * 5 * (size_t)n, bi = sizeof(int) * (size_t)n;
+  char *d = nullptr;
+  HIPCHK(hipMalloc(&d, 3 * b + 2 * bi));
+  double *dm = (double *)d, *dp = dm + 5 * (size_t)n, *dF = dp + 5 * (size_t)n;
+  int *dim_ = (int *)(dF + 5 * (size_t)n), *dip = dim_ + n;
+  HIPCHK(hipMemcpyAsync(dm, Vm, b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dp, Vp, b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dim_, idm, bi, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dip, idp, bi, hipMemcpyHostToDevice, c->stream));
+  const dim3 g((n + 255) / 256), bl(256);
+  hipLaunchKernelGGL(k_iface_flux_batch, g, bl, 0, c->stream, dir, n, dm, dim_, dp, dip, dF);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(F, dF, b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d));
+  return M2C_OK;
+}
+
 // the temperature-law constants of the materials (TLaw, physics.hpp)
 static TLaw build_tlaw(const m2c_config &cfg) {
   TLaw t;
@@ -1522,29 +1677,32 @@ static int setup_godunov(m2c_ctx *c) {
   return mp_alloc(c, &c->gscratch, (size_t)6 * c->mp.path_cap * c->gthreads, 0);
 }
 
-int m2c_riemann_batch(m2c_ctx *c, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
+static bool riemann_batch_args_ok(const m2c_ctx *c, int n, const int *idm, const int *idp) {
+  for (int t = 0; t < n; t++)
+    if (idm[t] < 0 || idm[t] >= c->cfg.nmat || idp[t] < 0 || idp[t] >= c->cfg.nmat) return false;
+  return true;
+}
+
+int m2c_riemann_batch(m2c_ctx *c, int n, const double *dir, const double *Vm, const int *idm, const double *Vp,
                       const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status) {
   if (!c || n < 0 || (n > 0 && (!dir || !Vm || !idm || !Vp || !idp || !Vs || !id || !Vsm || !Vsp || !status)))
     return M2C_ERR_ARG;
   if (n == 0) return M2C_OK;
-  for (int t = 0; t < n; t++) {
-    if (dir[t] < 0 || dir[t] > 2) return M2C_ERR_ARG;
-    if (idm[t] < 0 || idm[t] >= c->cfg.nmat || idp[t] < 0 || idp[t] >= c->cfg.nmat) return M2C_ERR_ARG;
-  }
+  if (!riemann_batch_args_ok(c, n, idm, idp)) return M2C_ERR_ARG;
   if (int rc0 = ensure_constants(c)) return rc0;
   if (int rc0 = ensure_riemann_constants(c)) return rc0;
   const int nthreads = (int)std::min<long long>(4096, ((long long)n + 63) / 64 * 64);
-  size_t b5 = sizeof(double) * 5 * (size_t)n, bi = sizeof(int) * (size_t)n;
+  size_t b5 = sizeof(double) * 5 * (size_t)n, b3 = sizeof(double) * 3 * (size_t)n, bi = sizeof(int) * (size_t)n;
   size_t sbytes = sizeof(double) * 6 * (size_t)c->mp.path_cap * nthreads;
   char *d = nullptr;
-  HIPCHK(hipMalloc(&d, 5 * b5 + 5 * bi + sbytes));
+  HIPCHK(hipMalloc(&d, 5 * b5 + b3 + 4 * bi + sbytes));
   double *dVm = (double *)d, *dVp = dVm + 5 * (size_t)n, *dVs = dVp + 5 * (size_t)n, *dVsm = dVs + 5 * (size_t)n,
-         *dVsp = dVsm + 5 * (size_t)n;
-  int *ddir = (int *)(dVsp + 5 * (size_t)n), *didm = ddir + n, *didp = didm + n, *did = didp + n, *dst = did + n;
-  double *scratch = (double *)(dst + n + (n & 1));
+         *dVsp = dVsm + 5 * (size_t)n, *ddir = dVsp + 5 * (size_t)n, *scratch = ddir + 3 * (size_t)n;
+  int *didm = (int *)(scratch + 6 * (size_t)c->mp.path_cap * nthreads), *didp = didm + n, *did = didp + n,
+      *dst = did + n;
   HIPCHK(hipMemcpyAsync(dVm, Vm, b5, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(dVp, Vp, b5, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(ddir, dir, bi, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(ddir, dir, b3, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(didm, idm, bi, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(didp, idp, bi, hipMemcpyHostToDevice, c->stream));
   int rc = m2c_mp_launch_riemann_batch(c->stream, n, ddir, dVm, didm, dVp, didp, dVs, did, dVsm, dVsp, dst, scratch,

@@ -1315,9 +1315,9 @@ __global__ __launch_bounds__(256) void k_state_finish(double *V, int8_t *IDn, co
   if (flags) atomicOr(&st->bits, flags);
 }
 
-// ComputeRiemannSolution for a batch of independent problems (pointwise checks): problem t uses
-// the unit normal of axis dir[t]
-__global__ __launch_bounds__(64) void k_riemann_batch(int n, const int *dir, const double *Vm, const int *idm,
+// ComputeRiemannSolution for a batch of independent problems: problem t is solved along the unit
+// normal dir[3t..3t+2] (ExactRiemannSolverBase.h:35-39 takes it as double *dir)
+__global__ __launch_bounds__(64) void k_riemann_batch(int n, const double *dir, const double *Vm, const int *idm,
                                                        const double *Vp, const int *idp, double *Vs, int *id,
                                                        double *Vsm, double *Vsp, int *status, double *scratch,
                                                        int nthreads) {
@@ -1332,8 +1332,15 @@ __global__ __launch_bounds__(64) void k_riemann_batch(int n, const int *dir, con
   S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
   S_.path3.n = 0;
   for (int t = tid; t < n; t += nthreads) {
-    double nrm[3] = {0.0, 0.0, 0.0};
-    nrm[dir[t]] = 1.0;
+    if (idm[t] < 0 || idm[t] >= c_grid.nmat || idp[t] < 0 || idp[t] >= c_grid.nmat) {
+      // material index outside the table (callers passing device arrays are not checked on the host)
+#pragma unroll
+      for (int q = 0; q < 5; q++) Vs[5 * t + q] = Vsm[5 * t + q] = Vsp[5 * t + q] = 0.0;
+      id[t] = -1;
+      status[t] = -1;
+      continue;
+    }
+    double nrm[3] = {dir[3 * t], dir[3 * t + 1], dir[3 * t + 2]};
     int mid = 0;
     S_.overflow = 0;
     int r = riemann_solve(S_, nrm, Vm + 5 * t, idm[t], Vp + 5 * t, idp[t], Vs + 5 * t, &mid, Vsm + 5 * t, Vsp + 5 * t);
@@ -1590,7 +1597,7 @@ extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_co
   return hchk(hipGetLastError());
 }
 
-extern "C" int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm,
+extern "C" int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const double *dir, const double *Vm, const int *idm,
                                            const double *Vp, const int *idp, double *Vs, int *id, double *Vsm,
                                            double *Vsp, int *status, double *scratch, int nthreads) {
   hipLaunchKernelGGL(k_riemann_batch, dim3((nthreads + 63) / 64), dim3(64), 0, s, n, dir, Vm, idm, Vp, idp, Vs, id,

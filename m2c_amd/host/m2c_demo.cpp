@@ -9,6 +9,10 @@
 //                                                        (material 0) and water (material 1): 12
 //                                                        doubles per problem (dir, V, id, wall
 //                                                        velocity) in, 12 out (status, id, Vs, Vsm)
+//   m2c_demo pointwise <problems.bin> <out.bin>          VarFcn, FluxFcn and ExactRiemannSolver calls on
+//                                                        the materials of pointwise_materials (air SG,
+//                                                        water SG, NASG, MG, JWL): 16 doubles per
+//                                                        problem in, 32 out (layout at run_pointwise)
 //
 // Writes the primitive state (nz*ny*nx*5 doubles), then the IDs (int32) and, with a level set, the
 // level set, to out.bin; prints steps and time.
@@ -199,17 +203,99 @@ static int run_onesided(const char *in, const char *out) {
   return 0;
 }
 
+// air, water (stiffened gases), NASG water, Mie-Grueneisen aluminium, JWL TNT products
+static m2c_config pointwise_materials() {
+  m2c_config c = default_config(4, 4, 4, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0);
+  add_stiffened_gas(c, 1.4, 0.0);
+  add_stiffened_gas(c, 6.12, 3.43e8);
+  const int kinds[3] = {M2C_EOS_NASG, M2C_EOS_MG, M2C_EOS_JWL};
+  const double params[3][6] = {{1.19, 6.217e8, 6.7212e-4, -1177788.0, 0, 0},
+                               {2785.0, 5328.0, 2.0, 1.338, 0.0, 0},
+                               {0.28, 3.712e11, 3.23e9, 4.15, 0.95, 1630.0}};
+  for (int k = 0; k < 3; k++) {
+    m2c_material &m = c.mat[c.nmat++];
+    m = c.mat[0];
+    m.eos = kinds[k];
+    for (int q = 0; q < 8; q++) m.p[q] = q < 6 ? params[k][q] : 0.0;
+  }
+  return c;
+}
+
+// One problem: in = id, rho, e, p, u, v, w | dir[3], idm, idp, Vm_u, Vp_u... laid out as
+//   [0] material id  [1] rho  [2] e  [3] p  [4..6] u, v, w      (VarFcn of that material)
+//   [7..9] unit normal  [10] axis  [11] rho_p  [12] p_p  [13] id_p  [14..15] unused
+// The Riemann problem and the fluxes pair the state (rho, u, v, w, p) of material id with
+// (rho_p, -u, w, v, p_p) of material id_p. out (32 doubles):
+//   [0] GetPressure(rho, e) [1] GetInternalEnergyPerUnitMass(rho, p) [2] GetDpdrho [3] GetBigGamma
+//   [4] ComputeSoundSpeedSquare [5] CheckState(rho, p) [6..10] PrimitiveToConservative(V)
+//   [11..15] ConservativeToPrimitive of that U [16] ComputeTotalEnthalpyPerUnitMass(V)
+//   [17] status of ComputeRiemannSolution along the normal (0/1), [18] its id, [19..23] Vsm
+//   [24..28] FluxFcn::ComputeNumericalFluxAtMaterialInterface along the axis (or, for equal
+//   materials, ComputeNumericalFluxAtCellInterface) [29] Vsp[0] [30] Vsp[4] [31] Vs[0]
+static int run_pointwise(const char *in, const char *out) {
+  std::vector<double> q;
+  {
+    FILE *f = fopen(in, "rb");
+    if (!f) return 1;
+    double v;
+    while (fread(&v, sizeof v, 1, f) == 1) q.push_back(v);
+    fclose(f);
+  }
+  if (q.size() % 16) return 1;
+  m2c_config c = pointwise_materials();
+  Context ctx(c);
+  ExactRiemannSolver rs(ctx);
+  FluxFcn ff(ctx);
+  std::vector<double> r;
+  for (size_t t = 0; t < q.size(); t += 16) {
+    const double *p = &q[t];
+    const int id = (int)p[0], idp = (int)p[13], axis = (int)p[10];
+    VarFcn vf(ctx, id);
+    double o[32] = {0};
+    const double rho = p[1], e = p[2], pr = p[3];
+    o[0] = vf.GetPressure(rho, e);
+    o[1] = vf.GetInternalEnergyPerUnitMass(rho, pr);
+    o[2] = vf.GetDpdrho(rho, e);
+    o[3] = vf.GetBigGamma(rho, e);
+    o[4] = vf.ComputeSoundSpeedSquare(rho, e);
+    o[5] = vf.CheckState(rho, pr) ? 1.0 : 0.0;
+    const double V[5] = {rho, p[4], p[5], p[6], pr};
+    vf.PrimitiveToConservative(V, o + 6);
+    vf.ConservativeToPrimitive(o + 6, o + 11);
+    o[16] = vf.ComputeTotalEnthalpyPerUnitMass(V);
+    const double Vp[5] = {p[11], -p[4], p[6], p[5], p[12]};
+    double Vs[5], Vsm[5], Vsp[5];
+    int rid = 0;
+    o[17] = rs.ComputeRiemannSolution(p + 7, V, id, Vp, idp, Vs, rid, Vsm, Vsp);
+    o[18] = rid;
+    for (int k = 0; k < 5; k++) o[19 + k] = Vsm[k];
+    if (id == idp) ff.ComputeNumericalFluxAtCellInterface(axis, V, Vp, id, o + 24);
+    else ff.ComputeNumericalFluxAtMaterialInterface(axis, V, id, Vp, idp, o + 24);
+    o[29] = Vsp[0];
+    o[30] = Vsp[4];
+    o[31] = Vs[0];
+    r.insert(r.end(), o, o + 32);
+  }
+  FILE *f = fopen(out, "wb");
+  if (!f) return 1;
+  fwrite(r.data(), sizeof(double), r.size(), f);
+  fclose(f);
+  printf("pointwise problems=%zu\n", q.size() / 16);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc < 4 || (strcmp(argv[1], "deck") && strcmp(argv[1], "onesided") && argc < 5)) {
+  if (argc < 4 || (strcmp(argv[1], "deck") && strcmp(argv[1], "onesided") && strcmp(argv[1], "pointwise") && argc < 5)) {
     fprintf(stderr,
             "usage: %s sod <n> <tmax> <out> | bubble <scale> <steps> <out> | deck <input> <out> | "
-            "onesided <in> <out>\n",
+            "onesided <in> <out> | pointwise <in> <out>\n",
             argv[0]);
     return 2;
   }
   try {
     if (!strcmp(argv[1], "deck")) return run_deck(argv[2], argv[3]);
     if (!strcmp(argv[1], "onesided")) return run_onesided(argv[2], argv[3]);
+    if (!strcmp(argv[1], "pointwise")) return run_pointwise(argv[2], argv[3]);
     if (!strcmp(argv[1], "sod")) return run_sod(atoi(argv[2]), atof(argv[3]), argv[4]);
     if (!strcmp(argv[1], "bubble")) return run_bubble(atof(argv[2]), atoi(argv[3]), argv[4]);
   } catch (const Error &e) {

@@ -8,6 +8,10 @@
 //   ExactRiemannSolver      ExactRiemannSolverBase.h:35-49 (ComputeRiemannSolution,
 //                           ComputeOneSidedRiemannSolution)
 //   LevelSetOperator        LevelSetOperator.h:86-90 (the level set Phi of one material)
+//   VarFcn                  VarFcnBase.h:62-294 (the pointwise equation-of-state set of one
+//                           material, evaluated on the device)
+//   FluxFcn                 FluxFcnBase.h:65-76 (ComputeNumericalFluxAtCellInterface and
+//                           ComputeNumericalFluxAtMaterialInterface)
 //
 // Arrays are the interior of SpaceVariable3D (k-j-i, dof innermost, no ghosts). Fatal conditions
 // that make the reference call exit() throw m2c_host::Error carrying the status code; a program
@@ -240,20 +244,17 @@ class TimeIntegrator {
   Context &c_;
 };
 
-// ExactRiemannSolverBase::ComputeRiemannSolution for mesh-axis normals (the default
-// MultiPhase.RiemannNormal = Mesh). Returns 0 on success, 1 on a solver failure (the reference
-// counts it); throws on an invalid input state (the reference exits).
+// ExactRiemannSolverBase::ComputeRiemannSolution along any unit normal dir (the mesh axis of
+// MultiPhase.RiemannNormal = Mesh, or the level-set / averaged normals of LevelSet / Average).
+// Returns 0 on success, 1 on a solver failure (the reference counts it); throws on an invalid input
+// state (the reference exits).
 class ExactRiemannSolver {
  public:
   explicit ExactRiemannSolver(Context &c) : c_(c) {}
   int ComputeRiemannSolution(const double *dir, const double *Vm, int idm, const double *Vp, int idp, double *Vs,
                              int &id, double *Vsm, double *Vsp) {
-    int axis = -1;
-    for (int d = 0; d < 3; d++)
-      if (dir[d] == 1.0 && dir[(d + 1) % 3] == 0.0 && dir[(d + 2) % 3] == 0.0) axis = d;
-    if (axis < 0) throw Error("ComputeRiemannSolution: only mesh-axis normals are provided", M2C_ERR_UNSUPPORTED);
     int st = 0;
-    check(m2c_riemann_batch(c_.get(), 1, &axis, Vm, &idm, Vp, &idp, Vs, &id, Vsm, Vsp, &st), "m2c_riemann_batch");
+    check(m2c_riemann_batch(c_.get(), 1, dir, Vm, &idm, Vp, &idp, Vs, &id, Vsm, Vsp, &st), "m2c_riemann_batch");
     if (st < 0) throw Error("ComputeRiemannSolution: invalid input state", M2C_ERR_STATE);
     return st == 0 ? 0 : 1;
   }
@@ -272,6 +273,98 @@ class ExactRiemannSolver {
     if (id != -1)
       for (int p = 0; p < 5; p++) Vs[p] = vs[p];
     return st == 0 ? 0 : 1;
+  }
+
+ private:
+  Context &c_;
+};
+
+// VarFcnBase of material `id` (VarFcnBase.h:62-294): the pointwise equation-of-state calls with the
+// reference's signatures, evaluated by the device kernels on the context's material table. Each call
+// is one device round trip; the *_n forms take n points at once. Where the reference exits
+// (ComputeSoundSpeed with c^2 <= 0, ComputeMachNumber with c^2 < 0) these throw.
+class VarFcn {
+ public:
+  VarFcn(Context &c, int id) : c_(c), id_(id) {
+    if (id < 0 || id >= c.config().nmat) throw Error("VarFcn: material index out of range", M2C_ERR_ARG);
+  }
+  int GetType() const { return c_.config().mat[id_].eos; }
+  double GetPressure(double rho, double e) { return scalar(M2C_VF_PRESSURE, rho, e); }
+  double GetInternalEnergyPerUnitMass(double rho, double p) { return scalar(M2C_VF_INTERNAL_ENERGY, rho, p); }
+  double GetDpdrho(double rho, double e) { return scalar(M2C_VF_DPDRHO, rho, e); }
+  double GetBigGamma(double rho, double e) { return scalar(M2C_VF_BIG_GAMMA, rho, e); }
+  double ComputeSoundSpeedSquare(double rho, double e) { return scalar(M2C_VF_SOUND_SPEED_SQUARE, rho, e); }
+  double ComputeSoundSpeed(double rho, double e) {
+    int32_t f = 0;
+    double c = scalar(M2C_VF_SOUND_SPEED, rho, e, &f);
+    if (f) throw Error("ComputeSoundSpeed: square root of a negative number", M2C_ERR_STATE);
+    return c;
+  }
+  double ComputeEnthalpyPerUnitMass(double rho, double p) { return scalar(M2C_VF_ENTHALPY, rho, p); }
+  double ComputeTotalEnthalpyPerUnitMass(const double *V) { return vec1(M2C_VF_TOTAL_ENTHALPY, V); }
+  double ComputeMachNumber(const double *V) {
+    int32_t f = 0;
+    double m = vec1(M2C_VF_MACH, V, &f);
+    if (f) throw Error("ComputeMachNumber: negative c^2", M2C_ERR_STATE);
+    return m;
+  }
+  bool CheckState(const double *V, bool /*silence*/ = false) { return vec1(M2C_VF_CHECK_STATE, V) != 0.0; }
+  bool CheckState(double rho, double p, bool /*silence*/ = false) {
+    const double V[5] = {rho, 0.0, 0.0, 0.0, p};
+    return vec1(M2C_VF_CHECK_STATE, V) != 0.0;
+  }
+  void ConservativeToPrimitive(const double *U, double *V) { vec5(M2C_VF_C2P, U, V); }
+  void PrimitiveToConservative(const double *V, double *U) { vec5(M2C_VF_P2C, V, U); }
+  bool ClipDensityAndPressure(double *V, double *U = nullptr) {
+    int32_t f = 0;
+    double out[5];
+    vec5(M2C_VF_CLIP, V, out, &f);
+    for (int q = 0; q < 5; q++) V[q] = out[q];
+    if (f && U) PrimitiveToConservative(V, U);
+    return f != 0;
+  }
+  // n points: in holds (rho, e) / (rho, p) pairs or 5-vectors as m2c_varfcn_batch describes
+  void Evaluate_n(int op, int n, const double *in, double *out, int32_t *flag = nullptr) {
+    std::vector<int32_t> ids((size_t)n, id_);
+    check(m2c_varfcn_batch(c_.get(), op, n, ids.data(), in, out, flag), "m2c_varfcn_batch");
+  }
+
+ private:
+  double scalar(int op, double a, double b, int32_t *flag = nullptr) {
+    const double in[2] = {a, b};
+    double out = 0.0;
+    int32_t f = 0;
+    check(m2c_varfcn_batch(c_.get(), op, 1, &id_, in, &out, &f), "m2c_varfcn_batch");
+    if (flag) *flag = f;
+    return out;
+  }
+  double vec1(int op, const double *V, int32_t *flag = nullptr) {
+    double out = 0.0;
+    int32_t f = 0;
+    check(m2c_varfcn_batch(c_.get(), op, 1, &id_, V, &out, &f), "m2c_varfcn_batch");
+    if (flag) *flag = f;
+    return out;
+  }
+  void vec5(int op, const double *in, double *out, int32_t *flag = nullptr) {
+    int32_t f = 0;
+    check(m2c_varfcn_batch(c_.get(), op, 1, &id_, in, out, &f), "m2c_varfcn_batch");
+    if (flag) *flag = f;
+  }
+  Context &c_;
+  int32_t id_;
+};
+
+// FluxFcnBase (FluxFcnBase.h:65-76) of the context's numerical flux: the face flux between two
+// states of one material, and the local Lax-Friedrichs flux between two materials.
+class FluxFcn {
+ public:
+  explicit FluxFcn(Context &c) : c_(c) {}
+  void ComputeNumericalFluxAtCellInterface(int dir, const double *Vm, const double *Vp, int id, double *F) {
+    check(m2c_flux_batch(c_.get(), dir, 1, Vm, Vp, id, F), "m2c_flux_batch");
+  }
+  void ComputeNumericalFluxAtMaterialInterface(int dir, const double *Vm, int idm, const double *Vp, int idp,
+                                               double *F) {
+    check(m2c_interface_flux_batch(c_.get(), dir, 1, Vm, &idm, Vp, &idp, F), "m2c_interface_flux_batch");
   }
 
  private:

@@ -551,3 +551,9 @@ void orc_eos_point(const orc_ctx *c, int id, double rho, double e, double p, dou
   out[4] = orc_c2(m, rho, e);
   out[5] = (double)orc_check_state_rp(m, rho, p);
 }
+
+/* FluxFcnLLF::ComputeNumericalFluxAtMaterialInterface (FluxFcnLLF.h:112-185) of materials idm | idp. */
+void orc_material_interface_flux(const orc_ctx *c, int dir, const double *Vm, int idm, const double *Vp, int idp,
+                                 double *F) {
+  orc_flux_llf2(&c->cfg.mat[idm], &c->cfg.mat[idp], dir, Vm, Vp, F);
+}

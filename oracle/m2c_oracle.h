@@ -148,6 +148,10 @@ int orc_store_riemann(orc_ctx *c, const double *V, const int *ID, int stamp);
  * Returns the number of unresolved cells. */
 long orc_update_states(orc_ctx *c, const int *IDn, const int *ID, double *V);
 
+/* FluxFcnLLF::ComputeNumericalFluxAtMaterialInterface (FluxFcnLLF.h:112-185) between materials idm, idp. */
+void orc_material_interface_flux(const orc_ctx *c, int dir, const double *Vm, int idm, const double *Vp, int idp,
+                                 double *F);
+
 /* Multiphase counters since creation: reinitialisation iterations (retries included), unresolved
  * cells of the state update, and the reinitialisation CFL number in force (divided by 1.5 on each
  * retry, LevelSetReinitializer.cpp:155-168). */

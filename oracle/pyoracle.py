@@ -164,6 +164,7 @@ def lib():
         L.orc_store_riemann.argtypes = [P, D, I, C.c_int]
         L.orc_update_states.argtypes = [P, I, I, D]
         L.orc_update_states.restype = C.c_long
+        L.orc_material_interface_flux.argtypes = [P, C.c_int, D, C.c_int, D, C.c_int, D]
         L.orc_mp_counters.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long), D]
         for f in ("orc_det_log", "orc_det_exp", "orc_pow_quarter", "orc_pow_fifth"):
             getattr(L, f).argtypes = [C.c_double]
@@ -335,6 +336,14 @@ class Oracle:
         """State update after the interface moved (IDn before, ID after); V in place. Returns the
         number of unresolved cells."""
         return self.L.orc_update_states(self.h, _ip(IDn), _ip(ID), _dp(V))
+
+    def interface_flux(self, d, Vm, idm, Vp, idp):
+        """Local Lax-Friedrichs flux between materials idm and idp along axis d."""
+        Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+        Vp = np.ascontiguousarray(Vp, dtype=np.float64)
+        F = np.zeros(5)
+        self.L.orc_material_interface_flux(self.h, d, _dp(Vm), idm, _dp(Vp), idp, _dp(F))
+        return F
 
     def mp_counters(self):
         """(reinitialisation iterations, unresolved cells, reinitialisation CFL in force)."""
