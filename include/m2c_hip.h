@@ -176,6 +176,18 @@ int m2c_eos_derived(m2c_ctx *ctx, int n, const double *rho, const double *p, con
 int m2c_riemann_batch(m2c_ctx *ctx, int n, const double *dir, const double *Vm, const int *idm, const double *Vp,
                       const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status);
 
+/* m2c_riemann_batch for callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. Every pointer
+ * is DEVICE memory owned by the caller (same layout as m2c_riemann_batch); the call allocates
+ * nothing: the exact solver's rarefaction-path cache lives in a fixed device array of the library
+ * (25 MB per GPU, held until the process ends) serving up to 512 concurrent solves, and n is
+ * unbounded (grid-stride). Material indices are checked on the device (status -1). Calls that
+ * overlap on one GPU share that array and race; the call returns after its stream has finished.
+ * M2C_ERR_ARG if the context's path cache exceeds the array's 1024 entries per side. */
+int m2c_riemann_batch_noalloc(m2c_ctx *ctx, int n, const double *dir, const double *Vm, const int *idm,
+                              const double *Vp, const int *idp, double *Vs, int *id, double *Vsm, double *Vsp,
+                              int *status);
+
 /* VarFcnBase's pointwise set (VarFcnBase.h:62-294) for n points, evaluated on the device with the
  * context's materials: point t uses material id[t]. The operation fixes the strides of in / out:
  *   (rho, e) -> 1: PRESSURE (GetPressure), DPDRHO (GetDpdrho), BIG_GAMMA (GetBigGamma),

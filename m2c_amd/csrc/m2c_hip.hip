@@ -1718,6 +1718,50 @@ int m2c_riemann_batch(m2c_ctx *c, int n, const double *dir, const double *Vm, co
   return M2C_OK;
 }
 
+// ---------------------------------------------------------------- m2c_riemann_batch_noalloc
+//
+// The exact solver's rarefaction-path cache for callers that cannot allocate: a fixed device
+// array (each GPU the library runs on holds its own copy until the process ends) serving
+// kNoAllocThreads concurrent solves of kNoAllocPathCap entries per side. Its device address is
+// looked up once per device; host code passes that address, never the array's name.
+constexpr int kNoAllocThreads = 512;
+constexpr int kNoAllocPathCap = 1024;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[(size_t)6 * kNoAllocPathCap * kNoAllocThreads];
+static double *g_scratch_ptr[kMaxDevices];
+
+static int noalloc_scratch(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) return M2C_ERR_ARG;
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (double *)ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return M2C_OK;
+}
+
+int m2c_riemann_batch_noalloc(m2c_ctx *c, int n, const double *dir, const double *Vm, const int *idm, const double *Vp,
+                              const int *idp, double *Vs, int *id, double *Vsm, double *Vsp, int *status) {
+  if (!c || n < 0 || (n > 0 && (!dir || !Vm || !idm || !Vp || !idp || !Vs || !id || !Vsm || !Vsp || !status)))
+    return M2C_ERR_ARG;
+  if (c->mp.path_cap > kNoAllocPathCap) {
+    fprintf(stderr, "m2c_riemann_batch_noalloc: rarefaction path cache of %d entries exceeds the fixed %d\n",
+            c->mp.path_cap, kNoAllocPathCap);
+    return M2C_ERR_ARG;
+  }
+  if (n == 0) return M2C_OK;
+  if (int rc0 = ensure_constants(c)) return rc0;   // sets the context's device current
+  if (int rc0 = ensure_riemann_constants(c)) return rc0;
+  double *scr = nullptr;
+  if (int rc0 = noalloc_scratch(c->cfg.device, &scr)) return rc0;
+  const int nthreads = (int)std::min<long long>(kNoAllocThreads, ((long long)n + 63) / 64 * 64);
+  if (m2c_mp_launch_riemann_batch(c->stream, n, dir, Vm, idm, Vp, idp, Vs, id, Vsm, Vsp, status, scr, nthreads))
+    return M2C_ERR_HIP;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return M2C_OK;
+}
+
 // shared by the one-sided entry points: face = 0 the plain solver, 1 a wall-cut face
 static int onesided_batch(m2c_ctx *c, int face, int n, const double *dir, const double *Vm, const int *idl,
                           const double *uw, const int *ax, const int *fb, int mode, double *Vs, int *id, double *Vsm,
