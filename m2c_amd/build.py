@@ -26,7 +26,10 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
 # dispatch; float = the mixed-precision form
 VARIANTS = [("hllc_sg", 2, 0, "double"), ("hllc_gen", 2, -1, "double"), ("roe_gen", 0, -1, "double"),
             ("llf_gen", 1, -1, "double"), ("hllc_sg_f32", 2, 0, "float"), ("hllc_gen_f32", 2, -1, "float"),
-            ("roe_gen_f32", 0, -1, "float"), ("llf_gen_f32", 1, -1, "float"), ("god_gen", 3, -1, "double")]
+            ("roe_gen_f32", 0, -1, "float"), ("llf_gen_f32", 1, -1, "float"), ("god_gen", 3, -1, "double"),
+            ("hllc_sg_mp", 2, 0, "double"), ("hllc_sg_mp_f32", 2, 0, "float")]
+# variants that keep the material-interface code with a compile-time equation of state
+MP_VARIANTS = {"hllc_sg_mp", "hllc_sg_mp_f32"}
 # The mixed-form variants (FP32 reconstruction and fluxes) take the hardware's FP32 reciprocal and
 # square root instead of the correctly rounded sequences; FP64 code is not affected by this flag.
 # batch.hip holds the same FP32 stage bodies and takes it too (batched and single runs stay equal).
@@ -34,7 +37,7 @@ FP32_FAST = ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", FP32_FAST),
          ("mlp", "mlp.hip", [])] + [
     (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"]
-     + (FP32_FAST if tc == "float" else []))
+     + (["-DM2C_MP=1"] if t in MP_VARIANTS else []) + (FP32_FAST if tc == "float" else []))
     for t, f, e, tc in VARIANTS]
 # host-only C++ (no device code), compiled with the host compiler
 HOST_UNITS = [("input_deck", "input_deck.cpp"), ("output", "output.cpp")]

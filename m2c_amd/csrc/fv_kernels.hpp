@@ -258,7 +258,7 @@ __device__ __forceinline__ void numerical_flux(const TC *Vm, const TC *Vp, int i
 
 // Flux of one face between cells with IDs idm (minus) and idp (plus). Interface faces solved by
 // the exact Riemann pass return 1 (their two fluxes are read from the interface-flux arrays).
-template <int FLUX, int EK, int DIR, typename TC>
+template <int FLUX, int EK, int DIR, typename TC, bool MP = (EK < 0)>
 __device__ __forceinline__ int face_flux(const TC *Vm, const TC *Vp, const TC *cm, const TC *cp, int idm, int idp,
                                          TC *F) {
   const Grid &g = M2C_G;
@@ -268,7 +268,7 @@ __device__ __forceinline__ int face_flux(const TC *Vm, const TC *Vp, const TC *c
   }
 #pragma unroll
   for (int p = 0; p < 5; p++) F[p] = TC(0);
-  if (EK < 0 && idm != idp && idm != g.inactive_id && idp != g.inactive_id) {
+  if (MP && idm != idp && idm != g.inactive_id && idp != g.inactive_id) {
     if (g.mp_flux == 2) {   // LLF at the material interface (SpaceOperator.cpp:2493-2501)
       const TC *am = g.mp_recon ? Vm : cm, *ap = g.mp_recon ? Vp : cp;
       flux_llf2<TC, -1, -1, DIR>(c_eos[idm], c_eos[idp], am, ap, F);
@@ -302,7 +302,7 @@ __device__ __forceinline__ void warp_min_atomic(unsigned long long *dst, double 
 
 // x (DIR 0) or y (DIR 1) face of the tile whose plus cell is at local (x, y); the flux goes to
 // sF[p*NF + fi].
-template <int FLUX, int EK, int DIR, typename TC>
+template <int FLUX, int EK, int DIR, typename TC, bool MP = (EK < 0)>
 __device__ __forceinline__ void tile_face(TC (*sV)[LY][LX], int (*sID)[LX], const TC *sS, TC *sF, int x, int y, int fi,
                                           int i0, int j0, unsigned long long &nrev) {
   const Grid &g = M2C_G;
@@ -337,7 +337,7 @@ __device__ __forceinline__ void tile_face(TC (*sV)[LY][LX], int (*sID)[LX], cons
       nrev += face_state<EK, true, TC>(vcm, sgm, idm, fm);
       nrev += face_state<EK, false, TC>(vcp, sgp, idp, fp);
     }
-    face_flux<FLUX, EK, DIR, TC>(fm, fp, vcm, vcp, idm, idp, F);
+    face_flux<FLUX, EK, DIR, TC, MP>(fm, fp, vcm, vcp, idm, idp, F);
   }
 #pragma unroll
   for (int p = 0; p < 5; p++) sF[p * NF + fi] = F[p];
@@ -355,7 +355,7 @@ template <typename TC> __device__ __forceinline__ void to_tc(const double *v, TC
 // the drop-in path; float for the mixed form, where the fetch (with ConservativeToPrimitive and
 // its p(e) inversion), the residual accumulation, the stage combination, ConservativeToPrimitive
 // of the result, clipping and the CFL reduction stay in double.
-template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double>
+template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double, bool MP = (EK < 0)>
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, const int by, const int bz) {
   const Grid &g = M2C_G;
   if (!a.ignore_active && !a.clk->active) return;
@@ -404,7 +404,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     TC szm[5];
     cell_slopes<2, TC>(vm2, vm1, w0, idm2, idm1, id0, szm);
     cell_slopes<2, TC>(vm1, w0, w1, idm1, id0, id1, sz0);
-    if constexpr (EK < 0) {   // a level set means two materials: the generic-EOS variants only
+    if constexpr (MP) {   // a level set means two materials: the multi-material variants only
       if (a.phin) {
         if (conrec_cell(a.phin, ci, cj, kbeg - 1)) zero5(szm);
         if (conrec_cell(a.phin, ci, cj, kbeg)) zero5(sz0);
@@ -417,7 +417,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     } else {
       nrev += face_state<EK, true, TC>(vm1, szm, idm1, fm);
     }
-    lo_iface = face_flux<FLUX, EK, 2, TC>(fm, fp, vm1, w0, idm1, id0, Flo);
+    lo_iface = face_flux<FLUX, EK, 2, TC, MP>(fm, fp, vm1, w0, idm1, id0, Flo);
     idlo = idm1;
     fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 2, d, id2, flags); to_tc(d, w2);
   }
@@ -468,7 +468,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     }
     // ---- z slope of k+1 (registers)
     cell_slopes<2, TC>(w0, w1, w2, id0, id1, id2, sz1);
-    if constexpr (EK < 0) {
+    if constexpr (MP) {
       if (a.phin && conrec_cell(a.phin, ci, cj, k + 1)) zero5(sz1);
     }
     __syncthreads();
@@ -488,7 +488,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
         }
         cell_slopes<0, TC>(vl, vc, vr, sID[y + HALO][x - 1 + HALO], sID[y + HALO][x + HALO],
                            sID[y + HALO][x + 1 + HALO], sg);
-        if constexpr (EK < 0) {
+        if constexpr (MP) {
           if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
         }
 #pragma unroll
@@ -505,7 +505,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
         }
         cell_slopes<1, TC>(vl, vc, vr, sID[y - 1 + HALO][x + HALO], sID[y + HALO][x + HALO],
                            sID[y + 1 + HALO][x + HALO], sg);
-        if constexpr (EK < 0) {
+        if constexpr (MP) {
           if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
         }
 #pragma unroll
@@ -516,8 +516,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
 
     // ---- x and y face fluxes: the west/south face of every column, then the east column of
     //      x faces and the north row of y faces
-    tile_face<FLUX, EK, 0, TC>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
-    tile_face<FLUX, EK, 1, TC>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
+    tile_face<FLUX, EK, 0, TC, MP>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
+    tile_face<FLUX, EK, 1, TC, MP>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
     {
       // the TY east x faces and the TX north y faces each go to one wave, two waves apart, and
       // the pair rotates with the block: the two other waves of the block skip the extra flux
@@ -526,9 +526,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       const int wave = tid >> 6, lane = tid & 63;
       const int rot = (bx + by + bz) & 3;
       if (wave == rot && lane < TY)
-        tile_face<FLUX, EK, 0, TC>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
+        tile_face<FLUX, EK, 0, TC, MP>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
       if (wave == ((rot + 2) & 3) && lane < TX)
-        tile_face<FLUX, EK, 1, TC>(sV, sID, sS, sFy, lane, TY, TY * TX + lane, i0, j0, nrev);
+        tile_face<FLUX, EK, 1, TC, MP>(sV, sID, sS, sFy, lane, TY, TY * TX + lane, i0, j0, nrev);
     }
 
     // ---- z face k+1/2 (registers)
@@ -542,7 +542,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       } else {
         nrev += face_state<EK, false, TC>(w1, sz1, id1, fp);
       }
-      hi_iface = face_flux<FLUX, EK, 2, TC>(fm, fp, w0, w1, id0, id1, Fhi);
+      hi_iface = face_flux<FLUX, EK, 2, TC, MP>(fm, fp, w0, w1, id0, id1, Fhi);
     }
     __syncthreads();
 
@@ -560,7 +560,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
         Fb[p] = double(Flo[p]);
         Ft[p] = double(Fhi[p]);
       }
-      if (EK < 0 && g.has_interfaces && a.Fi) {
+      if (MP && g.has_interfaces && a.Fi) {
         // interface faces: the plus cell takes localflux2 of its low face, the minus cell
         // localflux1 of its high face (SpaceOperator.cpp:2548-2556)
         const int me = sID[ty + HALO][tx + HALO];
@@ -745,12 +745,12 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
 }
 
 #ifndef M2C_BATCHED
-template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double>
+template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double, bool MP = (EK < 0)>
 #ifndef M2C_STAGE_BLOCKS
 #define M2C_STAGE_BLOCKS 2   // resident 256-thread blocks per CU the register budget is sized for
 #endif
 __global__ __launch_bounds__(NTHR, M2C_STAGE_BLOCKS) void stage_kernel(StageArgs a) {
-  stage_body<IN_CONS, MODE, FLUX, EK, TC>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+  stage_body<IN_CONS, MODE, FLUX, EK, TC, MP>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 #endif
 

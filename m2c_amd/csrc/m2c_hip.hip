@@ -53,7 +53,8 @@ struct m2c_ctx {
   Clock *clk = nullptr;
   Status *st = nullptr;
   int kchunk = 1;
-  int variant = 1;             // 0 HLLC+SG, 1 HLLC generic EOS, 2 Roe, 3 LLF; +4: mixed precision; 8 Godunov
+  int variant = 1;             // 0 HLLC+SG, 1 HLLC generic EOS, 2 Roe, 3 LLF; +4: mixed precision; 8 Godunov;
+                               // 9 HLLC+SG with material interfaces, 10 its mixed form
   dim3 grid3;
   // two materials / level set
   bool iface = false;          // exact Riemann pass at material interfaces
@@ -271,7 +272,8 @@ static int upload_constants(m2c_ctx *c) {
       m2c_stage_upload_god_gen(c->grid, c->eos) ||
       m2c_stage_upload_roe_gen(c->grid, c->eos) || m2c_stage_upload_llf_gen(c->grid, c->eos) ||
       m2c_stage_upload_hllc_sg_f32(c->grid, c->eos) || m2c_stage_upload_hllc_gen_f32(c->grid, c->eos) ||
-      m2c_stage_upload_roe_gen_f32(c->grid, c->eos) || m2c_stage_upload_llf_gen_f32(c->grid, c->eos))
+      m2c_stage_upload_roe_gen_f32(c->grid, c->eos) || m2c_stage_upload_llf_gen_f32(c->grid, c->eos) ||
+      m2c_stage_upload_hllc_sg_mp(c->grid, c->eos) || m2c_stage_upload_hllc_sg_mp_f32(c->grid, c->eos))
     return M2C_ERR_HIP;
   g_resident = c;
   return M2C_OK;
@@ -625,6 +627,8 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
   case 5: rc = m2c_stage_launch_hllc_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
   case 6: rc = m2c_stage_launch_roe_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
   case 7: rc = m2c_stage_launch_llf_gen_f32(in_cons, mode, c->grid3, c->stream, s); break;
+  case 9: rc = m2c_stage_launch_hllc_sg_mp(in_cons, mode, c->grid3, c->stream, s); break;
+  case 10: rc = m2c_stage_launch_hllc_sg_mp_f32(in_cons, mode, c->grid3, c->stream, s); break;
   default: rc = m2c_stage_launch_god_gen(in_cons, mode, c->grid3, c->stream, s); break;
   }
   if (rc == 1) return M2C_ERR_ARG;
@@ -772,6 +776,7 @@ static int setup_multiphase(m2c_ctx *c) {
   for (int sd = 0; sd < 6; sd++) mp.ls_bc[sd] = cfg->ls_bc[sd];
   mp.ls_neumann = 1;
   for (int sd = 0; sd < 6; sd++) mp.ls_neumann &= (cfg->ls_bc[sd] == 1);
+  L.ls_neumann = mp.ls_neumann;
   mp.reinit_freq = cfg->reinit_freq;
   mp.reinit_maxits = cfg->reinit_maxits;
   mp.reinit_tol = cfg->reinit_tol;
@@ -957,9 +962,11 @@ int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   bool all_sg = true;
   for (int m = 0; m < cfg->nmat; m++) all_sg = all_sg && cfg->mat[m].eos == M2C_EOS_SG;
   int v;
-  if (cfg->flux == M2C_FLUX_HLLC) v = (all_sg && cfg->inactive_id < 0 && !g.has_interfaces) ? 0 : 1;
+  // HLLC over stiffened gases only: the compile-time stiffened-gas variants, with the material-interface
+  // code (9, 10) when several materials meet
+  if (cfg->flux == M2C_FLUX_HLLC) v = (all_sg && cfg->inactive_id < 0) ? (g.has_interfaces ? 9 : 0) : 1;
   else v = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
-  if (cfg->precision == M2C_PREC_MIXED) v += 4;
+  if (cfg->precision == M2C_PREC_MIXED) v = (v == 9) ? 10 : v + 4;
   if (cfg->flux == M2C_FLUX_GODUNOV) v = 8;
   *variant = v;
   return M2C_OK;

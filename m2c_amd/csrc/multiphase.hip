@@ -821,6 +821,9 @@ __global__ void k_reinit_begin(MpState *ms, const Clock *clk, int retry) {
   }
   R.retry = 0;
   R.done = 0;
+  R.cur = 0;
+  R.acc_res_bits = 0ull;
+  R.acc_dphi_bits = 0ull;
   R.iter = c_mp.reinit_maxits;
   R.maxres_bits = 0ull;
   R.dphi_bits = 0x7FF0000000000000ull;   // +inf
@@ -858,14 +861,13 @@ __global__ void k_reinit_restore(double *phi, const double *phibk, const MpState
     phi[idx] = 0.0 * phi[idx] + 1.0 * phibk[idx];
 }
 
-// pseudo-time residual of the reinitialisation equation (LevelSetReinitializer.cpp:953-1035)
-__device__ __forceinline__ double reinit_rhs(const double *phi, const uint8_t *tag, const double *sign, int i, int j,
-                                             int k, long long idx, bool inner, double cfl, double *lres) {
+// pseudo-time residual of the reinitialisation equation (LevelSetReinitializer.cpp:953-1035) of cell
+// (i,j,k) = idx from its value p and its six face neighbours n (ghosts already applied)
+__device__ __forceinline__ double reinit_rhs_nb(double p, const Nb6 &n, int tg, double sg, int i, int j, int k,
+                                                double cfl, double *lres) {
   *lres = 0.0;
-  if (tag[idx] != 0) return 0.0;
+  if (tg != 0) return 0.0;
   const double dt = cfl * c_mp.min_dx;
-  const double p = phi[idx];
-  const Nb6 n = phi_nb6(phi, i, j, k, idx, inner);
   double a, b, c, d, e, f;
   if (c_mp.dx_pow2[0]) { a = (p - n.im) * c_mp.inv_dx[0]; b = (n.ip - p) * c_mp.inv_dx[0]; }
   else { a = (p - n.im) / (XC(0, i) - XC(0, i - 1)); b = (n.ip - p) / (XC(0, i + 1) - XC(0, i)); }
@@ -882,8 +884,225 @@ __device__ __forceinline__ double reinit_rhs(const double *phi, const uint8_t *t
   else
     lr = dsqrt<double>(smax(am * am, bp * bp) + smax(cm * cm, dp * dp) + smax(em * em, fp * fp)) - 1.0;
   *lres = dabs(lr);
-  if (dabs(lr) <= 1.0) return -dt * sign[idx] * lr;
-  return -dt * sign[idx] * lr / dabs(lr);
+  if (dabs(lr) <= 1.0) return -dt * sg * lr;
+  return -dt * sg * lr / dabs(lr);
+}
+
+__device__ __forceinline__ double reinit_rhs(const double *phi, const uint8_t *tag, const double *sign, int i, int j,
+                                             int k, long long idx, bool inner, double cfl, double *lres) {
+  if (tag[idx] != 0) { *lres = 0.0; return 0.0; }
+  return reinit_rhs_nb(phi[idx], phi_nb6(phi, i, j, k, idx, inner), 0, sign[idx], i, j, k, cfl, lres);
+}
+
+// ---- one whole reinitialisation iteration per launch (zero-Neumann level-set boundaries)
+//
+// The three RK3 stages of an iteration in one pass: a block owns RF_TX x RF_TY columns and marches
+// up a chunk of RF_KC planes. Step jz stores one plane of phi (tile + 3-cell halo) and the sign and
+// first-layer tag of one plane (tile + 2) into LDS rings, then evaluates stage 1 at plane jz + 2
+// (tile + 2 halo), stage 2 at jz + 1 (tile + 1) and the new value at jz (tile), each stage reading its
+// input from the ring of the previous one; the next step's global loads are issued before this
+// step's arithmetic. Ghosts are the clamped image cells (the zero-Neumann rule of
+// LevelSetOperator.cpp:934-1033 applied to Phi, Phi1 and the result), so a neighbour outside the box
+// is read at its clamped index. The arithmetic per cell is reinit_rhs_nb with the stage combinations
+// of k_reinit_A/B/C; the result goes to the other buffer of a pair (the residual test of stage 1 may
+// discard the iteration).
+constexpr int RF_TX = 32, RF_TY = 16, RF_KC = 16, RF_NT = RF_TX * RF_TY;
+constexpr int RF_W0 = RF_TX + 6, RF_H0 = RF_TY + 6;
+constexpr int RF_W1 = RF_TX + 4, RF_H1 = RF_TY + 4;
+constexpr int RF_W2 = RF_TX + 2, RF_H2 = RF_TY + 2;
+constexpr int RF_R0 = (RF_W0 * RF_H0 + RF_NT - 1) / RF_NT;   // load rounds of a phi plane
+constexpr int RF_R1 = (RF_W1 * RF_H1 + RF_NT - 1) / RF_NT;   // ... of a sign / tag plane
+
+__global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0, double *__restrict__ P1,
+                                                         const uint8_t *__restrict__ tag,
+                                                         const double *__restrict__ sign, MpState *ms) {
+  ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected || Rs.done) return;
+  const Grid &g = c_grid;
+  const double *__restrict__ in = Rs.cur ? P1 : P0;
+  double *__restrict__ out = Rs.cur ? P0 : P1;
+  const double cfl = Rs.cfl;
+  __shared__ double r0[4][RF_H0][RF_W0];   // phi, planes by p & 3
+  __shared__ double r1[3][RF_H1][RF_W1];   // stage 1, planes by p % 3
+  __shared__ double r2[3][RF_H2][RF_W2];   // stage 2
+  __shared__ double rs[3][RF_H1][RF_W1];   // sign, planes by p % 3
+  __shared__ uint8_t rt[3][RF_H1][RF_W1];  // first-layer tag
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RF_TX, j0 = blockIdx.y * RF_TY;
+  const int kb = blockIdx.z * RF_KC, ke = min(kb + RF_KC, g.nz);
+  const long long plane = (long long)g.nx * g.ny;
+  double mres = 0.0, mdphi = 0.0;
+  // prefetched values of the next step: phi of plane jz + 3 (region 0), sign and tag of jz + 2 (region 1)
+  double pf0[RF_R0], pfs[RF_R1];
+  uint8_t pft[RF_R1];
+  auto fetch = [&](int jz) {
+    const int p0 = jz + 3, p1 = jz + 2;
+    const bool l0 = p0 >= max(kb - 3, 0) && p0 < g.nz, l1 = p1 >= max(kb - 2, 0) && p1 < g.nz;
+#pragma unroll
+    for (int q = 0; q < RF_R0; q++) {
+      const int r = tid + q * RF_NT;
+      const int x = r % RF_W0, y = r / RF_W0;
+      const int i = i0 - 3 + x, j = j0 - 3 + y;
+      pf0[q] = (l0 && r < RF_W0 * RF_H0 && i >= 0 && i < g.nx && j >= 0 && j < g.ny)
+                   ? in[p0 * plane + (long long)j * g.nx + i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < RF_R1; q++) {
+      const int r = tid + q * RF_NT;
+      const int x = r % RF_W1, y = r / RF_W1;
+      const int i = i0 - 2 + x, j = j0 - 2 + y;
+      const bool ok = l1 && r < RF_W1 * RF_H1 && i >= 0 && i < g.nx && j >= 0 && j < g.ny;
+      const long long idx = p1 * plane + (long long)j * g.nx + i;
+      pfs[q] = ok ? sign[idx] : 0.0;
+      pft[q] = ok ? tag[idx] : (uint8_t)0;
+    }
+  };
+  // value of stage field `ring` (halo h) at global (i, j) of ring plane slot q, clamped into the box
+#define RF_AT(ring, h, q, ii, jj) ring[q][clampi(jj, 0, g.ny - 1) - (j0 - h)][clampi(ii, 0, g.nx - 1) - (i0 - h)]
+  // step jz stores phi(jz + 3) and evaluates stage 1 at jz + 2, stage 2 at jz + 1 and the result at jz;
+  // the first steps fill the rings: phi from kb - 3, stage 1 from kb - 2, stage 2 from kb - 1
+  fetch(kb - 6);
+  for (int jz = kb - 6; jz < ke; jz++) {
+    {
+      const int p0 = jz + 3, p1 = jz + 2;
+      const int q0 = p0 & 3, q1 = ((p1 % 3) + 3) % 3;
+#pragma unroll
+      for (int q = 0; q < RF_R0; q++) {
+        const int r = tid + q * RF_NT;
+        if (r < RF_W0 * RF_H0) r0[q0][r / RF_W0][r % RF_W0] = pf0[q];
+      }
+#pragma unroll
+      for (int q = 0; q < RF_R1; q++) {
+        const int r = tid + q * RF_NT;
+        if (r < RF_W1 * RF_H1) { rs[q1][r / RF_W1][r % RF_W1] = pfs[q]; rt[q1][r / RF_W1][r % RF_W1] = pft[q]; }
+      }
+      (void)p0;
+    }
+    if (jz + 1 < ke) fetch(jz + 1);
+    __syncthreads();
+    // stage 1 at plane jz + 2: rphi1a = phi + 1.0 * R(phi)
+    const int p1 = jz + 2;
+    if (p1 >= max(kb - 2, 0) && p1 < g.nz) {
+      const int q = p1 % 3, qc = p1 & 3, qm = max(p1 - 1, 0) & 3, qp = min(p1 + 1, g.nz - 1) & 3;
+      for (int r = tid; r < RF_W1 * RF_H1; r += RF_NT) {
+        const int x = r % RF_W1, y = r / RF_W1;
+        const int i = i0 - 2 + x, j = j0 - 2 + y;
+        if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) continue;
+        const double pc = r0[qc][y + 1][x + 1];
+        Nb6 n;
+        n.im = RF_AT(r0, 3, qc, i - 1, j); n.ip = RF_AT(r0, 3, qc, i + 1, j);
+        n.jm = RF_AT(r0, 3, qc, i, j - 1); n.jp = RF_AT(r0, 3, qc, i, j + 1);
+        n.km = r0[qm][y + 1][x + 1]; n.kp = r0[qp][y + 1][x + 1];
+        double lres;
+        const double R = reinit_rhs_nb(pc, n, rt[q][y][x], rs[q][y][x], i, j, p1, cfl, &lres);
+        if (lres > mres) mres = lres;
+        r1[q][y][x] = pc + 1.0 * R;
+      }
+    }
+    __syncthreads();
+    // stage 2 at plane jz + 1: rphi1b = (0.25 rphi1a + 0.75 phi) + 0.25 R(rphi1a)
+    const int p2 = jz + 1;
+    if (p2 >= max(kb - 1, 0) && p2 < g.nz) {
+      const int qc = p2 % 3, qm = max(p2 - 1, 0) % 3, qp = min(p2 + 1, g.nz - 1) % 3;
+      const int q0 = p2 & 3;
+      for (int r = tid; r < RF_W2 * RF_H2; r += RF_NT) {
+        const int x = r % RF_W2, y = r / RF_W2;
+        const int i = i0 - 1 + x, j = j0 - 1 + y;
+        if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) continue;
+        const double pc = r1[qc][y + 1][x + 1];
+        Nb6 n;
+        n.im = RF_AT(r1, 2, qc, i - 1, j); n.ip = RF_AT(r1, 2, qc, i + 1, j);
+        n.jm = RF_AT(r1, 2, qc, i, j - 1); n.jp = RF_AT(r1, 2, qc, i, j + 1);
+        n.km = r1[qm][y + 1][x + 1]; n.kp = r1[qp][y + 1][x + 1];
+        double lres;
+        const double R = reinit_rhs_nb(pc, n, rt[qc][y + 1][x + 1], rs[qc][y + 1][x + 1], i, j, p2, cfl, &lres);
+        const double t = 0.25 * pc + 0.75 * r0[q0][y + 2][x + 2];
+        r2[qc][y][x] = t + 0.25 * R;
+      }
+    }
+    __syncthreads();
+    // the new value at plane jz: (1/3 phi + 2/3 rphi1b) + 2/3 R(rphi1b), and the relative change
+    if (jz >= kb) {
+      const int qc = jz % 3, qm = max(jz - 1, 0) % 3, qp = min(jz + 1, g.nz - 1) % 3, q0 = jz & 3;
+      for (int r = tid; r < RF_TX * RF_TY; r += RF_NT) {
+        const int x = r % RF_TX, y = r / RF_TX;
+        const int i = i0 + x, j = j0 + y;
+        if (i >= g.nx || j >= g.ny) continue;
+        const long long idx = jz * plane + (long long)j * g.nx + i;
+        const double pc = r2[qc][y + 1][x + 1];
+        Nb6 n;
+        n.im = RF_AT(r2, 1, qc, i - 1, j); n.ip = RF_AT(r2, 1, qc, i + 1, j);
+        n.jm = RF_AT(r2, 1, qc, i, j - 1); n.jp = RF_AT(r2, 1, qc, i, j + 1);
+        n.km = r2[qm][y + 1][x + 1]; n.kp = r2[qp][y + 1][x + 1];
+        double lres;
+        const double R = reinit_rhs_nb(pc, n, rt[qc][y + 2][x + 2], rs[qc][y + 2][x + 2], i, j, jz, cfl, &lres);
+        const double po = r0[q0][y + 3][x + 3];
+        const double t = 1.0 / 3.0 * po + 2.0 / 3.0 * pc;
+        const double pn = t + 2.0 / 3.0 * R;
+        out[idx] = pn;
+        double le = dabs(pn - po);
+        if (pn != 0) le /= dabs(pn);
+        mdphi = smax(mdphi, le);
+      }
+    }
+    __syncthreads();
+  }
+#undef RF_AT
+  // block maxima of the stage-1 residual and the relative change
+  __shared__ double red[2][RF_NT / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    mres = smax(mres, __shfl_xor(mres, off, 64));
+    mdphi = smax(mdphi, __shfl_xor(mdphi, off, 64));
+  }
+  if ((tid & 63) == 0) { red[0][tid >> 6] = mres; red[1][tid >> 6] = mdphi; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = red[0][0], b = red[1][0];
+    for (int w = 1; w < RF_NT / 64; w++) { a = smax(a, red[0][w]); b = smax(b, red[1][w]); }
+    if (a > 0.0) atomic_max_pos(&Rs.acc_res_bits, a);
+    atomic_max_pos(&Rs.acc_dphi_bits, b);
+  }
+}
+
+// after iteration `it` of k_reinit_fused: the residual test of stage 1 (below tolerance: the
+// iteration is discarded and the loop ends, LevelSetReinitializer.cpp:109-117), else the new values
+// become current and the relative-change test may end the loop (:146-151)
+__global__ void k_reinit_decide(MpState *ms, int it) {
+  ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected || Rs.done) return;
+  const double res = bits_to_d(Rs.acc_res_bits), dphi = bits_to_d(Rs.acc_dphi_bits);
+  Rs.acc_res_bits = 0ull;
+  Rs.acc_dphi_bits = 0ull;
+  if (res < c_mp.reinit_tol) { Rs.done = 1; Rs.iter = it; return; }
+  Rs.cur ^= 1;
+  Rs.dphi_bits = __double_as_longlong(dphi);
+  if (dphi < c_mp.reinit_tol) { Rs.done = 1; Rs.iter = it; }
+}
+
+// the failsafe of LevelSetReinitializer.cpp:144-169 after the fused iterations
+__global__ void k_reinit_end_fused(MpState *ms, Status *st) {
+  ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected) return;
+  if (!Rs.done) Rs.iter = c_mp.reinit_maxits;
+  ms->reinit_iters += Rs.iter;
+  if (Rs.iter < c_mp.reinit_maxits) return;
+  if (bits_to_d(Rs.dphi_bits) < 0.02) return;
+  if (Rs.cfl < 0.02) {
+    atomicOr(&st->bits, (unsigned)ST_REINIT_FAIL);
+    return;
+  }
+  Rs.cfl = Rs.cfl / 1.5;
+  Rs.retry = 1;
+}
+
+// the current buffer of the pair back into phi
+__global__ void k_reinit_settle(double *phi, const double *P1, const MpState *ms) {
+  const ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected || !Rs.cur) return;
+  const Grid &g = c_grid;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x)
+    phi[idx] = P1[idx];
 }
 
 // RK3 stage 1 of reinitialisation iteration `it`: rphi1a = (0*rphi1a + 1*phi) + 1*R(phi)
@@ -1703,6 +1922,16 @@ extern "C" int m2c_mp_launch_reinit(const MpLaunch &L, double *phi, int maxits, 
   hipLaunchKernelGGL(k_reinit_begin, dim3(1), dim3(1), 0, L.stream, L.ms, L.clk, retry);
   if (retry) hipLaunchKernelGGL(k_reinit_restore, gr, bl, 0, L.stream, phi, L.phibk, L.ms);
   else hipLaunchKernelGGL(k_reinit_tag, gs, bs, 0, L.stream, phi, L.tag, L.sign, L.phibk, L.ms);
+  if (L.ls_neumann) {   // every boundary zero-Neumann: one fused launch per iteration
+    const dim3 gf((L.nx + RF_TX - 1) / RF_TX, (L.ny + RF_TY - 1) / RF_TY, (L.nz + RF_KC - 1) / RF_KC);
+    for (int it = 0; it < maxits; it++) {
+      hipLaunchKernelGGL(k_reinit_fused, gf, dim3(RF_NT), 0, L.stream, phi, L.rphi1a, L.tag, L.sign, L.ms);
+      hipLaunchKernelGGL(k_reinit_decide, dim3(1), dim3(1), 0, L.stream, L.ms, it);
+    }
+    hipLaunchKernelGGL(k_reinit_end_fused, dim3(1), dim3(1), 0, L.stream, L.ms, L.st);
+    hipLaunchKernelGGL(k_reinit_settle, gr, bl, 0, L.stream, phi, L.rphi1a, L.ms);
+    return hchk(hipGetLastError());
+  }
   for (int it = 0; it < maxits; it++) {
     hipLaunchKernelGGL(k_reinit_A, gs, bs, 0, L.stream, phi, L.rphi1a, L.tag, L.sign, L.ms, it);
     hipLaunchKernelGGL(k_reinit_B, gs, bs, 0, L.stream, phi, L.rphi1a, L.rphi1b, L.tag, L.sign, L.ms, it);

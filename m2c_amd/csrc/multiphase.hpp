@@ -38,9 +38,11 @@ struct MpParams {
 };
 
 struct ReinitState {
-  int active, detected, done, iter, retry, pad;
+  int active, detected, done, iter, retry;
+  int cur;                     // fused iterations: the buffer of the pair (phi, rphi1a) holding phi
   double cfl;                  // persists between calls (LevelSetReinitializer::cfl)
   unsigned long long maxres_bits, dphi_bits;
+  unsigned long long acc_res_bits, acc_dphi_bits;   // fused iterations: maxima of the running one
 };
 
 // device-resident per-step counters
@@ -60,6 +62,7 @@ struct MpLaunch {
   long long N, Nf;
   int nx, ny, nz;
   int flux, riemann_threads;
+  int ls_neumann;              // every level-set boundary zero-Neumann (the fused reinitialisation applies)
   int8_t *ID, *IDnew;
   unsigned *faces;
   MpState *ms;

@@ -20,3 +20,5 @@ M2C_DECLARE_VARIANT(hllc_gen_f32)
 M2C_DECLARE_VARIANT(roe_gen_f32)
 M2C_DECLARE_VARIANT(llf_gen_f32)
 M2C_DECLARE_VARIANT(god_gen)
+M2C_DECLARE_VARIANT(hllc_sg_mp)
+M2C_DECLARE_VARIANT(hllc_sg_mp_f32)

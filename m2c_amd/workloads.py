@@ -238,8 +238,9 @@ under Time {{ Type = Explicit; CFL = 0.5; under Explicit {{ Type = RungeKutta2; 
 BATCH_SEED = 4242
 
 
-def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0), p_range=(0.1, 100.0)):
-    """Returns a list of (settings, V, ID), one per instance.
+def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0), p_range=(0.1, 100.0), select=None):
+    """Returns a list of (settings, V, ID), one per instance (with `select`, only the instances of
+    those indices, drawn from the same random stream as the whole set).
 
     CFL is 0.4: the time step follows SpaceOperator.cpp:2287, dt = CFL * min over the three
     directions of dx/lambda, so an oblique wave sees the sum of the three directional Courant
@@ -247,13 +248,15 @@ def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0),
     oracle and the GPU agree on the blow-up)."""
     rng = np.random.default_rng(seed)
     out = []
-    for _ in range(n):
+    for q in range(n):
         nx, ny, nz = (int(v) for v in rng.integers(lo, hi + 1, size=3))
         normal = rng.normal(size=3)
         normal /= np.linalg.norm(normal)
         point = rng.uniform(0.0, 1.0, size=3) * np.array([nx, ny, nz])
         rho = np.exp(rng.uniform(np.log(rho_range[0]), np.log(rho_range[1]), size=2))
         p = np.exp(rng.uniform(np.log(p_range[0]), np.log(p_range[1]), size=2))
+        if select is not None and q not in select:
+            continue
         settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, float(nx), 0.0, float(ny), 0.0, float(nz)),
                         bc=[BC_SYMMETRY] * 6, mats=[material(EOS_SG, [1.4, 0.0])], flux=FLUX_HLLC,
                         limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.4)
