@@ -1,14 +1,20 @@
 """The mixed-precision form (FP32 reconstruction, limiters, face-state checks and fluxes; FP64
 fetch with ConservativeToPrimitive and its p(e) inversion, residual accumulation, RK stage
-combination, clipping, CFL, exact interface Riemann solves): L1 error per field against the FP64
-path, which is itself bit-identical to the oracle. The bar is relative L1 <= 1e-5 on density."""
+combination, clipping, CFL, exact interface Riemann solves) against the FP64 oracle (van Albada
+powers as dq*dq) on the same inputs: relative L1 and L-infinity per field, velocity components
+measured against the norms of the whole velocity (a component that is zero by symmetry would
+otherwise divide by ~0). The bar is relative L1 <= 1e-5 on every field; L-infinity is reported and
+held to 1e-4 (observed: a few 1e-6). The FP64 GPU run is checked to be the oracle's bit for bit."""
 import numpy as np
 import pytest
 
 import m2c_amd as M
 from m2c_amd import workloads as W
+from tests import cases
 
 pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "p")
 
 
 def run(settings, V0, ID0, phi0, steps, prec):
@@ -22,13 +28,19 @@ def run(settings, V0, ID0, phi0, steps, prec):
     return st, V, ID
 
 
-def l1(a, b):
-    den = np.abs(b).sum()
-    return float(np.abs(a - b).sum() / den) if den > 0 else float(np.abs(a - b).sum())
+def errors(V, Vo):
+    vel1, velinf = np.abs(Vo[..., 1:4]).sum(), np.abs(Vo[..., 1:4]).max()
+    out = {}
+    for f, name in enumerate(FIELDS):
+        d = np.abs(V[..., f] - Vo[..., f])
+        n1 = vel1 if 1 <= f <= 3 else np.abs(Vo[..., f]).sum()
+        ninf = velinf if 1 <= f <= 3 else np.abs(Vo[..., f]).max()
+        out[name] = (float(d.sum() / n1), float(d.max() / ninf))
+    return out
 
 
 @pytest.mark.parametrize("case", ["periodic", "shock_bubble"])
-def test_mixed_form_l1_error(case):
+def test_mixed_form_error_against_oracle(case):
     if case == "periodic":
         settings, V0, ID0 = W.periodic_modes(n=64)
         phi0, steps = None, 30
@@ -39,10 +51,15 @@ def test_mixed_form_l1_error(case):
     s64, V64, ID64 = run(settings, V0, ID0, phi0, steps, M.PREC_FP64)
     s32, V32, ID32 = run(settings, V0, ID0, phi0, steps, M.PREC_MIXED)
     assert s64.error == 0 and s32.error == 0
-    errs = [l1(V32[..., f], V64[..., f]) for f in range(5)]
-    print(f"  L1 rel per field {['%.2e' % e for e in errs]}, t {s64.t:.9e} vs {s32.t:.9e}, "
-          f"IDs differ {int((ID32 != ID64).sum())}", flush=True)
-    assert errs[0] <= 1e-5
-    assert errs[4] <= 1e-5
-    assert abs(s32.t - s64.t) <= 1e-5 * s64.t
+    r = cases.run_oracle(settings, V0, ID0, maxsteps=steps, phi0=phi0, vanalbada_pow=False)
+    ost, ot, on, Vo = r[0], r[1], r[2], r[3]
+    assert ost == 0 and on == steps
+    assert np.array_equal(V64, Vo) and s64.t == ot
+    e = errors(V32, Vo)
+    print("  vs oracle L1/Linf: " + ", ".join(f"{k} {a:.2e}/{b:.2e}" for k, (a, b) in e.items())
+          + f"; t {s32.t:.9e} vs {ot:.9e}, IDs differ {int((ID32 != ID64).sum())}", flush=True)
+    for name, (a, b) in e.items():
+        assert a <= 1e-5, name
+        assert b <= 1e-4, name
+    assert abs(s32.t - ot) <= 1e-5 * ot
     assert int((ID32 != ID64).sum()) <= max(1, int(0.01 * (ID64 != ID0).sum()))

@@ -45,7 +45,8 @@ def test_forward_matches_oracle():
     Yx, _ = R.forward(p.astype(np.float64), X.astype(np.float64), emulate=False)
     scale = 1.0 + np.abs(Ye).max()
     print(f"  forward: max |gpu - oracle(bf16)| {np.abs(Yg - Ye).max():.3e}, vs fp64 {np.abs(Yg - Yx).max():.3e}")
-    assert np.abs(Yg - Ye).max() <= 2e-3 * scale
+    # observed 1.4e-5: the bar is about ten times that
+    assert np.abs(Yg - Ye).max() <= 2e-4 * scale
     assert np.abs(Yg - Yx).max() <= 5e-2 * scale
     net.close()
 
@@ -62,14 +63,14 @@ def test_gradient_matches_oracle():
     le, ge = R.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64), emulate=True)
     lx, gx = R.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64), emulate=False)
     print(f"  loss gpu {lg:.6f} oracle(bf16) {le:.6f} fp64 {lx:.6f}")
-    assert abs(lg - le) <= 1e-3 * le
+    assert abs(lg - le) <= 2e-4 * le
     for l, ((wo, bo), (no, ni)) in enumerate(zip(R.OFFS, R.DIMS)):
         for name, sl in (("W", slice(wo, bo)), ("b", slice(bo, bo + no))):
             a, e = gg[sl].astype(np.float64), ge[sl]
             rel = np.linalg.norm(a - e) / max(np.linalg.norm(e), 1e-30)
             relx = np.linalg.norm(a - gx[sl]) / max(np.linalg.norm(gx[sl]), 1e-30)
             print(f"  layer {l} {name}: rel L2 vs oracle(bf16) {rel:.2e}, vs fp64 {relx:.2e}")
-            assert rel <= 1e-2
+            assert rel <= 3e-3   # observed at most 3.3e-4
             assert relx <= 6e-2
     net.close()
 
