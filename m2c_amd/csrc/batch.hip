@@ -321,21 +321,26 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
   }
   // HLLC (0, 1), Roe (2) and LLF (3) in FP64, HLLC in the mixed form (4, 5)
   if (b->variant < 0 || b->variant > 5) { delete b; return M2C_ERR_UNSUPPORTED; }
-  BHIP(hipSetDevice(cfgs[0].device));
-  BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  // from here on a failure releases everything allocated so far
+#define BHIP_D(x)                                                                                  \
+  do {                                                                                             \
+    if ((x) != hipSuccess) { m2c_batch_destroy(b); return M2C_ERR_HIP; }                            \
+  } while (0)
+  BHIP_D(hipSetDevice(cfgs[0].device));
+  BHIP_D(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   size_t bytes = sizeof(double) * 5 * (size_t)b->cells;
-  BHIP(hipMalloc(&b->V, bytes));
-  BHIP(hipMalloc(&b->U1, bytes));
-  if (b->nstage == 3) BHIP(hipMalloc(&b->U2, bytes));
-  BHIP(hipMalloc(&b->stage_buf, bytes));
-  BHIP(hipMalloc(&b->ID, (size_t)b->cells));
-  BHIP(hipMemset(b->ID, 0, (size_t)b->cells));
-  BHIP(hipMemset(b->V, 0, bytes));
-  BHIP(hipMalloc(&b->clk, sizeof(Clock) * n));
-  BHIP(hipMalloc(&b->st, sizeof(Status) * n));
-  BHIP(hipMemset(b->st, 0, sizeof(Status) * n));
-  BHIP(hipMalloc(&b->dinst, sizeof(BatchInst) * n));
-  BHIP(hipMalloc(&b->q, sizeof(TileQueue)));
+  BHIP_D(hipMalloc(&b->V, bytes));
+  BHIP_D(hipMalloc(&b->U1, bytes));
+  if (b->nstage == 3) BHIP_D(hipMalloc(&b->U2, bytes));
+  BHIP_D(hipMalloc(&b->stage_buf, bytes));
+  BHIP_D(hipMalloc(&b->ID, (size_t)b->cells));
+  BHIP_D(hipMemset(b->ID, 0, (size_t)b->cells));
+  BHIP_D(hipMemset(b->V, 0, bytes));
+  BHIP_D(hipMalloc(&b->clk, sizeof(Clock) * n));
+  BHIP_D(hipMalloc(&b->st, sizeof(Status) * n));
+  BHIP_D(hipMemset(b->st, 0, sizeof(Status) * n));
+  BHIP_D(hipMalloc(&b->dinst, sizeof(BatchInst) * n));
+  BHIP_D(hipMalloc(&b->q, sizeof(TileQueue)));
   // stage arguments of every instance (the same buffers as the single-instance context)
   for (int i = 0; i < n; i++) {
     BatchInst &B = b->hinst[i];
@@ -362,8 +367,8 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
       a2.S = U2; a2.Vn = V; a2.Us = U2; a2.Out = V; a2.a = 2.0 / 3.0; a2.b = 1.0 / 3.0; a2.c = 2.0 / 3.0;
     }
   }
-  if (int rc = batch_upload_tables(b)) return rc;
-  if (int rc = batch_layout(b, 8)) return rc;
+  if (int rc = batch_upload_tables(b)) { m2c_batch_destroy(b); return rc; }
+  if (int rc = batch_layout(b, 8)) { m2c_batch_destroy(b); return rc; }
   b->nblocks = 512;
   hipLaunchKernelGGL(k_batch_clock_init, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->clk, n, cfgs[0].cfl);
   for (int i = 0; i < n; i++)
@@ -373,21 +378,22 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
       h.cfl = cfgs[i].cfl;
       h.minbits = 0x7FEFFFFFFFFFFFFFull;
       h.active = 1;
-      BHIP(hipMemcpyAsync(b->clk + i, &h, sizeof(h), hipMemcpyHostToDevice, b->stream));
+      BHIP_D(hipMemcpyAsync(b->clk + i, &h, sizeof(h), hipMemcpyHostToDevice, b->stream));
     }
-  BHIP(hipStreamSynchronize(b->stream));
+  BHIP_D(hipStreamSynchronize(b->stream));
+#undef BHIP_D
   *out = b;
   return M2C_OK;
 }
 
 int m2c_batch_destroy(m2c_batch *b) {
   if (!b) return M2C_OK;
-  hipStreamSynchronize(b->stream);
+  if (b->stream) hipStreamSynchronize(b->stream);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   hipFree(b->V); hipFree(b->U1); hipFree(b->U2); hipFree(b->stage_buf); hipFree(b->backup);
   hipFree(b->ID); hipFree(b->clk); hipFree(b->st); hipFree(b->dinst); hipFree(b->q);
-  hipStreamDestroy(b->stream);
+  if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return M2C_OK;
 }
@@ -397,17 +403,16 @@ int m2c_batch_upload(m2c_batch *b, int i, const double *V, const int32_t *id) {
   BHIP(hipSetDevice(b->cfg[0].device));
   const BatchInst &B = b->hinst[i];
   const long long N = B.grid.N;
+  // the IDs are checked before anything is copied: M2C_ERR_ARG leaves the instance as it was
+  std::vector<int8_t> id8(id ? (size_t)N : 0);
+  for (long long c = 0; id && c < N; c++) {
+    if (id[c] < 0 || id[c] >= b->cfg[i].nmat) return M2C_ERR_ARG;
+    id8[c] = (int8_t)id[c];
+  }
   BHIP(hipMemcpyAsync(b->stage_buf, V, sizeof(double) * 5 * N, hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_batch_aos_to_soa, dim3(1024), dim3(256), 0, b->stream, b->stage_buf, b->V + 5 * B.cell0, N);
-  if (id) {
-    std::vector<int8_t> id8((size_t)N);
-    for (long long c = 0; c < N; c++) {
-      if (id[c] < 0 || id[c] >= b->cfg[i].nmat) return M2C_ERR_ARG;
-      id8[c] = (int8_t)id[c];
-    }
-    BHIP(hipMemcpyAsync(b->ID + B.cell0, id8.data(), (size_t)N, hipMemcpyHostToDevice, b->stream));
-    BHIP(hipStreamSynchronize(b->stream));
-  }
+  BHIP(hipGetLastError());
+  if (id) BHIP(hipMemcpyAsync(b->ID + B.cell0, id8.data(), (size_t)N, hipMemcpyHostToDevice, b->stream));
   BHIP(hipStreamSynchronize(b->stream));
   return M2C_OK;
 }

@@ -134,6 +134,8 @@ __global__ void k_clock_fixed_dt(Clock *clk, double dt) {
 }
 
 // dx/(|u|+c) minimum over the active cells (SpaceOperator.cpp:2128-2198)
+// EK: the equation of state when every material has it (stiffened gas), -1 = dispatch per cell
+template <int EK>
 __global__ __launch_bounds__(256) void k_cfl_min(const double *__restrict__ V, const int8_t *__restrict__ ID,
                                                  Clock *clk, Status *st) {
   const Grid &g = c_grid;
@@ -147,14 +149,22 @@ __global__ __launch_bounds__(256) void k_cfl_min(const double *__restrict__ V, c
     double v[5];
 #pragma unroll
     for (int p = 0; p < 5; p++) v[p] = V[p * g.N + idx];
-    double c2 = eos_c2<double, -1>(m, v[0], eos_e<double, -1>(m, v[0], v[4]));
+    double c2 = eos_c2<double, EK>(m, v[0], eos_e<double, EK>(m, v[0], v[4]));
     if (c2 < 0) { flags |= ST_NEG_C2; continue; }
     double cs = dsqrt(c2);
     double lf = dabs(v[1]) + cs, lg = dabs(v[2]) + cs, lh = dabs(v[3]) + cs;
     double loc = smin(g.dx / lf, smin(g.dy / lg, g.dz / lh));
     if (loc < mn) mn = loc;
   }
-  warp_min_atomic(&clk->minbits, mn);
+  // block minimum first: one atomic per block on the single address
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) mn = smin(mn, __shfl_xor(mn, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double m = smin(smin(red[0], red[1]), smin(red[2], red[3]));
+    atomicMin(&clk->minbits, (unsigned long long)__double_as_longlong(m));
+  }
   if (flags) atomicOr(&st->bits, flags);
 }
 
@@ -744,7 +754,10 @@ static int nstages(const m2c_ctx *c) { return c->cfg.ts == M2C_TS_FE ? 1 : (c->c
 
 static void launch_cfl(m2c_ctx *c) {
   int nb = (int)std::min<long long>((c->N + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_cfl_min, dim3(nb), dim3(256), 0, c->stream, c->V, c->ID, c->clk, c->st);
+  // the stiffened-gas kernel variants (0, 4, 9, 10) have only stiffened gases and no inactive material
+  const bool sg = c->variant == 0 || c->variant == 4 || c->variant == 9 || c->variant == 10;
+  if (sg) hipLaunchKernelGGL(k_cfl_min<EOS_SG>, dim3(nb), dim3(256), 0, c->stream, c->V, c->ID, c->clk, c->st);
+  else hipLaunchKernelGGL(k_cfl_min<-1>, dim3(nb), dim3(256), 0, c->stream, c->V, c->ID, c->clk, c->st);
 }
 
 
@@ -822,6 +835,12 @@ static int setup_multiphase(m2c_ctx *c) {
     if ((rc = mp_alloc(c, &L.phibk, (size_t)N, 0))) return rc;
     if ((rc = mp_alloc(c, &L.rphi1a, (size_t)N, 0))) return rc;
     if ((rc = mp_alloc(c, &L.rphi1b, (size_t)N, 0))) return rc;
+    if (cfg->precision == M2C_PREC_MIXED && L.ls_neumann) {   // the mixed form reinitialises in FP32
+      L.ls_f32 = 1;
+      if ((rc = mp_alloc(c, &L.phif0, (size_t)N, 0))) return rc;
+      if ((rc = mp_alloc(c, &L.phif1, (size_t)N, 0))) return rc;
+      if ((rc = mp_alloc(c, &L.signf, (size_t)N, 0))) return rc;
+    }
     if ((rc = mp_alloc(c, &L.chg, (size_t)N, 0))) return rc;
     if ((rc = mp_alloc(c, &L.cidx, (size_t)N, 0xff))) return rc;
     if ((rc = mp_alloc(c, &L.lvl, (size_t)N, 0))) return rc;

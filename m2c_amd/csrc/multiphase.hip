@@ -25,6 +25,7 @@
 // rounds, then an ordered single-thread pass for any chain that is still open), which reproduces
 // the in-order result exactly.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "fv_kernels.hpp"
 #include "multiphase.hpp"
@@ -681,8 +682,11 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *S, const int8_t 
     cell_velocity<IN_CONS>(S, IDn, idx, u);
     const bool inner2 = i >= 2 && i < g.nx - 2 && j >= 2 && j < g.ny - 2 && k >= 2 && k < g.nz - 2;
     double R;
-    if (c_mp.ls_neumann) R = ls_residual<2>(phiS, i, j, k, idx, inner2, u);
-    else R = inner2 ? ls_residual<1>(phiS, i, j, k, idx, true, u) : ls_residual<0>(phiS, i, j, k, idx, false, u);
+    // cells whose stencil stays in the box read their neighbours at fixed offsets (the same values the
+    // ghost rules would give them)
+    if (inner2) R = ls_residual<1>(phiS, i, j, k, idx, true, u);
+    else if (c_mp.ls_neumann) R = ls_residual<2>(phiS, i, j, k, idx, false, u);
+    else R = ls_residual<0>(phiS, i, j, k, idx, false, u);
     double t;
     if (MODE == LS_FIRST) {
       double x = X[idx];
@@ -822,6 +826,7 @@ __global__ void k_reinit_begin(MpState *ms, const Clock *clk, int retry) {
   R.retry = 0;
   R.done = 0;
   R.cur = 0;
+  R.accepted = 0;
   R.acc_res_bits = 0ull;
   R.acc_dphi_bits = 0ull;
   R.iter = c_mp.reinit_maxits;
@@ -863,29 +868,38 @@ __global__ void k_reinit_restore(double *phi, const double *phibk, const MpState
 
 // pseudo-time residual of the reinitialisation equation (LevelSetReinitializer.cpp:953-1035) of cell
 // (i,j,k) = idx from its value p and its six face neighbours n (ghosts already applied)
+// T = double: the drop-in arithmetic; T = float: the mixed form's reinitialisation
+template <typename T> struct NbT { T im, ip, jm, jp, km, kp; };
+
+template <typename T>
+__device__ __forceinline__ T reinit_rhs_t(T p, const NbT<T> &n, int tg, T sg, int i, int j, int k, T cfl, T *lres) {
+  *lres = T(0);
+  if (tg != 0) return T(0);
+  const T dt = cfl * T(c_mp.min_dx);
+  T a, b, c, d, e, f;
+  if (c_mp.dx_pow2[0]) { a = (p - n.im) * T(c_mp.inv_dx[0]); b = (n.ip - p) * T(c_mp.inv_dx[0]); }
+  else { a = (p - n.im) / T(XC(0, i) - XC(0, i - 1)); b = (n.ip - p) / T(XC(0, i + 1) - XC(0, i)); }
+  if (c_mp.dx_pow2[1]) { c = (p - n.jm) * T(c_mp.inv_dx[1]); d = (n.jp - p) * T(c_mp.inv_dx[1]); }
+  else { c = (p - n.jm) / T(XC(1, j) - XC(1, j - 1)); d = (n.jp - p) / T(XC(1, j + 1) - XC(1, j)); }
+  if (c_mp.dx_pow2[2]) { e = (p - n.km) * T(c_mp.inv_dx[2]); f = (n.kp - p) * T(c_mp.inv_dx[2]); }
+  else { e = (p - n.km) / T(XC(2, k) - XC(2, k - 1)); f = (n.kp - p) / T(XC(2, k + 1) - XC(2, k)); }
+  T ap = smax(a, T(0)), am = smin(a, T(0)), bp = smax(b, T(0)), bm = smin(b, T(0));
+  T cp = smax(c, T(0)), cm = smin(c, T(0)), dp = smax(d, T(0)), dm = smin(d, T(0));
+  T ep = smax(e, T(0)), em = smin(e, T(0)), fp = smax(f, T(0)), fm = smin(f, T(0));
+  T lr;
+  if (p >= T(0))
+    lr = dsqrt<T>(smax(ap * ap, bm * bm) + smax(cp * cp, dm * dm) + smax(ep * ep, fm * fm)) - T(1);
+  else
+    lr = dsqrt<T>(smax(am * am, bp * bp) + smax(cm * cm, dp * dp) + smax(em * em, fp * fp)) - T(1);
+  *lres = dabs(lr);
+  if (dabs(lr) <= T(1)) return -dt * sg * lr;
+  return -dt * sg * lr / dabs(lr);
+}
+
 __device__ __forceinline__ double reinit_rhs_nb(double p, const Nb6 &n, int tg, double sg, int i, int j, int k,
                                                 double cfl, double *lres) {
-  *lres = 0.0;
-  if (tg != 0) return 0.0;
-  const double dt = cfl * c_mp.min_dx;
-  double a, b, c, d, e, f;
-  if (c_mp.dx_pow2[0]) { a = (p - n.im) * c_mp.inv_dx[0]; b = (n.ip - p) * c_mp.inv_dx[0]; }
-  else { a = (p - n.im) / (XC(0, i) - XC(0, i - 1)); b = (n.ip - p) / (XC(0, i + 1) - XC(0, i)); }
-  if (c_mp.dx_pow2[1]) { c = (p - n.jm) * c_mp.inv_dx[1]; d = (n.jp - p) * c_mp.inv_dx[1]; }
-  else { c = (p - n.jm) / (XC(1, j) - XC(1, j - 1)); d = (n.jp - p) / (XC(1, j + 1) - XC(1, j)); }
-  if (c_mp.dx_pow2[2]) { e = (p - n.km) * c_mp.inv_dx[2]; f = (n.kp - p) * c_mp.inv_dx[2]; }
-  else { e = (p - n.km) / (XC(2, k) - XC(2, k - 1)); f = (n.kp - p) / (XC(2, k + 1) - XC(2, k)); }
-  double ap = smax(a, 0.0), am = smin(a, 0.0), bp = smax(b, 0.0), bm = smin(b, 0.0);
-  double cp = smax(c, 0.0), cm = smin(c, 0.0), dp = smax(d, 0.0), dm = smin(d, 0.0);
-  double ep = smax(e, 0.0), em = smin(e, 0.0), fp = smax(f, 0.0), fm = smin(f, 0.0);
-  double lr;
-  if (p >= 0)
-    lr = dsqrt<double>(smax(ap * ap, bm * bm) + smax(cp * cp, dm * dm) + smax(ep * ep, fm * fm)) - 1.0;
-  else
-    lr = dsqrt<double>(smax(am * am, bp * bp) + smax(cm * cm, dp * dp) + smax(em * em, fp * fp)) - 1.0;
-  *lres = dabs(lr);
-  if (dabs(lr) <= 1.0) return -dt * sg * lr;
-  return -dt * sg * lr / dabs(lr);
+  const NbT<double> m = {n.im, n.ip, n.jm, n.jp, n.km, n.kp};
+  return reinit_rhs_t<double>(p, m, tg, sg, i, j, k, cfl, lres);
 }
 
 __device__ __forceinline__ double reinit_rhs(const double *phi, const uint8_t *tag, const double *sign, int i, int j,
@@ -913,27 +927,28 @@ constexpr int RF_W2 = RF_TX + 2, RF_H2 = RF_TY + 2;
 constexpr int RF_R0 = (RF_W0 * RF_H0 + RF_NT - 1) / RF_NT;   // load rounds of a phi plane
 constexpr int RF_R1 = (RF_W1 * RF_H1 + RF_NT - 1) / RF_NT;   // ... of a sign / tag plane
 
-__global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0, double *__restrict__ P1,
+template <typename T>
+__global__ __launch_bounds__(RF_NT) void k_reinit_fused(T *__restrict__ P0, T *__restrict__ P1,
                                                          const uint8_t *__restrict__ tag,
-                                                         const double *__restrict__ sign, MpState *ms) {
+                                                         const T *__restrict__ sign, MpState *ms) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || Rs.done) return;
   const Grid &g = c_grid;
-  const double *__restrict__ in = Rs.cur ? P1 : P0;
-  double *__restrict__ out = Rs.cur ? P0 : P1;
-  const double cfl = Rs.cfl;
-  __shared__ double r0[4][RF_H0][RF_W0];   // phi, planes by p & 3
-  __shared__ double r1[3][RF_H1][RF_W1];   // stage 1, planes by p % 3
-  __shared__ double r2[3][RF_H2][RF_W2];   // stage 2
-  __shared__ double rs[3][RF_H1][RF_W1];   // sign, planes by p % 3
+  const T *__restrict__ in = Rs.cur ? P1 : P0;
+  T *__restrict__ out = Rs.cur ? P0 : P1;
+  const T cfl = T(Rs.cfl);
+  __shared__ T r0[4][RF_H0][RF_W0];        // phi, planes by p & 3
+  __shared__ T r1[3][RF_H1][RF_W1];        // stage 1, planes by p % 3
+  __shared__ T r2[3][RF_H2][RF_W2];        // stage 2
+  __shared__ T rs[3][RF_H1][RF_W1];        // sign, planes by p % 3
   __shared__ uint8_t rt[3][RF_H1][RF_W1];  // first-layer tag
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * RF_TX, j0 = blockIdx.y * RF_TY;
   const int kb = blockIdx.z * RF_KC, ke = min(kb + RF_KC, g.nz);
   const long long plane = (long long)g.nx * g.ny;
-  double mres = 0.0, mdphi = 0.0;
+  T mres = T(0), mdphi = T(0);
   // prefetched values of the next step: phi of plane jz + 3 (region 0), sign and tag of jz + 2 (region 1)
-  double pf0[RF_R0], pfs[RF_R1];
+  T pf0[RF_R0], pfs[RF_R1];
   uint8_t pft[RF_R1];
   auto fetch = [&](int jz) {
     const int p0 = jz + 3, p1 = jz + 2;
@@ -944,7 +959,7 @@ __global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0,
       const int x = r % RF_W0, y = r / RF_W0;
       const int i = i0 - 3 + x, j = j0 - 3 + y;
       pf0[q] = (l0 && r < RF_W0 * RF_H0 && i >= 0 && i < g.nx && j >= 0 && j < g.ny)
-                   ? in[p0 * plane + (long long)j * g.nx + i] : 0.0;
+                   ? in[p0 * plane + (long long)j * g.nx + i] : T(0);
     }
 #pragma unroll
     for (int q = 0; q < RF_R1; q++) {
@@ -953,12 +968,18 @@ __global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0,
       const int i = i0 - 2 + x, j = j0 - 2 + y;
       const bool ok = l1 && r < RF_W1 * RF_H1 && i >= 0 && i < g.nx && j >= 0 && j < g.ny;
       const long long idx = p1 * plane + (long long)j * g.nx + i;
-      pfs[q] = ok ? sign[idx] : 0.0;
+      pfs[q] = ok ? sign[idx] : T(0);
       pft[q] = ok ? tag[idx] : (uint8_t)0;
     }
   };
-  // value of stage field `ring` (halo h) at global (i, j) of ring plane slot q, clamped into the box
-#define RF_AT(ring, h, q, ii, jj) ring[q][clampi(jj, 0, g.ny - 1) - (j0 - h)][clampi(ii, 0, g.nx - 1) - (i0 - h)]
+  // value of stage field `ring` (halo h) at global (i, j) of ring plane slot q, clamped into the box (a
+  // block whose 3-cell halo lies inside the box needs no clamping and skips no cell)
+#define RF_AT(ring, h, q, ii, jj)                                                                       \
+  (EDGE ? ring[q][clampi(jj, 0, g.ny - 1) - (j0 - h)][clampi(ii, 0, g.nx - 1) - (i0 - h)]              \
+        : ring[q][(jj) - (j0 - h)][(ii) - (i0 - h)])
+  const bool edge = i0 < 3 || j0 < 3 || i0 + RF_TX + 3 > g.nx || j0 + RF_TY + 3 > g.ny;
+  auto march = [&](auto edge_c) {
+  constexpr bool EDGE = decltype(edge_c)::value;
   // step jz stores phi(jz + 3) and evaluates stage 1 at jz + 2, stage 2 at jz + 1 and the result at jz;
   // the first steps fill the rings: phi from kb - 3, stage 1 from kb - 2, stage 2 from kb - 1
   fetch(kb - 6);
@@ -987,16 +1008,16 @@ __global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0,
       for (int r = tid; r < RF_W1 * RF_H1; r += RF_NT) {
         const int x = r % RF_W1, y = r / RF_W1;
         const int i = i0 - 2 + x, j = j0 - 2 + y;
-        if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) continue;
-        const double pc = r0[qc][y + 1][x + 1];
-        Nb6 n;
+        if (EDGE && (i < 0 || i >= g.nx || j < 0 || j >= g.ny)) continue;
+        const T pc = r0[qc][y + 1][x + 1];
+        NbT<T> n;
         n.im = RF_AT(r0, 3, qc, i - 1, j); n.ip = RF_AT(r0, 3, qc, i + 1, j);
         n.jm = RF_AT(r0, 3, qc, i, j - 1); n.jp = RF_AT(r0, 3, qc, i, j + 1);
         n.km = r0[qm][y + 1][x + 1]; n.kp = r0[qp][y + 1][x + 1];
-        double lres;
-        const double R = reinit_rhs_nb(pc, n, rt[q][y][x], rs[q][y][x], i, j, p1, cfl, &lres);
+        T lres;
+        const T R = reinit_rhs_t<T>(pc, n, rt[q][y][x], rs[q][y][x], i, j, p1, cfl, &lres);
         if (lres > mres) mres = lres;
-        r1[q][y][x] = pc + 1.0 * R;
+        r1[q][y][x] = pc + T(1) * R;
       }
     }
     __syncthreads();
@@ -1008,16 +1029,16 @@ __global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0,
       for (int r = tid; r < RF_W2 * RF_H2; r += RF_NT) {
         const int x = r % RF_W2, y = r / RF_W2;
         const int i = i0 - 1 + x, j = j0 - 1 + y;
-        if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) continue;
-        const double pc = r1[qc][y + 1][x + 1];
-        Nb6 n;
+        if (EDGE && (i < 0 || i >= g.nx || j < 0 || j >= g.ny)) continue;
+        const T pc = r1[qc][y + 1][x + 1];
+        NbT<T> n;
         n.im = RF_AT(r1, 2, qc, i - 1, j); n.ip = RF_AT(r1, 2, qc, i + 1, j);
         n.jm = RF_AT(r1, 2, qc, i, j - 1); n.jp = RF_AT(r1, 2, qc, i, j + 1);
         n.km = r1[qm][y + 1][x + 1]; n.kp = r1[qp][y + 1][x + 1];
-        double lres;
-        const double R = reinit_rhs_nb(pc, n, rt[qc][y + 1][x + 1], rs[qc][y + 1][x + 1], i, j, p2, cfl, &lres);
-        const double t = 0.25 * pc + 0.75 * r0[q0][y + 2][x + 2];
-        r2[qc][y][x] = t + 0.25 * R;
+        T lres;
+        const T R = reinit_rhs_t<T>(pc, n, rt[qc][y + 1][x + 1], rs[qc][y + 1][x + 1], i, j, p2, cfl, &lres);
+        const T t = T(0.25) * pc + T(0.75) * r0[q0][y + 2][x + 2];
+        r2[qc][y][x] = t + T(0.25) * R;
       }
     }
     __syncthreads();
@@ -1027,29 +1048,32 @@ __global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0,
       for (int r = tid; r < RF_TX * RF_TY; r += RF_NT) {
         const int x = r % RF_TX, y = r / RF_TX;
         const int i = i0 + x, j = j0 + y;
-        if (i >= g.nx || j >= g.ny) continue;
+        if (EDGE && (i >= g.nx || j >= g.ny)) continue;
         const long long idx = jz * plane + (long long)j * g.nx + i;
-        const double pc = r2[qc][y + 1][x + 1];
-        Nb6 n;
+        const T pc = r2[qc][y + 1][x + 1];
+        NbT<T> n;
         n.im = RF_AT(r2, 1, qc, i - 1, j); n.ip = RF_AT(r2, 1, qc, i + 1, j);
         n.jm = RF_AT(r2, 1, qc, i, j - 1); n.jp = RF_AT(r2, 1, qc, i, j + 1);
         n.km = r2[qm][y + 1][x + 1]; n.kp = r2[qp][y + 1][x + 1];
-        double lres;
-        const double R = reinit_rhs_nb(pc, n, rt[qc][y + 2][x + 2], rs[qc][y + 2][x + 2], i, j, jz, cfl, &lres);
-        const double po = r0[q0][y + 3][x + 3];
-        const double t = 1.0 / 3.0 * po + 2.0 / 3.0 * pc;
-        const double pn = t + 2.0 / 3.0 * R;
+        T lres;
+        const T R = reinit_rhs_t<T>(pc, n, rt[qc][y + 2][x + 2], rs[qc][y + 2][x + 2], i, j, jz, cfl, &lres);
+        const T po = r0[q0][y + 3][x + 3];
+        const T t = T(1.0 / 3.0) * po + T(2.0 / 3.0) * pc;
+        const T pn = t + T(2.0 / 3.0) * R;
         out[idx] = pn;
-        double le = dabs(pn - po);
-        if (pn != 0) le /= dabs(pn);
+        T le = dabs(pn - po);
+        if (pn != T(0)) le /= dabs(pn);
         mdphi = smax(mdphi, le);
       }
     }
     __syncthreads();
   }
+  };
+  if (edge) march(std::true_type{});
+  else march(std::false_type{});
 #undef RF_AT
   // block maxima of the stage-1 residual and the relative change
-  __shared__ double red[2][RF_NT / 64];
+  __shared__ T red[2][RF_NT / 64];
   for (int off = 32; off > 0; off >>= 1) {
     mres = smax(mres, __shfl_xor(mres, off, 64));
     mdphi = smax(mdphi, __shfl_xor(mdphi, off, 64));
@@ -1057,10 +1081,10 @@ __global__ __launch_bounds__(RF_NT) void k_reinit_fused(double *__restrict__ P0,
   if ((tid & 63) == 0) { red[0][tid >> 6] = mres; red[1][tid >> 6] = mdphi; }
   __syncthreads();
   if (tid == 0) {
-    double a = red[0][0], b = red[1][0];
+    T a = red[0][0], b = red[1][0];
     for (int w = 1; w < RF_NT / 64; w++) { a = smax(a, red[0][w]); b = smax(b, red[1][w]); }
-    if (a > 0.0) atomic_max_pos(&Rs.acc_res_bits, a);
-    atomic_max_pos(&Rs.acc_dphi_bits, b);
+    if (a > T(0)) atomic_max_pos(&Rs.acc_res_bits, double(a));
+    atomic_max_pos(&Rs.acc_dphi_bits, double(b));
   }
 }
 
@@ -1075,6 +1099,7 @@ __global__ void k_reinit_decide(MpState *ms, int it) {
   Rs.acc_dphi_bits = 0ull;
   if (res < c_mp.reinit_tol) { Rs.done = 1; Rs.iter = it; return; }
   Rs.cur ^= 1;
+  Rs.accepted += 1;
   Rs.dphi_bits = __double_as_longlong(dphi);
   if (dphi < c_mp.reinit_tol) { Rs.done = 1; Rs.iter = it; }
 }
@@ -1093,6 +1118,29 @@ __global__ void k_reinit_end_fused(MpState *ms, Status *st) {
   }
   Rs.cfl = Rs.cfl / 1.5;
   Rs.retry = 1;
+}
+
+// the mixed form: phi and the sign function to FP32 for the iterations
+__global__ void k_reinit_to_f32(const double *phi, float *phif, const double *sign, float *signf, const MpState *ms) {
+  const ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected) return;
+  const Grid &g = c_grid;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x) {
+    phif[idx] = (float)phi[idx];
+    signf[idx] = (float)sign[idx];
+  }
+}
+
+// ... and the FP32 result back into phi when an iteration was accepted
+__global__ void k_reinit_settle_f32(double *phi, const float *phif0, const float *phif1, const MpState *ms) {
+  const ReinitState &Rs = ms->rs;
+  if (!Rs.active || !Rs.detected || !Rs.accepted) return;
+  const float *src = Rs.cur ? phif1 : phif0;
+  const Grid &g = c_grid;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
+       idx += (long long)gridDim.x * blockDim.x)
+    phi[idx] = (double)src[idx];
 }
 
 // the current buffer of the pair back into phi
@@ -1924,12 +1972,18 @@ extern "C" int m2c_mp_launch_reinit(const MpLaunch &L, double *phi, int maxits, 
   else hipLaunchKernelGGL(k_reinit_tag, gs, bs, 0, L.stream, phi, L.tag, L.sign, L.phibk, L.ms);
   if (L.ls_neumann) {   // every boundary zero-Neumann: one fused launch per iteration
     const dim3 gf((L.nx + RF_TX - 1) / RF_TX, (L.ny + RF_TY - 1) / RF_TY, (L.nz + RF_KC - 1) / RF_KC);
+    if (L.ls_f32) hipLaunchKernelGGL(k_reinit_to_f32, gr, bl, 0, L.stream, phi, L.phif0, L.sign, L.signf, L.ms);
     for (int it = 0; it < maxits; it++) {
-      hipLaunchKernelGGL(k_reinit_fused, gf, dim3(RF_NT), 0, L.stream, phi, L.rphi1a, L.tag, L.sign, L.ms);
+      if (L.ls_f32)
+        hipLaunchKernelGGL(k_reinit_fused<float>, gf, dim3(RF_NT), 0, L.stream, L.phif0, L.phif1, L.tag, L.signf,
+                           L.ms);
+      else
+        hipLaunchKernelGGL(k_reinit_fused<double>, gf, dim3(RF_NT), 0, L.stream, phi, L.rphi1a, L.tag, L.sign, L.ms);
       hipLaunchKernelGGL(k_reinit_decide, dim3(1), dim3(1), 0, L.stream, L.ms, it);
     }
     hipLaunchKernelGGL(k_reinit_end_fused, dim3(1), dim3(1), 0, L.stream, L.ms, L.st);
-    hipLaunchKernelGGL(k_reinit_settle, gr, bl, 0, L.stream, phi, L.rphi1a, L.ms);
+    if (L.ls_f32) hipLaunchKernelGGL(k_reinit_settle_f32, gr, bl, 0, L.stream, phi, L.phif0, L.phif1, L.ms);
+    else hipLaunchKernelGGL(k_reinit_settle, gr, bl, 0, L.stream, phi, L.rphi1a, L.ms);
     return hchk(hipGetLastError());
   }
   for (int it = 0; it < maxits; it++) {

@@ -39,7 +39,8 @@ struct MpParams {
 
 struct ReinitState {
   int active, detected, done, iter, retry;
-  int cur;                     // fused iterations: the buffer of the pair (phi, rphi1a) holding phi
+  int cur;                     // fused iterations: the buffer of the pair holding phi
+  int accepted, pad;           // fused iterations whose result became current
   double cfl;                  // persists between calls (LevelSetReinitializer::cfl)
   unsigned long long maxres_bits, dphi_bits;
   unsigned long long acc_res_bits, acc_dphi_bits;   // fused iterations: maxima of the running one
@@ -63,6 +64,8 @@ struct MpLaunch {
   int nx, ny, nz;
   int flux, riemann_threads;
   int ls_neumann;              // every level-set boundary zero-Neumann (the fused reinitialisation applies)
+  int ls_f32;                  // mixed form: the fused reinitialisation iterates in FP32
+  float *phif0, *phif1, *signf;   // its pair of level-set buffers and the sign function
   int8_t *ID, *IDnew;
   unsigned *faces;
   MpState *ms;
