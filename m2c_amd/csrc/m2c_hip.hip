@@ -652,9 +652,11 @@ static void launch_cfl(m2c_ctx *c);
 static int mp_chk(int rc) { return rc ? M2C_ERR_HIP : M2C_OK; }
 
 // Interface Riemann pass on the stage field S (first stage: keep the Riemann solutions).
-static int iface_pass(m2c_ctx *c, const double *S, int in_cons, int store, int ignore_active) {
+// relist = 0: the interface faces listed by an earlier stage of the same step (the IDs change only
+// at the end of a step)
+static int iface_pass(m2c_ctx *c, const double *S, int in_cons, int store, int ignore_active, int relist = 1) {
   if (!c->iface) return M2C_OK;
-  return mp_chk(m2c_mp_launch_iface(c->L, S, in_cons, store && c->ls, ignore_active));
+  return mp_chk(m2c_mp_launch_iface(c->L, S, in_cons, store && c->ls, ignore_active, relist));
 }
 
 // End-of-step tasks of TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
@@ -713,7 +715,7 @@ static int enqueue_step(m2c_ctx *c, int ignore_active, std::vector<hipEvent_t> *
   if (c->ls && mp_chk(m2c_mp_launch_ls(L, 0, LS_FIRST, 0, c->V, c->phiA, c->phiA, nullptr, c->phiB, 0, 0, 1.0,
                                        nullptr)))
     return M2C_ERR_HIP;
-  if ((rc = iface_pass(c, c->U1, 1, 0, ignore_active))) return rc;
+  if ((rc = iface_pass(c, c->U1, 1, 0, ignore_active, 0))) return rc;
   if (ts == M2C_TS_RK2) {
     // U(n+1) = 0.5 U1 + 0.5 U_n + 0.5 dt R(V1); Phi(n+1) = (0.5 Phi + 0.5 Phi1) + 0.5 dt R(Phi1; V1)
     rec(2);
@@ -736,7 +738,7 @@ static int enqueue_step(m2c_ctx *c, int ignore_active, std::vector<hipEvent_t> *
   if (c->ls && mp_chk(m2c_mp_launch_ls(L, 1, LS_AXBY, 0, c->U1, c->phiB, c->phiB, c->phiA, c->phiD, 0.25, 0.75, 0.25,
                                        nullptr)))
     return M2C_ERR_HIP;
-  if ((rc = iface_pass(c, c->U2, 1, 0, ignore_active))) return rc;
+  if ((rc = iface_pass(c, c->U2, 1, 0, ignore_active, 0))) return rc;
   rec(4);
   rc = launch_stage(c, 1, 2, c->U2, c->V, c->U2, c->V, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0, ignore_active);
   rec(5);

@@ -87,12 +87,16 @@ __device__ __forceinline__ long long train_sample(const FwdArgs &a, int q) {
   return (long long)((a.pa * (unsigned long long)(a.base + q) + a.pc) & a.pmask);
 }
 
+// one hidden layer of a tile, in place: every wave reads all K input columns of the tile's rows, so
+// the outputs are written only after the whole workgroup has finished reading
 template <int K, int TRAIN>
-__device__ __forceinline__ void fwd_hidden(const __bf16 *in, __bf16 *out, const __bf16 *W, const float *bias, __bf16 *HT,
-                                           __bf16 *DT, int B, int r0) {
+__device__ __forceinline__ void fwd_hidden(__bf16 *act, const __bf16 *W, const float *bias, __bf16 *HT, __bf16 *DT,
+                                           int B, int r0) {
   const int lane = threadIdx.x & 63, c0 = (threadIdx.x >> 6) * 64;
+  __bf16 *out = act;
   f32x4 acc[4][4];
-  tile_mma<K>(in, W, c0, acc);
+  tile_mma<K>(act, W, c0, acc);
+  __syncthreads();
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) {
     const int col = c0 + nt * 16 + (lane & 15);
@@ -117,9 +121,11 @@ __device__ __forceinline__ void fwd_hidden(const __bf16 *in, __bf16 *out, const 
   }
 }
 
+// one activation buffer (33 KB): inference runs four workgroups per CU, training (more live
+// registers for the stored activations) two
 template <int TRAIN>
-__global__ __launch_bounds__(256, 2) void k_mlp_fwd(MlpWeights w, FwdArgs a) {
-  __shared__ __attribute__((aligned(16))) __bf16 buf[2][MLP_TM * LDS_S];
+__global__ __launch_bounds__(256, TRAIN ? 2 : 4) void k_mlp_fwd(MlpWeights w, FwdArgs a) {
+  __shared__ __attribute__((aligned(16))) __bf16 buf[1][MLP_TM * LDS_S];
   const int count = a.n_dev ? min((int)*a.n_dev, a.n) : a.n;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, kq = (lane >> 4) * 8;
   for (int tile = blockIdx.x; tile * MLP_TM < count; tile += gridDim.x) {
@@ -136,13 +142,13 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fwd(MlpWeights w, FwdArgs a) {
       if (TRAIN && col < MLP_IN) a.XT[(size_t)col * a.B + q] = xb;
     }
     __syncthreads();
-    fwd_hidden<MLP_K0, TRAIN>(buf[0], buf[1], w.W[0], w.bias[0], a.HT[1], a.DT[1], a.B, r0);
+    fwd_hidden<MLP_K0, TRAIN>(buf[0], w.W[0], w.bias[0], a.HT[1], a.DT[1], a.B, r0);
     __syncthreads();
-    fwd_hidden<MLP_H, TRAIN>(buf[1], buf[0], w.W[1], w.bias[1], a.HT[2], a.DT[2], a.B, r0);
+    fwd_hidden<MLP_H, TRAIN>(buf[0], w.W[1], w.bias[1], a.HT[2], a.DT[2], a.B, r0);
     __syncthreads();
-    fwd_hidden<MLP_H, TRAIN>(buf[0], buf[1], w.W[2], w.bias[2], a.HT[3], a.DT[3], a.B, r0);
+    fwd_hidden<MLP_H, TRAIN>(buf[0], w.W[2], w.bias[2], a.HT[3], a.DT[3], a.B, r0);
     __syncthreads();
-    fwd_hidden<MLP_H, TRAIN>(buf[1], buf[0], w.W[3], w.bias[3], a.HT[4], a.DT[4], a.B, r0);
+    fwd_hidden<MLP_H, TRAIN>(buf[0], w.W[3], w.bias[3], a.HT[4], a.DT[4], a.B, r0);
     __syncthreads();
     // output layer: wave wv takes rows 16 wv .. +15, 16 (padded) output columns
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};

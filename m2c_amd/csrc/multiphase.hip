@@ -370,6 +370,8 @@ __global__ void k_mp_face_reset(MpState *ms) {
   ms->nfb = 0;
 }
 
+__global__ void k_mp_fb_reset(MpState *ms) { ms->nfb = 0; }
+
 // interface faces of the current IDs: (dir, plus cell) with different active materials on the two
 // sides; code = dir*Nf + face_key
 // One thread per (i,j) column of the box extended by one cell on the high sides, walking k; cell
@@ -1822,10 +1824,15 @@ static int nblocks(long long n, int bs, int cap) {
   return (int)b;
 }
 
-extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_cons, int store, int ignore_active) {
-  hipLaunchKernelGGL(k_mp_face_reset, dim3(1), dim3(1), 0, L.stream, L.ms);
-  hipLaunchKernelGGL(k_iface_list, sweep_grid(L.nx + 1, L.ny + 1, L.nz + 1), dim3(SWEEP_BX, SWEEP_BY), 0, L.stream,
-                     L.ID, L.faces, L.ms, L.clk, ignore_active);
+extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_cons, int store, int ignore_active,
+                                   int relist) {
+  if (relist) {
+    hipLaunchKernelGGL(k_mp_face_reset, dim3(1), dim3(1), 0, L.stream, L.ms);
+    hipLaunchKernelGGL(k_iface_list, sweep_grid(L.nx + 1, L.ny + 1, L.nz + 1), dim3(SWEEP_BX, SWEEP_BY), 0, L.stream,
+                       L.ID, L.faces, L.ms, L.clk, ignore_active);
+  } else {
+    hipLaunchKernelGGL(k_mp_fb_reset, dim3(1), dim3(1), 0, L.stream, L.ms);
+  }
   dim3 gr(L.riemann_threads / 64), bl(64);
   const float *pred = nullptr;
   if (L.sur_on) {   // network inputs and predictions of the listed faces (count read on the device)

@@ -111,7 +111,8 @@ int m2c_mp_launch_godunov_faces(hipStream_t s, int in_cons, const double *S, con
                                 const m2c::Clock *clk, int ignore_active, const double *crphi);
 int m2c_mp_launch_godunov_batch(hipStream_t s, int dir, int n, const double *Vm, const double *Vp, int id, double *F,
                                 double *scratch, int nthreads);
-int m2c_mp_launch_iface(const m2c::MpLaunch &L, const double *S, int in_cons, int store, int ignore_active);
+int m2c_mp_launch_iface(const m2c::MpLaunch &L, const double *S, int in_cons, int store, int ignore_active,
+                        int relist);
 int m2c_mp_launch_ls(const m2c::MpLaunch &L, int in_cons, int mode, int final_, const double *S, const double *phiS,
                      const double *X, const double *Y, double *out, double a, double b, double coef,
                      const double *phin);
