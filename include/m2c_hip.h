@@ -337,6 +337,11 @@ int m2c_time_diffusion_pass(m2c_ctx *ctx, double *ms);
  * counters. */
 int m2c_stream_copy(m2c_ctx *ctx, long long nbytes, int reps, double *ms_avg);
 
+/* Launch shape of the fused stage kernel (no reference counterpart): each 32x8 column tile marches
+ * up chunks of kchunk z planes; 0 restores the default (chunks such that the grid has about 2048
+ * workgroups). Results do not depend on it. */
+int m2c_set_kchunk(m2c_ctx *ctx, int kchunk);
+
 /* Raw device pointer of the primitive state (SoA, 5*nz*ny*nx doubles) for zero-copy callers. */
 int m2c_device_state(m2c_ctx *ctx, double **V_soa);
 

@@ -182,6 +182,7 @@ def lib():
         L.m2c_flux_batch.argtypes = [P, C.c_int, C.c_int, D, D, C.c_int, D]
         L.m2c_time_steps.argtypes = [P, C.c_int, D, D]
         L.m2c_device_state.argtypes = [P, C.POINTER(D)]
+        L.m2c_set_kchunk.argtypes = [P, C.c_int]
         L.m2c_stream_copy.argtypes = [P, C.c_longlong, C.c_int, D]
         I = C.POINTER(C.c_int)
         L.m2c_riemann_batch.argtypes = [P, C.c_int, D, D, I, D, I, D, I, D, D, I]
@@ -435,6 +436,10 @@ class Solver:
         _chk(self.L.m2c_eos_derived(self.h, rho.size, _dp(rho), _dp(p), ids.ctypes.data_as(C.POINTER(C.c_int32)),
                                     _dp(e), _dp(T)), "m2c_eos_derived")
         return e, T
+
+    def set_kchunk(self, kchunk):
+        """z planes per workgroup of the fused stage kernel (0: the default launch shape)."""
+        _chk(self.L.m2c_set_kchunk(self.h, kchunk), "m2c_set_kchunk")
 
     def time_steps(self, nsteps):
         a, b = C.c_double(), C.c_double()

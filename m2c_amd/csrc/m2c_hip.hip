@@ -1070,15 +1070,24 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   if (upload_constants(c)) return M2C_ERR_HIP;
   hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(1), 0, c->stream, c->clk, 0.0, 0.0, cfg->cfl, 0);
 
-  // launch shape: 32x8 column tiles; z split into chunks so that the grid has >= ~2048 blocks
-  int gx = (g.nx + TX - 1) / TX, gy = (g.ny + TY - 1) / TY;
-  long long tiles = (long long)gx * gy;
-  int nzc = (int)std::max<long long>(1, std::min<long long>(g.nz, (2048 + tiles - 1) / tiles));
-  c->kchunk = (g.nz + nzc - 1) / nzc;
-  nzc = (g.nz + c->kchunk - 1) / c->kchunk;
-  c->grid3 = dim3(gx, gy, nzc);
+  // launch shape: 32x8 column tiles marching up z chunks
+  m2c_set_kchunk(c, 0);
   HIPCHK(hipStreamSynchronize(c->stream));
   *out = c;
+  return M2C_OK;
+}
+
+int m2c_set_kchunk(m2c_ctx *c, int kchunk) {
+  if (!c || kchunk < 0) return M2C_ERR_ARG;
+  const Grid &g = c->grid;
+  const int gx = (g.nx + TX - 1) / TX, gy = (g.ny + TY - 1) / TY;
+  if (kchunk == 0) {   // the default: z chunks such that the grid has >= ~2048 blocks
+    const long long tiles = (long long)gx * gy;
+    const int nzc = (int)std::max<long long>(1, std::min<long long>(g.nz, (2048 + tiles - 1) / tiles));
+    kchunk = (g.nz + nzc - 1) / nzc;
+  }
+  c->kchunk = std::min(kchunk, g.nz);
+  c->grid3 = dim3(gx, gy, (g.nz + c->kchunk - 1) / c->kchunk);
   return M2C_OK;
 }
 
