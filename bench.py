@@ -70,6 +70,10 @@ BYTES_DIFF_STAGE_READ = 288.0
 # --no-mixed` (FETCH scaled by 2.0, WRITE by 1.0 after a copy-kernel calibration; summary in
 # profiles/stage_kernel_periodic256_rk3_fp64_traffic.md). Re-measure after changing the kernel.
 STAGE_TRAFFIC_PERIODIC256_FP64 = 3.430e9
+# The same for the two RK2 stage launches of the 384x192x192 shock-bubble FP64 run (mean of the two),
+# from `bench.py --steps 3 --warmup 1 --no-cpu --no-mixed --no-full` under the same two counters
+# (summary in profiles/stage_kernel_shock_bubble_384x192x192_fp64_traffic.md).
+STAGE_TRAFFIC_SHOCK_BUBBLE_FP64 = 3.013e9
 
 
 def host_cpu():
@@ -433,12 +437,16 @@ def bench_grid(M, W, args, workload, variants=True):
     total_steps = args.warmup + args.steps
     bpcu = BYTES_PER_CELL_UPDATE_MP + BYTES_PER_REINIT_ITER * reinit_its / max(total_steps, 1) if mp \
         else BYTES_PER_CELL_UPDATE + (3 * (BYTES_DIFF_PASS + BYTES_DIFF_STAGE_READ) if visc else 0.0)
-    traffic = STAGE_TRAFFIC_PERIODIC256_FP64 if (not mp and not visc and not god and n == 256) else None
+    traffic = None
+    if mp:
+        traffic = STAGE_TRAFFIC_SHOCK_BUBBLE_FP64
+    elif not visc and not god and n == 256:
+        traffic = STAGE_TRAFFIC_PERIODIC256_FP64
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "m2c::stage_kernel (fused MUSCL+" + ("Godunov-flux read" if god else "HLLC") + "+RK stage)",
             "avg_launch_ms": ms_stage}
-    if traffic is not None:
+    if traffic is not None and not mp:
         # the bound the kernel actually meets: FP64 issue and latency, against the vector FP64 peak
         tf = ISSUED_FP64_FLOP_PER_CELL_STAGE * cells / (ms_stage * 1e-3) / 1e12
         roof["fp64_valu"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
