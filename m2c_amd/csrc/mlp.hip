@@ -14,6 +14,7 @@
 // A row of ones appended to every H (column 10 of the padded input) makes the same GEMMs produce
 // the bias gradients.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <chrono>
 #include <cmath>
@@ -360,12 +361,62 @@ struct SampleArgs {
   int cap, nthreads;
   unsigned long long *next;  // work counter (zero at launch): waves claim 64 samples at a time
   long long s0;              // index of the first sample (the random stream is per global index)
+  const int *order;          // the chunk's samples in solve order (NULL: index order)
 };
 
-// Waves claim 64 consecutive samples at a time (solve times vary by orders of magnitude). Sides are a gas (gamma 1.1-1.7, pinf 0) or a liquid
-// (gamma 4-7.5, pinf 1e8-1e9) with even odds; p_L log-uniform in 1e4..1e9, p_R/p_L log-uniform in
-// 1e-4..1e4, rho_L log-uniform in 0.1..1e3, rho_R/rho_L log-uniform in 1e-3..1e3, u_R - u_L uniform in
-// +-(c_L + c_R). Targets are NaN where the exact solver fails or the star state is inadmissible.
+// Sample s of the stream: sides are a gas (gamma 1.1-1.7, pinf 0) or a liquid (gamma 4-7.5, pinf
+// 1e8-1e9) with even odds; p_L log-uniform in 1e4..1e9, p_R/p_L log-uniform in 1e-4..1e4, rho_L
+// log-uniform in 0.1..1e3, rho_R/rho_L log-uniform in 1e-3..1e3, u_R - u_L uniform in +-(c_L + c_R).
+__device__ __forceinline__ void sg_draw(unsigned long long sd, long long sg, SgSide &L, SgSide &R) {
+  const bool liqL = u01(sd, sg, 0) < 0.5, liqR = u01(sd, sg, 1) < 0.5;
+  L.gam = liqL ? 4.0 + 3.5 * u01(sd, sg, 2) : 1.1 + 0.6 * u01(sd, sg, 2);
+  R.gam = liqR ? 4.0 + 3.5 * u01(sd, sg, 3) : 1.1 + 0.6 * u01(sd, sg, 3);
+  L.pinf = liqL ? pow(10.0, 8.0 + u01(sd, sg, 4)) : 0.0;
+  R.pinf = liqR ? pow(10.0, 8.0 + u01(sd, sg, 5)) : 0.0;
+  L.p = pow(10.0, 4.0 + 5.0 * u01(sd, sg, 6));
+  R.p = L.p * pow(10.0, 4.0 * (2.0 * u01(sd, sg, 7) - 1.0));
+  L.rho = pow(10.0, -1.0 + 4.0 * u01(sd, sg, 8));
+  R.rho = L.rho * pow(10.0, 3.0 * (2.0 * u01(sd, sg, 9) - 1.0));
+  L.u = 0.0; R.u = 0.0;
+  sg_side_init(L);
+  sg_side_init(R);
+  const double cs = L.c + R.c;
+  L.u = cs * (u01(sd, sg, 11) - 0.5);
+  R.u = L.u + cs * (2.0 * u01(sd, sg, 10) - 1.0);
+}
+
+// Solve-order key of a sample (ascending = solved first). Exact solves differ in cost by three
+// orders of magnitude, and a wave of 64 lanes takes as long as its slowest lane, so the samples are
+// solved in the order of a predicted cost: lanes of one wave then finish close together. The slow
+// solves are near-vacuum expansions, where the two rarefaction curves (Toro 4.7 in effective
+// pressure) have not met at a pressure of 1e-6 sqrt(p_L p_R): their key is in [-2, -1), lower for
+// a gas side holding a smaller share of c_L + c_R, which measured slower. Every other sample keys on
+// how far the curves are from meeting there (>= -1). Only the order depends on the key; each
+// sample's result does not.
+__global__ void k_sg_keys(long long n, unsigned long long seed, long long s0, float *key, int *idx) {
+  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  SgSide L, R;
+  sg_draw(seed, s0 + s, L, R);
+  const double cs = L.c + R.c;
+  const double pf = 1e-6 * sqrt(L.p * R.p);
+  auto rare = [&](const SgSide &q) {
+    return 2.0 * q.c / (q.gam - 1.0) * (pow((pf + q.pinf) / q.P, (q.gam - 1.0) / (2.0 * q.gam)) - 1.0);
+  };
+  const double gap = -(rare(L) + rare(R) + (R.u - L.u)) / cs;
+  double k;
+  if (gap < 0.0) {
+    const double gshare = L.pinf == 0.0 ? L.c / cs : (R.pinf == 0.0 ? R.c / cs : 1.0);
+    k = -2.0 + 0.999 * gshare;
+  } else {
+    k = fmin(gap, 1e30) - 1.0;
+  }
+  key[s] = (float)k;
+  idx[s] = (int)s;
+}
+
+// Waves claim 64 samples at a time, in the chunk's solve order. Targets are NaN where the exact
+// solver fails or the star state is inadmissible.
 __global__ __launch_bounds__(64) void k_sg_samples(SampleArgs a) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= a.nthreads) return;
@@ -383,26 +434,13 @@ __global__ __launch_bounds__(64) void k_sg_samples(SampleArgs a) {
     if (lane == 0) base = atomicAdd(a.next, 64ull);
     base = __shfl(base, 0, 64);
     if (base >= (unsigned long long)a.n) break;
-    const long long s = (long long)base + lane;
-    if (s < a.n) {   // the claim above is reached by the whole wave
+    const long long q = (long long)base + lane;
+    if (q < a.n) {   // the claim above is reached by the whole wave
+      const long long s = a.order ? (long long)a.order[q] : q;
       const unsigned long long sd = a.seed;
       const long long sg = a.s0 + s;   // global sample index: the random draws
-      const bool liqL = u01(sd, sg, 0) < 0.5, liqR = u01(sd, sg, 1) < 0.5;
       SgSide L, R;
-      L.gam = liqL ? 4.0 + 3.5 * u01(sd, sg, 2) : 1.1 + 0.6 * u01(sd, sg, 2);
-      R.gam = liqR ? 4.0 + 3.5 * u01(sd, sg, 3) : 1.1 + 0.6 * u01(sd, sg, 3);
-      L.pinf = liqL ? pow(10.0, 8.0 + u01(sd, sg, 4)) : 0.0;
-      R.pinf = liqR ? pow(10.0, 8.0 + u01(sd, sg, 5)) : 0.0;
-      L.p = pow(10.0, 4.0 + 5.0 * u01(sd, sg, 6));
-      R.p = L.p * pow(10.0, 4.0 * (2.0 * u01(sd, sg, 7) - 1.0));
-      L.rho = pow(10.0, -1.0 + 4.0 * u01(sd, sg, 8));
-      R.rho = L.rho * pow(10.0, 3.0 * (2.0 * u01(sd, sg, 9) - 1.0));
-      L.u = 0.0; R.u = 0.0;
-      sg_side_init(L);
-      sg_side_init(R);
-      const double cs = L.c + R.c;
-      L.u = cs * (u01(sd, sg, 11) - 0.5);
-      R.u = L.u + cs * (2.0 * u01(sd, sg, 10) - 1.0);
+      sg_draw(sd, sg, L, R);
       sg_eos(mats[0], L.gam, L.pinf);
       sg_eos(mats[1], R.gam, R.pinf);
       const double Vm[5] = {L.rho, L.u, 0.0, 0.0, L.p}, Vp[5] = {R.rho, R.u, 0.0, 0.0, R.p};
@@ -759,25 +797,41 @@ static RiemannCfg default_rcfg() {
   return c;
 }
 
-// samples in launches of at most 2^19 (a progress line on stderr after each when verbose)
+// samples in launches of at most 2^19 (a progress line on stderr after each when verbose); each
+// launch solves its samples in the order of k_sg_keys
 static int gen_samples_dev(hipStream_t s, long long n, unsigned long long seed, float *X, float *T, double *raw,
                            int verbose = 0) {
   // one wave per SIMD at this kernel's register use: 4 SIMDs x 256 CUs x 64 lanes
   const int cap = 1024, nthreads = 65536;
   const long long chunk = 1ll << 19;
-  double *scratch = nullptr;
   const size_t sb = sizeof(double) * 6 * (size_t)cap * nthreads;
-  MCHK(hipMalloc(&scratch, sb + 256));
-  unsigned long long *next = (unsigned long long *)((char *)scratch + sb);
+  size_t tb = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const float *)nullptr, (float *)nullptr, (const int *)nullptr,
+                                         (int *)nullptr, (int)chunk, 0, 32, s) != hipSuccess)
+    return M2C_ERR_HIP;
+  const size_t kb = align256(sizeof(float) * chunk), ib = align256(sizeof(int) * chunk);
+  char *buf = nullptr;
+  MCHK(hipMalloc(&buf, sb + 256 + 2 * kb + 2 * ib + align256(tb)));
+  double *scratch = (double *)buf;
+  unsigned long long *next = (unsigned long long *)(buf + sb);
+  float *key = (float *)(buf + sb + 256), *key_sorted = (float *)((char *)key + kb);
+  int *idx = (int *)((char *)key_sorted + kb), *order = (int *)((char *)idx + ib);
+  void *tmp = (char *)order + ib;
   int rc = M2C_OK;
   auto t0 = std::chrono::steady_clock::now();
   for (long long s0 = 0; s0 < n && !rc; s0 += chunk) {
     const long long m = n - s0 < chunk ? n - s0 : chunk;
     if (hipMemsetAsync(next, 0, sizeof(unsigned long long), s) != hipSuccess) { rc = M2C_ERR_HIP; break; }
+    hipLaunchKernelGGL(k_sg_keys, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, seed, s0, key, idx);
+    if (hipGetLastError() != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key_sorted, idx, order, (int)m, 0, 32, s) != hipSuccess) {
+      rc = M2C_ERR_HIP;
+      break;
+    }
     SampleArgs a;
     a.n = m; a.seed = seed; a.X = X + (size_t)s0 * MLP_IN; a.T = T + (size_t)s0 * MLP_OUT;
     a.raw = raw ? raw + (size_t)s0 * RAW : nullptr; a.cfg = default_rcfg(); a.scratch = scratch;
-    a.cap = cap; a.nthreads = nthreads; a.next = next; a.s0 = s0;
+    a.cap = cap; a.nthreads = nthreads; a.next = next; a.s0 = s0; a.order = order;
     hipLaunchKernelGGL(k_sg_samples, dim3(nthreads / 64), dim3(64), 0, s, a);
     rc = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess ? M2C_OK : M2C_ERR_HIP;
     if (verbose) {
@@ -786,7 +840,7 @@ static int gen_samples_dev(hipStream_t s, long long n, unsigned long long seed, 
       fflush(stderr);
     }
   }
-  hipFree(scratch);
+  hipFree(buf);
   return rc;
 }
 

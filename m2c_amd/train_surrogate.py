@@ -1,6 +1,6 @@
 """Train the learned interface solver on the GPU and store its parameters.
 
-    python m2c_amd/train_surrogate.py [--log2-samples 23] [--epochs 5] [--out m2c_amd/data/sg_riemann_mlp.npy]
+    python m2c_amd/train_surrogate.py [--log2-samples 24] [--epochs 5] [--out m2c_amd/data/sg_riemann_mlp.npy]
 
 Training samples are stiffened-gas Riemann problems solved by the library's exact two-material
 solver (the device restatement of ExactRiemannSolverBase::ComputeRiemannSolution); the network is
@@ -21,7 +21,7 @@ import m2c_amd as M  # noqa: E402
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--log2-samples", type=int, default=23)
+    ap.add_argument("--log2-samples", type=int, default=24)
     ap.add_argument("--holdout", type=int, default=1 << 16)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--epochs", type=int, default=5)
