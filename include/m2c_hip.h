@@ -235,6 +235,23 @@ int m2c_wall_face_flux_batch(m2c_ctx *ctx, int n, const int *axis, const int *si
                              const double *nwall, const double *vwall, const double *V, const int *id, double *F,
                              double *Vstar, int *status);
 
+/* Mesh faces cut by embedded walls, fluxed inside every later stage of m2c_step / m2c_run /
+ * m2c_compute_residual as SpaceOperator::ComputeAdvectionFluxes does when it is given
+ * EmbeddedBoundaryDataSets (SpaceOperator.cpp:2438-2484 i-1/2, :2569-2611 j-1/2, :2696-2737 k-1/2):
+ * each active side of a cut face solves its one-sided problem against the wall and takes the flux
+ * between its state and the clipped star state; an inactive side's flux is zero. With
+ * Reconstruction SlopeNearInterface Zero, slopes of cells with a cut face along an axis are zero
+ * along it (Reconstructor.cpp:656-664). Replaces the intersections that FindEdgeSurfaceIntersections
+ * (SpaceOperator.cpp:3434-3500) reads from the EmbeddedBoundaryDataSets: face t is normal to axis
+ * face[4t] with plus cell (face[4t+1], face[4t+2], face[4t+3]), and wall[12t..12t+11] holds that
+ * function's nwallf, vwallf (the intersection seen from the minus cell) and nwallb, vwallb (seen
+ * from the plus cell). normal_mode: EmbeddedBoundaryMethod RiemannNormal (0 EMBEDDED_SURFACE,
+ * 1 MESH, 2 AVERAGE); recon: ReconstructionAtInterface (0 CONSTANT, 1 LINEAR). Faces must be
+ * interior and listed once; the walls stay until replaced (n = 0 removes them). Contexts with a
+ * level set, material interfaces, viscous or heat-diffusion terms or a periodic boundary return
+ * M2C_ERR_UNSUPPORTED. Host arrays. */
+int m2c_set_embedded_faces(m2c_ctx *ctx, int n, const int *face, const double *wall, int normal_mode, int recon);
+
 /* ---- many independent instances (batched) ------------------------------------------------
  * A batch holds n single-material instances (each its own grid, boundaries, state and
  * CFL-adaptive time step; one material table, flux, precision and RK2/RK3 scheme for all; the

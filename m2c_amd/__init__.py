@@ -191,6 +191,7 @@ def lib():
                                                C.POINTER(C.c_int32), D]
         L.m2c_onesided_riemann_batch.argtypes = [P, C.c_int, D, D, I, D, D, I, D, I]
         L.m2c_wall_face_flux_batch.argtypes = [P, C.c_int, I, I, C.c_int, D, D, D, I, D, D, I]
+        L.m2c_set_embedded_faces.argtypes = [P, C.c_int, I, D, C.c_int, C.c_int]
         L.m2c_upload_levelset.argtypes = [P, D]
         L.m2c_download_levelset.argtypes = [P, D]
         Pb = C.c_void_p
@@ -358,6 +359,17 @@ class Solver:
         _chk(self.L.m2c_onesided_riemann_batch(self.h, n, _dp(dd), _dp(Vm), cp(im), _dp(us), _dp(Vs), cp(mid),
                                                _dp(Vsm), cp(st)), "m2c_onesided_riemann_batch")
         return st, Vs, mid, Vsm
+
+    def set_embedded_faces(self, faces, walls, normal_mode=1, recon=0):
+        """Mesh faces cut by embedded walls, fluxed in every later stage: faces (n, 4) = axis and the
+        plus cell's i, j, k; walls (n, 12) = nwallf, vwallf, nwallb, vwallb. normal_mode 0 wall normal,
+        1 mesh axis, 2 their average; recon 0 constant, 1 linear. An empty list removes the walls."""
+        f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).reshape(-1, 4))
+        w = np.ascontiguousarray(np.asarray(walls, dtype=np.float64).reshape(-1, 12))
+        if f.shape[0] != w.shape[0]:
+            raise ValueError("faces and walls differ in length")
+        _chk(self.L.m2c_set_embedded_faces(self.h, f.shape[0], f.ctypes.data_as(C.POINTER(C.c_int)), _dp(w),
+                                           normal_mode, recon), "m2c_set_embedded_faces")
 
     def wall_face_flux(self, axis, side, normal_mode, nwall, vwall, V, ids):
         """Fluxes on the fluid side of n wall-cut faces: (status, F, Vstar). side +1: the fluid cell is

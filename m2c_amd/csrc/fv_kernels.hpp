@@ -95,6 +95,9 @@ struct StageArgs {
   const double *G[3] = {nullptr, nullptr, nullptr};
   // the level set at t_n when a constant-reconstruction depth is set (null otherwise)
   const double *phin = nullptr;
+  // faces cut by embedded walls (null: none): bit d of Wm[face key] marks the d-face whose plus
+  // cell has that key; its two fluxes are read from Fi like an interface face's
+  const uint8_t *Wm = nullptr;
 };
 
 // Each translation unit that launches kernels holds its own copy of these tables (internal
@@ -290,6 +293,14 @@ __device__ __forceinline__ long long face_key(int i, int j, int k) {
   return ((long long)k * (g.ny + 1) + j) * (g.nx + 1) + i;
 }
 
+// 1 when the d-face whose plus cell is (i, j, k) is cut by an embedded wall; faces outside the
+// key range (ghost cells) never are
+__device__ __forceinline__ unsigned wall_cut(const uint8_t *W, int d, int i, int j, int k) {
+  const Grid &g = M2C_G;
+  if (i < 0 || j < 0 || k < 0 || i > g.nx || j > g.ny || k > g.nz) return 0u;
+  return ((unsigned)W[face_key(i, j, k)] >> d) & 1u;
+}
+
 __device__ __forceinline__ void warp_min_atomic(unsigned long long *dst, double v) {
   // wavefront min (64 lanes), then one atomic per wave
   unsigned long long b = __double_as_longlong(v);
@@ -359,6 +370,9 @@ template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double, bool MP
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, const int by, const int bz) {
   const Grid &g = M2C_G;
   if (!a.ignore_active && !a.clk->active) return;
+  // faces cut by embedded walls: the run-time equation-of-state variants only (the stiffened-gas
+  // variants keep their registers for the interface code)
+  constexpr bool WALLS = MP && (EK < 0);
 
   __shared__ TC sV[5][LY][LX];
   __shared__ int sID[LY][LX];
@@ -408,6 +422,11 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       if (a.phin) {
         if (conrec_cell(a.phin, ci, cj, kbeg - 1)) zero5(szm);
         if (conrec_cell(a.phin, ci, cj, kbeg)) zero5(sz0);
+      }
+      // slopes along z of cells with a cut low or high z face (Reconstructor.cpp:656-664)
+      if (WALLS && a.Wm && g.slope_zero) {
+        if (wall_cut(a.Wm, 2, ci, cj, kbeg - 1) | wall_cut(a.Wm, 2, ci, cj, kbeg)) zero5(szm);
+        if (wall_cut(a.Wm, 2, ci, cj, kbeg) | wall_cut(a.Wm, 2, ci, cj, kbeg + 1)) zero5(sz0);
       }
     }
     TC fm[5], fp[5];
@@ -470,6 +489,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     cell_slopes<2, TC>(w0, w1, w2, id0, id1, id2, sz1);
     if constexpr (MP) {
       if (a.phin && conrec_cell(a.phin, ci, cj, k + 1)) zero5(sz1);
+      if (WALLS && a.Wm && g.slope_zero && (wall_cut(a.Wm, 2, ci, cj, k + 1) | wall_cut(a.Wm, 2, ci, cj, k + 2)))
+        zero5(sz1);
     }
     __syncthreads();
 
@@ -490,6 +511,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
                            sID[y + HALO][x + 1 + HALO], sg);
         if constexpr (MP) {
           if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
+          if (WALLS && a.Wm && g.slope_zero &&
+              (wall_cut(a.Wm, 0, i0 + x, j0 + y, k) | wall_cut(a.Wm, 0, i0 + x + 1, j0 + y, k)))
+            zero5(sg);
         }
 #pragma unroll
         for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
@@ -507,6 +531,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
                            sID[y + 1 + HALO][x + HALO], sg);
         if constexpr (MP) {
           if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
+          if (WALLS && a.Wm && g.slope_zero &&
+              (wall_cut(a.Wm, 1, i0 + x, j0 + y, k) | wall_cut(a.Wm, 1, i0 + x, j0 + y + 1, k)))
+            zero5(sg);
         }
 #pragma unroll
         for (int p = 0; p < 5; p++) sS[p * NSL + it] = sg[p];
@@ -616,6 +643,29 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
             if (no) Fn[p] = a.G[1][p * my + gys + g.nx];
             if (bo) Fb[p] = a.G[2][p * mz + gzb];
             if (to) Ft[p] = a.G[2][p * mz + gzb + mxy];
+          }
+        }
+      }
+      if constexpr (WALLS) {
+        if (a.Wm && a.Fi) {
+          // faces cut by an embedded wall, after the Godunov fluxes: as at an interface face, the
+          // plus cell takes localflux2 of its low face, the minus cell localflux1 of its high face
+          // (SpaceOperator.cpp:2438-2484)
+          const long long Nf = a.Nf;
+          const long long kw = face_key(gi, gj, k);
+          const unsigned m0 = a.Wm[kw];
+          const bool cw = m0 & 1u, cs = (m0 >> 1) & 1u, cb = (m0 >> 2) & 1u;
+          const bool ce = a.Wm[kw + 1] & 1u;
+          const bool cn = ((unsigned)a.Wm[kw + (g.nx + 1)] >> 1) & 1u;
+          const bool ct = ((unsigned)a.Wm[kw + (long long)(g.nx + 1) * (g.ny + 1)] >> 2) & 1u;
+#pragma unroll
+          for (int p = 0; p < 5; p++) {
+            if (cw) Fw[p] = a.Fi[0 * 10 * Nf + (5 + p) * Nf + kw];
+            if (ce) Fe[p] = a.Fi[0 * 10 * Nf + p * Nf + kw + 1];
+            if (cs) Fs[p] = a.Fi[1 * 10 * Nf + (5 + p) * Nf + kw];
+            if (cn) Fn[p] = a.Fi[1 * 10 * Nf + p * Nf + kw + (g.nx + 1)];
+            if (cb) Fb[p] = a.Fi[2 * 10 * Nf + (5 + p) * Nf + kw];
+            if (ct) Ft[p] = a.Fi[2 * 10 * Nf + p * Nf + kw + (long long)(g.nx + 1) * (g.ny + 1)];
           }
         }
       }

@@ -82,11 +82,19 @@ struct m2c_ctx {
   long long gnf[3] = {0, 0, 0};
   double *gscratch = nullptr;
   int gthreads = 0;
+  // mesh faces cut by embedded walls (m2c_set_embedded_faces): face codes dir * Nf + key, wall
+  // records (12 doubles each), the per-face-key cut mask, the exact solver's path cache
+  int eb_n = 0, eb_mode = 1, eb_recon = 0, eb_threads = 0;
+  int variant0 = 1;            // the kernel variant chosen at creation (walls force a generic one)
+  unsigned *eb_faces = nullptr;
+  double *eb_wall = nullptr, *eb_scratch = nullptr;
+  uint8_t *eb_mask = nullptr;
 };
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
 
 static int ensure_riemann_constants(m2c_ctx *c);
+static void riemann_only_params(const m2c_config &cfg, MpParams &mp);
 static bool riemann_batch_args_ok(const m2c_ctx *c, int n, const int *idm, const int *idp);
 static int setup_godunov(m2c_ctx *c);
 static TLaw build_tlaw(const m2c_config &cfg);
@@ -277,7 +285,7 @@ static m2c_ctx *g_resident = nullptr;
 static int upload_constants(m2c_ctx *c) {
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_grid), &c->grid, sizeof(Grid)));
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_eos), c->eos, sizeof(c->eos)));
-  if ((c->iface || c->ls || c->godunov) && m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
+  if ((c->iface || c->ls || c->godunov || c->eb_n) && m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
   if (m2c_stage_upload_hllc_sg(c->grid, c->eos) || m2c_stage_upload_hllc_gen(c->grid, c->eos) ||
       m2c_stage_upload_god_gen(c->grid, c->eos) ||
       m2c_stage_upload_roe_gen(c->grid, c->eos) || m2c_stage_upload_llf_gen(c->grid, c->eos) ||
@@ -618,6 +626,12 @@ static int launch_stage(m2c_ctx *c, int in_cons, int mode, const double *S, cons
       return M2C_ERR_HIP;
     if (prof) hipEventRecord((*c->diff_ev)[c->diff_ev_i++], c->stream);
     for (int d = 0; d < 3; d++) s.G[d] = c->G[d];
+  }
+  if (c->eb_n) {   // the two fluxes of every wall-cut face, before the stage reads them
+    if (m2c_mp_launch_walls(c->stream, c->cfg.flux, in_cons, S, c->ID, c->eb_faces, c->eb_wall, c->eb_n, c->eb_mode,
+                            c->eb_recon, c->Fi, c->Nf, c->eb_scratch, c->eb_threads, c->st, c->clk, ignore_active))
+      return M2C_ERR_HIP;
+    s.Wm = c->eb_mask;
   }
   s.phin = conrec_phi(c);
   s.no_cfl = c->ls ? 1 : 0;
@@ -1005,6 +1019,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   c->grid = g0;
   memcpy(c->eos, e0, sizeof(e0));
   c->variant = v0;
+  c->variant0 = v0;
   c->N = g0.N;
   Grid &g = c->grid;
   c->iface = g.has_interfaces && cfg->mp_flux != 2;
@@ -1098,9 +1113,86 @@ int m2c_ctx_destroy(m2c_ctx *c) {
   hipFree(c->V); hipFree(c->U1); hipFree(c->U2); hipFree(c->R);
   hipFree(c->ID); hipFree(c->clk); hipFree(c->st);
   for (void *p : c->mp_bufs) hipFree(p);
+  hipFree(c->eb_faces); hipFree(c->eb_wall); hipFree(c->eb_mask); hipFree(c->eb_scratch);
   hipStreamDestroy(c->stream);
   delete c;
   return M2C_OK;
+}
+
+static void free_walls(m2c_ctx *c) {
+  hipFree(c->eb_faces); hipFree(c->eb_wall); hipFree(c->eb_mask); hipFree(c->eb_scratch);
+  c->eb_faces = nullptr; c->eb_wall = nullptr; c->eb_mask = nullptr; c->eb_scratch = nullptr;
+  c->eb_n = 0;
+  c->eb_threads = 0;
+  c->variant = c->variant0;
+}
+
+int m2c_set_embedded_faces(m2c_ctx *c, int n, const int *face, const double *wall, int normal_mode, int recon) {
+  if (!c || n < 0 || (n > 0 && (!face || !wall))) return M2C_ERR_ARG;
+  if (normal_mode < 0 || normal_mode > 2 || recon < 0 || recon > 1) return M2C_ERR_ARG;
+  const Grid &g = c->grid;
+  if (n > 0) {
+    // one active material beside walls: no interfaces or level set, no diffusion terms, no
+    // periodic axis (a wrapped ghost cell's cut faces are not tracked)
+    if (c->iface || c->ls || c->diff) return M2C_ERR_UNSUPPORTED;
+    for (int sd = 0; sd < 6; sd++)
+      if (c->cfg.bc[sd] == M2C_BC_PERIODIC) return M2C_ERR_UNSUPPORTED;
+  }
+  const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  if (3 * Nf >= (1ll << 32)) return M2C_ERR_UNSUPPORTED;
+  std::vector<uint8_t> mask;
+  std::vector<unsigned> codes((size_t)n);
+  if (n > 0) {
+    mask.assign((size_t)Nf, 0);
+    const int N[3] = {g.nx, g.ny, g.nz};
+    for (int t = 0; t < n; t++) {
+      const int d = face[4 * t], q[3] = {face[4 * t + 1], face[4 * t + 2], face[4 * t + 3]};
+      if (d < 0 || d > 2) return M2C_ERR_ARG;
+      for (int a = 0; a < 3; a++) {
+        const bool ok = a == d ? (q[a] >= 1 && q[a] <= N[a] - 1) : (q[a] >= 0 && q[a] < N[a]);
+        if (!ok) return M2C_ERR_ARG;   // faces on the domain boundary are not cut faces
+      }
+      for (int v = 0; v < 12; v++)
+        if (!std::isfinite(wall[12 * t + v])) return M2C_ERR_ARG;
+      const long long key = ((long long)q[2] * (g.ny + 1) + q[1]) * (g.nx + 1) + q[0];
+      if (mask[(size_t)key] & (1u << d)) return M2C_ERR_ARG;   // listed twice
+      mask[(size_t)key] |= (uint8_t)(1u << d);
+      codes[(size_t)t] = (unsigned)((long long)d * Nf + key);
+    }
+  }
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  free_walls(c);
+  if (n == 0) {
+    g_resident = nullptr;
+    return ensure_constants(c);
+  }
+  const int threads = (int)std::min<long long>(16384, ((long long)n + 63) / 64 * 64);
+  if (!c->godunov) riemann_only_params(c->cfg, c->mp);
+  int rc = M2C_OK;
+  if (!c->Fi) {
+    c->Nf = Nf;
+    if ((rc = mp_alloc(c, &c->Fi, (size_t)(30 * Nf), 0))) return rc;
+  }
+  if (hipMalloc(&c->eb_faces, sizeof(unsigned) * (size_t)n) != hipSuccess ||
+      hipMalloc(&c->eb_wall, sizeof(double) * 12 * (size_t)n) != hipSuccess ||
+      hipMalloc(&c->eb_mask, (size_t)Nf) != hipSuccess ||
+      hipMalloc(&c->eb_scratch, sizeof(double) * 6 * (size_t)c->mp.path_cap * threads) != hipSuccess ||
+      hipMemcpy(c->eb_faces, codes.data(), sizeof(unsigned) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->eb_wall, wall, sizeof(double) * 12 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->eb_mask, mask.data(), (size_t)Nf, hipMemcpyHostToDevice) != hipSuccess) {
+    free_walls(c);
+    return M2C_ERR_HIP;
+  }
+  c->eb_n = n;
+  c->eb_mode = normal_mode;
+  c->eb_recon = recon;
+  c->eb_threads = threads;
+  // the compile-time stiffened-gas variant has no cut-face code: the generic one of the same flux
+  if (c->variant == 0) c->variant = 1;
+  else if (c->variant == 4) c->variant = 5;
+  g_resident = nullptr;   // the exact solver's constants go up with the context's
+  return ensure_constants(c);
 }
 
 int m2c_upload_state(m2c_ctx *c, const double *V, const int32_t *id) {
@@ -1688,7 +1780,7 @@ static void riemann_only_params(const m2c_config &cfg, MpParams &mp) {
 }
 
 static int ensure_riemann_constants(m2c_ctx *c) {
-  if (c->iface || c->ls || c->godunov) return M2C_OK;   // uploaded with the context's constants
+  if (c->iface || c->ls || c->godunov || c->eb_n) return M2C_OK;   // uploaded with the context's constants
   riemann_only_params(c->cfg, c->mp);
   if (m2c_mp_upload(c->grid, c->eos, c->mp)) return M2C_ERR_HIP;
   return M2C_OK;

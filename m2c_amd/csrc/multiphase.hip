@@ -1670,6 +1670,104 @@ __global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir,
   }
 }
 
+// ---------------------------------------------------------------- faces cut by embedded walls
+
+// One side of a wall-cut face: the one-sided problem of the side's state v (material m) along
+// GetNormalForOneSidedRiemann's direction (fb = +1: the cell is on the minus side), the clipped star
+// state, then the numerical flux along DIR between the state and the star state (F(v, v*) on the
+// minus side, F(v*, v) on the plus side). Where the reference exits (a negative status) the flux
+// is zero. Returns 1 on a solver error (counted as the sweep counts them).
+template <int FLUX, int DIR>
+__device__ __forceinline__ int wall_side(RSolver &S, int fb, int mode, const double *nwall, const double *vwall,
+                                         const double *v, int m, double *F, unsigned &flags) {
+  double nrm[3], vs[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, vsm[5];
+  onesided_normal(DIR, fb, mode, nwall, nrm);
+  int mid = 0;
+  S.overflow = 0;
+  const int r = onesided_solve(S, nrm, v, m, vwall, vs, &mid, vsm);
+  if (S.overflow) flags |= ST_RIEMANN_PATH;
+#pragma unroll
+  for (int p = 0; p < 5; p++) F[p] = 0.0;
+  if (r >= 0) {
+    eos_clip<double, -1>(c_eos[m], vsm);
+    const double *a = fb > 0 ? v : vsm, *b = fb > 0 ? vsm : v;
+    if (FLUX == FLUX_GODUNOV) godunov_flux<DIR>(S, a, b, m, F);
+    else numerical_flux<FLUX, -1, DIR>(a, b, m, F);
+  }
+  return r != 0;
+}
+
+// The two fluxes of every mesh face cut by an embedded wall, for the stage that follows
+// (SpaceOperator.cpp:2438-2484 i-1/2, :2569-2611 j-1/2, :2696-2737 k-1/2): faces[t] = dir * Nf + key
+// (the plus cell's face key), wall[12t..] = nwallf, vwallf, nwallb, vwallb. An active minus cell
+// solves against the forward intersection and gives localflux1 (Fi components 0-4), an active plus
+// cell against the backward one and gives localflux2 (components 5-9); an inactive side's flux is
+// zero. The sides' states are the cell values (ReconstructionAtInterface CONSTANT) or the
+// reconstructed face states (LINEAR), which are the cell values too where slopes next to cut faces
+// are zeroed. One thread per face (grid-stride), the rarefaction path cache in the thread's slice.
+template <int FLUX, int IN_CONS>
+__global__ __launch_bounds__(64) void k_wall_faces(const double *S, const int8_t *ID, const unsigned *faces,
+                                                   const double *wall, int n, int mode, int recon, double *Fi,
+                                                   long long Nf, double *scratch, Status *st, const Clock *clk,
+                                                   int ignore_active) {
+  if (!ignore_active && !step_active(clk)) return;
+  const Grid &g = c_grid;
+  const int T = gridDim.x * blockDim.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  RSolver S_;
+  S_.mat = c_eos;
+  S_.cfg = c_mp.rcfg;
+  S_.overflow = 0;
+  const int cap = c_mp.path_cap;
+  S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = T; S_.path1.n = 0;
+  S_.path3.base = scratch + (long long)3 * cap * T + tid; S_.path3.cap = cap; S_.path3.stride = T; S_.path3.n = 0;
+  unsigned flags = 0;
+  unsigned long long nerr = 0;
+  for (int t = tid; t < n; t += T) {
+    const unsigned code = faces[t];
+    const int dir = (int)(code / Nf);
+    const long long key = code - (long long)dir * Nf;
+    const int i = (int)(key % (g.nx + 1));
+    const long long r = key / (g.nx + 1);
+    const int j = (int)(r % (g.ny + 1));
+    const int k = (int)(r / (g.ny + 1));
+    double vm[5], vp[5];
+    int idm, idp;
+    unsigned fl = 0;
+    fetch_cell<IN_CONS, -1>(S, ID, i - (dir == 0), j - (dir == 1), k - (dir == 2), vm, idm, fl);
+    fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp, idp, fl);
+    flags |= fl;
+    if (recon == 1 && !g.slope_zero) {
+      double fm[5], fp[5];
+      if (dir == 0) face_states<0, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, nullptr);
+      else if (dir == 1) face_states<1, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, nullptr);
+      else face_states<2, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, nullptr);
+#pragma unroll
+      for (int p = 0; p < 5; p++) { vm[p] = fm[p]; vp[p] = fp[p]; }
+    }
+    const double *w = wall + 12 * (long long)t;
+    double F1[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, F2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (dir == 0) {
+      if (idm != g.inactive_id) nerr += wall_side<FLUX, 0>(S_, 1, mode, w, w + 3, vm, idm, F1, flags);
+      if (idp != g.inactive_id) nerr += wall_side<FLUX, 0>(S_, -1, mode, w + 6, w + 9, vp, idp, F2, flags);
+    } else if (dir == 1) {
+      if (idm != g.inactive_id) nerr += wall_side<FLUX, 1>(S_, 1, mode, w, w + 3, vm, idm, F1, flags);
+      if (idp != g.inactive_id) nerr += wall_side<FLUX, 1>(S_, -1, mode, w + 6, w + 9, vp, idp, F2, flags);
+    } else {
+      if (idm != g.inactive_id) nerr += wall_side<FLUX, 2>(S_, 1, mode, w, w + 3, vm, idm, F1, flags);
+      if (idp != g.inactive_id) nerr += wall_side<FLUX, 2>(S_, -1, mode, w + 6, w + 9, vp, idp, F2, flags);
+    }
+    double *f = Fi + (long long)dir * 10 * Nf + key;
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+      f[p * Nf] = F1[p];
+      f[(5 + p) * Nf] = F2[p];
+    }
+  }
+  if (nerr) atomicAdd(&st->riemann_err, nerr);
+  if (flags) atomicOr(&st->bits, flags);
+}
+
 // ---------------------------------------------------------------- Godunov flux on the grid
 
 // FluxFcnGodunov (FluxFcnGodunov.h:37-60) on every face of direction DIR between two cells of one
@@ -1893,6 +1991,30 @@ extern "C" int m2c_mp_launch_onesided_batch(hipStream_t s, int flux, int face, i
   else if (flux == FLUX_GODUNOV) M2C_OS(FLUX_GODUNOV, 1);
   else M2C_OS(FLUX_LLF, 1);
 #undef M2C_OS
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_walls(hipStream_t s, int flux, int in_cons, const double *S, const int8_t *ID,
+                                   const unsigned *faces, const double *wall, int n, int mode, int recon, double *Fi,
+                                   long long Nf, double *scratch, int nthreads, Status *st, const Clock *clk,
+                                   int ignore_active) {
+  if (n <= 0) return 0;
+  const dim3 gr((nthreads + 63) / 64), bl(64);
+#define M2C_WF(FL, IC)                                                                                        \
+  hipLaunchKernelGGL((k_wall_faces<FL, IC>), gr, bl, 0, s, S, ID, faces, wall, n, mode, recon, Fi, Nf, scratch, \
+                     st, clk, ignore_active)
+  if (in_cons) {
+    if (flux == FLUX_HLLC) M2C_WF(FLUX_HLLC, 1);
+    else if (flux == FLUX_ROE) M2C_WF(FLUX_ROE, 1);
+    else if (flux == FLUX_GODUNOV) M2C_WF(FLUX_GODUNOV, 1);
+    else M2C_WF(FLUX_LLF, 1);
+  } else {
+    if (flux == FLUX_HLLC) M2C_WF(FLUX_HLLC, 0);
+    else if (flux == FLUX_ROE) M2C_WF(FLUX_ROE, 0);
+    else if (flux == FLUX_GODUNOV) M2C_WF(FLUX_GODUNOV, 0);
+    else M2C_WF(FLUX_LLF, 0);
+  }
+#undef M2C_WF
   return hchk(hipGetLastError());
 }
 

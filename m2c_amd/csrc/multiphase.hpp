@@ -109,6 +109,9 @@ int m2c_mp_launch_onesided_batch(hipStream_t s, int flux, int face, int n, const
 int m2c_mp_launch_godunov_faces(hipStream_t s, int in_cons, const double *S, const int8_t *ID, double *const *G,
                                 const long long *nf, double *scratch, int nthreads, m2c::Status *st,
                                 const m2c::Clock *clk, int ignore_active, const double *crphi);
+int m2c_mp_launch_walls(hipStream_t s, int flux, int in_cons, const double *S, const int8_t *ID, const unsigned *faces,
+                        const double *wall, int n, int mode, int recon, double *Fi, long long Nf, double *scratch,
+                        int nthreads, m2c::Status *st, const m2c::Clock *clk, int ignore_active);
 int m2c_mp_launch_godunov_batch(hipStream_t s, int dir, int n, const double *Vm, const double *Vp, int id, double *F,
                                 double *scratch, int nthreads);
 int m2c_mp_launch_iface(const m2c::MpLaunch &L, const double *S, int in_cons, int store, int ignore_active,

@@ -190,6 +190,16 @@ class SpaceOperator {
   void ComputeTimeStepSize(double &dt, double &cfl) {
     check(m2c_time_step_size(c_.get(), &dt, &cfl), "ComputeTimeStepSize");
   }
+  // The EmbeddedBoundaryDataSet argument of ComputeResidual: faces cut by embedded walls, as
+  // FindEdgeSurfaceIntersections reports them (face[4t] axis, then the plus cell's i, j, k;
+  // wall[12t..] nwallf, vwallf, nwallb, vwallb), kept for every later stage until replaced
+  void SetEmbeddedFaces(const std::vector<int> &face, const std::vector<double> &wall, int riemann_normal = 1,
+                        int recon = 0) {
+    if (face.size() % 4 || wall.size() != 3 * face.size())
+      throw std::runtime_error("SetEmbeddedFaces: 4 ints and 12 doubles per face");
+    check(m2c_set_embedded_faces(c_.get(), (int)(face.size() / 4), face.data(), wall.data(), riemann_normal, recon),
+          "m2c_set_embedded_faces");
+  }
 
  private:
   Context &c_;

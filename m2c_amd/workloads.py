@@ -15,6 +15,10 @@ m2c_amd.make_config, V the primitive state (nz, ny, nx, 5) and ID the material I
                     (van Albada), RK2, CFL 0.5. `scale` shrinks every length for small test runs.
   periodic_viscous  periodic_modes with constant viscosity (mu 0.01, bulk 0.002) and heat diffusion
                     (kappa 0.02, T = p / rho)
+  sphere_obstacle   a supersonic gas stream (Mach 2 along x, gamma 1.4) past a sphere bounded by an
+                    embedded wall: cells inside the sphere are inactive and every mesh face between an
+                    inside and an outside cell is cut, its wall records holding the sphere's outward
+                    normal and the wall velocity at the crossing of the cell-centre edge
   batched_riemann   independent single-material Riemann-type instances: edge lengths drawn
                     uniformly from lo..hi per axis, left/right states log-uniform (rho in
                     [0.1, 10], p in [0.1, 100], u = 0) split by a random plane (normal uniform on
@@ -25,8 +29,8 @@ input format (IoData.h:266-267 has no periodic code); they are the documented ex
 """
 import numpy as np
 
-from . import (BC_INLET, BC_OUTLET, BC_PERIODIC, BC_SYMMETRY, EOS_SG, FLUX_HLLC, LIM_VANALBADA, TS_RK2,
-               TS_RK3, VISC_CONSTANT, material)
+from . import (BC_INLET, BC_OUTLET, BC_PERIODIC, BC_SLIPWALL, BC_SYMMETRY, EOS_SG, FLUX_HLLC, LIM_VANALBADA,
+               TS_RK2, TS_RK3, VISC_CONSTANT, material)
 
 FOURIER_SEED = 2024
 
@@ -236,6 +240,57 @@ under Time {{ Type = Explicit; CFL = 0.5; under Explicit {{ Type = RungeKutta2; 
 
 
 BATCH_SEED = 4242
+
+
+def sphere_obstacle(nx=48, ny=32, nz=32, radius=6.0, centre=None, vwall=(0.0, 0.0, 0.0), seed=5, amp=0.05):
+    """A Mach-2 gas stream past a sphere of `radius` cells: (settings, V, ID, faces, walls). Material 1
+    is inactive (the inside of the sphere); `faces` (n, 4) lists every axis, i, j, k (plus cell) of a
+    face between an inside and an outside cell, `walls` (n, 12) its forward and backward wall records
+    (unit outward normal at the crossing of the segment between the two cell centres, wall velocity).
+    The free stream carries a seeded perturbation of relative amplitude `amp`."""
+    cx, cy, cz = centre if centre is not None else (nx / 3.0, ny / 2.0, nz / 2.0)
+    gas = material(EOS_SG, [1.4, 0.0])
+    rho, p = 1.0, 1.0
+    u = 2.0 * np.sqrt(1.4 * p / rho)
+    settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, float(nx), 0.0, float(ny), 0.0, float(nz)),
+                    bc=[BC_INLET, BC_OUTLET, BC_SLIPWALL, BC_SLIPWALL, BC_SLIPWALL, BC_SLIPWALL],
+                    mats=[gas, gas], inactive_id=1, inlet=(rho, u, 0.0, 0.0, p), outlet=(rho, u, 0.0, 0.0, p),
+                    flux=FLUX_HLLC, limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5)
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    inside = (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 < radius * radius
+    ID = inside.astype(np.int32)
+    rng = np.random.default_rng(seed)
+    V = np.zeros((nz, ny, nx, 5))
+    V[..., 0] = rho * (1.0 + amp * rng.uniform(-1.0, 1.0, (nz, ny, nx)))
+    V[..., 1] = u * (1.0 + amp * rng.uniform(-1.0, 1.0, (nz, ny, nx)))
+    V[..., 2] = amp * u * rng.uniform(-1.0, 1.0, (nz, ny, nx))
+    V[..., 4] = p * (1.0 + amp * rng.uniform(-1.0, 1.0, (nz, ny, nx)))
+    V[inside, 1:4] = 0.0
+    faces, walls = [], []
+    C = np.array([cx, cy, cz])
+    for d in range(3):
+        sl_m = [slice(None)] * 3
+        sl_p = [slice(None)] * 3
+        sl_m[2 - d] = slice(0, -1)
+        sl_p[2 - d] = slice(1, None)
+        cut = inside[tuple(sl_m)] != inside[tuple(sl_p)]
+        for kk, jj, ii in zip(*np.nonzero(cut)):
+            q = [ii, jj, kk]
+            q[d] += 1                                     # the plus cell
+            a = np.array([q[0] + 0.5, q[1] + 0.5, q[2] + 0.5])
+            a[d] -= 1.0                                    # minus cell centre
+            e = np.zeros(3)
+            e[d] = 1.0
+            # |a + t e - C|^2 = r^2 on t in (0, 1)
+            w = a - C
+            bq, cq = w[d], float(w @ w) - radius * radius
+            disc = np.sqrt(max(bq * bq - cq, 0.0))
+            t = -bq + disc if cq < 0.0 else -bq - disc
+            pt = a + t * e
+            nrm = (pt - C) / np.sqrt(float((pt - C) @ (pt - C)))
+            faces.append([d, q[0], q[1], q[2]])
+            walls.append(np.concatenate([nrm, vwall, nrm, vwall]))
+    return settings, V, ID, np.array(faces, dtype=np.int32).reshape(-1, 4), np.array(walls).reshape(-1, 12)
 
 
 def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0), p_range=(0.1, 100.0), select=None):

@@ -85,7 +85,8 @@ void orc_destroy(orc_ctx *c) {
   for (int f = 0; f < 6; f++) free(c->face[f]);
   free(c->Un); free(c->U1); free(c->R); free(c->V1); free(c->V2);
   free(c->Fx); free(c->Fy); free(c->Fz); free(c->ferr);
-  for (int d = 0; d < 3; d++) { free(c->Dv[d]); free(c->Dh[d]); }
+  for (int d = 0; d < 3; d++) { free(c->Dv[d]); free(c->Dh[d]); free(c->eb_rec[d]); }
+  free(c->eb_wall);
   orc_ls_free(c);
   free(c);
 }
@@ -94,6 +95,45 @@ void orc_counters(const orc_ctx *c, long *rev, long *clip, long *rerr) {
   if (rev) *rev = c->n_reverted;
   if (clip) *clip = c->n_clipped;
   if (rerr) *rerr = c->n_riemann_err;
+}
+
+int orc_set_embedded_faces(orc_ctx *c, int n, const int *face, const double *wall, int normal_mode, int recon) {
+  const long nf = (long)(c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+  for (int d = 0; d < 3; d++) { free(c->eb_rec[d]); c->eb_rec[d] = NULL; }
+  free(c->eb_wall);
+  c->eb_wall = NULL;
+  c->eb_n = 0;
+  if (n <= 0) return 0;
+  if (normal_mode < 0 || normal_mode > 2 || recon < 0 || recon > 1) return -1;
+  for (int d = 0; d < 3; d++) {
+    c->eb_rec[d] = (int *)malloc(sizeof(int) * nf);
+    for (long f = 0; f < nf; f++) c->eb_rec[d][f] = -1;
+  }
+  c->eb_wall = (double *)malloc(sizeof(double) * 12 * n);
+  memcpy(c->eb_wall, wall, sizeof(double) * 12 * n);
+  const int N[3] = {c->nx, c->ny, c->nz};
+  for (int t = 0; t < n; t++) {
+    const int d = face[4 * t], q[3] = {face[4 * t + 1], face[4 * t + 2], face[4 * t + 3]};
+    int ok = d >= 0 && d <= 2;
+    for (int a = 0; ok && a < 3; a++)
+      ok = a == d ? (q[a] >= 1 && q[a] <= N[a] - 1) : (q[a] >= 0 && q[a] < N[a]);
+    if (!ok || c->eb_rec[d][ORC_FKEY(c, q[0], q[1], q[2])] >= 0) {
+      orc_set_embedded_faces(c, 0, NULL, NULL, 0, 0);
+      return -1;
+    }
+    c->eb_rec[d][ORC_FKEY(c, q[0], q[1], q[2])] = t;
+  }
+  c->eb_n = n;
+  c->eb_normal_mode = normal_mode;
+  c->eb_recon = recon;
+  return 0;
+}
+
+/* the wall record of face (d, plus cell i,j,k), or NULL */
+static inline const double *eb_face(const orc_ctx *c, int d, int i, int j, int k) {
+  if (!c->eb_n || i < 0 || j < 0 || k < 0 || i > c->nx || j > c->ny || k > c->nz) return NULL;
+  const int r = c->eb_rec[d][ORC_FKEY(c, i, j, k)];
+  return r >= 0 ? c->eb_wall + 12 * r : NULL;
 }
 
 void orc_numerical_flux(const orc_ctx *c, int dir, const double *Vm, const double *Vp, int id, double *F) {
@@ -233,6 +273,11 @@ static void recon_cell(orc_ctx *c, const double *V, const int *ID, int i, int j,
     }
     if (g->slope_zero_near_interface && (myid != ID[lo] || myid != ID[hi]))
       for (int p = 0; p < 5; p++) sig[d][p] = 0.0;
+    /* Reconstructor.cpp:656-664: also where the cell's low or high face along d is cut by an
+     * embedded wall */
+    if (g->slope_zero_near_interface && c->eb_n &&
+        (eb_face(c, d, i, j, k) || eb_face(c, d, i + (d == 0), j + (d == 1), k + (d == 2))))
+      for (int p = 0; p < 5; p++) sig[d][p] = 0.0;
   }
   for (int p = 0; p < 5; p++) {
     out[0][p] = v[p] - 0.5 * sig[0][p];
@@ -333,13 +378,37 @@ static int check_reconstructed(orc_ctx *c, const double *V, const int *ID, long 
 
 /* ------------------------------------------------------------------ residual ------------ */
 
-/* one face: localflux1 (to the minus cell) and localflux2 (to the plus cell) */
-static int face_flux(orc_ctx *c, int dir, const double *V, const int *ID, long lo, long hi,
+/* one face: localflux1 (to the minus cell) and localflux2 (to the plus cell); the number of
+ * Riemann-solver errors */
+static int face_flux(orc_ctx *c, int dir, const double *V, const int *ID, long lo, long hi, const double *wall,
                      double *lf1, double *lf2) {
   const orc_config *g = &c->cfg;
   int nbid = ID[lo], myid = ID[hi];
   const double *vm = c->face[2 * dir + 1] + 5 * lo;   /* vr / vt / vf of the minus cell */
   const double *vp = c->face[2 * dir] + 5 * hi;       /* vl / vb / vk of the plus cell  */
+  if (wall) {
+    /* a face cut by an embedded wall (SpaceOperator.cpp:2438-2484): each active side solves its
+     * one-sided problem against the wall (forward intersection for the minus cell, backward for
+     * the plus cell) and takes the flux between its state and the clipped star state. Where the
+     * reference exits (a negative status) the side's flux is zero and an error is counted. */
+    double Vs[5];
+    int nerr = 0;
+    memset(lf1, 0, 5 * sizeof(double));
+    memset(lf2, 0, 5 * sizeof(double));
+    if (nbid != g->inactive_id) {
+      const double *am = c->eb_recon == 0 ? V + 5 * lo : vm;
+      int e = orc_wall_face_flux(c, dir, 1, c->eb_normal_mode, wall, wall + 3, am, nbid, lf1, Vs);
+      if (e < 0) memset(lf1, 0, 5 * sizeof(double));
+      nerr += e != 0;
+    }
+    if (myid != g->inactive_id) {
+      const double *ap = c->eb_recon == 0 ? V + 5 * hi : vp;
+      int e = orc_wall_face_flux(c, dir, -1, c->eb_normal_mode, wall + 6, wall + 9, ap, myid, lf2, Vs);
+      if (e < 0) memset(lf2, 0, 5 * sizeof(double));
+      nerr += e != 0;
+    }
+    return nerr;
+  }
   if (nbid != myid) {
     if (nbid != g->inactive_id && myid != g->inactive_id) {
       const double *am = (g->mp_recon == ORC_REC_CONSTANT) ? V + 5 * lo : vm;
@@ -349,7 +418,7 @@ static int face_flux(orc_ctx *c, int dir, const double *V, const int *ID, long l
         memcpy(lf2, lf1, 5 * sizeof(double));
         return 0;
       }
-      return orc_interface_flux(c, dir, am, nbid, ap, myid, lo, hi, lf1, lf2);
+      return orc_interface_flux(c, dir, am, nbid, ap, myid, lo, hi, lf1, lf2) != 0;
     }
     memset(lf1, 0, 5 * sizeof(double));
     memset(lf2, 0, 5 * sizeof(double));
@@ -384,15 +453,18 @@ int orc_compute_residual(orc_ctx *c, const double *V, const int *ID, double *R) 
       for (int i = 0; i <= nx; i++) {
         if (j < ny && k < nz) {      /* x face i-1/2 */
           double *F = c->Fx + 10 * (((long)k * ny + j) * (nx + 1) + i);
-          rerr += face_flux(c, 0, V, ID, ORC_IDX(c, i - 1, j, k), ORC_IDX(c, i, j, k), F, F + 5) != 0;
+          rerr += face_flux(c, 0, V, ID, ORC_IDX(c, i - 1, j, k), ORC_IDX(c, i, j, k), eb_face(c, 0, i, j, k),
+                            F, F + 5);
         }
         if (i < nx && k < nz) {      /* y face j-1/2 */
           double *F = c->Fy + 10 * (((long)k * (ny + 1) + j) * nx + i);
-          rerr += face_flux(c, 1, V, ID, ORC_IDX(c, i, j - 1, k), ORC_IDX(c, i, j, k), F, F + 5) != 0;
+          rerr += face_flux(c, 1, V, ID, ORC_IDX(c, i, j - 1, k), ORC_IDX(c, i, j, k), eb_face(c, 1, i, j, k),
+                            F, F + 5);
         }
         if (i < nx && j < ny) {      /* z face k-1/2 */
           double *F = c->Fz + 10 * (((long)k * ny + j) * nx + i);
-          rerr += face_flux(c, 2, V, ID, ORC_IDX(c, i, j, k - 1), ORC_IDX(c, i, j, k), F, F + 5) != 0;
+          rerr += face_flux(c, 2, V, ID, ORC_IDX(c, i, j, k - 1), ORC_IDX(c, i, j, k), eb_face(c, 2, i, j, k),
+                            F, F + 5);
         }
       }
   }

@@ -130,6 +130,15 @@ void orc_onesided_normal(int d, int fb, int mode, const double *nwall, double *d
 int orc_wall_face_flux(const orc_ctx *c, int axis, int side, int mode, const double *nwall, const double *vwall,
                        const double *V, int id, double *F, double *Vstar);
 
+/* Mesh faces cut by embedded walls, for orc_compute_residual and orc_run (SpaceOperator.cpp:
+ * 2438-2484, 2569-2611, 2696-2737; FindEdgeSurfaceIntersections :3434-3500 is the caller's): face t
+ * is normal to axis face[4t] with plus cell (face[4t+1], face[4t+2], face[4t+3]); wall[12t..] holds
+ * nwallf, vwallf (forward intersection, seen from the minus cell) and nwallb, vwallb (backward,
+ * seen from the plus cell). normal_mode is EmbeddedBoundaryMethod RiemannNormal (0 EMBEDDED_SURFACE,
+ * 1 MESH, 2 AVERAGE), recon its ReconstructionAtInterface (0 CONSTANT, 1 LINEAR). Faces on the domain
+ * boundary are rejected. n = 0 removes the walls. Returns 0, or -1 on a bad face. */
+int orc_set_embedded_faces(orc_ctx *c, int n, const int *face, const double *wall, int normal_mode, int recon);
+
 /* Pointwise EOS of material id: PrimitiveToConservative / ConservativeToPrimitive
  * (VarFcnBase.h:169-197) and the squared sound speed at (rho, p) (VarFcnBase.h:215-219). */
 void orc_eos_p2c(const orc_ctx *c, int id, const double *V, double *U);

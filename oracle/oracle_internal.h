@@ -34,7 +34,15 @@ struct orc_ctx {
   long n_reinit_iters, n_unresolved;
   int steps_done;           /* time steps of earlier orc_run calls: the numbering continues */
   double t_done;            /* and their time */
+  /* mesh faces cut by embedded walls (orc_set_embedded_faces): per direction and face key
+   * ((k*(ny+1)+j)*(nx+1)+i, keyed by the plus cell) the wall record or -1; 12 doubles per record
+   * (nwallf, vwallf, nwallb, vwallb) */
+  int *eb_rec[3];
+  double *eb_wall;
+  int eb_n, eb_normal_mode, eb_recon;
 };
+
+#define ORC_FKEY(c, i, j, k) ((((long)(k)) * ((c)->ny + 1) + (j)) * (long)((c)->nx + 1) + (i))
 
 #define ORC_IDX(c, i, j, k) ((((long)(k) + 1) * (c)->gy + ((j) + 1)) * (long)(c)->gx + ((i) + 1))
 

@@ -156,6 +156,7 @@ def lib():
         L.orc_riemann_onesided.argtypes = [P, D, D, C.c_int, D, D, I, D]
         L.orc_onesided_normal.argtypes = [C.c_int, C.c_int, C.c_int, D, D]
         L.orc_wall_face_flux.argtypes = [P, C.c_int, C.c_int, C.c_int, D, D, D, C.c_int, D, D]
+        L.orc_set_embedded_faces.argtypes = [P, C.c_int, I, D, C.c_int, C.c_int]
         L.orc_eos_p2c.argtypes = [P, C.c_int, D, D]
         L.orc_eos_c2p.argtypes = [P, C.c_int, D, D]
         L.orc_eos_c2.argtypes = [P, C.c_int, C.c_double, C.c_double]
@@ -299,6 +300,13 @@ class Oracle:
         F, Vs = np.zeros(5), np.zeros(5)
         st = self.L.orc_wall_face_flux(self.h, axis, side, mode, _dp(nw), _dp(vw), _dp(V), mid, _dp(F), _dp(Vs))
         return st, F, Vs
+
+    def set_embedded_faces(self, faces, walls, normal_mode=1, recon=0):
+        """Wall-cut faces for later residuals and runs: faces (n, 4) = axis, i, j, k of the plus cell;
+        walls (n, 12) = nwallf, vwallf, nwallb, vwallb. Returns the oracle's status."""
+        f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).reshape(-1, 4))
+        w = np.ascontiguousarray(np.asarray(walls, dtype=np.float64).reshape(-1, 12))
+        return self.L.orc_set_embedded_faces(self.h, f.shape[0], _ip(f), _dp(w), normal_mode, recon)
 
     def interface_normal(self, phi, d, i, j, k, myid, nbid):
         """Interface Riemann direction at the d-face of (i,j,k) from the ghosted level set phi."""

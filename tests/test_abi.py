@@ -29,7 +29,8 @@ def test_header_declares_the_entry_points():
     for n in ["m2c_ctx_create", "m2c_ctx_destroy", "m2c_upload_state", "m2c_download_state",
               "m2c_compute_residual", "m2c_time_step_size", "m2c_advance_dt", "m2c_run",
               "m2c_get_status", "m2c_flux_batch", "m2c_riemann_batch", "m2c_upload_levelset",
-              "m2c_download_levelset", "m2c_varfcn_batch", "m2c_interface_flux_batch", "m2c_set_kchunk"]:
+              "m2c_download_levelset", "m2c_varfcn_batch", "m2c_interface_flux_batch", "m2c_set_kchunk",
+              "m2c_set_embedded_faces"]:
         assert n in names
 
 
