@@ -112,6 +112,32 @@ static __constant__ Grid c_grid;
 #define M2C_G c_grid
 #endif
 
+// Material constants of a cell with ID id. The stiffened-gas variants (one material, or two with
+// IDs 0 and 1) read only the seven constants assigned below; they come from uniform loads of the
+// table and a select by ID instead of a per-lane load of the entry, which would put a memory
+// latency inside every face-state check and flux.
+template <int EK, bool MP> __device__ __forceinline__ decltype(auto) mat_of(int id) {
+  if constexpr (EK == EOS_SG) {
+    DevEos m{};
+    const DevEos &a = c_eos[0];
+    if constexpr (MP) {
+      const DevEos &b = c_eos[1];
+      const bool o = id != 0;
+      m.rhomin = o ? b.rhomin : a.rhomin; m.pmin = o ? b.pmin : a.pmin;
+      m.rhomax = o ? b.rhomax : a.rhomax; m.pmax = o ? b.pmax : a.pmax;
+      m.pstiff = o ? b.pstiff : a.pstiff; m.gam1 = o ? b.gam1 : a.gam1;
+      m.gam_pstiff = o ? b.gam_pstiff : a.gam_pstiff;
+    } else {
+      m.rhomin = a.rhomin; m.pmin = a.pmin; m.rhomax = a.rhomax; m.pmax = a.pmax;
+      m.pstiff = a.pstiff; m.gam1 = a.gam1; m.gam_pstiff = a.gam_pstiff;
+    }
+    m.type = EOS_SG;
+    return m;
+  } else {
+    return (c_eos[id]);
+  }
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // image index of a ghost cell (SpaceOperator.cpp:449-480); periodic wraps
@@ -121,38 +147,62 @@ __device__ __forceinline__ int image_idx(int i, int n, bool per) {
   return i;
 }
 
-// Load the stencil value of (possibly ghost) cell (i,j,k).
-template <int IN_CONS, int EK>
-__device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const int8_t *__restrict__ ID, int i,
-                                           int j, int k, double *V, int &id, unsigned &flags) {
+// A cell fetch in two halves: fetch_begin issues the loads of the (possibly ghost) cell (i,j,k),
+// fetch_end turns what arrived into the stencil value (ConservativeToPrimitive of a conservative
+// stage field, the boundary rule's sign flips), so that a prefetch can be converted once its loads
+// have landed. `inner`: the caller knows (i,j,k) lies in the box (no ghost rules apply).
+struct Fetched {
+  double raw[5];
+  int id, bc, side;
+};
+
+__device__ __forceinline__ void fetch_begin(const double *__restrict__ S, const int8_t *__restrict__ ID, int i,
+                                            int j, int k, Fetched &f, bool inner = false) {
   const Grid &g = M2C_G;
-  int oi = (i < 0) ? 0 : (i >= g.nx ? 1 : -1);
-  int oj = (j < 0) ? 2 : (j >= g.ny ? 3 : -1);
-  int ok = (k < 0) ? 4 : (k >= g.nz ? 5 : -1);
-  int nout = (oi >= 0) + (oj >= 0) + (ok >= 0);
-  int side = oi >= 0 ? oi : (oj >= 0 ? oj : ok);
-  int bc = (nout == 1) ? g.bc[side] : 0;
-  int ii = clampi(image_idx(i, g.nx, oi >= 0 && g.bc[oi] == BC_PERIODIC), 0, g.nx - 1);
-  int jj = clampi(image_idx(j, g.ny, oj >= 0 && g.bc[oj] == BC_PERIODIC), 0, g.ny - 1);
-  int kk = clampi(image_idx(k, g.nz, ok >= 0 && g.bc[ok] == BC_PERIODIC), 0, g.nz - 1);
-  long long idx = ((long long)kk * g.ny + jj) * g.nx + ii;
-  id = ID[idx];
+  long long idx;
+  int bc = 0, side = -1;
+  if (inner) {
+    idx = ((long long)k * g.ny + j) * g.nx + i;
+  } else {
+    int oi = (i < 0) ? 0 : (i >= g.nx ? 1 : -1);
+    int oj = (j < 0) ? 2 : (j >= g.ny ? 3 : -1);
+    int ok = (k < 0) ? 4 : (k >= g.nz ? 5 : -1);
+    int nout = (oi >= 0) + (oj >= 0) + (ok >= 0);
+    side = oi >= 0 ? oi : (oj >= 0 ? oj : ok);
+    bc = (nout == 1) ? g.bc[side] : 0;
+    int ii = clampi(image_idx(i, g.nx, oi >= 0 && g.bc[oi] == BC_PERIODIC), 0, g.nx - 1);
+    int jj = clampi(image_idx(j, g.ny, oj >= 0 && g.bc[oj] == BC_PERIODIC), 0, g.ny - 1);
+    int kk = clampi(image_idx(k, g.nz, ok >= 0 && g.bc[ok] == BC_PERIODIC), 0, g.nz - 1);
+    idx = ((long long)kk * g.ny + jj) * g.nx + ii;
+  }
+  f.id = ID[idx];
+  f.bc = bc;
+  f.side = side;
   if (bc == BC_INLET || bc == BC_OUTLET) {
     const double *s = (bc == BC_INLET) ? g.inlet : g.outlet;
 #pragma unroll
-    for (int p = 0; p < 5; p++) V[p] = s[p];
+    for (int p = 0; p < 5; p++) f.raw[p] = s[p];
     return;
   }
-  double raw[5];
 #pragma unroll
-  for (int p = 0; p < 5; p++) raw[p] = S[p * g.N + idx];
+  for (int p = 0; p < 5; p++) f.raw[p] = S[p * g.N + idx];
+}
+
+template <int IN_CONS, int EK, bool MP = (EK < 0)>
+__device__ __forceinline__ void fetch_end(const Fetched &f, double *V, unsigned &flags) {
+  const int bc = f.bc, side = f.side;
+  if (bc == BC_INLET || bc == BC_OUTLET) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) V[p] = f.raw[p];
+    return;
+  }
   if (IN_CONS) {
-    const DevEos &m = c_eos[id];
-    eos_c2p<EK>(m, raw, V);
+    const auto &m = mat_of<EK, MP>(f.id);
+    eos_c2p<EK>(m, f.raw, V);
     if (eos_clip<double, EK>(m, V)) flags |= ST_CLIP_INTERMEDIATE;
   } else {
 #pragma unroll
-    for (int p = 0; p < 5; p++) V[p] = raw[p];
+    for (int p = 0; p < 5; p++) V[p] = f.raw[p];
   }
   if (bc == BC_SYMMETRY || bc == BC_SLIPWALL) {
     if (side < 2) V[1] *= -1.0;
@@ -161,6 +211,16 @@ __device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const i
   } else if (bc == BC_STICKWALL) {
     V[1] = -1.0 * V[1]; V[2] = -1.0 * V[2]; V[3] = -1.0 * V[3];
   }
+}
+
+// Load the stencil value of (possibly ghost) cell (i,j,k).
+template <int IN_CONS, int EK, bool MP = (EK < 0)>
+__device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const int8_t *__restrict__ ID, int i,
+                                           int j, int k, double *V, int &id, unsigned &flags, bool inner = false) {
+  Fetched f;
+  fetch_begin(S, ID, i, j, k, f, inner);
+  fetch_end<IN_CONS, EK, MP>(f, V, flags);
+  id = f.id;
 }
 
 template <int DIM, typename TC> __device__ __forceinline__ TC limited_slope(TC dq0, TC dq1) {
@@ -220,11 +280,11 @@ __device__ __forceinline__ void cell_slopes(const TC *vl, const TC *vc, const TC
 }
 
 // Face state v +/- sigma/2 with the revert rule; returns 1 if reverted.
-template <int EK, bool PLUS, typename TC>
+template <int EK, bool PLUS, typename TC, bool MP = (EK < 0)>
 __device__ __forceinline__ int face_state(const TC *vc, const TC *sig, int id, TC *out) {
 #pragma unroll
   for (int p = 0; p < 5; p++) out[p] = PLUS ? vc[p] + TC(0.5) * sig[p] : vc[p] - TC(0.5) * sig[p];
-  const DevEos &m = c_eos[id];
+  const auto &m = mat_of<EK, MP>(id);
   if (id != M2C_G.inactive_id && (eos_bad<TC, EK>(m, out) || eos_would_clip<TC, EK>(m, out))) {
 #pragma unroll
     for (int p = 0; p < 5; p++) out[p] = vc[p];
@@ -248,9 +308,9 @@ __device__ __forceinline__ void ghost_face(int bc, const TC *img_face, const TC 
   }
 }
 
-template <int FLUX, int EK, int DIR, typename TC = double>
+template <int FLUX, int EK, int DIR, typename TC = double, bool MP = (EK < 0)>
 __device__ __forceinline__ void numerical_flux(const TC *Vm, const TC *Vp, int id, TC *F) {
-  const DevEos &m = c_eos[id];
+  const auto &m = mat_of<EK, MP>(id);
   if (FLUX == FLUX_HLLC) flux_hllc<TC, EK, DIR>(m, Vm, Vp, F);
   else if (FLUX == FLUX_ROE) flux_roe<TC, EK, DIR>(m, Vm, Vp, TC(M2C_G.roe_delta), F);
   else if (FLUX == FLUX_GODUNOV) {   // the Godunov pass supplies these faces (G arrays)
@@ -266,7 +326,7 @@ __device__ __forceinline__ int face_flux(const TC *Vm, const TC *Vp, const TC *c
                                          TC *F) {
   const Grid &g = M2C_G;
   if (idm == idp && idp != g.inactive_id) {
-    numerical_flux<FLUX, EK, DIR, TC>(Vm, Vp, idp, F);
+    numerical_flux<FLUX, EK, DIR, TC, MP>(Vm, Vp, idp, F);
     return 0;
   }
 #pragma unroll
@@ -339,14 +399,14 @@ __device__ __forceinline__ void tile_face(TC (*sV)[LY][LX], int (*sID)[LX], cons
     const int idm = sID[ym + HALO][xm + HALO], idp = sID[y + HALO][x + HALO];
     const bool per = g.bc[2 * DIR] == BC_PERIODIC;
     if (gpl == 0 && !per) {
-      nrev += face_state<EK, false, TC>(vcp, sgp, idp, fp);
+      nrev += face_state<EK, false, TC, MP>(vcp, sgp, idp, fp);
       ghost_face<DIR, TC>(g.bc[2 * DIR], fp, vcm, fm);
     } else if (gpl == gn && !per) {
-      nrev += face_state<EK, true, TC>(vcm, sgm, idm, fm);
+      nrev += face_state<EK, true, TC, MP>(vcm, sgm, idm, fm);
       ghost_face<DIR, TC>(g.bc[2 * DIR + 1], fm, vcp, fp);
     } else {
-      nrev += face_state<EK, true, TC>(vcm, sgm, idm, fm);
-      nrev += face_state<EK, false, TC>(vcp, sgp, idp, fp);
+      nrev += face_state<EK, true, TC, MP>(vcm, sgm, idm, fm);
+      nrev += face_state<EK, false, TC, MP>(vcp, sgp, idp, fp);
     }
     face_flux<FLUX, EK, DIR, TC, MP>(fm, fp, vcm, vcp, idm, idp, F);
   }
@@ -392,6 +452,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
   // rules), so a partial tile's neighbours of the last cells are correct
   const int ci = min(gi, g.nx + 1), cj = min(gj, g.ny + 1);
   const bool zper = g.bc[4] == BC_PERIODIC;
+  // the block's x/y window (tile and halo) lies in the box: its fetches need no ghost rules
+  const bool xy_in = i0 - HALO >= 0 && i0 + TX + HALO <= g.nx && j0 - HALO >= 0 && j0 + TY + HALO <= g.ny;
   unsigned flags = 0;
   unsigned long long nrev = 0;
   double cfl_min = 1.7976931348623157e308;
@@ -411,10 +473,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     double d[5];
     TC vm2[5], vm1[5];
     int idm2, idm1;
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 2, d, idm2, flags); to_tc(d, vm2);
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg - 1, d, idm1, flags); to_tc(d, vm1);
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg, w0d, id0, flags); to_tc(w0d, w0);
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 1, d, id1, flags); to_tc(d, w1);
+    fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, kbeg - 2, d, idm2, flags); to_tc(d, vm2);
+    fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, kbeg - 1, d, idm1, flags); to_tc(d, vm1);
+    fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, kbeg, w0d, id0, flags); to_tc(w0d, w0);
+    fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, kbeg + 1, d, id1, flags); to_tc(d, w1);
     TC szm[5];
     cell_slopes<2, TC>(vm2, vm1, w0, idm2, idm1, id0, szm);
     cell_slopes<2, TC>(vm1, w0, w1, idm1, id0, id1, sz0);
@@ -430,15 +492,15 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       }
     }
     TC fm[5], fp[5];
-    nrev += face_state<EK, false, TC>(w0, sz0, id0, fp);
+    nrev += face_state<EK, false, TC, MP>(w0, sz0, id0, fp);
     if (kbeg == 0 && !zper) {
       ghost_face<2, TC>(g.bc[4], fp, vm1, fm);
     } else {
-      nrev += face_state<EK, true, TC>(vm1, szm, idm1, fm);
+      nrev += face_state<EK, true, TC, MP>(vm1, szm, idm1, fm);
     }
     lo_iface = face_flux<FLUX, EK, 2, TC, MP>(fm, fp, vm1, w0, idm1, id0, Flo);
     idlo = idm1;
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, kbeg + 2, d, id2, flags); to_tc(d, w2);
+    fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, kbeg + 2, d, id2, flags); to_tc(d, w2);
   }
 
   // halo cell owned by this thread (x halo: 4 columns x TY rows, then y halo: 4 rows x TX columns)
@@ -458,7 +520,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
   int hid = 0;
   if (tid < NHALO) {
     double d[5];
-    fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, kbeg, d, hid, flags);
+    fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, hi, hj, kbeg, d, hid, flags);
     to_tc(d, hv);
   }
 
@@ -467,13 +529,12 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     double w3d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     TC hvn[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
     int id3 = 0, hidn = 0;
-    if (k + 1 < kend) {
-      fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 3, w3d, id3, flags);
-      if (tid < NHALO) {
-        double d[5];
-        fetch_cell<IN_CONS, EK>(a.S, a.ID, hi, hj, k + 1, d, hidn, flags);
-        to_tc(d, hvn);
-      }
+    const bool in3 = xy_in && k + 3 < g.nz, in1 = xy_in && k + 1 < g.nz;
+    const bool more = k + 1 < kend;
+    Fetched f3, fh;
+    if (more) {
+      fetch_begin(a.S, a.ID, ci, cj, k + 3, f3, in3);
+      if (tid < NHALO) fetch_begin(a.S, a.ID, hi, hj, k + 1, fh, in1);
     }
 
     // ---- plane k into LDS: own cells and the halo cell from registers
@@ -563,11 +624,11 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     int hi_iface;
     {
       TC fm[5], fp[5];
-      nrev += face_state<EK, true, TC>(w0, sz0, id0, fm);
+      nrev += face_state<EK, true, TC, MP>(w0, sz0, id0, fm);
       if (k + 1 == g.nz && !zper) {
         ghost_face<2, TC>(g.bc[5], fm, w1, fp);
       } else {
-        nrev += face_state<EK, false, TC>(w1, sz1, id1, fp);
+        nrev += face_state<EK, false, TC, MP>(w1, sz1, id1, fp);
       }
       hi_iface = face_flux<FLUX, EK, 2, TC, MP>(fm, fp, w0, w1, id0, id1, Fhi);
     }
@@ -673,7 +734,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       // viscous and heat-diffusion faces of this cell (SpaceOperator.cpp:3367-3371): added after the
       // advective sum, own faces first, then the east, north and front faces (ViscosityOperator.cpp:
       // 259-393, HeatDiffusionOperator.cpp:96-131)
-      const bool diff = a.Dv[0] != nullptr;
+            const bool diff = a.Dv[0] != nullptr;
       const long long fxw = ((long long)k * g.ny + gj) * (g.nx + 1) + gi;
       const long long fys = ((long long)k * (g.ny + 1) + gj) * g.nx + gi;
       const long long fzb = ((long long)k * g.ny + gj) * g.nx + gi;
@@ -712,7 +773,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
 #pragma unroll
         for (int p = 0; p < 5; p++) a.Out[p * g.N + idx] = R[p];
       } else {
-        const DevEos &m = c_eos[id0];
+        const auto &m = mat_of<EK, MP>(id0);
         double vn[5], un[5], U[5];
         if (IN_CONS) {
 #pragma unroll
@@ -761,6 +822,17 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       }
     }
 
+    // ---- the prefetched cells, converted now that their loads have landed
+    if (more) {
+      fetch_end<IN_CONS, EK, MP>(f3, w3d, flags);
+      id3 = f3.id;
+      if (tid < NHALO) {
+        double d[5];
+        fetch_end<IN_CONS, EK, MP>(fh, d, flags);
+        to_tc(d, hvn);
+        hidn = fh.id;
+      }
+    }
     // ---- shift the z window
 #pragma unroll
     for (int p = 0; p < 5; p++) {
@@ -776,7 +848,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       // V_n exactly)
         int dummy;
         unsigned fl = 0;
-        fetch_cell<IN_CONS, EK>(a.S, a.ID, ci, cj, k + 1, w0d, dummy, fl);
+        fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, k + 1, w0d, dummy, fl);
       }
     }
     lo_iface = hi_iface;

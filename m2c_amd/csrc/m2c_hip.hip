@@ -997,9 +997,9 @@ int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   bool all_sg = true;
   for (int m = 0; m < cfg->nmat; m++) all_sg = all_sg && cfg->mat[m].eos == M2C_EOS_SG;
   int v;
-  // HLLC over stiffened gases only: the compile-time stiffened-gas variants, with the material-interface
-  // code (9, 10) when several materials meet
-  if (cfg->flux == M2C_FLUX_HLLC) v = (all_sg && cfg->inactive_id < 0) ? (g.has_interfaces ? 9 : 0) : 1;
+  // HLLC over one or two stiffened gases: the compile-time stiffened-gas variants, with the
+  // material-interface code (9, 10) when the two materials meet
+  if (cfg->flux == M2C_FLUX_HLLC) v = (all_sg && cfg->inactive_id < 0 && cfg->nmat <= 2) ? (g.has_interfaces ? 9 : 0) : 1;
   else v = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
   if (cfg->precision == M2C_PREC_MIXED) v = (v == 9) ? 10 : v + 4;
   if (cfg->flux == M2C_FLUX_GODUNOV) v = 8;
