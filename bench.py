@@ -21,8 +21,9 @@ Beside it, in the same line:
 Fourier modes of amplitude 1e-2 and a Gaussian pressure pulse), gamma = 1.4, MUSCL + van Albada,
 HLLC, RK3, CFL 0.4, FP64, as its own line.
 
---workload batched_riemann: 64 independent instances (edges 48..160), mixed precision, one
-persistent launch per RK stage, hipGraph step replay, autotuned launch shape.
+--workload batched_riemann: 64 independent instances (edges 48..160), every 4th one two-material
+(level set, exact interface Riemann solves), mixed precision, one persistent launch per RK stage,
+hipGraph step replay, autotuned launch shape.
 
 --workload learned: the learned interface solver's kernels (inference roofline against the bf16
 MFMA peak, training steps) and its per-problem cost against the exact solver.
@@ -265,20 +266,18 @@ def variant_form(M, W, settings, V0, ID0, phi0, args, V64, value64, precision, n
     return out
 
 
-def bench_batched(M, W, args):
-    """64 independent Riemann-type instances (edges 48..160 per axis), mixed precision, RK2:
-    one persistent launch per RK stage, a hipGraph per step, autotuned launch shape. Compared with
-    the same instances advanced one context at a time."""
-    insts = W.batched_riemann(n=64)
-    cells = sum(V.shape[0] * V.shape[1] * V.shape[2] for _, V, _ in insts)
-    cfgs = [W.config_from(s, precision=M.PREC_MIXED) for s, _, _ in insts]
-    b = M.Batch(cfgs)
-    for i, (s, V0, ID0) in enumerate(insts):
-        b.upload(i, V0, ID0)
-    print(f"bench: batch of {len(insts)} instances, {cells} cells uploaded", file=sys.stderr, flush=True)
+def _run_batch(M, W, args, insts):
+    """Autotune, warm up and time one batch; returns (cells/s, ms per step, graph ms per step,
+    tuning, errors)."""
+    cells = sum(x[1].shape[0] * x[1].shape[1] * x[1].shape[2] for x in insts)
+    b = M.Batch([W.config_from(x[0], precision=M.PREC_MIXED) for x in insts])
+    for i, x in enumerate(insts):
+        b.upload(i, x[1], x[2])
+        if len(x) == 4:
+            b.upload_levelset(i, x[3])
     tune = b.autotune(trial_steps=2)
-    print(f"bench: autotune kchunk {tune.kchunk} blocks {tune.blocks} ({tune.ms_per_step:.2f} ms/step)",
-          file=sys.stderr, flush=True)
+    print(f"bench: autotune kchunk {tune.kchunk} blocks {tune.blocks} tile {tune.tile_x}x{tune.tile_y} "
+          f"({tune.ms_per_step:.2f} ms/step)", file=sys.stderr, flush=True)
     if args.warmup:
         b.run(args.warmup)
     t0 = time.perf_counter()
@@ -286,28 +285,50 @@ def bench_batched(M, W, args):
     el = time.perf_counter() - t0
     errs = [b.status(i).error for i in range(len(insts))]
     b.close()
-    value = cells * args.steps / el
-    # the same instances one context at a time (a bounded sample of 8 instances)
+    tuning = {"kchunk": tune.kchunk, "blocks": tune.blocks, "tile": [tune.tile_x, tune.tile_y],
+              "ms_per_step": tune.ms_per_step, "candidates": tune.candidates}
+    return cells * args.steps / el, el / args.steps * 1e3, ms / args.steps, tuning, int(sum(1 for e in errs if e))
+
+
+def bench_batched(M, W, args):
+    """64 independent Riemann-type instances (edges 48..160 per axis), every 4th one two-material
+    (a second gas behind its split plane, level set reinitialised every 5 steps, exact Riemann solves
+    at the interface faces), mixed precision, RK2: one persistent launch per RK stage, hipGraph step
+    replay, autotuned tile shape and launch shape. Beside it: the same 64 instances with one gas
+    each, and a sample of the instances advanced one context at a time."""
+    insts = W.batched_riemann(n=64, two_every=4)
+    cells = sum(x[1].shape[0] * x[1].shape[1] * x[1].shape[2] for x in insts)
+    ntwo = sum(len(x) == 4 for x in insts)
+    print(f"bench: batch of {len(insts)} instances ({ntwo} two-material), {cells} cells", file=sys.stderr,
+          flush=True)
+    value, ms_step, graph_ms, tuning, nerr = _run_batch(M, W, args, insts)
+    single = W.batched_riemann(n=64)
+    v1, ms1, g1, t1, e1 = _run_batch(M, W, args, single)
+    # the same instances one context at a time (a bounded sample of 8 instances, two of them
+    # two-material)
     nsolo, solo_steps = 8, 10
     tsolo, csolo = 0.0, 0
-    for s, V0, ID0 in insts[:nsolo]:
+    for x in insts[:nsolo]:
+        s, V0, ID0 = x[:3]
         sv = M.Solver(W.config_from(s, precision=M.PREC_MIXED))
         sv.upload(V0, ID0)
+        if len(x) == 4:
+            sv.upload_levelset(x[3])
         sv.run(1e300, 2, sync_every=0)
-        t1 = time.perf_counter()
+        t0 = time.perf_counter()
         sv.run(1e300, solo_steps, sync_every=0)
-        tsolo += time.perf_counter() - t1
+        tsolo += time.perf_counter() - t0
         csolo += V0.shape[0] * V0.shape[1] * V0.shape[2] * solo_steps
         sv.close()
     line = {"metric": "cell-updates/s on 1 GPU: plain FP64 vs mixed form at L1 err<=1e-5; % HBM peak",
             "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64+f32", "data": "synthetic",
             "config": {"workload": "batched_riemann_64x[48..160]^3_rk2_hllc_vanalbada_mixed", "cells": cells,
-                       "instances": len(insts), "errors": int(sum(1 for e in errs if e)),
-                       "autotune": {"kchunk": tune.kchunk, "blocks": tune.blocks,
-                                    "ms_per_step": tune.ms_per_step, "candidates": tune.candidates},
-                       "graph_ms_per_step": ms / args.steps,
+                       "instances": len(insts), "two_material_instances": ntwo, "errors": nerr,
+                       "autotune": tuning, "graph_ms_per_step": graph_ms,
+                       "single_material_batch": {"value": v1, "ms_per_step": ms1, "graph_ms_per_step": g1,
+                                                 "autotune": t1, "errors": e1},
                        "one_context_at_a_time": {"value": csolo / tsolo, "instances": nsolo, "steps": solo_steps}}}
     print(json.dumps(line), flush=True)
 

@@ -253,30 +253,42 @@ int m2c_wall_face_flux_batch(m2c_ctx *ctx, int n, const int *axis, const int *si
 int m2c_set_embedded_faces(m2c_ctx *ctx, int n, const int *face, const double *wall, int normal_mode, int recon);
 
 /* ---- many independent instances (batched) ------------------------------------------------
- * A batch holds n single-material instances (each its own grid, boundaries, state and
- * CFL-adaptive time step; one material table, flux, precision and RK2/RK3 scheme for all; the
- * flux HLLC, or Roe or LLF in FP64, else M2C_ERR_UNSUPPORTED). Every
- * RK stage of every instance runs in one persistent launch draining per-XCD tile queues; a time
- * step is captured once in a hipGraph and replayed. An instance evolves bit for bit as it would
- * in its own context (no counterpart in the reference, which runs one problem per process). */
+ * A batch holds n instances, each its own grid, boundaries, state and CFL-adaptive time step; one
+ * flux, precision and RK2/RK3 scheme for all, and one material table (the materials an instance
+ * defines are the leading entries of the first instance's); the flux HLLC, or Roe or LLF in FP64,
+ * else M2C_ERR_UNSUPPORTED. An instance is single-material, or two-material: materials meeting at
+ * interface faces (exact Riemann solves there), with a level set (advection, reinitialisation,
+ * material-ID and state update of swept cells). Every RK stage of every instance runs in one
+ * persistent launch draining per-XCD tile queues; the two-material instances' interface passes,
+ * level-set stages and end-of-step tasks run around those launches on the batch's stream. A time
+ * step is captured once in a hipGraph and replayed (with two-material instances, a
+ * reinitialisation step is replayed as two graphs with the host's check of the reinitialisation
+ * failsafe between them). An instance evolves bit for bit as it would in its own context (no
+ * counterpart in the reference, which runs one problem per process). */
 typedef struct m2c_batch m2c_batch;
 typedef struct {
   int kchunk;                       /* z planes per work item */
   int blocks;                       /* resident persistent workgroups */
   double ms_per_step;               /* best time per batch step among the candidates */
   int candidates;
+  int tile_x, tile_y;               /* columns per work item: 32 x 8 or 16 x 16 */
 } m2c_batch_tuning;
 int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out);
 int m2c_batch_destroy(m2c_batch *b);
 int m2c_batch_upload(m2c_batch *b, int i, const double *V, const int32_t *id);
 int m2c_batch_download(m2c_batch *b, int i, double *V, int32_t *id);
+/* level set of a two-material instance with one (nz*ny*nx doubles, as m2c_upload_levelset) */
+int m2c_batch_upload_levelset(m2c_batch *b, int i, const double *phi);
+int m2c_batch_download_levelset(m2c_batch *b, int i, double *phi);
 /* nsteps steps of every instance; use_graph: replay the captured step graph; ms: elapsed (HIP events) */
 int m2c_batch_run(m2c_batch *b, int nsteps, int use_graph, double *ms);
 int m2c_batch_status(m2c_batch *b, int i, m2c_status *st);
-/* times (z-chunk depth, resident workgroups) candidates for trial_steps steps each on a copy of
- * the state, keeps the fastest; the state, clocks and counters are restored */
+/* times (tile shape, z-chunk depth, resident workgroups) candidates for trial_steps steps each on
+ * a copy of the state, keeps the fastest; the state, clocks and counters are restored */
 int m2c_batch_autotune(m2c_batch *b, int trial_steps, m2c_batch_tuning *out);
 int m2c_batch_set_tuning(m2c_batch *b, int kchunk, int blocks);
+/* the tile shape of the work items: 32 x 8 or 16 x 16 columns (else M2C_ERR_UNSUPPORTED) */
+int m2c_batch_set_tile(m2c_batch *b, int tile_x, int tile_y);
 
 /* Learned interface solver (no counterpart in the reference). A 10-256-256-256-256-3 SiLU network
  * predicts the star state of a two-stiffened-gas Riemann problem along a mesh axis; the predicted

@@ -61,7 +61,8 @@ class DeckRun(C.Structure):
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("kchunk", C.c_int), ("blocks", C.c_int), ("ms_per_step", C.c_double), ("candidates", C.c_int)]
+    _fields_ = [("kchunk", C.c_int), ("blocks", C.c_int), ("ms_per_step", C.c_double), ("candidates", C.c_int),
+                ("tile_x", C.c_int), ("tile_y", C.c_int)]
 
 
 class Status(C.Structure):
@@ -199,10 +200,13 @@ def lib():
         L.m2c_batch_destroy.argtypes = [Pb]
         L.m2c_batch_upload.argtypes = [Pb, C.c_int, D, C.POINTER(C.c_int32)]
         L.m2c_batch_download.argtypes = [Pb, C.c_int, D, C.POINTER(C.c_int32)]
+        L.m2c_batch_upload_levelset.argtypes = [Pb, C.c_int, D]
+        L.m2c_batch_download_levelset.argtypes = [Pb, C.c_int, D]
         L.m2c_batch_run.argtypes = [Pb, C.c_int, C.c_int, D]
         L.m2c_batch_status.argtypes = [Pb, C.c_int, C.POINTER(Status)]
         L.m2c_batch_autotune.argtypes = [Pb, C.c_int, C.POINTER(BatchTuning)]
         L.m2c_batch_set_tuning.argtypes = [Pb, C.c_int, C.c_int]
+        L.m2c_batch_set_tile.argtypes = [Pb, C.c_int, C.c_int]
         F = C.POINTER(C.c_float)
         L.m2c_mlp_create.argtypes = [C.c_int, C.c_ulonglong, C.POINTER(P)]
         L.m2c_mlp_destroy.argtypes = [P]
@@ -571,8 +575,8 @@ def read_deck(path):
 
 
 class Batch:
-    """Many independent single-material instances advanced together (one persistent launch per
-    RK stage, a hipGraph per step)."""
+    """Many independent single- or two-material instances advanced together (one persistent launch
+    per RK stage, a hipGraph per step)."""
 
     def __init__(self, cfgs):
         self.L = lib()
@@ -608,6 +612,16 @@ class Batch:
              "m2c_batch_download")
         return V, ID
 
+    def upload_levelset(self, i, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        assert phi.shape == self.shapes[i]
+        _chk(self.L.m2c_batch_upload_levelset(self.h, i, _dp(phi)), "m2c_batch_upload_levelset")
+
+    def download_levelset(self, i):
+        phi = np.empty(self.shapes[i], dtype=np.float64)
+        _chk(self.L.m2c_batch_download_levelset(self.h, i, _dp(phi)), "m2c_batch_download_levelset")
+        return phi
+
     def run(self, nsteps, graph=True):
         ms = C.c_double()
         _chk(self.L.m2c_batch_run(self.h, nsteps, int(graph), C.byref(ms)), "m2c_batch_run")
@@ -625,6 +639,9 @@ class Batch:
 
     def set_tuning(self, kchunk, blocks):
         _chk(self.L.m2c_batch_set_tuning(self.h, kchunk, blocks), "m2c_batch_set_tuning")
+
+    def set_tile(self, tile_x, tile_y):
+        _chk(self.L.m2c_batch_set_tile(self.h, tile_x, tile_y), "m2c_batch_set_tile")
 
 
 NPARAM = 200963

@@ -32,14 +32,22 @@ VARIANTS = [("hllc_sg", 2, 0, "double"), ("hllc_gen", 2, -1, "double"), ("roe_ge
 MP_VARIANTS = {"hllc_sg_mp", "hllc_sg_mp_f32"}
 # The mixed-form variants (FP32 reconstruction and fluxes) take the hardware's FP32 reciprocal and
 # square root instead of the correctly rounded sequences; FP64 code is not affected by this flag.
-# batch.hip holds the same FP32 stage bodies and takes it too (batched and single runs stay equal).
+# The batched stage units holding FP32 stage bodies take it too (batched and single runs stay equal).
 FP32_FAST = ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 # The FP64 stage kernels are bound by the latency of their dependent FP64 chains at two waves per
 # SIMD; the scheduler that maximises instruction-level parallelism shortens the two-material
 # launch by ~2 % (rocprofv3, 384x192x192 shock-bubble) and leaves the results unchanged.
 ILP_SCHED = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
-UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", FP32_FAST),
+# the batched runner: its host unit, and one unit per stage variant holding both tile shapes
+BATCH_VARIANTS = [("hllc_sg", 2, 0, "double", 0), ("hllc_gen", 2, -1, "double", 1), ("roe_gen", 0, -1, "double", 1),
+                  ("llf_gen", 1, -1, "double", 1), ("hllc_sg_f32", 2, 0, "float", 0),
+                  ("hllc_gen_f32", 2, -1, "float", 1), ("hllc_sg_mp", 2, 0, "double", 1),
+                  ("hllc_sg_mp_f32", 2, 0, "float", 1)]
+UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", []),
          ("mlp", "mlp.hip", [])] + [
+    (f"bstage_{t}", "batch_stage.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}",
+                                        f"-DM2C_MP={mp}"] + (FP32_FAST if tc == "float" else []))
+    for t, f, e, tc, mp in BATCH_VARIANTS] + [
     (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"]
      + (["-DM2C_MP=1"] if t in MP_VARIANTS else []) + (FP32_FAST if tc == "float" else []) + (ILP_SCHED if t in ("hllc_sg", "hllc_sg_mp") else []))
     for t, f, e, tc in VARIANTS]

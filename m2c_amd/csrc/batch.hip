@@ -1,7 +1,11 @@
 // batch.hip -- many independent instances advanced by one persistent launch per RK stage.
 //
-// Each instance is a single-material finite-volume problem with its own grid, boundary
-// conditions, state and CFL-adaptive time step (its own device clock). A stage launch drains a
+// Each instance is a single- or two-material finite-volume problem with its own grid, boundary
+// conditions, state and CFL-adaptive time step (its own device clock). A two-material instance
+// (level set, exact Riemann solves at its material interface) keeps a single-instance context for
+// its interface-face lists, level set and end-of-step buffers (batch_ctx.hpp): its fluid stages run
+// in the same persistent launches as every other instance, and the batch runs its interface pass,
+// level-set stages and end-of-step tasks around them. A stage launch drains a
 // queue of (instance, tile, z-chunk) work items: one queue per XCD (a workgroup's XCD is its
 // block index modulo 8, the hardware's round-robin), holding a contiguous range of work items so
 // that neighbouring tiles of an instance share an XCD's L2; a block whose queue is empty steals
@@ -22,75 +26,10 @@
 #include <vector>
 
 #include "../../include/m2c_hip.h"
-#include "fv_kernels.hpp"
+#include "batch_ctx.hpp"
+#include "batch_kern.hpp"
 
 namespace m2c {
-
-constexpr int NXCD = 8;
-
-struct BatchInst {
-  Grid grid;
-  StageArgs args[3];           // per stage of the time scheme
-  int gx, gy, nzc;             // tiles in x, y; z chunks
-  unsigned tile0;              // first work item of this instance
-  long long cell0;             // offset of the instance's cells in the concatenated fields
-};
-
-struct TileQueue {
-  unsigned head[NXCD];
-  unsigned start[NXCD];
-  unsigned end[NXCD];
-};
-
-__device__ __forceinline__ void load_grid(const Grid &src) {
-  const int n = (int)(sizeof(Grid) / sizeof(int));
-  const int *s = reinterpret_cast<const int *>(&src);
-  int *d = reinterpret_cast<int *>(&s_bgrid);
-  for (int t = threadIdx.x; t < n; t += blockDim.x) d[t] = s[t];
-}
-
-__device__ __forceinline__ int find_instance(const BatchInst *in, int n, unsigned tile) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (in[mid].tile0 <= tile) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-template <int IN_CONS, int MODE, int FLUX, int EK, typename TC>
-__global__ __launch_bounds__(NTHR, 2) void k_batch_stage(const BatchInst *insts, int ninst, TileQueue *q,
-                                                          int stage) {
-  __shared__ int s_tile, s_inst;
-  const int myq = blockIdx.x % NXCD;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      int t = -1;
-      for (int r = 0; r < NXCD && t < 0; r++) {
-        const int qq = (myq + r) % NXCD;
-        if (q->head[qq] >= q->end[qq]) continue;
-        unsigned v = atomicAdd(&q->head[qq], 1u);
-        if (v < q->end[qq]) t = (int)v;
-      }
-      s_tile = t;
-      s_inst = t >= 0 ? find_instance(insts, ninst, (unsigned)t) : 0;
-    }
-    __syncthreads();
-    const int t = s_tile;
-    if (t < 0) break;
-    const BatchInst &B = insts[s_inst];
-    load_grid(B.grid);
-    __syncthreads();
-    const unsigned local = (unsigned)t - B.tile0;
-    const int per = B.gx * B.gy;
-    const int bz = (int)(local / per);
-    const int rem = (int)(local % per);
-    const StageArgs a = B.args[stage];
-    stage_body<IN_CONS, MODE, FLUX, EK, TC>(a, rem % B.gx, rem / B.gx, bz);
-    __syncthreads();
-  }
-}
 
 __global__ void k_queue_reset(TileQueue *q) {
   if (threadIdx.x < NXCD) q->head[threadIdx.x] = q->start[threadIdx.x];
@@ -130,15 +69,16 @@ __global__ void k_batch_clock_finalize(Clock *clk, int n, int advance) {
   c.minbits = 0x7FEFFFFFFFFFFFFFull;
 }
 
-// CFL minimum of every instance; blockIdx.y = instance (SpaceOperator.cpp:2128-2198)
-__global__ __launch_bounds__(256) void k_batch_cfl(const BatchInst *insts, const double *V, const int8_t *ID,
-                                                    Clock *clk, Status *st) {
-  const BatchInst &B = insts[blockIdx.y];
+// CFL minimum of every instance (list null) or of the listed ones; blockIdx.y = instance or list
+// entry (SpaceOperator.cpp:2128-2198)
+__global__ __launch_bounds__(256) void k_batch_cfl(const BatchInst *insts, const int *list, Clock *clk, Status *st) {
+  const int inst = list ? list[blockIdx.y] : (int)blockIdx.y;
+  const BatchInst &B = insts[inst];
   load_grid(B.grid);
   __syncthreads();
   const Grid &g = s_bgrid;
-  const double *Vi = V + 5 * B.cell0;
-  const int8_t *IDi = ID + B.cell0;
+  const double *Vi = B.Vc;
+  const int8_t *IDi = B.IDc;
   double mn = DBL_MAX;
   unsigned flags = 0;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
@@ -156,8 +96,8 @@ __global__ __launch_bounds__(256) void k_batch_cfl(const BatchInst *insts, const
     double loc = smin(g.dx / lf, smin(g.dy / lg, g.dz / lh));
     if (loc < mn) mn = loc;
   }
-  warp_min_atomic(&clk[blockIdx.y].minbits, mn);
-  if (flags) atomicOr(&st[blockIdx.y].bits, flags);
+  warp_min_atomic(&clk[inst].minbits, mn);
+  if (flags) atomicOr(&st[inst].bits, flags);
 }
 
 __global__ void k_batch_copy(const double *src, double *dst, long long n) {
@@ -189,7 +129,7 @@ struct m2c_batch {
   int n = 0, ts = M2C_TS_RK2, variant = 0, nstage = 2, flux = M2C_FLUX_HLLC;
   DevEos eos[8];
   hipStream_t stream = nullptr;
-  long long cells = 0;
+  long long cells = 0;          // cells of the single-material instances (concatenated fields)
   double *V = nullptr, *U1 = nullptr, *U2 = nullptr, *stage_buf = nullptr, *backup = nullptr;
   int8_t *ID = nullptr;
   Clock *clk = nullptr;
@@ -197,9 +137,17 @@ struct m2c_batch {
   BatchInst *dinst = nullptr;
   TileQueue *q = nullptr;
   unsigned ntiles = 0;
-  int kchunk = 8, nblocks = 512;
+  int kchunk = 8, nblocks = 512, shape = 0;   // shape: 0 = 32 x 8 column tiles, 1 = 16 x 16
+  // two-material instances: their contexts (null for a single-material instance) and their indices
+  std::vector<m2c_ctx *> mpc;
+  std::vector<int> mp_list;
+  int *d_mp_list = nullptr;
+  long long steps = 0;          // time steps taken (the reinitialisation schedule)
+  // captured steps: every step (single-material batches) or, with two-material instances, a step
+  // without reinitialisation, and the parts before and after the host's reinitialisation retries
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec_rhead = nullptr, exec_rtail = nullptr;
 };
 
 #define BHIP(x)                                                                   \
@@ -213,17 +161,32 @@ struct m2c_batch {
 
 static int batch_upload_tables(m2c_batch *b) {
   BHIP(hipMemcpyToSymbol(HIP_SYMBOL(c_eos), b->eos, sizeof(b->eos)));
+  if (m2c_bstage_upload_hllc_sg(b->eos) || m2c_bstage_upload_hllc_gen(b->eos) || m2c_bstage_upload_roe_gen(b->eos) ||
+      m2c_bstage_upload_llf_gen(b->eos) || m2c_bstage_upload_hllc_sg_f32(b->eos) ||
+      m2c_bstage_upload_hllc_gen_f32(b->eos) || m2c_bstage_upload_hllc_sg_mp(b->eos) ||
+      m2c_bstage_upload_hllc_sg_mp_f32(b->eos))
+    return M2C_ERR_HIP;
   return M2C_OK;
 }
 
-// work items and queues for z chunks of depth kchunk
-static int batch_layout(m2c_batch *b, int kchunk) {
+static void drop_graphs(m2c_batch *b) {
+  if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->exec_rhead) { hipGraphExecDestroy(b->exec_rhead); b->exec_rhead = nullptr; }
+  if (b->exec_rtail) { hipGraphExecDestroy(b->exec_rtail); b->exec_rtail = nullptr; }
+  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+
+// work items and queues for tiles of the given shape and z chunks of depth kchunk
+static int batch_layout(m2c_batch *b, int kchunk, int shape) {
+  int tx, ty;
+  batch_tile_dims(shape, &tx, &ty);
+  b->shape = shape;
   unsigned t = 0;
   for (int i = 0; i < b->n; i++) {
     BatchInst &B = b->hinst[i];
     const Grid &g = B.grid;
-    B.gx = (g.nx + TX - 1) / TX;
-    B.gy = (g.ny + TY - 1) / TY;
+    B.gx = (g.nx + tx - 1) / tx;
+    B.gy = (g.ny + ty - 1) / ty;
     B.nzc = (g.nz + kchunk - 1) / kchunk;
     B.tile0 = t;
     for (int s = 0; s < 3; s++) B.args[s].kchunk = kchunk;
@@ -239,49 +202,136 @@ static int batch_layout(m2c_batch *b, int kchunk) {
   }
   BHIP(hipMemcpy(b->dinst, b->hinst.data(), sizeof(BatchInst) * b->n, hipMemcpyHostToDevice));
   BHIP(hipMemcpy(b->q, &hq, sizeof(hq), hipMemcpyHostToDevice));
-  if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+  drop_graphs(b);
   return M2C_OK;
 }
 
-template <int FLUX, int EK, typename TC>
-static void launch_stage_t(m2c_batch *b, int in_cons, int mode, int stage) {
-  dim3 gr(b->nblocks), bl(NTHR);
-  if (in_cons == 0 && mode == 1)
-    hipLaunchKernelGGL((k_batch_stage<0, 1, FLUX, EK, TC>), gr, bl, 0, b->stream, b->dinst, b->n, b->q, stage);
-  else if (in_cons == 1 && mode == 1)
-    hipLaunchKernelGGL((k_batch_stage<1, 1, FLUX, EK, TC>), gr, bl, 0, b->stream, b->dinst, b->n, b->q, stage);
-  else
-    hipLaunchKernelGGL((k_batch_stage<1, 2, FLUX, EK, TC>), gr, bl, 0, b->stream, b->dinst, b->n, b->q, stage);
-}
-
-static void launch_stage(m2c_batch *b, int in_cons, int mode, int stage) {
-  // the single-instance context's variants: Roe and LLF run the generic-EOS kernels
+static int launch_stage(m2c_batch *b, int in_cons, int mode, int stage) {
+  // the single-instance context's variants: Roe and LLF run the generic-EOS kernels; 9 and 10 are
+  // the stiffened-gas variants with the material-interface code (two-material instances)
+  const int sh = b->shape;
+  int rc;
   switch (b->variant) {
-  case 0: launch_stage_t<FLUX_HLLC, 0, double>(b, in_cons, mode, stage); break;
-  case 1: launch_stage_t<FLUX_HLLC, -1, double>(b, in_cons, mode, stage); break;
-  case 2: launch_stage_t<FLUX_ROE, -1, double>(b, in_cons, mode, stage); break;
-  case 3: launch_stage_t<FLUX_LLF, -1, double>(b, in_cons, mode, stage); break;
-  case 4: launch_stage_t<FLUX_HLLC, 0, float>(b, in_cons, mode, stage); break;
-  default: launch_stage_t<FLUX_HLLC, -1, float>(b, in_cons, mode, stage); break;
+#define M2C_BL(tag) m2c_bstage_launch_##tag(sh, in_cons, mode, b->nblocks, b->stream, b->dinst, b->n, b->q, stage)
+  case 0: rc = M2C_BL(hllc_sg); break;
+  case 1: rc = M2C_BL(hllc_gen); break;
+  case 2: rc = M2C_BL(roe_gen); break;
+  case 3: rc = M2C_BL(llf_gen); break;
+  case 4: rc = M2C_BL(hllc_sg_f32); break;
+  case 9: rc = M2C_BL(hllc_sg_mp); break;
+  case 10: rc = M2C_BL(hllc_sg_mp_f32); break;
+  default: rc = M2C_BL(hllc_gen_f32); break;
+#undef M2C_BL
   }
+  return rc ? M2C_ERR_HIP : M2C_OK;
 }
 
-// one time step of every instance: RK2 (TimeIntegrator.cpp:291-428) or RK3 (:496-690)
-static void enqueue_batch_step(m2c_batch *b) {
+static bool reinit_step(const m2c_batch *b, long long time_step) {
+  for (int i : b->mp_list) {
+    const m2c_config &c = b->cfg[i];
+    if (c.ls_matid >= 0 && c.reinit_freq > 0 && time_step > 0 && time_step % c.reinit_freq == 0) return true;
+  }
+  return false;
+}
+
+// the fluid stages of every instance (RK2, TimeIntegrator.cpp:291-428; RK3, :496-690), each
+// two-material instance's interface pass before and level-set stage after each of them, and the
+// head of its end-of-step tasks (IDs of swept nodes, the first reinitialisation attempt)
+static int enqueue_batch_head(m2c_batch *b, long long time_step) {
   for (int s = 0; s < b->nstage; s++) {
+    for (int i : b->mp_list)
+      if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_before(b->mpc[i], s)) return M2C_ERR_HIP;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->q);
     const int in_cons = s == 0 ? 0 : 1;
     const int mode = s == b->nstage - 1 ? 2 : 1;
-    launch_stage(b, in_cons, mode, s);
+    if (int rc = launch_stage(b, in_cons, mode, s)) return rc;
+    for (int i : b->mp_list)
+      if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_after(b->mpc[i], s)) return M2C_ERR_HIP;
   }
+  for (int i : b->mp_list)
+    if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_end_head(b->mpc[i], time_step)) return M2C_ERR_HIP;
+  return M2C_OK;
+}
+
+// the tail: two-material instances' state update after interface motion and CFL minimum (the
+// single-material instances' minimum comes from their final stage), then every instance's dt
+static int enqueue_batch_tail(m2c_batch *b) {
+  for (int i : b->mp_list)
+    if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_end_tail(b->mpc[i])) return M2C_ERR_HIP;
+  if (!b->mp_list.empty())
+    hipLaunchKernelGGL(k_batch_cfl, dim3(64, (unsigned)b->mp_list.size()), dim3(256), 0, b->stream, b->dinst,
+                       b->d_mp_list, b->clk, b->st);
   hipLaunchKernelGGL(k_batch_clock_finalize, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->clk, b->n, 1);
+  return M2C_OK;
+}
+
+static int batch_retries(m2c_batch *b, long long time_step) {
+  for (int i : b->mp_list)
+    if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_end_retries(b->mpc[i], time_step)) return M2C_ERR_HIP;
+  return M2C_OK;
 }
 
 static void enqueue_batch_cfl(m2c_batch *b, int maxsteps_more) {
   hipLaunchKernelGGL(k_batch_clock_set, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->clk, b->n, maxsteps_more);
-  hipLaunchKernelGGL(k_batch_cfl, dim3(64, b->n), dim3(256), 0, b->stream, b->dinst, b->V, b->ID, b->clk, b->st);
+  hipLaunchKernelGGL(k_batch_cfl, dim3(64, b->n), dim3(256), 0, b->stream, b->dinst, (const int *)nullptr, b->clk,
+                     b->st);
   hipLaunchKernelGGL(k_batch_clock_finalize, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->clk, b->n, 0);
+}
+
+// capture what `fn` enqueues on the batch stream into an executable graph
+template <class F> static int capture(m2c_batch *b, hipGraphExec_t *exec, F fn) {
+  hipGraph_t g = nullptr;
+  BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = fn();
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  BHIP(e);
+  e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  BHIP(e);
+  return M2C_OK;
+}
+
+// one time step of every instance
+static int batch_step(m2c_batch *b, int use_graph) {
+  const long long ts = b->steps + 1;
+  int rc;
+  if (b->mp_list.empty()) {
+    if (use_graph) {
+      if (!b->exec && (rc = capture(b, &b->exec, [&] {
+            int r = enqueue_batch_head(b, ts);
+            return r ? r : enqueue_batch_tail(b);
+          })))
+        return rc;
+      BHIP(hipGraphLaunch(b->exec, b->stream));
+    } else if ((rc = enqueue_batch_head(b, ts)) || (rc = enqueue_batch_tail(b))) {
+      return rc;
+    }
+  } else if (!reinit_step(b, ts)) {
+    if (use_graph) {
+      if (!b->exec && (rc = capture(b, &b->exec, [&] {
+            int r = enqueue_batch_head(b, ts);
+            return r ? r : enqueue_batch_tail(b);
+          })))
+        return rc;
+      BHIP(hipGraphLaunch(b->exec, b->stream));
+    } else if ((rc = enqueue_batch_head(b, ts)) || (rc = enqueue_batch_tail(b))) {
+      return rc;
+    }
+  } else {
+    // a reinitialisation step: the host reads each instance's failsafe flag between head and tail
+    if (use_graph) {
+      if (!b->exec_rhead && (rc = capture(b, &b->exec_rhead, [&] { return enqueue_batch_head(b, ts); }))) return rc;
+      if (!b->exec_rtail && (rc = capture(b, &b->exec_rtail, [&] { return enqueue_batch_tail(b); }))) return rc;
+      BHIP(hipGraphLaunch(b->exec_rhead, b->stream));
+      if ((rc = batch_retries(b, ts))) return rc;
+      BHIP(hipGraphLaunch(b->exec_rtail, b->stream));
+    } else if ((rc = enqueue_batch_head(b, ts)) || (rc = batch_retries(b, ts)) || (rc = enqueue_batch_tail(b))) {
+      return rc;
+    }
+  }
+  b->steps += 1;
+  return M2C_OK;
 }
 
 extern "C" {
@@ -296,11 +346,16 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
   if (b->ts != M2C_TS_RK2 && b->ts != M2C_TS_RK3) { delete b; return M2C_ERR_UNSUPPORTED; }
   b->nstage = b->ts == M2C_TS_RK2 ? 2 : 3;
   b->hinst.resize(n);
+  b->mpc.assign(n, nullptr);
+  std::vector<int> variants(n);
+  int nmat_max = 0;
   for (int i = 0; i < n; i++) {
     const m2c_config &c = cfgs[i];
-    // one kernel variant and one material table for the whole batch
-    if (c.ts != b->ts || c.flux != b->flux || c.precision != cfgs[0].precision || c.nmat != cfgs[0].nmat ||
-        c.device != cfgs[0].device || memcmp(c.mat, cfgs[0].mat, sizeof(c.mat)) != 0 || c.ls_matid >= 0) {
+    // one kernel variant and one material table for the whole batch: the materials an instance
+    // defines are the leading entries of the batch's table
+    const int nm = std::min(c.nmat, cfgs[0].nmat);
+    if (c.ts != b->ts || c.flux != b->flux || c.precision != cfgs[0].precision || c.device != cfgs[0].device ||
+        c.nmat < 1 || c.nmat > M2C_MAX_MAT || memcmp(c.mat, cfgs[0].mat, sizeof(m2c_material) * nm) != 0) {
       delete b;
       return M2C_ERR_UNSUPPORTED;
     }
@@ -309,18 +364,32 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
         delete b;
         return M2C_ERR_UNSUPPORTED;
       }
-    int variant = 0;
     DevEos eos[8];
-    int rc = m2c_fill_grid(&c, &b->hinst[i].grid, eos, &variant);
+    int rc = m2c_fill_grid(&c, &b->hinst[i].grid, eos, &variants[i]);
     if (rc) { delete b; return rc; }
-    if (b->hinst[i].grid.has_interfaces) { delete b; return M2C_ERR_UNSUPPORTED; }
-    if (i == 0) { b->variant = variant; memcpy(b->eos, eos, sizeof(eos)); }
-    else if (variant != b->variant) { delete b; return M2C_ERR_UNSUPPORTED; }
+    // two-material instances: materials meeting at interface faces, or a level set
+    if (b->hinst[i].grid.has_interfaces || c.ls_matid >= 0) b->mp_list.push_back(i);
+    if (c.nmat > nmat_max) { nmat_max = c.nmat; memcpy(b->eos, eos, sizeof(eos)); }
+  }
+  // with two-material instances every instance runs the variant with the material-interface code
+  // (the stiffened-gas ones: 9 for FP64, 10 for the mixed form; the run-time-EOS variants have it)
+  const bool anymp = !b->mp_list.empty();
+  for (int i = 0; i < n; i++) {
+    int v = variants[i];
+    if (anymp && v == 0) v = 9;
+    if (anymp && v == 4) v = 10;
+    if (i == 0) b->variant = v;
+    else if (v != b->variant) { delete b; return M2C_ERR_UNSUPPORTED; }
+  }
+  // HLLC (0, 1, 9), Roe (2) and LLF (3) in FP64, HLLC in the mixed form (4, 5, 10)
+  if (b->variant < 0 || (b->variant > 5 && b->variant != 9 && b->variant != 10)) { delete b; return M2C_ERR_UNSUPPORTED; }
+  std::vector<char> two(n, 0);
+  for (int i : b->mp_list) two[i] = 1;
+  for (int i = 0; i < n; i++) {
+    if (two[i]) continue;
     b->hinst[i].cell0 = b->cells;
     b->cells += b->hinst[i].grid.N;
   }
-  // HLLC (0, 1), Roe (2) and LLF (3) in FP64, HLLC in the mixed form (4, 5)
-  if (b->variant < 0 || b->variant > 5) { delete b; return M2C_ERR_UNSUPPORTED; }
   // from here on a failure releases everything allocated so far
 #define BHIP_D(x)                                                                                  \
   do {                                                                                             \
@@ -328,27 +397,47 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
   } while (0)
   BHIP_D(hipSetDevice(cfgs[0].device));
   BHIP_D(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  size_t bytes = sizeof(double) * 5 * (size_t)b->cells;
+  const long long cells = std::max<long long>(b->cells, 1);
+  size_t bytes = sizeof(double) * 5 * (size_t)cells;
   BHIP_D(hipMalloc(&b->V, bytes));
   BHIP_D(hipMalloc(&b->U1, bytes));
   if (b->nstage == 3) BHIP_D(hipMalloc(&b->U2, bytes));
-  BHIP_D(hipMalloc(&b->stage_buf, bytes));
-  BHIP_D(hipMalloc(&b->ID, (size_t)b->cells));
-  BHIP_D(hipMemset(b->ID, 0, (size_t)b->cells));
+  BHIP_D(hipMalloc(&b->ID, (size_t)cells));
+  BHIP_D(hipMemset(b->ID, 0, (size_t)cells));
   BHIP_D(hipMemset(b->V, 0, bytes));
+  long long stage_cells = cells;   // the host-transfer buffer fits the largest instance
+  for (int i = 0; i < n; i++) stage_cells = std::max<long long>(stage_cells, b->hinst[i].grid.N);
+  BHIP_D(hipMalloc(&b->stage_buf, sizeof(double) * 5 * (size_t)stage_cells));
   BHIP_D(hipMalloc(&b->clk, sizeof(Clock) * n));
   BHIP_D(hipMalloc(&b->st, sizeof(Status) * n));
   BHIP_D(hipMemset(b->st, 0, sizeof(Status) * n));
   BHIP_D(hipMalloc(&b->dinst, sizeof(BatchInst) * n));
   BHIP_D(hipMalloc(&b->q, sizeof(TileQueue)));
+  BHIP_D(hipMalloc(&b->d_mp_list, sizeof(int) * std::max<size_t>(1, b->mp_list.size())));
+  if (anymp)
+    BHIP_D(hipMemcpy(b->d_mp_list, b->mp_list.data(), sizeof(int) * b->mp_list.size(), hipMemcpyHostToDevice));
   // stage arguments of every instance (the same buffers as the single-instance context)
   for (int i = 0; i < n; i++) {
     BatchInst &B = b->hinst[i];
+    if (two[i]) {
+      m2c_ctx *c = nullptr;
+      int rc = m2c_ctx_create(&cfgs[i], &c);
+      if (rc) { m2c_batch_destroy(b); return rc; }
+      b->mpc[i] = c;
+      if ((rc = m2c_ctx_batch_attach(c, b->stream, b->clk + i, b->st + i))) { m2c_batch_destroy(b); return rc; }
+      int v = 0;
+      for (int s = 0; s < b->nstage; s++)
+        if ((rc = m2c_ctx_batch_stage_args(c, s, &B.args[s], &B.Vc, &B.IDc, &v))) { m2c_batch_destroy(b); return rc; }
+      if (v != b->variant) { m2c_batch_destroy(b); return M2C_ERR_UNSUPPORTED; }
+      continue;
+    }
     const long long o = 5 * B.cell0;
     double *V = b->V + o, *U1 = b->U1 + o, *U2 = b->U2 ? b->U2 + o : nullptr;
+    B.Vc = V;
+    B.IDc = b->ID + B.cell0;
     for (int s = 0; s < 3; s++) {
       StageArgs &a = B.args[s];
-      memset(&a, 0, sizeof(a));
+      a = StageArgs();
       a.ID = b->ID + B.cell0;
       a.clk = b->clk + i;
       a.st = b->st + i;
@@ -368,7 +457,7 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
     }
   }
   if (int rc = batch_upload_tables(b)) { m2c_batch_destroy(b); return rc; }
-  if (int rc = batch_layout(b, 8)) { m2c_batch_destroy(b); return rc; }
+  if (int rc = batch_layout(b, 8, 0)) { m2c_batch_destroy(b); return rc; }
   b->nblocks = 512;
   hipLaunchKernelGGL(k_batch_clock_init, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->clk, n, cfgs[0].cfl);
   for (int i = 0; i < n; i++)
@@ -379,6 +468,7 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
       h.minbits = 0x7FEFFFFFFFFFFFFFull;
       h.active = 1;
       BHIP_D(hipMemcpyAsync(b->clk + i, &h, sizeof(h), hipMemcpyHostToDevice, b->stream));
+      BHIP_D(hipStreamSynchronize(b->stream));
     }
   BHIP_D(hipStreamSynchronize(b->stream));
 #undef BHIP_D
@@ -389,10 +479,10 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
 int m2c_batch_destroy(m2c_batch *b) {
   if (!b) return M2C_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  for (m2c_ctx *c : b->mpc) m2c_ctx_destroy(c);
+  drop_graphs(b);
   hipFree(b->V); hipFree(b->U1); hipFree(b->U2); hipFree(b->stage_buf); hipFree(b->backup);
-  hipFree(b->ID); hipFree(b->clk); hipFree(b->st); hipFree(b->dinst); hipFree(b->q);
+  hipFree(b->ID); hipFree(b->clk); hipFree(b->st); hipFree(b->dinst); hipFree(b->q); hipFree(b->d_mp_list);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return M2C_OK;
@@ -400,6 +490,7 @@ int m2c_batch_destroy(m2c_batch *b) {
 
 int m2c_batch_upload(m2c_batch *b, int i, const double *V, const int32_t *id) {
   if (!b || i < 0 || i >= b->n || !V) return M2C_ERR_ARG;
+  if (b->mpc[i]) return m2c_upload_state(b->mpc[i], V, id);
   BHIP(hipSetDevice(b->cfg[0].device));
   const BatchInst &B = b->hinst[i];
   const long long N = B.grid.N;
@@ -419,6 +510,7 @@ int m2c_batch_upload(m2c_batch *b, int i, const double *V, const int32_t *id) {
 
 int m2c_batch_download(m2c_batch *b, int i, double *V, int32_t *id) {
   if (!b || i < 0 || i >= b->n) return M2C_ERR_ARG;
+  if (b->mpc[i]) return m2c_download_state(b->mpc[i], V, id);
   BHIP(hipSetDevice(b->cfg[0].device));
   const BatchInst &B = b->hinst[i];
   const long long N = B.grid.N;
@@ -436,6 +528,16 @@ int m2c_batch_download(m2c_batch *b, int i, double *V, int32_t *id) {
   return M2C_OK;
 }
 
+int m2c_batch_upload_levelset(m2c_batch *b, int i, const double *phi) {
+  if (!b || i < 0 || i >= b->n || !b->mpc[i]) return M2C_ERR_ARG;
+  return m2c_upload_levelset(b->mpc[i], phi);
+}
+
+int m2c_batch_download_levelset(m2c_batch *b, int i, double *phi) {
+  if (!b || i < 0 || i >= b->n || !b->mpc[i]) return M2C_ERR_ARG;
+  return m2c_download_levelset(b->mpc[i], phi);
+}
+
 int m2c_batch_run(m2c_batch *b, int nsteps, int use_graph, double *ms) {
   if (!b || nsteps < 0) return M2C_ERR_ARG;
   BHIP(hipSetDevice(b->cfg[0].device));
@@ -445,18 +547,9 @@ int m2c_batch_run(m2c_batch *b, int nsteps, int use_graph, double *ms) {
   BHIP(hipEventCreate(&e0));
   BHIP(hipEventCreate(&e1));
   BHIP(hipEventRecord(e0, b->stream));
-  if (use_graph) {
-    if (!b->exec) {
-      BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_batch_step(b);
-      BHIP(hipStreamEndCapture(b->stream, &b->graph));
-      BHIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
-    for (int s = 0; s < nsteps; s++) BHIP(hipGraphLaunch(b->exec, b->stream));
-  } else {
-    for (int s = 0; s < nsteps; s++) enqueue_batch_step(b);
-  }
-  BHIP(hipGetLastError());
+  int rc = M2C_OK;
+  for (int s = 0; s < nsteps && !rc; s++) rc = batch_step(b, use_graph);
+  if (!rc && hipGetLastError() != hipSuccess) rc = M2C_ERR_HIP;
   BHIP(hipEventRecord(e1, b->stream));
   BHIP(hipEventSynchronize(e1));
   float t = 0.f;
@@ -464,11 +557,16 @@ int m2c_batch_run(m2c_batch *b, int nsteps, int use_graph, double *ms) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (ms) *ms = t;
-  return M2C_OK;
+  return rc;
 }
 
 int m2c_batch_status(m2c_batch *b, int i, m2c_status *out) {
   if (!b || i < 0 || i >= b->n || !out) return M2C_ERR_ARG;
+  if (b->mpc[i]) {   // the context reads the clock and status the batch owns
+    int rc = m2c_get_status(b->mpc[i], out);
+    out->cfl = b->cfg[i].cfl;
+    return rc;
+  }
   Status s;
   Clock k;
   BHIP(hipMemcpyAsync(&s, b->st + i, sizeof(Status), hipMemcpyDeviceToHost, b->stream));
@@ -489,38 +587,50 @@ int m2c_batch_status(m2c_batch *b, int i, m2c_status *out) {
 int m2c_batch_autotune(m2c_batch *b, int trial_steps, m2c_batch_tuning *out) {
   if (!b || trial_steps < 1) return M2C_ERR_ARG;
   BHIP(hipSetDevice(b->cfg[0].device));
-  const size_t bytes = sizeof(double) * 5 * (size_t)b->cells;
+  const size_t bytes = sizeof(double) * 5 * (size_t)std::max<long long>(b->cells, 1);
   if (!b->backup) BHIP(hipMalloc(&b->backup, bytes));
-  // tune on a copy of the state; the clocks and counters are restored afterwards
+  // tune on a copy of the state; the clocks, counters and step count are restored afterwards
   std::vector<Clock> hclk(b->n);
   std::vector<Status> hst(b->n);
   BHIP(hipMemcpy(hclk.data(), b->clk, sizeof(Clock) * b->n, hipMemcpyDeviceToHost));
   BHIP(hipMemcpy(hst.data(), b->st, sizeof(Status) * b->n, hipMemcpyDeviceToHost));
+  const long long steps0 = b->steps;
   hipLaunchKernelGGL(k_batch_copy, dim3(4096), dim3(256), 0, b->stream, b->V, b->backup, 5 * b->cells);
+  for (int i : b->mp_list)
+    if (int rc = m2c_ctx_batch_backup(b->mpc[i], 0)) return rc;
+  auto restore = [&]() -> int {
+    hipLaunchKernelGGL(k_batch_copy, dim3(4096), dim3(256), 0, b->stream, b->backup, b->V, 5 * b->cells);
+    for (int i : b->mp_list)
+      if (int rc = m2c_ctx_batch_backup(b->mpc[i], 1)) return rc;
+    BHIP(hipStreamSynchronize(b->stream));
+    BHIP(hipMemcpy(b->clk, hclk.data(), sizeof(Clock) * b->n, hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->st, hst.data(), sizeof(Status) * b->n, hipMemcpyHostToDevice));
+    b->steps = steps0;
+    return M2C_OK;
+  };
   const int kchunks[] = {4, 8, 16, 32};
   const int blocks[] = {256, 512, 1024};
   double best = 1e300;
-  int bk = b->kchunk, bb = b->nblocks, ncand = 0;
-  for (int kc : kchunks)
-    for (int nb : blocks) {
-      if (int rc = batch_layout(b, kc)) return rc;
-      b->nblocks = nb;
-      hipLaunchKernelGGL(k_batch_copy, dim3(4096), dim3(256), 0, b->stream, b->backup, b->V, 5 * b->cells);
-      BHIP(hipMemcpy(b->clk, hclk.data(), sizeof(Clock) * b->n, hipMemcpyHostToDevice));
-      double ms = 0.0;
-      if (int rc = m2c_batch_run(b, trial_steps, 1, &ms)) return rc;
-      ncand++;
-      if (ms < best) { best = ms; bk = kc; bb = nb; }
-    }
-  if (int rc = batch_layout(b, bk)) return rc;
+  int bk = b->kchunk, bb = b->nblocks, bs = b->shape, ncand = 0;
+  for (int sh = 0; sh < BATCH_TILES; sh++)
+    for (int kc : kchunks)
+      for (int nb : blocks) {
+        if (int rc = batch_layout(b, kc, sh)) return rc;
+        b->nblocks = nb;
+        if (int rc = restore()) return rc;
+        double ms = 0.0;
+        if (int rc = m2c_batch_run(b, trial_steps, 1, &ms)) return rc;
+        ncand++;
+        if (ms < best) { best = ms; bk = kc; bb = nb; bs = sh; }
+      }
+  if (int rc = batch_layout(b, bk, bs)) return rc;
   b->nblocks = bb;
-  hipLaunchKernelGGL(k_batch_copy, dim3(4096), dim3(256), 0, b->stream, b->backup, b->V, 5 * b->cells);
-  BHIP(hipMemcpy(b->clk, hclk.data(), sizeof(Clock) * b->n, hipMemcpyHostToDevice));
-  BHIP(hipMemcpy(b->st, hst.data(), sizeof(Status) * b->n, hipMemcpyHostToDevice));
+  if (int rc = restore()) return rc;
   BHIP(hipStreamSynchronize(b->stream));
   if (out) {
     out->kchunk = bk;
     out->blocks = bb;
+    batch_tile_dims(bs, &out->tile_x, &out->tile_y);
     out->ms_per_step = best / trial_steps;
     out->candidates = ncand;
   }
@@ -529,9 +639,19 @@ int m2c_batch_autotune(m2c_batch *b, int trial_steps, m2c_batch_tuning *out) {
 
 int m2c_batch_set_tuning(m2c_batch *b, int kchunk, int blocks) {
   if (!b || kchunk < 1 || blocks < 1) return M2C_ERR_ARG;
-  if (int rc = batch_layout(b, kchunk)) return rc;
+  if (int rc = batch_layout(b, kchunk, b->shape)) return rc;
   b->nblocks = blocks;
   return M2C_OK;
+}
+
+int m2c_batch_set_tile(m2c_batch *b, int tile_x, int tile_y) {
+  if (!b) return M2C_ERR_ARG;
+  for (int sh = 0; sh < BATCH_TILES; sh++) {
+    int tx, ty;
+    batch_tile_dims(sh, &tx, &ty);
+    if (tx == tile_x && ty == tile_y) return batch_layout(b, b->kchunk, sh);
+  }
+  return M2C_ERR_UNSUPPORTED;
 }
 
 }  // extern "C"

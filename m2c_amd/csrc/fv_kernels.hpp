@@ -34,6 +34,23 @@ constexpr int NFY = TX * (TY + 1);             // y faces: x in [0, TX), y in [0
 constexpr int NHALO = 2 * HALO * TY + 2 * HALO * TX;
 constexpr int SLOPE_ROUNDS = (NSL + NTHR - 1) / NTHR;
 
+// A tile of TX_ x TY_ columns (NTHR threads, one column each) and its LDS extents; the constants
+// above are the 32 x 8 tile of the single-instance kernel, the batched kernel also runs 16 x 16.
+template <int TX_, int TY_> struct TileShape {
+  static_assert(TX_ * TY_ == NTHR, "one thread per column");
+  static constexpr int TX = TX_, TY = TY_, LX = TX_ + 2 * HALO, LY = TY_ + 2 * HALO;
+  static constexpr int NSX = (TX_ + 2) * TY_, NSY = TX_ * (TY_ + 2), NSL = NSX + NSY;
+  static constexpr int NFX = (TX_ + 1) * TY_, NFY = TX_ * (TY_ + 1);
+  static constexpr int NHALO = 2 * HALO * TY_ + 2 * HALO * TX_;
+  static_assert(NHALO <= NTHR, "one halo cell per thread");
+  static constexpr int SLOPE_ROUNDS = (NSL + NTHR - 1) / NTHR;
+};
+using DefaultTile = TileShape<TX, TY>;
+#define M2C_TILE_CONSTANTS(TS)                                                                 \
+  constexpr int TX = TS::TX, TY = TS::TY, LX = TS::LX, LY = TS::LY, NSX = TS::NSX, NSL = TS::NSL, \
+                NFX = TS::NFX, NFY = TS::NFY, NHALO = TS::NHALO, SLOPE_ROUNDS = TS::SLOPE_ROUNDS;   \
+  (void)LX; (void)LY; (void)NSX; (void)NSL; (void)NFX; (void)NFY; (void)NHALO; (void)SLOPE_ROUNDS
+
 enum { BC_INLET = 1, BC_OUTLET = 2, BC_SLIPWALL = 3, BC_STICKWALL = 4, BC_SYMMETRY = 5, BC_OVERSET = 6,
        BC_PERIODIC = 7 };
 
@@ -107,6 +124,7 @@ static __constant__ DevEos c_eos[8];
 #ifdef M2C_BATCHED
 static __shared__ Grid s_bgrid;
 #define M2C_G s_bgrid
+#define M2C_BATCHED_LAYOUT 1
 #else
 static __constant__ Grid c_grid;
 #define M2C_G c_grid
@@ -373,9 +391,10 @@ __device__ __forceinline__ void warp_min_atomic(unsigned long long *dst, double 
 
 // x (DIR 0) or y (DIR 1) face of the tile whose plus cell is at local (x, y); the flux goes to
 // sF[p*NF + fi].
-template <int FLUX, int EK, int DIR, typename TC, bool MP = (EK < 0)>
-__device__ __forceinline__ void tile_face(TC (*sV)[LY][LX], int (*sID)[LX], const TC *sS, TC *sF, int x, int y, int fi,
-                                          int i0, int j0, unsigned long long &nrev) {
+template <int FLUX, int EK, int DIR, typename TC, bool MP = (EK < 0), class TS = DefaultTile>
+__device__ __forceinline__ void tile_face(TC (*sV)[TS::LY][TS::LX], int (*sID)[TS::LX], const TC *sS, TC *sF, int x,
+                                          int y, int fi, int i0, int j0, unsigned long long &nrev) {
+  M2C_TILE_CONSTANTS(TS);
   const Grid &g = M2C_G;
   constexpr int SOFF = DIR == 0 ? 0 : NSX;
   constexpr int NF = DIR == 0 ? NFX : NFY;
@@ -426,8 +445,9 @@ template <typename TC> __device__ __forceinline__ void to_tc(const double *v, TC
 // the drop-in path; float for the mixed form, where the fetch (with ConservativeToPrimitive and
 // its p(e) inversion), the residual accumulation, the stage combination, ConservativeToPrimitive
 // of the result, clipping and the CFL reduction stay in double.
-template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double, bool MP = (EK < 0)>
+template <int IN_CONS, int MODE, int FLUX, int EK, typename TC = double, bool MP = (EK < 0), class TS = DefaultTile>
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, const int by, const int bz) {
+  M2C_TILE_CONSTANTS(TS);
   const Grid &g = M2C_G;
   if (!a.ignore_active && !a.clk->active) return;
   // faces cut by embedded walls: the run-time equation-of-state variants only (the stiffened-gas
@@ -604,8 +624,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
 
     // ---- x and y face fluxes: the west/south face of every column, then the east column of
     //      x faces and the north row of y faces
-    tile_face<FLUX, EK, 0, TC, MP>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
-    tile_face<FLUX, EK, 1, TC, MP>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
+    tile_face<FLUX, EK, 0, TC, MP, TS>(sV, sID, sS, sFx, tx, ty, ty * (TX + 1) + tx, i0, j0, nrev);
+    tile_face<FLUX, EK, 1, TC, MP, TS>(sV, sID, sS, sFy, tx, ty, ty * TX + tx, i0, j0, nrev);
     {
       // the TY east x faces and the TX north y faces each go to one wave, two waves apart, and
       // the pair rotates with the block: the two other waves of the block skip the extra flux
@@ -614,9 +634,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       const int wave = tid >> 6, lane = tid & 63;
       const int rot = (bx + by + bz) & 3;
       if (wave == rot && lane < TY)
-        tile_face<FLUX, EK, 0, TC, MP>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
+        tile_face<FLUX, EK, 0, TC, MP, TS>(sV, sID, sS, sFx, TX, lane, lane * (TX + 1) + TX, i0, j0, nrev);
       if (wave == ((rot + 2) & 3) && lane < TX)
-        tile_face<FLUX, EK, 1, TC, MP>(sV, sID, sS, sFy, lane, TY, TY * TX + lane, i0, j0, nrev);
+        tile_face<FLUX, EK, 1, TC, MP, TS>(sV, sID, sS, sFy, lane, TY, TY * TX + lane, i0, j0, nrev);
     }
 
     // ---- z face k+1/2 (registers)

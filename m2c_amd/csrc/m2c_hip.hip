@@ -13,6 +13,7 @@
 
 #include "../../include/m2c_hip.h"
 #include "multiphase.hpp"
+#include "batch_ctx.hpp"
 #include "stage_launch.hpp"
 
 using namespace m2c;
@@ -89,6 +90,20 @@ struct m2c_ctx {
   unsigned *eb_faces = nullptr;
   double *eb_wall = nullptr, *eb_scratch = nullptr;
   uint8_t *eb_mask = nullptr;
+  // attached to a batch (m2c_ctx_batch_attach): the context's own stream, clock and status wait here
+  // while the batch's are in use; pinned copies of the interface and level-set constants for the
+  // asynchronous switch between instances; the autotuner's copy of the state
+  hipStream_t own_stream = nullptr;
+  Clock *own_clk = nullptr;
+  Status *own_st = nullptr;
+  struct Bound {
+    Grid g;
+    DevEos eos[8];
+    MpParams mp;
+  } *bound = nullptr;
+  double *bk_V = nullptr, *bk_phi = nullptr;
+  int8_t *bk_ID = nullptr;
+  MpState *bk_ms = nullptr;
 };
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
@@ -676,22 +691,94 @@ static int iface_pass(m2c_ctx *c, const double *S, int in_cons, int store, int i
 // End-of-step tasks of TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
 // 695-797): IDs of swept nodes, reinitialisation, states of cells whose material changed, clip,
 // then the CFL minimum of the updated field. `reinit_now`: the host's view of whether this step
-// is a reinitialisation step (the device decides again from its own step counter).
-static int end_of_step(m2c_ctx *c, bool reinit_now) {
+// is a reinitialisation step (the device decides again from its own step counter). The head (IDs
+// and the first reinitialisation attempt), the host-side retries of the reinitialisation failsafe
+// and the tail (state update) are separate so that a batch can run the head of every instance
+// before it waits on any of them.
+static int end_of_step_head(m2c_ctx *c, bool reinit_now) {
   if (mp_chk(m2c_mp_launch_id_remaining(c->L, c->id_single ? 1 : 0))) return M2C_ERR_HIP;
-  if (reinit_now) {
-    if (mp_chk(m2c_mp_launch_reinit(c->L, c->phiA, c->cfg.reinit_maxits, 0))) return M2C_ERR_HIP;
-    for (int attempt = 0; attempt < 64; attempt++) {
-      MpState h;
-      HIPCHK(hipMemcpyAsync(&h, c->L.ms, sizeof(MpState), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-      if (!h.rs.retry) break;
-      if (mp_chk(m2c_mp_launch_reinit(c->L, c->phiA, c->cfg.reinit_maxits, 1))) return M2C_ERR_HIP;
-    }
+  if (reinit_now && mp_chk(m2c_mp_launch_reinit(c->L, c->phiA, c->cfg.reinit_maxits, 0))) return M2C_ERR_HIP;
+  return M2C_OK;
+}
+
+static int end_of_step_retries(m2c_ctx *c, bool reinit_now) {
+  if (!reinit_now) return M2C_OK;
+  for (int attempt = 0; attempt < 64; attempt++) {
+    MpState h;
+    HIPCHK(hipMemcpyAsync(&h, c->L.ms, sizeof(MpState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!h.rs.retry) break;
+    if (mp_chk(m2c_mp_launch_reinit(c->L, c->phiA, c->cfg.reinit_maxits, 1))) return M2C_ERR_HIP;
   }
-  if (mp_chk(m2c_mp_launch_state_update(c->L, c->V, 4))) return M2C_ERR_HIP;
+  return M2C_OK;
+}
+
+static int end_of_step_tail(m2c_ctx *c) {
+  return mp_chk(m2c_mp_launch_state_update(c->L, c->V, 4));
+}
+
+static int end_of_step(m2c_ctx *c, bool reinit_now) {
+  int rc;
+  if ((rc = end_of_step_head(c, reinit_now)) || (rc = end_of_step_retries(c, reinit_now)) ||
+      (rc = end_of_step_tail(c)))
+    return rc;
   launch_cfl(c);
   return M2C_OK;
+}
+
+// Stage s of an RK2 (TimeIntegrator.cpp:291-428) or RK3 (:496-690) step: the fluid stage's fields
+// and coefficients, U_out = (a U_s + b U_n) + c dt R(S), and the level-set stage interleaved with it
+// (phi_out = (a' X + b' Y) + c' dt R(phi_S; V(S)); the last stage also updates the material IDs).
+struct StageSpec {
+  int in_cons, mode;           // mode 1: intermediate stage (U out), 2: final stage (V out)
+  const double *S, *Vn, *Us;
+  double *Out;
+  double a, b, c;
+  int first;
+  // level set
+  int ls_mode, ls_final;
+  const double *phiS, *X, *Y;
+  double *phiOut;
+  double la, lb, lc;
+  const double *phin;
+};
+
+static int nstages(const m2c_ctx *c);
+
+static void stage_spec(const m2c_ctx *c, int s, StageSpec &p) {
+  memset(&p, 0, sizeof(p));
+  const bool rk2 = c->cfg.ts == M2C_TS_RK2;
+  if (s == 0) {
+    // U1 = U_n + dt R(V_n); Phi1 = (0*Phi1 + 1*Phi) + dt R(Phi; V_n)
+    p = {0, 1, c->V, c->V, nullptr, c->U1, 0.0, 0.0, 1.0, 1,
+         LS_FIRST, 0, c->phiA, c->phiA, nullptr, c->phiB, 0.0, 0.0, 1.0, nullptr};
+  } else if (rk2) {
+    // U(n+1) = 0.5 U1 + 0.5 U_n + 0.5 dt R(V1); Phi(n+1) = (0.5 Phi + 0.5 Phi1) + 0.5 dt R(Phi1; V1)
+    p = {1, 2, c->U1, c->V, c->U1, c->V, 0.5, 0.5, 0.5, 0,
+         LS_AXBY, 1, c->phiB, c->phiA, c->phiB, c->phiA, 0.5, 0.5, 0.5, c->phiA};
+  } else if (s == 1) {
+    // U2 = 0.25 U1 + 0.75 U_n + 0.25 dt R(V1)
+    p = {1, 1, c->U1, c->V, c->U1, c->U2, 0.25, 0.75, 0.25, 0,
+         LS_AXBY, 0, c->phiB, c->phiB, c->phiA, c->phiD, 0.25, 0.75, 0.25, nullptr};
+  } else {
+    // U(n+1) = 2/3 U2 + 1/3 U_n + 2/3 dt R(V2)
+    p = {1, 2, c->U2, c->V, c->U2, c->V, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0,
+         LS_AXBY, 1, c->phiD, c->phiA, c->phiD, c->phiA, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, c->phiA};
+  }
+}
+
+// what precedes the fluid stage s: the step's start and the interface Riemann pass (the first stage
+// keeps the Riemann solutions and lists the interface faces; later stages reuse the list)
+static int stage_before(m2c_ctx *c, int s, const StageSpec &p, int ignore_active) {
+  if (s == 0 && (c->iface || c->ls) && mp_chk(m2c_mp_step_begin(c->L))) return M2C_ERR_HIP;
+  return iface_pass(c, p.S, p.in_cons, s == 0 ? 1 : 0, ignore_active, s == 0 ? 1 : 0);
+}
+
+// what follows the fluid stage s: the level-set stage
+static int stage_after(m2c_ctx *c, const StageSpec &p) {
+  if (!c->ls) return M2C_OK;
+  return mp_chk(m2c_mp_launch_ls(c->L, p.in_cons, p.ls_mode, p.ls_final, p.S, p.phiS, p.X, p.Y, p.phiOut, p.la,
+                                 p.lb, p.lc, p.phin));
 }
 
 // One time step: the Runge-Kutta stages (TimeIntegrator.cpp FE :150-226, RK2 :291-428,
@@ -702,9 +789,9 @@ static int enqueue_step(m2c_ctx *c, int ignore_active, std::vector<hipEvent_t> *
   int rc;
   auto rec = [&](int i) { if (ev) hipEventRecord((*ev)[i], c->stream); };
   const bool reinit_now = c->ls && c->cfg.reinit_freq > 0 && time_step > 0 && time_step % c->cfg.reinit_freq == 0;
-  if (c->iface || c->ls) if (mp_chk(m2c_mp_step_begin(c->L))) return M2C_ERR_HIP;
   const MpLaunch &L = c->L;
   if (ts == M2C_TS_FE) {
+    if (c->iface || c->ls) if (mp_chk(m2c_mp_step_begin(c->L))) return M2C_ERR_HIP;
     if ((rc = iface_pass(c, c->V, 0, 1, ignore_active))) return rc;
     rec(0);
     rc = launch_stage(c, 0, 2, c->V, c->V, nullptr, c->U1, 0, 0, 1.0, 1, ignore_active);
@@ -720,50 +807,18 @@ static int enqueue_step(m2c_ctx *c, int ignore_active, std::vector<hipEvent_t> *
     }
     return M2C_OK;
   }
-  // stage 1: U1 = U_n + dt R(V_n); Phi1 = (0*Phi1 + 1*Phi) + dt R(Phi; V_n)
-  if ((rc = iface_pass(c, c->V, 0, 1, ignore_active))) return rc;
-  rec(0);
-  rc = launch_stage(c, 0, 1, c->V, c->V, nullptr, c->U1, 0, 0, 1.0, 1, ignore_active);
-  rec(1);
-  if (rc) return rc;
-  if (c->ls && mp_chk(m2c_mp_launch_ls(L, 0, LS_FIRST, 0, c->V, c->phiA, c->phiA, nullptr, c->phiB, 0, 0, 1.0,
-                                       nullptr)))
-    return M2C_ERR_HIP;
-  if ((rc = iface_pass(c, c->U1, 1, 0, ignore_active, 0))) return rc;
-  if (ts == M2C_TS_RK2) {
-    // U(n+1) = 0.5 U1 + 0.5 U_n + 0.5 dt R(V1); Phi(n+1) = (0.5 Phi + 0.5 Phi1) + 0.5 dt R(Phi1; V1)
-    rec(2);
-    rc = launch_stage(c, 1, 2, c->U1, c->V, c->U1, c->V, 0.5, 0.5, 0.5, 0, ignore_active);
-    rec(3);
+  const int ns = nstages(c);
+  for (int s = 0; s < ns; s++) {
+    StageSpec p;
+    stage_spec(c, s, p);
+    if ((rc = stage_before(c, s, p, ignore_active))) return rc;
+    rec(2 * s);
+    rc = launch_stage(c, p.in_cons, p.mode, p.S, p.Vn, p.Us, p.Out, p.a, p.b, p.c, p.first, ignore_active);
+    rec(2 * s + 1);
     if (rc) return rc;
-    if (c->ls) {
-      if (mp_chk(m2c_mp_launch_ls(L, 1, LS_AXBY, 1, c->U1, c->phiB, c->phiA, c->phiB, c->phiA, 0.5, 0.5, 0.5,
-                                  c->phiA)))
-        return M2C_ERR_HIP;
-      return end_of_step(c, reinit_now);
-    }
-    return M2C_OK;
+    if ((rc = stage_after(c, p))) return rc;
   }
-  // RK3: U2 = 0.25 U1 + 0.75 U_n + 0.25 dt R(V1); U(n+1) = 2/3 U2 + 1/3 U_n + 2/3 dt R(V2)
-  rec(2);
-  rc = launch_stage(c, 1, 1, c->U1, c->V, c->U1, c->U2, 0.25, 0.75, 0.25, 0, ignore_active);
-  rec(3);
-  if (rc) return rc;
-  if (c->ls && mp_chk(m2c_mp_launch_ls(L, 1, LS_AXBY, 0, c->U1, c->phiB, c->phiB, c->phiA, c->phiD, 0.25, 0.75, 0.25,
-                                       nullptr)))
-    return M2C_ERR_HIP;
-  if ((rc = iface_pass(c, c->U2, 1, 0, ignore_active, 0))) return rc;
-  rec(4);
-  rc = launch_stage(c, 1, 2, c->U2, c->V, c->U2, c->V, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, 0, ignore_active);
-  rec(5);
-  if (rc) return rc;
-  if (c->ls) {
-    if (mp_chk(m2c_mp_launch_ls(L, 1, LS_AXBY, 1, c->U2, c->phiD, c->phiA, c->phiD, c->phiA, 1.0 / 3.0, 2.0 / 3.0,
-                                2.0 / 3.0, c->phiA)))
-      return M2C_ERR_HIP;
-    return end_of_step(c, reinit_now);
-  }
-  return M2C_OK;
+  return c->ls ? end_of_step(c, reinit_now) : M2C_OK;
 }
 
 static int nstages(const m2c_ctx *c) { return c->cfg.ts == M2C_TS_FE ? 1 : (c->cfg.ts == M2C_TS_RK2 ? 2 : 3); }
@@ -1110,6 +1165,13 @@ int m2c_ctx_destroy(m2c_ctx *c) {
   if (!c) return M2C_OK;
   if (g_resident == c) g_resident = nullptr;
   hipStreamSynchronize(c->stream);
+  if (c->own_stream) {   // attached to a batch: take back the context's own stream, clock and status
+    c->stream = c->L.stream = c->own_stream;
+    c->clk = c->L.clk = c->own_clk;
+    c->st = c->L.st = c->own_st;
+  }
+  if (c->bound) hipHostFree(c->bound);
+  hipFree(c->bk_V); hipFree(c->bk_phi); hipFree(c->bk_ID); hipFree(c->bk_ms);
   hipFree(c->V); hipFree(c->U1); hipFree(c->U2); hipFree(c->R);
   hipFree(c->ID); hipFree(c->clk); hipFree(c->st);
   for (void *p : c->mp_bufs) hipFree(p);
@@ -1976,6 +2038,107 @@ int m2c_stream_copy(m2c_ctx *c, long long nbytes, int reps, double *ms_avg) {
 int m2c_device_state(m2c_ctx *c, double **V) {
   if (!c || !V) return M2C_ERR_ARG;
   *V = c->V;
+  return M2C_OK;
+}
+
+
+// ---------------------------------------------------------------- two-material instances of a batch
+
+int m2c_ctx_batch_attach(m2c_ctx *c, hipStream_t s, Clock *clk, Status *st) {
+  if (!c || !s || !clk || !st || c->own_stream) return M2C_ERR_ARG;
+  if (c->diff || c->godunov || c->eb_n || c->L.sur_on || c->cfg.ts == M2C_TS_FE) return M2C_ERR_UNSUPPORTED;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipHostMalloc((void **)&c->bound, sizeof(m2c_ctx::Bound), hipHostMallocDefault));
+  c->bound->g = c->grid;
+  memcpy(c->bound->eos, c->eos, sizeof(c->eos));
+  c->bound->mp = c->mp;
+  c->own_stream = c->stream;
+  c->own_clk = c->clk;
+  c->own_st = c->st;
+  c->stream = c->L.stream = s;
+  c->clk = c->L.clk = clk;
+  c->st = c->L.st = st;
+  if (g_resident == c) g_resident = nullptr;
+  return M2C_OK;
+}
+
+int m2c_ctx_batch_stage_args(m2c_ctx *c, int s, StageArgs *a, const double **V, const int8_t **ID, int *variant) {
+  if (!c || !a || s < 0 || s >= nstages(c)) return M2C_ERR_ARG;
+  StageSpec p;
+  stage_spec(c, s, p);
+  StageArgs q;
+  q.phin = conrec_phi(c);
+  q.no_cfl = c->ls ? 1 : 0;
+  q.S = p.S; q.Vn = p.Vn; q.Us = p.Us; q.Out = p.Out; q.ID = c->ID;
+  q.Fi = c->Fi; q.Nf = c->Nf;
+  q.clk = c->clk; q.st = c->st;
+  q.a = p.a; q.b = p.b; q.c = p.c; q.first = p.first;
+  q.kchunk = c->kchunk;
+  q.ignore_active = 0;
+  *a = q;
+  if (V) *V = c->V;
+  if (ID) *ID = c->ID;
+  if (variant) *variant = c->variant;
+  return M2C_OK;
+}
+
+int m2c_ctx_batch_bind(m2c_ctx *c) {
+  if (!c || !c->bound) return M2C_ERR_ARG;
+  // the single-context entry points re-upload their tables on their next call
+  g_resident = nullptr;
+  return mp_chk(m2c_mp_upload_async(c->stream, &c->bound->g, c->bound->eos, &c->bound->mp));
+}
+
+int m2c_ctx_batch_before(m2c_ctx *c, int s) {
+  StageSpec p;
+  stage_spec(c, s, p);
+  return stage_before(c, s, p, 0);
+}
+
+int m2c_ctx_batch_after(m2c_ctx *c, int s) {
+  StageSpec p;
+  stage_spec(c, s, p);
+  return stage_after(c, p);
+}
+
+static bool reinit_step(const m2c_ctx *c, long long time_step) {
+  return c->ls && c->cfg.reinit_freq > 0 && time_step > 0 && time_step % c->cfg.reinit_freq == 0;
+}
+
+int m2c_ctx_batch_end_head(m2c_ctx *c, long long time_step) {
+  return c->ls ? end_of_step_head(c, reinit_step(c, time_step)) : M2C_OK;
+}
+
+int m2c_ctx_batch_end_retries(m2c_ctx *c, long long time_step) {
+  return c->ls ? end_of_step_retries(c, reinit_step(c, time_step)) : M2C_OK;
+}
+
+int m2c_ctx_batch_end_tail(m2c_ctx *c) { return c->ls ? end_of_step_tail(c) : M2C_OK; }
+
+int m2c_ctx_batch_backup(m2c_ctx *c, int restore) {
+  if (!c) return M2C_ERR_ARG;
+  const size_t nv = sizeof(double) * 5 * (size_t)c->N;
+  if (!c->bk_V) {
+    if (restore) return M2C_ERR_ARG;
+    HIPCHK(hipMalloc(&c->bk_V, nv));
+    HIPCHK(hipMalloc(&c->bk_ID, (size_t)c->N));
+    if (c->ls) HIPCHK(hipMalloc(&c->bk_phi, sizeof(double) * (size_t)c->N));
+    if (c->iface || c->ls) HIPCHK(hipMalloc(&c->bk_ms, sizeof(MpState)));
+  }
+  auto cp = [&](void *dst, const void *src, size_t n) {
+    return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, c->stream);
+  };
+  if (!restore) {
+    HIPCHK(cp(c->bk_V, c->V, nv));
+    HIPCHK(cp(c->bk_ID, c->ID, (size_t)c->N));
+    if (c->ls) HIPCHK(cp(c->bk_phi, c->phiA, sizeof(double) * (size_t)c->N));
+    if (c->bk_ms) HIPCHK(cp(c->bk_ms, c->L.ms, sizeof(MpState)));
+  } else {
+    HIPCHK(cp(c->V, c->bk_V, nv));
+    HIPCHK(cp(c->ID, c->bk_ID, (size_t)c->N));
+    if (c->ls) HIPCHK(cp(c->phiA, c->bk_phi, sizeof(double) * (size_t)c->N));
+    if (c->bk_ms) HIPCHK(cp(c->L.ms, c->bk_ms, sizeof(MpState)));
+  }
   return M2C_OK;
 }
 

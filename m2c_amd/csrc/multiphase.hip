@@ -1915,6 +1915,18 @@ extern "C" int m2c_mp_upload(const Grid &grid, const DevEos *eos, const MpParams
   return 0;
 }
 
+// the same three tables copied on a stream (sources in pinned host memory that outlive the copy):
+// the switch between the two-material instances of a batch
+extern "C" int m2c_mp_upload_async(hipStream_t s, const Grid *grid, const DevEos *eos, const MpParams *mp) {
+  if (hipMemcpyToSymbolAsync(HIP_SYMBOL(c_grid), grid, sizeof(Grid), 0, hipMemcpyHostToDevice, s) != hipSuccess)
+    return 2;
+  if (hipMemcpyToSymbolAsync(HIP_SYMBOL(c_eos), eos, sizeof(DevEos) * 8, 0, hipMemcpyHostToDevice, s) != hipSuccess)
+    return 2;
+  if (hipMemcpyToSymbolAsync(HIP_SYMBOL(c_mp), mp, sizeof(MpParams), 0, hipMemcpyHostToDevice, s) != hipSuccess)
+    return 2;
+  return 0;
+}
+
 static int nblocks(long long n, int bs, int cap) {
   long long b = (n + bs - 1) / bs;
   if (b < 1) b = 1;

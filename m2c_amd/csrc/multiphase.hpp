@@ -99,6 +99,7 @@ int m2c_mp_launch_sur_batch(hipStream_t s, int n, const int *dir, const double *
                             double *Vs, int *id, double *Vsm, double *Vsp, int *status, int *path, double *scratch,
                             int nthreads, unsigned *fbl, unsigned *nfb);
 int m2c_mp_upload(const m2c::Grid &grid, const m2c::DevEos *eos, const m2c::MpParams &mp);
+int m2c_mp_upload_async(hipStream_t s, const m2c::Grid *grid, const m2c::DevEos *eos, const m2c::MpParams *mp);
 int m2c_mp_step_begin(const m2c::MpLaunch &L);
 int m2c_mp_launch_riemann_batch(hipStream_t s, int n, const double *dir, const double *Vm, const int *idm,
                                 const double *Vp, const int *idp, double *Vs, int *id, double *Vsm, double *Vsp,

@@ -293,9 +293,20 @@ def sphere_obstacle(nx=48, ny=32, nz=32, radius=6.0, centre=None, vwall=(0.0, 0.
     return settings, V, ID, np.array(faces, dtype=np.int32).reshape(-1, 4), np.array(walls).reshape(-1, 12)
 
 
-def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0), p_range=(0.1, 100.0), select=None):
-    """Returns a list of (settings, V, ID), one per instance (with `select`, only the instances of
-    those indices, drawn from the same random stream as the whole set).
+# second gas of the two-material batched instances (stiffened gas, gamma 1.67, no stiffening)
+GAS2 = (1.67, 0.0)
+
+
+def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0), p_range=(0.1, 100.0), select=None,
+                    two_every=0, reinit_freq=5):
+    """Returns a list of (settings, V, ID) or, for two-material instances, (settings, V, ID, phi),
+    one per instance (with `select`, only the instances of those indices, drawn from the same random
+    stream as the whole set).
+
+    two_every = k > 0: every k-th instance (indices k-1, 2k-1, ...) is two-material: the gas of the
+    left side of its split plane is a second gas (GAS2, material 1, tracked by a level set, the
+    signed distance to the plane, reinitialised every `reinit_freq` steps), with exact Riemann
+    solves at the interface faces; the others keep one gas (material 0, gamma 1.4).
 
     CFL is 0.4: the time step follows SpaceOperator.cpp:2287, dt = CFL * min over the three
     directions of dx/lambda, so an oblique wave sees the sum of the three directional Courant
@@ -312,15 +323,22 @@ def batched_riemann(n=64, lo=48, hi=160, seed=BATCH_SEED, rho_range=(0.1, 10.0),
         p = np.exp(rng.uniform(np.log(p_range[0]), np.log(p_range[1]), size=2))
         if select is not None and q not in select:
             continue
+        two = two_every > 0 and (q + 1) % two_every == 0
         settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, float(nx), 0.0, float(ny), 0.0, float(nz)),
                         bc=[BC_SYMMETRY] * 6, mats=[material(EOS_SG, [1.4, 0.0])], flux=FLUX_HLLC,
                         limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.4)
         z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
-        left = ((x - point[0]) * normal[0] + (y - point[1]) * normal[1] + (z - point[2]) * normal[2]) < 0.0
+        dist = (x - point[0]) * normal[0] + (y - point[1]) * normal[1] + (z - point[2]) * normal[2]
+        left = dist < 0.0
         V = np.zeros((nz, ny, nx, 5))
         V[..., 0] = np.where(left, rho[0], rho[1])
         V[..., 4] = np.where(left, p[0], p[1])
-        out.append((settings, V, np.zeros((nz, ny, nx), dtype=np.int32)))
+        if two:
+            settings.update(mats=[material(EOS_SG, [1.4, 0.0]), material(EOS_SG, list(GAS2))], ls_matid=1,
+                            reinit_freq=reinit_freq)
+            out.append((settings, V, np.where(left, 1, 0).astype(np.int32), dist.copy()))
+        else:
+            out.append((settings, V, np.zeros((nz, ny, nx), dtype=np.int32)))
     return out
 
 

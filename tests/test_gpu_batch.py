@@ -81,3 +81,38 @@ def test_batch_autotune_restores_state_and_keeps_results():
     for i, (s, V0, ID0) in enumerate(insts):
         st, V = solo(s, V0, ID0, 5, precision=M.PREC_MIXED)
         assert np.array_equal(b.download(i)[0], V)
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED])
+def test_batch_tile_shapes_match_single_instances(prec):
+    # the 16 x 16 column tiles (partial tiles at every instance's edges: edges 9..37) and the 32 x 8 ones
+    insts = W.batched_riemann(n=5, lo=9, hi=37, seed=13, two_every=3, reinit_freq=2)
+    b = M.Batch([W.config_from(x[0], precision=prec) for x in insts])
+    for i, x in enumerate(insts):
+        b.upload(i, x[1], x[2])
+        if len(x) == 4:
+            b.upload_levelset(i, x[3])
+    b.set_tile(16, 16)
+    b.run(2, graph=False)
+    b.set_tile(32, 8)
+    b.run(2, graph=True)
+    b.set_tile(16, 16)
+    b.set_tuning(4, 256)
+    b.run(2, graph=True)
+    with pytest.raises(M.M2CError):
+        b.set_tile(64, 4)
+    for i, x in enumerate(insts):
+        s = M.Solver(W.config_from(x[0], precision=prec))
+        s.upload(x[1], x[2])
+        if len(x) == 4:
+            s.upload_levelset(x[3])
+        st = s.run(1e300, 6, sync_every=0)
+        V, ID = s.download()
+        phi = s.download_levelset() if len(x) == 4 else None
+        s.close()
+        Vb, IDb = b.download(i)
+        assert b.status(i).steps == st.steps == 6 and b.status(i).t == st.t, i
+        assert np.array_equal(Vb, V) and np.array_equal(IDb, ID), i
+        if phi is not None:
+            assert np.array_equal(b.download_levelset(i), phi), i
+    b.close()
