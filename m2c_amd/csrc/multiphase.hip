@@ -670,7 +670,10 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
 // out = t + (coef*dt) R(phiS; velocity of S), with t = 0*out + 1*X (LS_FIRST), a*X + b*Y (LS_AXBY)
 // or X (LS_X). FINAL != 0: also the material-ID update of the step (MultiPhaseOperator.cpp:127-213)
 // from sign(phi_n) = sign(X... phin) and the new value; out may alias phin in that case.
-template <int IN_CONS, int MODE, int FINAL>
+// NEUMANN: every level-set boundary is zero-Neumann, so every cell reads its neighbours at their
+// clamped indices (for a cell whose stencil stays in the box those are its fixed offsets); one
+// path keeps the kernel's registers low. Otherwise cells near the boundary take the ghost rules.
+template <int IN_CONS, int MODE, int FINAL, int NEUMANN>
 __global__ __launch_bounds__(256) void k_ls_stage(const double *S, const int8_t *IDn, const double *phiS,
                                                    const double *X, const double *Y, double *out, double a, double b,
                                                    double coef, const double *phin, int8_t *IDnew, unsigned *rem,
@@ -686,8 +689,8 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *S, const int8_t 
     double R;
     // cells whose stencil stays in the box read their neighbours at fixed offsets (the same values the
     // ghost rules would give them)
-    if (inner2) R = ls_residual<1>(phiS, i, j, k, idx, true, u);
-    else if (c_mp.ls_neumann) R = ls_residual<2>(phiS, i, j, k, idx, false, u);
+    if (NEUMANN) R = ls_residual<2>(phiS, i, j, k, idx, inner2, u);
+    else if (inner2) R = ls_residual<1>(phiS, i, j, k, idx, true, u);
     else R = ls_residual<0>(phiS, i, j, k, idx, false, u);
     double t;
     if (MODE == LS_FIRST) {
@@ -2076,9 +2079,15 @@ extern "C" int m2c_mp_launch_ls(const MpLaunch &L, int in_cons, int mode, int fi
                                 const double *phiS, const double *X, const double *Y, double *out, double a, double b,
                                 double coef, const double *phin) {
   const dim3 gr = sweep_grid(L.nx, L.ny, L.nz), bl(SWEEP_BX, SWEEP_BY);
-#define M2C_LS(IC, MO, FI)                                                                                      \
-  hipLaunchKernelGGL((k_ls_stage<IC, MO, FI>), gr, bl, 0, L.stream, S, L.ID, phiS, X, Y, out, a, b, coef, phin, \
-                     L.IDnew, L.rem, L.ms, L.clk)
+#define M2C_LS(IC, MO, FI)                                                                                   \
+  do {                                                                                                       \
+    if (L.ls_neumann)                                                                                        \
+      hipLaunchKernelGGL((k_ls_stage<IC, MO, FI, 1>), gr, bl, 0, L.stream, S, L.ID, phiS, X, Y, out, a, b,    \
+                         coef, phin, L.IDnew, L.rem, L.ms, L.clk);                                           \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_ls_stage<IC, MO, FI, 0>), gr, bl, 0, L.stream, S, L.ID, phiS, X, Y, out, a, b,    \
+                         coef, phin, L.IDnew, L.rem, L.ms, L.clk);                                           \
+  } while (0)
   if (final_) {
     if (in_cons) {
       if (mode == LS_AXBY) M2C_LS(1, LS_AXBY, 1); else if (mode == LS_X) M2C_LS(1, LS_X, 1); else M2C_LS(1, LS_FIRST, 1);
