@@ -70,11 +70,11 @@ BYTES_DIFF_STAGE_READ = 288.0
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --steps 3 --warmup 1 --no-cpu
 # --no-mixed` (FETCH scaled by 2.0, WRITE by 1.0 after a copy-kernel calibration; summary in
 # profiles/stage_kernel_periodic256_rk3_fp64_traffic.md). Re-measure after changing the kernel.
-STAGE_TRAFFIC_PERIODIC256_FP64 = 3.430e9
+STAGE_TRAFFIC_PERIODIC256_FP64 = 3.413e9
 # The same for the two RK2 stage launches of the 384x192x192 shock-bubble FP64 run (mean of the two),
 # from `bench.py --steps 3 --warmup 1 --no-cpu --no-mixed --no-full` under the same two counters
 # (summary in profiles/stage_kernel_shock_bubble_384x192x192_fp64_traffic.md).
-STAGE_TRAFFIC_SHOCK_BUBBLE_FP64 = 3.013e9
+STAGE_TRAFFIC_SHOCK_BUBBLE_FP64 = 2.935e9
 
 
 def host_cpu():
