@@ -68,8 +68,10 @@ typedef struct {
   double surrogate_residual_tol;
 } m2c_riemann_params;                /* ExactRiemannSolverData, IoData.cpp:1345-1357 */
 
+#define M2C_MAX_MESH_PTS 16
+
 typedef struct {
-  int nx, ny, nz;                   /* uniform Cartesian mesh (MeshGenerator.cpp:146-158) */
+  int nx, ny, nz;                   /* cells along each axis (with control points: as m2c_mesh_1d) */
   double x0, x1, y0, y1, z0, z1;
   int bc[6];                        /* x0, xmax, y0, ymax, z0, zmax */
   double inlet[5], outlet[5];       /* BoundaryConditions.Inlet / Outlet (primitive) */
@@ -105,7 +107,27 @@ typedef struct {
    * |phi(t_n)| below it use constant reconstruction (SpaceOperator::TagNodesOutsideConRecDepth);
    * 0 = none (the default) */
   double mp_conrec_depth;
+  /* Mesh.ControlPointX/Y/Z[i] (MeshData, IoData.cpp:486-495; MeshResolution1DPointData: Coordinate,
+   * CellWidth): mesh_npts[d] > 0 stretches axis d as MeshGenerator::ComputeMesh1DNonUniform does
+   * (MeshGenerator.cpp:163-259), and n[d] must equal the number of cells that generates
+   * (m2c_mesh_1d); 0 = the uniform axis of n[d] cells (ComputeMesh1DUniform). Stretched meshes run
+   * with the reference's per-cell limiter coefficients A, B, K (Reconstructor.cpp:53-121), face
+   * areas and volumes (SpaceOperator.cpp:160-177), CFL widths, level-set coordinates and
+   * reinitialisation steps; not with periodic axes, viscous or heat-diffusion terms, embedded walls
+   * or batches (M2C_ERR_UNSUPPORTED). */
+  int mesh_npts[3];
+  double mesh_pts[3][M2C_MAX_MESH_PTS][2];
 } m2c_config;
+
+/* The cells of one mesh axis (MeshGenerator.cpp:99-259, the reference's steps and rounding):
+ * npts = 0: n uniform cells of [x0, xmax] (ComputeMesh1DUniform); npts > 0: the cells
+ * ComputeMesh1DNonUniform generates from the control points pts (npts pairs of coordinate and cell
+ * width; n unused). Writes up to cap cell centres x and widths dx (either may be NULL) and the
+ * number of cells to *nout; M2C_ERR_ARG where the reference stops with an error (control points
+ * outside the axis, widths in conflict with the points) and for widths that are not positive and
+ * finite or an axis of 1e9 cells or more (where the reference divides by zero or overflows). */
+int m2c_mesh_1d(double x0, double xmax, int n, int npts, const double *pts, int cap, double *x, double *dx,
+                int *nout);
 
 typedef struct {
   int error;                        /* sticky m2c_status_code of device-detected conditions */
@@ -404,7 +426,8 @@ typedef struct {
  * success. */
 int m2c_config_from_deck(const char *text, m2c_config *cfg, m2c_deck_run *run, char *msg, int msglen);
 
-/* Initial state of a deck on its uniform mesh: SpaceOperator::SetInitialCondition
+/* Initial state of a deck on its mesh (cell centres as m2c_mesh_1d generates them):
+ * SpaceOperator::SetInitialCondition
  * (SpaceOperator.cpp:696-960) and, for the material of its level set, LevelSetOperator::
  * SetInitialCondition (LevelSetOperator.cpp:291-556). DefaultInitialState everywhere, then the
  * GeometricEntities Plane, CylinderAndCone, CylinderWithSphericalCaps, Sphere and Parallelepiped

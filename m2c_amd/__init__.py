@@ -52,7 +52,8 @@ class Config(C.Structure):
                 ("precision", C.c_int), ("device", C.c_int),
                 ("ls_matid", C.c_int), ("ls_bc", C.c_int * 6), ("reinit_freq", C.c_int),
                 ("reinit_maxits", C.c_int), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
-                ("mp_riemann_normal", C.c_int), ("mp_conrec_depth", C.c_double)]
+                ("mp_riemann_normal", C.c_int), ("mp_conrec_depth", C.c_double),
+                ("mesh_npts", C.c_int * 3), ("mesh_pts", (C.c_double * 2 * 16) * 3)]
 
 
 class DeckRun(C.Structure):
@@ -124,10 +125,12 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
                 limiter=LIM_VANALBADA, ts=TS_RK2, cfl=0.5, rec_linear=True, inactive_id=-1, roe_delta=0.2,
                 gminmod_alpha=1.2, slope_zero=False, mp_flux=0, mp_recon=0, precision=PREC_FP64, device=0,
                 ls_matid=None, ls_bc=(1, 1, 1, 1, 1, 1), reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8,
-                reinit_tol=2.0e-4, mp_normal=1, conrec_depth=0.0):
+                reinit_tol=2.0e-4, mp_normal=1, conrec_depth=0.0, control_points=None):
     """Solver options; the level-set ones follow LevelSetSchemeData / LevelSetReinitializationData
     (IoData.cpp:1205-1266): ls_matid=None disables the level set. mp_normal: MultiPhase.RiemannNormal
-    (1 Mesh, 0 LevelSet, 2 Average); conrec_depth: MultiPhase.ConstantReconstructionDepth."""
+    (1 Mesh, 0 LevelSet, 2 Average); conrec_depth: MultiPhase.ConstantReconstructionDepth.
+    control_points: per axis None (uniform) or a list of (coordinate, cell width) pairs, Mesh.
+    ControlPointX/Y/Z; the axis's cell count must be the one mesh_1d returns for them."""
     c = Config()
     c.nx, c.ny, c.nz = nx, ny, nz
     c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
@@ -154,7 +157,29 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
     c.reinit_cfl, c.reinit_tol = reinit_cfl, reinit_tol
     c.mp_riemann_normal = mp_normal
     c.mp_conrec_depth = conrec_depth
+    for d, pts in enumerate(control_points or (None, None, None)):
+        pts = list(pts or [])
+        if len(pts) > 16:
+            raise ValueError("at most 16 control points per axis")
+        c.mesh_npts[d] = len(pts)
+        for q, (x, h) in enumerate(pts):
+            c.mesh_pts[d][q][0], c.mesh_pts[d][q][1] = x, h
     return c
+
+
+def mesh_1d(x0, xmax, n=0, points=None):
+    """Cell centres and widths of one mesh axis (m2c_mesh_1d: MeshGenerator's uniform axis of n
+    cells, or its stretched axis from (coordinate, cell width) control points)."""
+    L = lib()
+    pts = np.ascontiguousarray(np.array(points or [], dtype=np.float64).reshape(-1))
+    m = C.c_int(0)
+    _chk(L.m2c_mesh_1d(x0, xmax, n, len(pts) // 2, _dp(pts) if len(pts) else None, 0, None, None, C.byref(m)),
+         "m2c_mesh_1d")
+    x = np.empty(m.value)
+    w = np.empty(m.value)
+    _chk(L.m2c_mesh_1d(x0, xmax, n, len(pts) // 2, _dp(pts) if len(pts) else None, m.value, _dp(x), _dp(w),
+                       C.byref(m)), "m2c_mesh_1d")
+    return x, w
 
 
 _lib = None
@@ -201,6 +226,7 @@ def lib():
         L.m2c_batch_upload.argtypes = [Pb, C.c_int, D, C.POINTER(C.c_int32)]
         L.m2c_batch_download.argtypes = [Pb, C.c_int, D, C.POINTER(C.c_int32)]
         L.m2c_batch_upload_levelset.argtypes = [Pb, C.c_int, D]
+        L.m2c_mesh_1d.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, D, C.c_int, D, D, C.POINTER(C.c_int)]
         L.m2c_batch_download_levelset.argtypes = [Pb, C.c_int, D]
         L.m2c_batch_run.argtypes = [Pb, C.c_int, C.c_int, D]
         L.m2c_batch_status.argtypes = [Pb, C.c_int, C.POINTER(Status)]
@@ -236,12 +262,17 @@ def lib():
 
 
 class M2CError(RuntimeError):
-    pass
+    """A library call that returned an error status: `code` is ERR_ARG, ERR_HIP, ERR_STATE or
+    ERR_UNSUPPORTED."""
+
+    def __init__(self, message, code=None):
+        super().__init__(message)
+        self.code = code
 
 
 def _chk(rc, what):
     if rc != OK:
-        raise M2CError(f"{what} failed with status {rc}")
+        raise M2CError(f"{what} failed with status {rc}", rc)
 
 
 def axis_normals(dirs, n):
@@ -493,8 +524,7 @@ class DeckError(M2CError):
     ERR_UNSUPPORTED (valid for the reference, not provided here)."""
 
     def __init__(self, code, message):
-        super().__init__(message)
-        self.code = code
+        super().__init__(message, code)
 
 
 def config_from_deck(text, msglen=1 << 16):

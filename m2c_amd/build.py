@@ -52,7 +52,7 @@ UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), (
      + (["-DM2C_MP=1"] if t in MP_VARIANTS else []) + (FP32_FAST if tc == "float" else []) + (ILP_SCHED if t in ("hllc_sg", "hllc_sg_mp") else []))
     for t, f, e, tc in VARIANTS]
 # host-only C++ (no device code), compiled with the host compiler
-HOST_UNITS = [("input_deck", "input_deck.cpp"), ("output", "output.cpp")]
+HOST_UNITS = [("input_deck", "input_deck.cpp"), ("output", "output.cpp"), ("mesh", "mesh.cpp")]
 CXX = os.environ.get("CXX", "g++")
 
 

@@ -367,6 +367,7 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
     DevEos eos[8];
     int rc = m2c_fill_grid(&c, &b->hinst[i].grid, eos, &variants[i]);
     if (rc) { delete b; return rc; }
+    if (b->hinst[i].grid.stretched) { delete b; return M2C_ERR_UNSUPPORTED; }   // uniform meshes only
     // two-material instances: materials meeting at interface faces, or a level set
     if (b->hinst[i].grid.has_interfaces || c.ls_matid >= 0) b->mp_list.push_back(i);
     if (c.nmat > nmat_max) { nmat_max = c.nmat; memcpy(b->eos, eos, sizeof(eos)); }

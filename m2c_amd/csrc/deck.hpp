@@ -69,4 +69,8 @@ void gett(const Node *n, const char *name, int &out, std::initializer_list<std::
 // elements of an array object in increasing index order
 std::vector<const Node *> elements(const Node *arr);
 
+// cell centres x[d] and widths w[d] of the three axes of a configuration (uniform, or stretched by
+// its control points: m2c_mesh_1d); an m2c status
+int mesh_axes(const m2c_config &c, std::vector<double> x[3], std::vector<double> w[3]);
+
 }  // namespace m2c_deck

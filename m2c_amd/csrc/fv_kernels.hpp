@@ -71,6 +71,14 @@ struct Grid {
   double conrec_depth;         // |phi(t_n)| below this: constant reconstruction (0: none)
   int inactive_id, nmat, mp_flux, mp_recon;
   int has_interfaces;          // more than one active material
+  // stretched mesh (0: the uniform values above hold): per axis d the cell widths w[d][i + 2] for
+  // i in [-2, n+1] (a ghost takes its boundary cell's width, SpaceOperator.cpp:219-300) and the
+  // limiter coefficients cA, cB, cK of the cells 0..n-1 (Reconstructor.cpp:53-121). Read by the
+  // run-time-EOS variants of the kernels; a stretched mesh selects one of them.
+  int stretched;
+  const double *w[3];
+  const double *cA[3], *cB[3];
+  const int *cK[3];
 };
 
 struct Status {
@@ -241,11 +249,29 @@ __device__ __forceinline__ void fetch_cell(const double *__restrict__ S, const i
   id = f.id;
 }
 
-template <int DIM, typename TC> __device__ __forceinline__ TC limited_slope(TC dq0, TC dq1) {
+// Limiter coefficients A, B, K of the cell with index c along DIM (Reconstructor.cpp:53-121): the
+// uniform mesh's (A = B = 1), or with STR and a stretched mesh the cell's own (a cell beyond the box
+// takes its boundary cell's; such a slope never reaches a face state, the boundary rules give the
+// ghost's face value)
+struct SlopeCoef {
+  double A, B;
+  int K;
+};
+template <int DIM, bool STR> __device__ __forceinline__ SlopeCoef slope_coef(int c) {
+  const Grid &g = M2C_G;
+  if (STR && g.stretched) {
+    const int n = DIM == 0 ? g.nx : (DIM == 1 ? g.ny : g.nz);
+    const int q = clampi(c, 0, n - 1);
+    return {g.cA[DIM][q], g.cB[DIM][q], g.cK[DIM][q]};
+  }
+  return {g.A[DIM], g.B[DIM], g.K[DIM]};
+}
+
+template <typename TC> __device__ __forceinline__ TC limited_slope(TC dq0, TC dq1, const SlopeCoef &cf) {
   const Grid &g = M2C_G;
   switch (g.limiter) {
   case LIM_GENMINMOD: {
-    TC A = TC(g.A[DIM]), B = TC(g.B[DIM]), al = TC(g.alpha);
+    TC A = TC(cf.A), B = TC(cf.B), al = TC(g.alpha);
     if (dq0 < TC(0) && dq1 < TC(0)) return smax(al * dq0, smax(B / (A + TC(1)) * (dq0 + dq1), al * dq1));
     if (dq0 > TC(0) && dq1 > TC(0)) return smin(al * dq0, smin(B / (A + TC(1)) * (dq0 + dq1), al * dq1));
     return TC(0);
@@ -254,8 +280,8 @@ template <int DIM, typename TC> __device__ __forceinline__ TC limited_slope(TC d
     if (dq0 * dq1 <= TC(0)) return TC(0);
     if (dabs(dq0 * dq1) < TC(1e-16)) return TC(0);
     TC d0k = dq0, d1k = dq1;
-    for (int t = 1; t < g.K[DIM]; t++) { d0k *= dq0; d1k *= dq1; }
-    return TC(g.B[DIM]) * (dq1 * d0k + dq0 * d1k) / (d0k + TC(g.A[DIM]) * d1k);
+    for (int t = 1; t < cf.K; t++) { d0k *= dq0; d1k *= dq1; }
+    return TC(cf.B) * (dq1 * d0k + dq0 * d1k) / (d0k + TC(cf.A) * d1k);
   }
   default: return TC(0.5) * (dq0 + dq1);
   }
@@ -278,22 +304,23 @@ template <typename TC> __device__ __forceinline__ void zero5(TC *s) {
   for (int p = 0; p < 5; p++) s[p] = TC(0);
 }
 
-// Slopes of one cell in direction DIM from its two neighbours.
-template <int DIM, typename TC>
+// Slopes of the cell with index c along DIM from its two neighbours (STR: stretched meshes served).
+template <int DIM, typename TC, bool STR = false>
 __device__ __forceinline__ void cell_slopes(const TC *vl, const TC *vc, const TC *vr, int idl, int idc, int idr,
-                                            TC *sig) {
+                                            TC *sig, int c) {
   const Grid &g = M2C_G;
   if (!g.rec_linear || idc == g.inactive_id || (g.slope_zero && (idc != idl || idc != idr))) {
 #pragma unroll
     for (int p = 0; p < 5; p++) sig[p] = TC(0);
     return;
   }
+  const SlopeCoef cf = slope_coef<DIM, STR>(c);
   bool zl = (idl == g.inactive_id), zr = (idr == g.inactive_id);
 #pragma unroll
   for (int p = 0; p < 5; p++) {
     TC dq0 = zl ? TC(0) : vc[p] - vl[p];
     TC dq1 = zr ? TC(0) : vr[p] - vc[p];
-    sig[p] = limited_slope<DIM, TC>(dq0, dq1);
+    sig[p] = limited_slope<TC>(dq0, dq1, cf);
   }
 }
 
@@ -450,9 +477,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
   M2C_TILE_CONSTANTS(TS);
   const Grid &g = M2C_G;
   if (!a.ignore_active && !a.clk->active) return;
-  // faces cut by embedded walls: the run-time equation-of-state variants only (the stiffened-gas
-  // variants keep their registers for the interface code)
+  // faces cut by embedded walls and stretched meshes: the run-time equation-of-state variants only
+  // (the stiffened-gas variants keep their registers for the interface code)
   constexpr bool WALLS = MP && (EK < 0);
+  constexpr bool STR = EK < 0;
 
   __shared__ TC sV[5][LY][LX];
   __shared__ int sID[LY][LX];
@@ -498,8 +526,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
     fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, kbeg, w0d, id0, flags); to_tc(w0d, w0);
     fetch_cell<IN_CONS, EK, MP>(a.S, a.ID, ci, cj, kbeg + 1, d, id1, flags); to_tc(d, w1);
     TC szm[5];
-    cell_slopes<2, TC>(vm2, vm1, w0, idm2, idm1, id0, szm);
-    cell_slopes<2, TC>(vm1, w0, w1, idm1, id0, id1, sz0);
+    cell_slopes<2, TC, STR>(vm2, vm1, w0, idm2, idm1, id0, szm, kbeg - 1);
+    cell_slopes<2, TC, STR>(vm1, w0, w1, idm1, id0, id1, sz0, kbeg);
     if constexpr (MP) {   // a level set means two materials: the multi-material variants only
       if (a.phin) {
         if (conrec_cell(a.phin, ci, cj, kbeg - 1)) zero5(szm);
@@ -567,7 +595,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       sID[hy + HALO][hx + HALO] = hid;
     }
     // ---- z slope of k+1 (registers)
-    cell_slopes<2, TC>(w0, w1, w2, id0, id1, id2, sz1);
+    cell_slopes<2, TC, STR>(w0, w1, w2, id0, id1, id2, sz1, k + 1);
     if constexpr (MP) {
       if (a.phin && conrec_cell(a.phin, ci, cj, k + 1)) zero5(sz1);
       if (WALLS && a.Wm && g.slope_zero && (wall_cut(a.Wm, 2, ci, cj, k + 1) | wall_cut(a.Wm, 2, ci, cj, k + 2)))
@@ -588,8 +616,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
           vc[p] = sV[p][y + HALO][x + HALO];
           vr[p] = sV[p][y + HALO][x + 1 + HALO];
         }
-        cell_slopes<0, TC>(vl, vc, vr, sID[y + HALO][x - 1 + HALO], sID[y + HALO][x + HALO],
-                           sID[y + HALO][x + 1 + HALO], sg);
+        cell_slopes<0, TC, STR>(vl, vc, vr, sID[y + HALO][x - 1 + HALO], sID[y + HALO][x + HALO],
+                                sID[y + HALO][x + 1 + HALO], sg, i0 + x);
         if constexpr (MP) {
           if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
           if (WALLS && a.Wm && g.slope_zero &&
@@ -608,8 +636,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
           vc[p] = sV[p][y + HALO][x + HALO];
           vr[p] = sV[p][y + 1 + HALO][x + HALO];
         }
-        cell_slopes<1, TC>(vl, vc, vr, sID[y - 1 + HALO][x + HALO], sID[y + HALO][x + HALO],
-                           sID[y + 1 + HALO][x + HALO], sg);
+        cell_slopes<1, TC, STR>(vl, vc, vr, sID[y - 1 + HALO][x + HALO], sID[y + HALO][x + HALO],
+                                sID[y + 1 + HALO][x + HALO], sg, j0 + y);
         if constexpr (MP) {
           if (a.phin && conrec_cell(a.phin, i0 + x, j0 + y, k)) zero5(sg);
           if (WALLS && a.Wm && g.slope_zero &&
@@ -760,15 +788,22 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
       const long long fzb = ((long long)k * g.ny + gj) * g.nx + gi;
       const long long nfx = (long long)(g.nx + 1) * g.ny * g.nz, nfy = (long long)g.nx * (g.ny + 1) * g.nz,
                       nfz = (long long)g.nx * g.ny * (g.nz + 1), nxy = (long long)g.nx * g.ny;
+      // face areas and volume (SpaceOperator.cpp:160-177, 2554): the cell's own widths
+      double ax = g.ax, ay = g.ay, az = g.az, inv_mvol = g.inv_mvol;
+      if (STR && g.stretched) {
+        const double wx = g.w[0][gi + 2], wy = g.w[1][gj + 2], wz = g.w[2][k + 2];
+        ax = wy * wz; ay = wx * wz; az = wx * wy;
+        inv_mvol = 1.0 / (-(wx * wy * wz));
+      }
 #pragma unroll
       for (int p = 0; p < 5; p++) {
         double s = 0.0;
-        s -= Fw[p] * g.ax;
-        s -= Fs[p] * g.ay;
-        s -= Fb[p] * g.az;
-        s += Fe[p] * g.ax;
-        s += Fn[p] * g.ay;
-        s += Ft[p] * g.az;
+        s -= Fw[p] * ax;
+        s -= Fs[p] * ay;
+        s -= Fb[p] * az;
+        s += Fe[p] * ax;
+        s += Fn[p] * ay;
+        s += Ft[p] * az;
         if (diff) {
           s += a.Dv[0][p * nfx + fxw];
           s += a.Dv[1][p * nfy + fys];
@@ -785,7 +820,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
             s -= a.Dh[2][fzb + nxy];
           }
         }
-        R[p] = s * g.inv_mvol;
+        R[p] = s * inv_mvol;
         if (!dfinite(R[p])) flags |= ST_NAN_FLUX;
       }
       const long long idx = ((long long)k * g.ny + gj) * g.nx + gi;
@@ -834,7 +869,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int bx, con
             if (c2 < 0) flags |= ST_NEG_C2;
             double cs = dsqrt(c2);
             double lf = dabs(V[1]) + cs, lg = dabs(V[2]) + cs, lh = dabs(V[3]) + cs;
-            double loc = smin(g.dx / lf, smin(g.dy / lg, g.dz / lh));
+            double hx = g.dx, hy = g.dy, hz = g.dz;
+            if (STR && g.stretched) { hx = g.w[0][gi + 2]; hy = g.w[1][gj + 2]; hz = g.w[2][k + 2]; }
+            double loc = smin(hx / lf, smin(hy / lg, hz / lh));
             if (loc < cfl_min) cfl_min = loc;
           }
           if (clipped) atomicAdd(&a.st->clipped, 1ull);

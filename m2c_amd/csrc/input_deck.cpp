@@ -459,12 +459,41 @@ struct Mapper {
     getd(m, "Y0", c.y0); getd(m, "Ymax", c.y1);
     getd(m, "Z0", c.z0); getd(m, "Zmax", c.z1);
     geti(m, "NumberOfCellsX", c.nx); geti(m, "NumberOfCellsY", c.ny); geti(m, "NumberOfCellsZ", c.nz);
-    for (const char *cp : {"ControlPointX", "ControlPointY", "ControlPointZ"})
-      if (m && m->find(cp))
-        fail(M2C_ERR_UNSUPPORTED, m->find(cp)->line, std::string("Mesh.") + cp + ": only uniform meshes are provided");
-    // MeshGenerator.cpp:41-44: every direction needs a number of cells
-    if (c.nx <= 0 || c.ny <= 0 || c.nz <= 0)
-      fail(M2C_ERR_ARG, 0, "Mesh.NumberOfCellsX/Y/Z must be given (positive): unable to create the mesh");
+    // Mesh.ControlPointX/Y/Z[i] { Coordinate; CellWidth } (MeshResolution1DPointData, IoData.cpp:
+    // 430-447; both 0 unless given); MeshGenerator.cpp:41-48: an axis takes either a number of cells
+    // or control points, and with control points its cell count is the generator's
+    const char *const cpn[3] = {"ControlPointX", "ControlPointY", "ControlPointZ"};
+    int *const nn[3] = {&c.nx, &c.ny, &c.nz};
+    const double lo[3] = {c.x0, c.y0, c.z0}, hi[3] = {c.x1, c.y1, c.z1};
+    if (c.x1 < c.x0 || c.y1 < c.y0 || c.z1 < c.z0) fail(M2C_ERR_ARG, 0, "Mesh: the domain's dimensions are incorrect");
+    for (int d = 0; d < 3; d++) {
+      const Node *cp = m ? m->find(cpn[d]) : nullptr;
+      const std::vector<const Node *> el = elements(cp);
+      c.mesh_npts[d] = 0;
+      if (el.empty()) {
+        if (*nn[d] <= 0) fail(M2C_ERR_ARG, 0, "Mesh.NumberOfCellsX/Y/Z must be given (positive): unable to create the mesh");
+        continue;
+      }
+      if (*nn[d] > 0)
+        fail(M2C_ERR_ARG, cp->line, std::string("Mesh.") + cpn[d] + " together with a number of cells (conflicting input)");
+      if ((int)el.size() > M2C_MAX_MESH_PTS)
+        fail(M2C_ERR_UNSUPPORTED, cp->line,
+             std::string("Mesh.") + cpn[d] + ": more than " + std::to_string(M2C_MAX_MESH_PTS) + " control points");
+      for (size_t q = 0; q < el.size(); q++) {
+        double x = 0.0, h = 0.0;
+        getd(el[q], "Coordinate", x);
+        getd(el[q], "CellWidth", h);
+        c.mesh_pts[d][q][0] = x;
+        c.mesh_pts[d][q][1] = h;
+        ignore_rest(el[q], std::string("Mesh.") + cpn[d] + "[].", {"Coordinate", "CellWidth"});
+      }
+      c.mesh_npts[d] = (int)el.size();
+      int ncell = 0;
+      if (m2c_mesh_1d(lo[d], hi[d], 0, c.mesh_npts[d], &c.mesh_pts[d][0][0], 0, nullptr, nullptr, &ncell) != M2C_OK)
+        fail(M2C_ERR_ARG, cp->line,
+             std::string("Mesh.") + cpn[d] + ": control points outside the domain, or widths in conflict with them");
+      *nn[d] = ncell;
+    }
     for (int s = 0; s < 6; s++) {
       gett(m, kBcNames[s], c.bc[s],
            {{"None", 0}, {"Inlet", M2C_BC_INLET}, {"Outlet", M2C_BC_OUTLET}, {"Farfield0", M2C_BC_INLET},
@@ -477,7 +506,7 @@ struct Mapper {
       if (c.bc[s] == 0) fail(M2C_ERR_ARG, 0, std::string("Mesh.") + kBcNames[s] + " is None (or not given)");
     }
     ignore_rest(m, "Mesh.", {"Type", "X0", "Xmax", "Y0", "Ymax", "Z0", "Zmax", "NumberOfCellsX", "NumberOfCellsY",
-                             "NumberOfCellsZ", kBcNames[0], kBcNames[1], kBcNames[2], kBcNames[3], kBcNames[4],
+                             "NumberOfCellsZ", cpn[0], cpn[1], cpn[2], kBcNames[0], kBcNames[1], kBcNames[2], kBcNames[3], kBcNames[4],
                              kBcNames[5]});
   }
 
@@ -1096,13 +1125,15 @@ int m2c_initial_condition_from_deck(const char *text, double *V, int32_t *id, do
   put_msg(report, msg, msglen);
   if (rc != M2C_OK) return rc;
   const int nx = c.nx, ny = c.ny, nz = c.nz;
-  const double hx = (c.x1 - c.x0) / nx, hy = (c.y1 - c.y0) / ny, hz = (c.z1 - c.z0) / nz;
+  // cell centres: MeshGenerator's (x0 + h/2 + i h on a uniform axis)
+  std::vector<double> ax[3], aw[3];
+  if (int mrc = mesh_axes(c, ax, aw)) return mrc;
   const double diag = std::sqrt(std::pow(c.x1 - c.x0, 2) + std::pow(c.y1 - c.y0, 2) + std::pow(c.z1 - c.z0, 2));
   const bool want_phi = phi && c.ls_matid >= 0;
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) {
-        const V3 p{c.x0 + 0.5 * hx + i * hx, c.y0 + 0.5 * hy + j * hy, c.z0 + 0.5 * hz + k * hz};
+        const V3 p{ax[0][i], ax[1][j], ax[2][k]};
         const size_t cell = ((size_t)k * ny + j) * nx + i;
         double *v = V + 5 * cell;
         for (int q = 0; q < 5; q++) v[q] = r.initial_state[q];

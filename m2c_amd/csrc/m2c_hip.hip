@@ -104,6 +104,11 @@ struct m2c_ctx {
   double *bk_V = nullptr, *bk_phi = nullptr;
   int8_t *bk_ID = nullptr;
   MpState *bk_ms = nullptr;
+  // mesh: cell centres and widths of each axis (host), and on a stretched mesh the device tables
+  // the kernels read (widths with two ghosts per side, limiter coefficients per cell)
+  std::vector<double> mx[3], mw[3];
+  double *d_mesh = nullptr;
+  int *d_meshK = nullptr;
 };
 
 static const unsigned long long kDblMaxBits = 0x7FEFFFFFFFFFFFFFull;
@@ -176,7 +181,12 @@ __global__ __launch_bounds__(256) void k_cfl_min(const double *__restrict__ V, c
     if (c2 < 0) { flags |= ST_NEG_C2; continue; }
     double cs = dsqrt(c2);
     double lf = dabs(v[1]) + cs, lg = dabs(v[2]) + cs, lh = dabs(v[3]) + cs;
-    double loc = smin(g.dx / lf, smin(g.dy / lg, g.dz / lh));
+    double hx = g.dx, hy = g.dy, hz = g.dz;
+    if (EK < 0 && g.stretched) {   // the cell's own widths
+      const long long r = idx / g.nx;
+      hx = g.w[0][idx - r * g.nx + 2]; hy = g.w[1][r % g.ny + 2]; hz = g.w[2][r / g.ny + 2];
+    }
+    double loc = smin(hx / lf, smin(hy / lg, hz / lh));
     if (loc < mn) mn = loc;
   }
   // block minimum first: one atomic per block on the single address
@@ -832,6 +842,59 @@ static void launch_cfl(m2c_ctx *c) {
 }
 
 
+// Cell centres and widths of every axis (MeshGenerator::ComputeMeshCoordinatesAndDeltas,
+// MeshGenerator.cpp:12-131); on a stretched mesh also the device tables of the stage kernels: per
+// axis the widths of cells -2..n+1 (a ghost takes its boundary cell's width,
+// SpaceOperator::PopulateGhostBoundaryCoordinates, SpaceOperator.cpp:219-300) and the limiter
+// coefficients of Reconstructor::Setup (Reconstructor.cpp:53-121) per cell.
+static int setup_mesh(m2c_ctx *c) {
+  const m2c_config &cfg = c->cfg;
+  const double lo[3] = {cfg.x0, cfg.y0, cfg.z0}, hi[3] = {cfg.x1, cfg.y1, cfg.z1};
+  const int n[3] = {cfg.nx, cfg.ny, cfg.nz};
+  for (int d = 0; d < 3; d++) {
+    c->mx[d].assign((size_t)n[d], 0.0);
+    c->mw[d].assign((size_t)n[d], 0.0);
+    int m = 0;
+    if (int rc = m2c_mesh_1d(lo[d], hi[d], n[d], cfg.mesh_npts[d], &cfg.mesh_pts[d][0][0], n[d], c->mx[d].data(),
+                             c->mw[d].data(), &m))
+      return rc;
+    if (m != n[d]) return M2C_ERR_ARG;
+  }
+  if (!c->grid.stretched) return M2C_OK;
+  size_t nd = 0, nk = 0;
+  for (int d = 0; d < 3; d++) { nd += (size_t)(n[d] + 4) + 2 * (size_t)n[d]; nk += (size_t)n[d]; }
+  std::vector<double> h(nd);
+  std::vector<int> hk(nk);
+  size_t ow[3], oa[3], ob[3], ok[3], q = 0, qk = 0;
+  for (int d = 0; d < 3; d++) {
+    const std::vector<double> &w = c->mw[d];
+    auto W = [&](int i) { return w[(size_t)std::min(std::max(i, 0), n[d] - 1)]; };   // ghosts: boundary widths
+    ow[d] = q;
+    for (int i = -2; i <= n[d] + 1; i++) h[q++] = W(i);
+    oa[d] = q; q += (size_t)n[d];
+    ob[d] = q; q += (size_t)n[d];
+    ok[d] = qk; qk += (size_t)n[d];
+    for (int i = 0; i < n[d]; i++) {
+      const double A = (W(i - 1) + W(i)) / (W(i) + W(i + 1));
+      const double B = 2.0 * W(i) / (W(i) + W(i + 1));
+      h[oa[d] + i] = A;
+      h[ob[d] + i] = B;
+      hk[ok[d] + i] = cfg.limiter == M2C_LIM_VANALBADA ? slope_k(A, B) : 0;
+    }
+  }
+  HIPCHK(hipMalloc(&c->d_mesh, sizeof(double) * nd));
+  HIPCHK(hipMalloc(&c->d_meshK, sizeof(int) * nk));
+  HIPCHK(hipMemcpy(c->d_mesh, h.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_meshK, hk.data(), sizeof(int) * nk, hipMemcpyHostToDevice));
+  for (int d = 0; d < 3; d++) {
+    c->grid.w[d] = c->d_mesh + ow[d];
+    c->grid.cA[d] = c->d_mesh + oa[d];
+    c->grid.cB[d] = c->d_mesh + ob[d];
+    c->grid.cK[d] = c->d_meshK + ok[d];
+  }
+  return M2C_OK;
+}
+
 // Buffers and tables of the interface pass and the level set (per-cell arrays without a ghost
 // frame; face arrays keyed by the face's plus cell).
 static int setup_multiphase(m2c_ctx *c) {
@@ -921,15 +984,14 @@ static int setup_multiphase(m2c_ctx *c) {
     // coordinates x[-1..n] (MeshGenerator.cpp:146-158, ghosts SpaceOperator.cpp:188-230) and the
     // FD3 coefficients of every index along each axis (GradientCalculatorFD3.cpp:300-469)
     const int n[3] = {g.nx, g.ny, g.nz};
-    const double lo[3] = {cfg->x0, cfg->y0, cfg->z0}, h[3] = {g.dx, g.dy, g.dz};
     std::vector<double> xc, fd;
-    size_t xo[3], lo_off[3], ro_off[3], r1y_off;
+    size_t xo[3], wo[3], lo_off[3], ro_off[3], r1y_off;
     for (int d = 0; d < 3; d++) {
       xo[d] = xc.size();
       std::vector<double> x(n[d] + 2);
-      for (int i = 0; i < n[d]; i++) x[i + 1] = lo[d] + 0.5 * h[d] + i * h[d];
-      x[0] = x[1] - h[d];
-      x[n[d] + 1] = x[n[d]] + h[d];
+      for (int i = 0; i < n[d]; i++) x[i + 1] = c->mx[d][i];
+      x[0] = x[1] - c->mw[d][0];
+      x[n[d] + 1] = x[n[d]] + c->mw[d][n[d] - 1];
       xc.insert(xc.end(), x.begin(), x.end());
       const double *xi = x.data() + 1;
       lo_off[d] = fd.size();
@@ -957,6 +1019,13 @@ static int setup_multiphase(m2c_ctx *c) {
         fd.insert(fd.end(), w, w + 4);
       }
     }
+    // cell widths with one ghost per side (the boundary cell's), index i + 1
+    for (int d = 0; d < 3; d++) {
+      wo[d] = xc.size();
+      xc.push_back(c->mw[d][0]);
+      xc.insert(xc.end(), c->mw[d].begin(), c->mw[d].end());
+      xc.push_back(c->mw[d][n[d] - 1]);
+    }
     r1y_off = fd.size();
     {
       const double *yj = xc.data() + xo[1] + 1;
@@ -970,8 +1039,10 @@ static int setup_multiphase(m2c_ctx *c) {
     if ((rc = mp_alloc(c, &c->fd, fd.size(), 0))) return rc;
     HIPCHK(hipMemcpy(c->xc, xc.data(), sizeof(double) * xc.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->fd, fd.data(), sizeof(double) * fd.size(), hipMemcpyHostToDevice));
+    mp.stretched = g.stretched;
     for (int d = 0; d < 3; d++) {
       mp.xc[d] = c->xc + xo[d];
+      mp.wc[d] = c->xc + wo[d];
       mp.fdl[d] = c->fd + lo_off[d];
       mp.fdr[d] = c->fd + ro_off[d];
     }
@@ -1024,6 +1095,23 @@ int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   g.N = (long long)cfg->nx * cfg->ny * cfg->nz;
   for (int s = 0; s < 6; s++) g.bc[s] = cfg->bc[s];
   for (int p = 0; p < 5; p++) { g.inlet[p] = cfg->inlet[p]; g.outlet[p] = cfg->outlet[p]; }
+  // stretched axes (Mesh.ControlPointX/Y/Z): the configured cell count must be the generator's
+  {
+    const double lo[3] = {cfg->x0, cfg->y0, cfg->z0}, hi[3] = {cfg->x1, cfg->y1, cfg->z1};
+    const int n[3] = {cfg->nx, cfg->ny, cfg->nz};
+    for (int d = 0; d < 3; d++) {
+      if (cfg->mesh_npts[d] < 0 || cfg->mesh_npts[d] > M2C_MAX_MESH_PTS) return M2C_ERR_ARG;
+      if (cfg->mesh_npts[d] == 0) continue;
+      int m = 0;
+      if (int rc = m2c_mesh_1d(lo[d], hi[d], n[d], cfg->mesh_npts[d], &cfg->mesh_pts[d][0][0], 0, nullptr, nullptr, &m))
+        return rc;
+      if (m != n[d]) return M2C_ERR_ARG;
+      g.stretched = 1;
+    }
+    // the reference has no periodic axes; their ghost widths would be the far side's
+    for (int sd = 0; sd < 6 && g.stretched; sd++)
+      if (cfg->bc[sd] == M2C_BC_PERIODIC) return M2C_ERR_UNSUPPORTED;
+  }
   // uniform mesh, MeshGenerator.cpp:146-158; volume and areas as SpaceOperator.cpp:166-177
   g.dx = (cfg->x1 - cfg->x0) / cfg->nx;
   g.dy = (cfg->y1 - cfg->y0) / cfg->ny;
@@ -1054,7 +1142,9 @@ int m2c_fill_grid(const m2c_config *cfg, Grid *gp, DevEos *eos, int *variant) {
   int v;
   // HLLC over one or two stiffened gases: the compile-time stiffened-gas variants, with the
   // material-interface code (9, 10) when the two materials meet
-  if (cfg->flux == M2C_FLUX_HLLC) v = (all_sg && cfg->inactive_id < 0 && cfg->nmat <= 2) ? (g.has_interfaces ? 9 : 0) : 1;
+  // (a stretched mesh runs the run-time-EOS variants, which read the per-cell mesh tables)
+  if (cfg->flux == M2C_FLUX_HLLC)
+    v = (all_sg && cfg->inactive_id < 0 && cfg->nmat <= 2 && !g.stretched) ? (g.has_interfaces ? 9 : 0) : 1;
   else v = cfg->flux == M2C_FLUX_ROE ? 2 : 3;
   if (cfg->precision == M2C_PREC_MIXED) v = (v == 9) ? 10 : v + 4;
   if (cfg->flux == M2C_FLUX_GODUNOV) v = 8;
@@ -1117,6 +1207,10 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
     for (int m = 0; needT && m < cfg->nmat; m++)
       if (cfg->mat[m].eos == M2C_EOS_JWL || cfg->mat[m].eos == M2C_EOS_DUMMY) bad = bad ? bad : M2C_ERR_ARG;
   }
+  if (g.stretched) {   // stretched meshes: no diffusion terms (their face tables assume one spacing)
+    for (int m = 0; m < cfg->nmat; m++)
+      if (cfg->mat[m].viscosity != 0 || cfg->mat[m].kappa != 0.0) bad = bad ? bad : M2C_ERR_UNSUPPORTED;
+  }
   if (bad) {
     delete c;
     return bad;
@@ -1134,6 +1228,7 @@ int m2c_ctx_create(const m2c_config *cfg, m2c_ctx **out) {
   HIPCHK(hipMalloc(&c->clk, sizeof(Clock)));
   HIPCHK(hipMalloc(&c->st, sizeof(Status)));
   HIPCHK(hipMemset(c->st, 0, sizeof(Status)));
+  if (int rc = setup_mesh(c)) return rc;
   if (int rc = setup_multiphase(c)) return rc;
   if (int rc = setup_diffusion(c)) return rc;
   if (int rc = setup_godunov(c)) return rc;
@@ -1171,6 +1266,7 @@ int m2c_ctx_destroy(m2c_ctx *c) {
     c->st = c->L.st = c->own_st;
   }
   if (c->bound) hipHostFree(c->bound);
+  hipFree(c->d_mesh); hipFree(c->d_meshK);
   hipFree(c->bk_V); hipFree(c->bk_phi); hipFree(c->bk_ID); hipFree(c->bk_ms);
   hipFree(c->V); hipFree(c->U1); hipFree(c->U2); hipFree(c->R);
   hipFree(c->ID); hipFree(c->clk); hipFree(c->st);
@@ -1193,6 +1289,7 @@ int m2c_set_embedded_faces(m2c_ctx *c, int n, const int *face, const double *wal
   if (!c || n < 0 || (n > 0 && (!face || !wall))) return M2C_ERR_ARG;
   if (normal_mode < 0 || normal_mode > 2 || recon < 0 || recon > 1) return M2C_ERR_ARG;
   const Grid &g = c->grid;
+  if (n > 0 && g.stretched) return M2C_ERR_UNSUPPORTED;
   if (n > 0) {
     // one active material beside walls: no interfaces or level set, no diffusion terms, no
     // periodic axis (a wrapped ghost cell's cut faces are not tracked)

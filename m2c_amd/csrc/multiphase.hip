@@ -44,6 +44,13 @@ __device__ __forceinline__ long long cidx3(int i, int j, int k) {
 
 __device__ __forceinline__ double XC(int d, int i) { return c_mp.xc[d][i + 1]; }
 
+// min(dx, dy, dz) of cell (i,j,k): the reinitialisation's pseudo-time step / cfl and the width of
+// its sign function (LevelSetReinitializer.cpp:600-603, 988-989)
+__device__ __forceinline__ double cell_min_width(int i, int j, int k) {
+  if (!c_mp.stretched) return c_mp.min_dx;
+  return smin(c_mp.wc[0][i + 1], smin(c_mp.wc[1][j + 1], c_mp.wc[2][k + 1]));
+}
+
 // wave-aggregated append; every lane of the wavefront must call it
 __device__ __forceinline__ unsigned wave_append(unsigned *counter, bool pred) {
   unsigned long long m = __ballot(pred);
@@ -137,7 +144,6 @@ template <int D>
 __device__ void grad_at_face(const double *phi, const int q[3], double *dir) {
   const Grid &g = c_grid;
   const int n[3] = {g.nx, g.ny, g.nz};
-  const double h[3] = {g.dx, g.dy, g.dz};
   int m[3] = {q[0], q[1], q[2]};
   m[D] -= 1;
   auto P = [&](const int *c, int t, int s) {
@@ -146,7 +152,7 @@ __device__ void grad_at_face(const double *phi, const int q[3], double *dir) {
     return phi_nrm(phi, r[0], r[1], r[2]);
   };
   dir[D] = (P(q, D, 0) - P(m, D, 0)) / (XC(D, q[D]) - XC(D, m[D]));
-  const double x_half = XC(D, q[D]) - 0.5 * h[D];
+  const double x_half = XC(D, q[D]) - 0.5 * c_mp.wc[D][q[D] + 1];
   const double dd = XC(D, q[D]) - XC(D, m[D]);
   const double c1 = (XC(D, q[D]) - x_half) / dd;
   const double c2 = (x_half - XC(D, m[D])) / dd;
@@ -513,18 +519,18 @@ __device__ void face_states(const double *S, const int8_t *ID, int i, int j, int
   // crphi: the level set at t_n when a constant-reconstruction depth is set
   const bool cm = conrec_cell(crphi, i - di, j - dj, k - dk), cp = conrec_cell(crphi, i, j, k);
   if (gpl == 0 && !per) {
-    cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+    cell_slopes<DIR, double, true>(vm1, vp0, vp1, idm1, idp0, idp1, sgp, gpl);
     if (cp) zero5(sgp);
     face_state<-1, false, double>(vp0, sgp, idp0, fp);
     ghost_face<DIR, double>(g.bc[2 * DIR], fp, vm1, fm);
   } else if (gpl == gn && !per) {
-    cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
+    cell_slopes<DIR, double, true>(vm2, vm1, vp0, idm2, idm1, idp0, sgm, gpl - 1);
     if (cm) zero5(sgm);
     face_state<-1, true, double>(vm1, sgm, idm1, fm);
     ghost_face<DIR, double>(g.bc[2 * DIR + 1], fm, vp0, fp);
   } else {
-    cell_slopes<DIR, double>(vm2, vm1, vp0, idm2, idm1, idp0, sgm);
-    cell_slopes<DIR, double>(vm1, vp0, vp1, idm1, idp0, idp1, sgp);
+    cell_slopes<DIR, double, true>(vm2, vm1, vp0, idm2, idm1, idp0, sgm, gpl - 1);
+    cell_slopes<DIR, double, true>(vm1, vp0, vp1, idm1, idp0, idp1, sgp, gpl);
     if (cm) zero5(sgm);
     if (cp) zero5(sgp);
     face_state<-1, true, double>(vm1, sgm, idm1, fm);
@@ -856,7 +862,7 @@ __global__ __launch_bounds__(256) void k_reinit_tag(const double *phi, uint8_t *
     else if (p * n.kp <= 0) t = 1;
     tag[idx] = (uint8_t)t;
     det |= t;
-    const double f = c_mp.sign_factor;
+    const double f = 1.0 * cell_min_width(i, j, k);
     sign[idx] = p / dsqrt<double>(p * p + f * f);
     const double ob = phibk[idx];
     phibk[idx] = 0.0 * ob + 1.0 * p;
@@ -880,7 +886,7 @@ template <typename T>
 __device__ __forceinline__ T reinit_rhs_t(T p, const NbT<T> &n, int tg, T sg, int i, int j, int k, T cfl, T *lres) {
   *lres = T(0);
   if (tg != 0) return T(0);
-  const T dt = cfl * T(c_mp.min_dx);
+  const T dt = cfl * T(cell_min_width(i, j, k));
   T a, b, c, d, e, f;
   if (c_mp.dx_pow2[0]) { a = (p - n.im) * T(c_mp.inv_dx[0]); b = (n.ip - p) * T(c_mp.inv_dx[0]); }
   else { a = (p - n.im) / T(XC(0, i) - XC(0, i - 1)); b = (n.ip - p) / T(XC(0, i + 1) - XC(0, i)); }

@@ -21,6 +21,8 @@ struct MpParams {
   double sign_factor;          // smoothing width of the sign function: 1 * min(dx, dy, dz)
   double min_dx;               // min(dx, dy, dz): pseudo-time step = cfl * min_dx
   const double *xc[3];         // cell-centre coordinates, index -1..n stored at [i+1]
+  const double *wc[3];         // cell widths, index -1..n stored at [i+1] (ghosts: boundary widths)
+  int stretched;               // the widths vary: the reinitialisation's steps and sign widths per cell
   const double *fdl[3], *fdr[3], *fdr1y;   // FD3 coefficients per index along each axis
   // fd_uni[d]: the coefficients of fdl[d] and fdr[d] are the same for every index 2..n-3 (cells whose
   // stencil stays in the box); they are then also held in fdl_u/fdr_u. Likewise fdr1y.

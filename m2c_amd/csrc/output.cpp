@@ -58,13 +58,14 @@ struct Field {
   const double *V;
   const int32_t *ID;
   const double *phi;
+  const std::vector<double> *mx, *mw;   // cell centres and widths per axis (MeshGenerator's)
+  // centre of index q along axis d; the ghosts one width beyond the boundary cells
+  // (SpaceOperator::PopulateGhostBoundaryCoordinates, SpaceOperator.cpp:219-300)
   double xc(int d, int q) const {
-    const double lo[3] = {c.x0, c.y0, c.z0}, hi[3] = {c.x1, c.y1, c.z1};
-    const int n[3] = {c.nx, c.ny, c.nz};
-    const double h = (hi[d] - lo[d]) / n[d];
-    if (q < 0) return (lo[d] + 0.5 * h + 0 * h) - h;
-    if (q >= n[d]) return (lo[d] + 0.5 * h + (n[d] - 1) * h) + h;
-    return lo[d] + 0.5 * h + q * h;
+    const int n = (int)mx[d].size();
+    if (q < 0) return mx[d][0] - mw[d][0];
+    if (q >= n) return mx[d][n - 1] + mw[d][n - 1];
+    return mx[d][q];
   }
   size_t idx(int i, int j, int k) const { return ((size_t)k * c.ny + j) * c.nx + i; }
   // the outside direction of a face ghost: 2*dim + (0 low, 1 high), -1 inside
@@ -152,6 +153,7 @@ struct Line {
 
 struct m2c_output {
   m2c_config cfg;
+  std::vector<double> mx[3], mw[3];   // cell centres and widths of the mesh's axes
   int ls_index = -1;                 // the deck's LevelSet[i] index, -1 without a level set
   std::string prefix, base = "solution";
   int frequency = 0, frame = 0;
@@ -175,8 +177,9 @@ struct m2c_output {
 namespace {
 
 // ProbeOutput::SetupInterpolation for one point; false when it lies outside the ghosted lattice
-bool locate(const m2c_config &c, const double p[3], Stencil &s) {
-  const Field F{c, nullptr, nullptr, nullptr};
+bool locate(const m2c_output &o, const double p[3], Stencil &s) {
+  const m2c_config &c = o.cfg;
+  const Field F{c, nullptr, nullptr, nullptr, o.mx, o.mw};
   const int n[3] = {c.nx, c.ny, c.nz};
   int ijk[3];
   for (int d = 0; d < 3; d++) {
@@ -209,6 +212,7 @@ int read_output(const char *text, m2c_output &o, std::string &report) {
   m2c_deck_run run;
   int rc = read_config(text, root, o.cfg, run, report);
   if (rc != M2C_OK) return rc;
+  if ((rc = mesh_axes(o.cfg, o.mx, o.mw)) != M2C_OK) return rc;
   report.clear();
   try {
     const Node *sp = root.find("Space");
@@ -255,7 +259,7 @@ int read_output(const char *text, m2c_output &o, std::string &report) {
         getd(el[q], "Y", p[1]);
         getd(el[q], "Z", p[2]);
         Stencil s;
-        if (!locate(o.cfg, p, s)) fail(M2C_ERR_ARG, el[q]->line, "cannot locate probe node " + std::to_string(q) + " in the domain");
+        if (!locate(o, p, s)) fail(M2C_ERR_ARG, el[q]->line, "cannot locate probe node " + std::to_string(q) + " in the domain");
         o.probes.push_back(s);
       }
       std::string names[P_NVAR];
@@ -306,7 +310,7 @@ int read_output(const char *text, m2c_output &o, std::string &report) {
         double p[3];
         for (int d = 0; d < 3; d++) p[d] = L.x0[d] + i * ((L.x1[d] - L.x0[d]) / (L.n - 1));
         Stencil s;
-        if (!locate(o.cfg, p, s)) fail(M2C_ERR_ARG, n->line, "cannot locate line-plot point " + std::to_string(i));
+        if (!locate(o, p, s)) fail(M2C_ERR_ARG, n->line, "cannot locate line-plot point " + std::to_string(i));
         L.st.push_back(s);
       }
       if (L.n > 1 && !L.base.empty()) o.needs_eos = true;
@@ -361,7 +365,7 @@ void put_array(std::string &xml, std::vector<char> &blob, const char *name, int 
 int write_snapshot(m2c_output &o, m2c_ctx *ctx, double time, const double *V, const int32_t *ID, const double *phi) {
   const m2c_config &c = o.cfg;
   const size_t N = (size_t)c.nx * c.ny * c.nz;
-  const Field F{c, V, ID, phi};
+  const Field F{c, V, ID, phi, o.mx, o.mw};
   std::vector<char> blob;
   std::string coords, fields;
   const int n[3] = {c.nx, c.ny, c.nz};
@@ -502,7 +506,7 @@ int corner_temperatures(m2c_ctx *ctx, const std::vector<Corner> &cs, const std::
 
 int write_probes(m2c_output &o, m2c_ctx *ctx, double time, int step, const double *V, const int32_t *ID,
                  const double *phi) {
-  const Field F{o.cfg, V, ID, phi};
+  const Field F{o.cfg, V, ID, phi, o.mx, o.mw};
   std::vector<Corner> cs;
   for (const Stencil &s : o.probes) cs.push_back(corners(F, s, phi != nullptr));
   std::vector<double> T;
@@ -526,7 +530,7 @@ int write_probes(m2c_output &o, m2c_ctx *ctx, double time, int step, const doubl
 int write_line(m2c_output &o, Line &L, m2c_ctx *ctx, double time, int step, const double *V, const int32_t *ID,
                const double *phi) {
   if (L.base.empty()) return M2C_OK;
-  const Field F{o.cfg, V, ID, phi};
+  const Field F{o.cfg, V, ID, phi, o.mx, o.mw};
   std::vector<Corner> cs;
   for (const Stencil &s : L.st) cs.push_back(corners(F, s, phi != nullptr));
   std::vector<double> T;

@@ -13,6 +13,11 @@ m2c_amd.make_config, V the primitive state (nz, ny, nx, 5) and ID the material I
                     in the water at x=64 cells with post-shock p=1e9 Pa; exact two-material Riemann
                     solves at interface faces, level-set reinitialisation every 5 steps, HLLC+MUSCL
                     (van Albada), RK2, CFL 0.5. `scale` shrinks every length for small test runs.
+  shock_bubble_stretched
+                    shock_bubble on a mesh stretched by control points (Mesh.ControlPointX/Y/Z):
+                    cells of width 0.75 h across the bubble, growing linearly to 1.5 h at the box's
+                    faces (h = the uniform run's width; box, materials, bubble and shock as
+                    shock_bubble)
   periodic_viscous  periodic_modes with constant viscosity (mu 0.01, bulk 0.002) and heat diffusion
                     (kappa 0.02, T = p / rho)
   sphere_obstacle   a supersonic gas stream (Mach 2 along x, gamma 1.4) past a sphere bounded by an
@@ -30,7 +35,7 @@ input format (IoData.h:266-267 has no periodic code); they are the documented ex
 import numpy as np
 
 from . import (BC_INLET, BC_OUTLET, BC_PERIODIC, BC_SLIPWALL, BC_SYMMETRY, EOS_SG, FLUX_HLLC, LIM_VANALBADA,
-               TS_RK2, TS_RK3, VISC_CONSTANT, material)
+               TS_RK2, TS_RK3, VISC_CONSTANT, material, mesh_1d)
 
 FOURIER_SEED = 2024
 
@@ -134,6 +139,36 @@ def shock_bubble(nx=384, ny=192, nz=192, radius=24.0, cx=160.0, shock_x=64.0, p_
     return settings, V, ID, phi
 
 
+def shock_bubble_stretched(nx=384, ny=192, nz=192, radius=24.0, cx=160.0, shock_x=64.0, p_shock=1e9, h=1.0,
+                           reinit_freq=5, scale=1.0, fine=0.75, coarse=1.5):
+    """shock_bubble on a stretched mesh: width fine*h over the bubble's extent along each axis,
+    linear to coarse*h at the box's faces; the cell counts are the mesh generator's."""
+    settings, _, _, _ = shock_bubble(nx, ny, nz, radius, cx, shock_x, p_shock, h, reinit_freq, scale)
+    if scale != 1.0:
+        radius, cx, shock_x = radius * scale, cx * scale, shock_x * scale
+    box = settings["box"]
+    ctr = (cx * h, 0.5 * (box[2] + box[3]), 0.5 * (box[4] + box[5]))
+    pts, axes = [], []
+    for d in range(3):
+        lo, hi = box[2 * d], box[2 * d + 1]
+        p = [(lo, coarse * h), (ctr[d] - radius * h, fine * h), (ctr[d] + radius * h, fine * h), (hi, coarse * h)]
+        pts.append(p)
+        axes.append(mesh_1d(lo, hi, points=p)[0])
+    settings.update(nx=len(axes[0]), ny=len(axes[1]), nz=len(axes[2]), control_points=pts)
+    zc, yc, xc = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    phi = np.sqrt((xc - ctr[0]) ** 2 + (yc - ctr[1]) ** 2 + (zc - ctr[2]) ** 2) - radius * h
+    ID = np.where(phi < 0.0, 1, 0).astype(np.int32)
+    rho_w, p0 = 1000.0, 1e5
+    post = settings["inlet"]
+    V = np.zeros(ID.shape + (5,))
+    V[..., 0] = np.where(ID == 1, 1.2, rho_w)
+    V[..., 4] = p0
+    shocked = xc < shock_x * h
+    for c in range(5):
+        V[..., c] = np.where(shocked, post[c], V[..., c])
+    return settings, V, ID, phi
+
+
 def _num(x):
     """A number as deck text that reads back to the same double (17 significant digits)."""
     return repr(float(x))
@@ -189,17 +224,23 @@ under Output {{ Prefix = ""; Solution = "sod"; Frequency = 100; Density = On; Ve
 """
 
 
-def shock_bubble_deck(scale=1.0, reinit_freq=5):
-    """shock_bubble as an M2C input deck (h = 1): water, a level-set sphere of air, a planar shock."""
+def shock_bubble_deck(scale=1.0, reinit_freq=5, stretched=False):
+    """shock_bubble as an M2C input deck (h = 1): water, a level-set sphere of air, a planar shock;
+    stretched=True: shock_bubble_stretched's mesh, given by Mesh.ControlPointX/Y/Z."""
     settings, _, _, _ = shock_bubble(scale=scale, reinit_freq=reinit_freq)
     nx, ny, nz = settings["nx"], settings["ny"], settings["nz"]
     radius, cx, shock_x = 24.0, 160.0, 64.0
     if scale != 1.0:
         radius, cx, shock_x = radius * scale, cx * scale, shock_x * scale
+    cells = f"NumberOfCellsX = {nx}; NumberOfCellsY = {ny}; NumberOfCellsZ = {nz};"
+    if stretched:
+        pts = shock_bubble_stretched(scale=scale, reinit_freq=reinit_freq)[0]["control_points"]
+        cells = "\n  ".join(f"under ControlPoint{a}[{q}] {{ Coordinate = {_num(x)}; CellWidth = {_num(w)}; }}"
+                             for a, p in zip("XYZ", pts) for q, (x, w) in enumerate(p))
     return f"""// shock-bubble: water (stiffened gas) around an air bubble, planar shock at x = {_num(shock_x)}
 under Mesh {{
   X0 = 0; Xmax = {nx}; Y0 = 0; Ymax = {ny}; Z0 = 0; Zmax = {nz};
-  NumberOfCellsX = {nx}; NumberOfCellsY = {ny}; NumberOfCellsZ = {nz};
+  {cells}
   BoundaryConditionX0 = Inlet; BoundaryConditionXmax = Outlet;
   BoundaryConditionY0 = Symmetry; BoundaryConditionYmax = Symmetry;
   BoundaryConditionZ0 = Symmetry; BoundaryConditionZmax = Symmetry;

@@ -50,15 +50,32 @@ orc_ctx *orc_create(const orc_config *cfg) {
   c->dx = (cfg->x1 - cfg->x0) / cfg->nx;
   c->dy = (cfg->y1 - cfg->y0) / cfg->ny;
   c->dz = (cfg->z1 - cfg->z0) / cfg->nz;
-  c->vol = c->dx * c->dy * c->dz;
-  c->area[0] = c->dy * c->dz;
-  c->area[1] = c->dx * c->dz;
-  c->area[2] = c->dx * c->dy;
-  double d[3] = {c->dx, c->dy, c->dz};
-  for (int a = 0; a < 3; a++) {
-    c->A[a] = (d[a] + d[a]) / (d[a] + d[a]);
-    c->B[a] = 2.0 * d[a] / (d[a] + d[a]);
-    c->K[a] = (cfg->limiter == ORC_LIM_VANALBADA) ? slope_k(c->A[a], c->B[a]) : 0;
+  /* the mesh (MeshGenerator.cpp:12-131): widths and centres per axis; ghosts take the boundary
+   * cell's width (SpaceOperator.cpp:219-300); limiter coefficients per cell from the widths of the
+   * cell and its neighbours (Reconstructor.cpp:53-121) */
+  {
+    const double lo[3] = {cfg->x0, cfg->y0, cfg->z0}, hi[3] = {cfg->x1, cfg->y1, cfg->z1};
+    const int N[3] = {cfg->nx, cfg->ny, cfg->nz};
+    for (int a = 0; a < 3; a++) {
+      const int n = N[a];
+      c->w[a] = (double *)calloc((size_t)n + 2, sizeof(double));
+      c->mx[a] = (double *)calloc((size_t)n, sizeof(double));
+      c->cA[a] = (double *)calloc((size_t)n, sizeof(double));
+      c->cB[a] = (double *)calloc((size_t)n, sizeof(double));
+      c->cK[a] = (int *)calloc((size_t)n, sizeof(int));
+      int m = 0;
+      if (orc_mesh_1d(lo[a], hi[a], n, cfg->mesh_npts[a], &cfg->mesh_pts[a][0][0], n, c->mx[a], c->w[a] + 1, &m) ||
+          m != n)
+        return NULL;
+      c->w[a][0] = c->w[a][1];
+      c->w[a][n + 1] = c->w[a][n];
+      for (int i = 0; i < n; i++) {
+        const double wm = ORC_W(c, a, i - 1), w0 = ORC_W(c, a, i), wp = ORC_W(c, a, i + 1);
+        c->cA[a][i] = (wm + w0) / (w0 + wp);
+        c->cB[a][i] = 2.0 * w0 / (w0 + wp);
+        c->cK[a][i] = (cfg->limiter == ORC_LIM_VANALBADA) ? slope_k(c->cA[a][i], c->cB[a][i]) : 0;
+      }
+    }
   }
   long n = orc_ncells_ghosted(c);
   for (int f = 0; f < 6; f++) c->face[f] = (double *)calloc(n * 5, sizeof(double));
@@ -86,6 +103,7 @@ void orc_destroy(orc_ctx *c) {
   free(c->Un); free(c->U1); free(c->R); free(c->V1); free(c->V2);
   free(c->Fx); free(c->Fy); free(c->Fz); free(c->ferr);
   for (int d = 0; d < 3; d++) { free(c->Dv[d]); free(c->Dh[d]); free(c->eb_rec[d]); }
+  for (int d = 0; d < 3; d++) { free(c->w[d]); free(c->mx[d]); free(c->cA[d]); free(c->cB[d]); free(c->cK[d]); }
   free(c->eb_wall);
   orc_ls_free(c);
   free(c);
@@ -258,15 +276,19 @@ static void recon_cell(orc_ctx *c, const double *V, const int *ID, int i, int j,
   long nb[6] = {ORC_IDX(c, i - 1, j, k), ORC_IDX(c, i + 1, j, k), ORC_IDX(c, i, j - 1, k),
                 ORC_IDX(c, i, j + 1, k), ORC_IDX(c, i, j, k - 1), ORC_IDX(c, i, j, k + 1)};
   double sig[3][5];
+  const int ijk[3] = {i, j, k}, nn[3] = {c->nx, c->ny, c->nz};
   for (int d = 0; d < 3; d++) {
     long lo = nb[2 * d], hi = nb[2 * d + 1];
+    const int q = ijk[d] < 0 ? 0 : (ijk[d] >= nn[d] ? nn[d] - 1 : ijk[d]);
+    const double cA = c->cA[d][q], cB = c->cB[d][q];
+    const int cK = c->cK[d][q];
     for (int p = 0; p < 5; p++) {
       double dq0 = (ID[lo] == g->inactive_id) ? 0.0 : v[p] - V[5 * lo + p];
       double dq1 = (ID[hi] == g->inactive_id) ? 0.0 : V[5 * hi + p] - v[p];
       double s;
       switch (g->limiter) {
-      case ORC_LIM_GENMINMOD: s = gen_minmod(c->A[d], c->B[d], g->gminmod_alpha, dq0, dq1); break;
-      case ORC_LIM_VANALBADA: s = van_albada(c->A[d], c->B[d], c->K[d], dq0, dq1, g->vanalbada_pow); break;
+      case ORC_LIM_GENMINMOD: s = gen_minmod(cA, cB, g->gminmod_alpha, dq0, dq1); break;
+      case ORC_LIM_VANALBADA: s = van_albada(cA, cB, cK, dq0, dq1, g->vanalbada_pow); break;
       default: s = 0.5 * (dq0 + dq1); break;
       }
       sig[d][p] = s;
@@ -483,12 +505,14 @@ int orc_compute_residual(orc_ctx *c, const double *V, const int *ID, double *R) 
     orc_diffusion(c, V, ID, c->Dv, c->Dh);
   }
 
-  const double ax = c->area[0], ay = c->area[1], az = c->area[2];
-  const double inv = 1.0 / (-c->vol);
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) {
+        /* face areas and volume of the cell (SpaceOperator.cpp:160-177, 2554) */
+        const double wx = ORC_W(c, 0, i), wy = ORC_W(c, 1, j), wz = ORC_W(c, 2, k);
+        const double ax = wy * wz, ay = wx * wz, az = wx * wy;
+        const double inv = 1.0 / (-(wx * wy * wz));
         const long fxw = ((long)k * ny + j) * (nx + 1) + i, fys = ((long)k * (ny + 1) + j) * nx + i,
                    fzb = ((long)k * ny + j) * nx + i;
         const long fxe = fxw + 1, fyn = fys + nx, fzt = fzb + (long)nx * ny;
@@ -550,7 +574,7 @@ int orc_time_step(const orc_ctx *c, const double *V, const int *ID, double *dt) 
         if (c2 < 0) { bad = 1; continue; }
         double cs = sqrt(c2);
         double lf = fabs(v[1]) + cs, lg = fabs(v[2]) + cs, lh = fabs(v[3]) + cs;
-        double a = c->dx / lf, b = c->dy / lg, cc = c->dz / lh;
+        double a = ORC_W(c, 0, i) / lf, b = ORC_W(c, 1, j) / lg, cc = ORC_W(c, 2, k) / lh;
         double m2 = (cc < b) ? cc : b;
         double m1 = (m2 < a) ? m2 : a;
         mn = (m1 < mn) ? m1 : mn;

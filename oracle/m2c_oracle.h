@@ -59,7 +59,16 @@ typedef struct {
   double reinit_cfl, reinit_tol;
   int mp_normal;            /* MultiPhase.RiemannNormal: 0 LevelSet, 1 Mesh, 2 Average */
   double conrec_depth;      /* MultiPhase.ConstantReconstructionDepth (0: none) */
+  /* Mesh.ControlPointX/Y/Z: mesh_npts[d] > 0 stretches axis d (MeshGenerator.cpp:163-259), with
+   * n[d] the number of cells that generates; 0 = uniform */
+  int mesh_npts[3];
+  double mesh_pts[3][16][2];
 } orc_config;
+
+/* cells of one mesh axis (oracle/mesh.c): n uniform cells, or the stretched cells of npts control
+ * points (coordinate, width); up to cap centres x / widths dx, their count in *nout; -1 on error */
+int orc_mesh_1d(double x0, double xmax, int n, int npts, const double *pts, int cap, double *x, double *dx,
+                int *nout);
 
 typedef struct orc_ctx orc_ctx;
 

@@ -52,17 +52,15 @@ static void fd_weights(const double *d, int cidx, double *w) {
 void orc_ls_init(orc_ctx *c) {
   long n = orc_ncells_ghosted(c);
   int N[3] = {c->nx, c->ny, c->nz};
-  double lo[3] = {c->cfg.x0, c->cfg.y0, c->cfg.z0};
-  double h[3] = {c->dx, c->dy, c->dz};
   for (int d = 0; d < 3; d++) {
     c->rsV[d] = (double *)calloc((size_t)n * 10, sizeof(double));
     c->rsId[d] = (int *)calloc((size_t)n * 3, sizeof(int));
     for (long q = 0; q < n; q++) c->rsId[d][3 * q + 2] = -1;
     /* coordinates x[-1..N] (MeshGenerator.cpp:153-157, SpaceOperator.cpp:188-230) */
     double *x = (double *)calloc((size_t)N[d] + 2, sizeof(double));
-    for (int i = 0; i < N[d]; i++) x[i + 1] = lo[d] + 0.5 * h[d] + i * h[d];
-    x[0] = x[1] - h[d];
-    x[N[d] + 1] = x[N[d]] + h[d];
+    for (int i = 0; i < N[d]; i++) x[i + 1] = c->mx[d][i];
+    x[0] = x[1] - ORC_W(c, d, 0);
+    x[N[d] + 1] = x[N[d]] + ORC_W(c, d, N[d] - 1);
     c->xc[d] = x;
     /* FD3 coefficients per index along the axis */
     double *L = (double *)calloc((size_t)N[d] * 4, sizeof(double));
@@ -245,7 +243,7 @@ static double reinit_residual(orc_ctx *c, const double *phi, double *R, double c
       for (int i = 0; i < c->nx; i++) {
         long me = ORC_IDX(c, i, j, k);
         if (c->tag[me] != 0) { R[me] = 0.0; continue; }
-        double dx = dmin(c->dx, dmin(c->dy, c->dz));
+        double dx = dmin(ORC_W(c, 0, i), dmin(ORC_W(c, 1, j), ORC_W(c, 2, k)));
         double dt = cfl * dx;
         double p = phi[me];
         double a = (p - phi[ORC_IDX(c, i - 1, j, k)]) / (X(c, 0, i) - X(c, 0, i - 1));
@@ -293,8 +291,13 @@ int orc_ls_reinitialize(orc_ctx *c, double *phi) {
       }
   if (!detected) return 0;
   /* sign function (:594-615), eps = 1 */
-  double factor = 1.0 * dmin(c->dx, dmin(c->dy, c->dz));
-  for (long q = 0; q < n; q++) c->sign[q] = phi[q] / sqrt(phi[q] * phi[q] + factor * factor);
+  for (int k = -1; k <= c->nz; k++)   /* every cell of the ghosted box, with its own min width */
+    for (int j = -1; j <= c->ny; j++)
+      for (int i = -1; i <= c->nx; i++) {
+        const long q = ORC_IDX(c, i, j, k);
+        const double factor = 1.0 * dmin(ORC_W(c, 0, i), dmin(ORC_W(c, 1, j), ORC_W(c, 2, k)));
+        c->sign[q] = phi[q] / sqrt(phi[q] * phi[q] + factor * factor);
+      }
   for (long q = 0; q < n; q++) c->phibk[q] = 0.0 * c->phibk[q] + 1.0 * phi[q];
   for (;;) {
     double residual = 0.0, dphi_max = 0.0;

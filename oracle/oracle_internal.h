@@ -11,9 +11,11 @@ typedef struct { orc_rs_entry *e; long n, cap; } orc_rs_list;
 struct orc_ctx {
   orc_config cfg;
   int nx, ny, nz, gx, gy, gz;
-  double dx, dy, dz, vol, area[3];
-  double A[3], B[3];
-  int K[3];
+  double dx, dy, dz;
+  /* mesh per axis: cell widths with one ghost per side (the boundary cell's), index i + 1; cell
+   * centres of cells 0..n-1; limiter coefficients per cell (Reconstructor.cpp:53-121) */
+  double *w[3], *mx[3], *cA[3], *cB[3];
+  int *cK[3];
   double *face[6];          /* vl, vr, vb, vt, vk, vf (ghosted, AoS) */
   double *Un, *U1, *R, *V1, *V2;
   double *Fx, *Fy, *Fz;     /* per face: localflux1[5], localflux2[5] */
@@ -41,6 +43,8 @@ struct orc_ctx {
   double *eb_wall;
   int eb_n, eb_normal_mode, eb_recon;
 };
+
+#define ORC_W(c, d, i) ((c)->w[d][(i) + 1])
 
 #define ORC_FKEY(c, i, j, k) ((((long)(k)) * ((c)->ny + 1) + (j)) * (long)((c)->nx + 1) + (i))
 

@@ -52,7 +52,8 @@ class Config(C.Structure):
                 ("ls_on", C.c_int), ("ls_matid", C.c_int), ("ls_bc", C.c_int * 6),
                 ("reinit_freq", C.c_int), ("reinit_maxits", C.c_int),
                 ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double), ("mp_normal", C.c_int),
-                ("conrec_depth", C.c_double)]
+                ("conrec_depth", C.c_double),
+                ("mesh_npts", C.c_int * 3), ("mesh_pts", (C.c_double * 2 * 16) * 3)]
 
 
 def default_riemann():
@@ -91,7 +92,9 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
                 vanalbada_pow=True, inactive_id=-1, nthreads=0, roe_delta=0.2, gminmod_alpha=1.2,
                 slope_zero=False, mp_flux=0, mp_recon=0, ls_matid=None, ls_bc=(1, 1, 1, 1, 1, 1),
                 reinit_freq=-1, reinit_maxits=30, reinit_cfl=0.8, reinit_tol=2.0e-4, mp_normal=1,
-                conrec_depth=0.0):
+                conrec_depth=0.0, control_points=None):
+    """control_points: None, or per axis None / a list of (coordinate, cell width) pairs
+    (Mesh.ControlPointX/Y/Z); nx/ny/nz of a stretched axis must be its generated cell count."""
     c = Config()
     c.nx, c.ny, c.nz = nx, ny, nz
     c.x0, c.x1, c.y0, c.y1, c.z0, c.z1 = box
@@ -121,6 +124,11 @@ def make_config(nx, ny, nz, box, bc, mats, inlet=(1, 0, 0, 0, 1), outlet=(1, 0, 
     c.reinit_cfl, c.reinit_tol = reinit_cfl, reinit_tol
     c.mp_normal = mp_normal
     c.conrec_depth = conrec_depth
+    for d, pts in enumerate(control_points or (None, None, None)):
+        if pts:
+            c.mesh_npts[d] = len(pts)
+            for q, (x, w) in enumerate(pts):
+                c.mesh_pts[d][q][0], c.mesh_pts[d][q][1] = x, w
     return c
 
 
@@ -171,8 +179,24 @@ def lib():
             getattr(L, f).argtypes = [C.c_double]
             getattr(L, f).restype = C.c_double
         L.orc_counters.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.orc_mesh_1d.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, D, C.c_int, D, D, I]
         _lib = L
     return _lib
+
+
+def mesh_1d(x0, xmax, n=0, control_points=None):
+    """Cell centres and widths of one mesh axis (MeshGenerator.cpp:99-259): n uniform cells, or the
+    stretched cells of a list of (coordinate, width) control points."""
+    pts = np.ascontiguousarray(np.array(control_points or [], dtype=np.float64).reshape(-1))
+    cnt = C.c_int(0)
+    L = lib()
+    pp = _dp(pts) if pts.size else None
+    if L.orc_mesh_1d(x0, xmax, n, pts.size // 2, pp, 0, None, None, C.byref(cnt)) != 0:
+        raise ValueError("mesh_1d: control points and widths in conflict, or a bad axis")
+    x = np.zeros(cnt.value)
+    dx = np.zeros(cnt.value)
+    L.orc_mesh_1d(x0, xmax, n, pts.size // 2, pp, cnt.value, _dp(x), _dp(dx), C.byref(cnt))
+    return x, dx
 
 
 def _dp(a):

@@ -1193,7 +1193,6 @@ static double cdiff_local(double p0, double p1, double p2, double x0, double x1,
  * the three face directions as there */
 static void grad_at_face(const orc_ctx *c, const double *phi, int d, int i, int j, int k, double *dir) {
   int q[3] = {i, j, k}, m[3] = {i, j, k}, N[3] = {c->nx, c->ny, c->nz};
-  double h[3] = {c->dx, c->dy, c->dz};
   m[d] -= 1;
   dir[0] = dir[1] = dir[2] = 0.0;
   dir[d] = (phi_get(c, phi, q[0], q[1], q[2]) - phi_get(c, phi, m[0], m[1], m[2])) / (XCO(c, d, q[d]) - XCO(c, d, m[d]));
@@ -1218,7 +1217,7 @@ static void grad_at_face(const orc_ctx *c, const double *phi, int d, int i, int 
                           XCO(c, t, qt - 1), XCO(c, t, qt), XCO(c, t, qt + 1));
     }
   }
-  double x_half = XCO(c, d, q[d]) - 0.5 * h[d];
+  double x_half = XCO(c, d, q[d]) - 0.5 * ORC_W(c, d, q[d]);
   double dd = XCO(c, d, q[d]) - XCO(c, d, m[d]);
   double c1 = (XCO(c, d, q[d]) - x_half) / dd;
   double c2 = (x_half - XCO(c, d, m[d])) / dd;

@@ -41,7 +41,7 @@ def oracle_config(settings, vanalbada_pow=True, nthreads=0, **over):
                           ls_bc=s.get("ls_bc", (1, 1, 1, 1, 1, 1)), reinit_freq=s.get("reinit_freq", -1),
                           reinit_maxits=s.get("reinit_maxits", 30), reinit_cfl=s.get("reinit_cfl", 0.8),
                           reinit_tol=s.get("reinit_tol", 2.0e-4), mp_normal=s.get("mp_normal", 1),
-                          conrec_depth=s.get("conrec_depth", 0.0))
+                          conrec_depth=s.get("conrec_depth", 0.0), control_points=s.get("control_points"))
 
 
 def run_oracle(settings, V0, ID0, tmax=1e300, maxsteps=10, vanalbada_pow=True, phi0=None, **over):

@@ -70,6 +70,35 @@ def test_shock_bubble_deck_reads_back_to_the_workload(scale):
     assert np.array_equal(V, V0) and np.array_equal(ID, ID0) and np.array_equal(phi, phi0)
 
 
+@pytest.mark.parametrize("scale", [0.0625, 0.125])
+def test_stretched_shock_bubble_deck_reads_back_to_the_workload(scale):
+    # Mesh.ControlPointX/Y/Z: the cell counts, control points, cell centres of the shapes and the
+    # sphere's level set are the workload's, bit for bit
+    text = W.shock_bubble_deck(scale, reinit_freq=3, stretched=True)
+    assert "ControlPointY[2]" in text and "NumberOfCells" not in text
+    cfg, run, ignored = M.config_from_deck(text)
+    settings, V0, ID0, phi0 = W.shock_bubble_stretched(scale=scale, reinit_freq=3)
+    assert (cfg.nx, cfg.ny, cfg.nz) == (settings["nx"], settings["ny"], settings["nz"])
+    assert same_config(cfg, W.config_from(settings))
+    assert ignored == []
+    V, ID, phi = M.initial_condition_from_deck(text)
+    assert np.array_equal(V, V0) and np.array_equal(ID, ID0) and np.array_equal(phi, phi0)
+
+
+def test_control_points_set_the_cell_count():
+    base = MESH.replace("NumberOfCellsX = 8; NumberOfCellsY = 4; NumberOfCellsZ = 2;",
+                        "NumberOfCellsY = 4; NumberOfCellsZ = 2;")
+    cfg, _, _ = read("Mesh.ControlPointX[3].Coordinate = 2.0; Mesh.ControlPointX[3].CellWidth = 0.25;", base=base)
+    assert cfg.nx == 32 and cfg.mesh_npts[0] == 1 and cfg.mesh_npts[1] == 0
+    assert (cfg.mesh_pts[0][0][0], cfg.mesh_pts[0][0][1]) == (2.0, 0.25)
+    # MeshGenerator.cpp:41-48: an axis needs cells or control points, not both, and points inside it
+    assert err("Mesh.ControlPointX[0].CellWidth = 0.5;").code == M.ERR_ARG
+    assert err("", base=base).code == M.ERR_ARG
+    assert err("Mesh.ControlPointX[0].Coordinate = 9.0; Mesh.ControlPointX[0].CellWidth = 0.5;", base=base).code == M.ERR_ARG
+    # a width left at its default (0) has no cell count
+    assert err("Mesh.ControlPointX[0].Coordinate = 2.0;", base=base).code == M.ERR_ARG
+
+
 # ------------------------------------------------------------------ syntax (Parser.y, Lexer.l)
 
 def test_integer_and_double_arithmetic():
@@ -270,7 +299,7 @@ def test_missing_mesh_and_materials():
 
 @pytest.mark.parametrize("extra", [
     "Time.Type = Implicit;", "Time.LocalTimeStepping = On;", "Time.ConvergenceTolerance = 1e-6;",
-    "Mesh.Type = Cylindrical;", "Mesh.ControlPointX[0].Coordinate = 0.5;", "Mesh.BoundaryConditionX0 = Overset;",
+    "Mesh.Type = Cylindrical;", "Mesh.BoundaryConditionX0 = Overset;",
     "Equations.Material[0].EquationOfState = ANEOSBirchMurnaghanDebye;",
     "Equations.Material[0].ViscosityModel.Type = Artificial;",
     "Equations.MaterialTransition[0].FromMaterialID = 0;",

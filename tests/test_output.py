@@ -169,6 +169,35 @@ def test_probe_interpolation_is_exact_for_linear_fields(tmp_path):
         assert abs(p[q] - (2.0 - 0.1 * px + 0.3 * py + 0.7 * pz)) < 1e-8
 
 
+def test_stretched_mesh_coordinates_and_probes(tmp_path):
+    # Mesh.ControlPointX / Y: the snapshot's coordinates are the generated cell centres, and the
+    # probes' trilinear stencils sit on them (a linear field is reproduced at any point inside)
+    d = str(tmp_path)
+    mesh = MESH.format(bx0="Inlet").replace(
+        "NumberOfCellsX = 8; NumberOfCellsY = 4;",
+        "under ControlPointX[0] { Coordinate = 0; CellWidth = 0.5; } "
+        "under ControlPointX[1] { Coordinate = 8; CellWidth = 1.5; } "
+        "under ControlPointY[0] { Coordinate = 2; CellWidth = 0.25; }")
+    pts = [(2.3, 1.7, 0.9), (6.1, 2.2, 1.4), (0.4, 0.6, 1.1)]
+    o = M.Output(mesh + f'under Output {{ Prefix = "{d}/"; Density = On; '
+                 + PROBES.format(nodes=probe_nodes(pts)) + " }\n")
+    xs, _ = M.mesh_1d(0.0, 8.0, points=[(0.0, 0.5), (8.0, 1.5)])
+    ys, _ = M.mesh_1d(0.0, 4.0, points=[(2.0, 0.25)])
+    assert len(ys) == 16 and not np.allclose(np.diff(xs), np.diff(xs)[0])
+    z, y, x = np.meshgrid([0.5, 1.5], ys, xs, indexing="ij")
+    V = np.zeros(x.shape + (5,))
+    V[..., 0] = 1.0 + 0.25 * x - 0.5 * y + 0.125 * z
+    V[..., 4] = 2.0
+    o.write(0.0, 0.0, 0, V, np.zeros(x.shape, np.int32), force=True)
+    o.close()
+    s = read_vtr(os.path.join(d, "solution_0000.vtr"))
+    assert np.array_equal(s["Xcoord"], xs) and np.array_equal(s["Ycoord"], ys)
+    assert np.array_equal(s["density"], V[..., 0].ravel())
+    rho = probe_values(d, "rho.txt")[0]
+    for q, (px, py, pz) in enumerate(pts):
+        assert abs(rho[q] - (1.0 + 0.25 * px - 0.5 * py + 0.125 * pz)) < 1e-8
+
+
 def test_probe_ghost_rules(tmp_path):
     d = str(tmp_path)
     # x0 inlet (7, 3, 0, 0, 11); y0 slip wall, ymax stick wall (velocity mirrored); the z faces symmetric
