@@ -122,8 +122,9 @@ __device__ __forceinline__ void fwd_hidden(__bf16 *act, const __bf16 *W, const f
   }
 }
 
-// one activation buffer (33 KB): inference runs four workgroups per CU, training (more live
-// registers for the stored activations) two
+// the 64-sample forward with one activation buffer (33 KB): TRAIN = 1 is the training forward (two
+// workgroups per CU, more live registers for the stored activations); TRAIN = 0 evaluates the same
+// network at four workgroups per CU (inference runs k_mlp_infer below)
 template <int TRAIN>
 __global__ __launch_bounds__(256, TRAIN ? 2 : 4) void k_mlp_fwd(MlpWeights w, FwdArgs a) {
   __shared__ __attribute__((aligned(16))) __bf16 buf[1][MLP_TM * LDS_S];
@@ -187,6 +188,144 @@ __global__ __launch_bounds__(256, TRAIN ? 2 : 4) void k_mlp_fwd(MlpWeights w, Fw
       if (lane == 0) atomicAdd(a.loss, sq);
     }
     __syncthreads();   // buf[0] is restaged by the next tile
+  }
+}
+
+// ---- inference: 128-sample tiles in MFMA fragment order -----------------------------------------
+//
+// The product is formed transposed, Z^T = W X^T: the A operand is 16 output features x 32 inputs of
+// W, the B operand 32 inputs x 16 samples of the activation tile, so a lane's accumulator holds
+// four consecutive features of one sample. Weights (MlpWeights::WF) and the LDS activation tile are
+// both stored in fragment order: every A or B fragment a wave reads is one contiguous kilobyte (64
+// lanes x 16 bytes), and the epilogue's four features of a sample are one 8-byte LDS store. Each
+// wave computes 64 features for all 128 samples (4 x 8 MFMA tiles, 128 accumulator registers):
+// every weight fragment read from L2 feeds eight MFMAs, and the fragments of consecutive K steps
+// alternate between two register sets so that a step waits only for loads issued a step earlier.
+constexpr int MLP_TI = 128;   // samples per inference tile
+
+// bf16 offset of (sample s, feature f) in a fragment-ordered tile with ks_n K steps of 32 features:
+// B fragment (sample tile s/16, K step f/32) at 512 (s/16 * ks_n + f/32), lane s%16 + 16 (f%32/8)
+__device__ __forceinline__ int frag_off(int s, int f, int ks_n) {
+  return (((s >> 4) * ks_n + (f >> 5)) * 64 + (s & 15) + 16 * ((f & 31) >> 3)) * 8 + (f & 7);
+}
+
+template <int KS>
+__device__ __forceinline__ void mma_step_inf(const __bf16 *act, int ks, const bf16x8 (&a)[4], f32x4 (&acc)[4][8]) {
+  const int lane = threadIdx.x & 63;
+  bf16x8 b[8];
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++) b[nt] = *(const bf16x8 *)(act + ((nt * KS + ks) * 64 + lane) * 8);
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++)
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) acc[mt][nt] = mfma16(a[mt], b[nt], acc[mt][nt]);
+}
+
+// acc = W[f0 .. f0+63][0..K) x act[0..127][0..K)^T
+template <int K>
+__device__ __forceinline__ void tile_mma_inf(const __bf16 *act, const __bf16 *WF, int f0, f32x4 (&acc)[4][8]) {
+  constexpr int KS = K / 32;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // the fragment of feature tile f0/16 + mt and K step ks: 1 KB at wp + (mt KS + ks) 512
+  const __bf16 *wp = WF + ((size_t)(f0 / 16) * KS * 64 + lane) * 8;
+  bf16x8 a0[4], a1[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) a0[mt] = *(const bf16x8 *)(wp + (size_t)mt * KS * 512);
+#pragma unroll 1
+  for (int ks = 0; ks < KS; ks += 2) {
+    if (ks + 1 < KS) {
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) a1[mt] = *(const bf16x8 *)(wp + (size_t)(mt * KS + ks + 1) * 512);
+    }
+    mma_step_inf<KS>(act, ks, a0, acc);
+    if (ks + 1 < KS) {
+      if (ks + 2 < KS) {
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) a0[mt] = *(const bf16x8 *)(wp + (size_t)(mt * KS + ks + 2) * 512);
+      }
+      mma_step_inf<KS>(act, ks + 1, a1, acc);
+    }
+  }
+}
+
+// SiLU for inference: the hardware exponential and reciprocal (about one FP32 ulp), then bf16
+__device__ __forceinline__ float silu_inf(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+// one hidden layer of an inference tile, in place (outputs, 256 features in fragment order, written
+// after every wave has read the layer's inputs)
+template <int K>
+__device__ __forceinline__ void inf_hidden(__bf16 *act, const __bf16 *WF, const float *bias) {
+  const int lane = threadIdx.x & 63, f0 = (threadIdx.x >> 6) * 64;
+  f32x4 acc[4][8];
+  tile_mma_inf<K>(act, WF, f0, acc);
+  __syncthreads();
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    const int f = f0 + mt * 16 + 4 * (lane >> 4);
+    const float b0 = bias[f], b1 = bias[f + 1], b2 = bias[f + 2], b3 = bias[f + 3];
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++) {
+      bf16x4 hv;
+      hv[0] = (__bf16)silu_inf(acc[mt][nt][0] + b0);
+      hv[1] = (__bf16)silu_inf(acc[mt][nt][1] + b1);
+      hv[2] = (__bf16)silu_inf(acc[mt][nt][2] + b2);
+      hv[3] = (__bf16)silu_inf(acc[mt][nt][3] + b3);
+      *(bf16x4 *)(act + frag_off(nt * 16 + (lane & 15), f, MLP_H / 32)) = hv;
+    }
+  }
+}
+
+// persistent inference: two 128-sample tiles per CU at a time (64 KB of LDS each)
+__global__ __launch_bounds__(256, 2) void k_mlp_infer(MlpWeights w, FwdArgs a) {
+  __shared__ __attribute__((aligned(16))) __bf16 buf[MLP_TI * MLP_H];
+  const int count = a.n_dev ? min((int)*a.n_dev, a.n) : a.n;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int tile = blockIdx.x; tile * MLP_TI < count; tile += gridDim.x) {
+    const int r0 = tile * MLP_TI;
+    for (int t = threadIdx.x; t < MLP_TI * MLP_K0; t += 256) {   // inputs: one K step of 32
+      const int row = t / MLP_K0, col = t % MLP_K0, q = r0 + row;
+      const float x = (q < count && col < MLP_IN) ? a.X[(size_t)q * MLP_IN + col] : 0.f;
+      buf[frag_off(row, col, 1)] = (__bf16)x;
+    }
+    __syncthreads();
+    inf_hidden<MLP_K0>(buf, w.WF[0], w.bias[0]);
+    __syncthreads();
+    inf_hidden<MLP_H>(buf, w.WF[1], w.bias[1]);
+    __syncthreads();
+    inf_hidden<MLP_H>(buf, w.WF[2], w.bias[2]);
+    __syncthreads();
+    inf_hidden<MLP_H>(buf, w.WF[3], w.bias[3]);
+    __syncthreads();
+    // output layer: the 16 (padded) output rows of W4 as A, wave wv's sample tiles 2 wv, 2 wv + 1
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll 4
+    for (int ks = 0; ks < MLP_H / 32; ks++) {
+      const bf16x8 av = *(const bf16x8 *)(w.W[4] + (size_t)r * MLP_H + 32 * ks + kq);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const bf16x8 bv = *(const bf16x8 *)(buf + (((2 * wv + h) * (MLP_H / 32) + ks) * 64 + lane) * 8);
+        acc[h] = mfma16(av, bv, acc[h]);
+      }
+    }
+    // lane: sample 32 wv + 16 h + lane%16, outputs 4 (lane/16) .. +3; outputs 0..2 are the network's
+    if ((lane >> 4) == 0) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int q = r0 + wv * 32 + h * 16 + r;
+        if (q < count) {
+#pragma unroll
+          for (int o = 0; o < MLP_OUT; o++) a.Y[(size_t)q * MLP_OUT + o] = acc[h][o] + w.bias[4][o];
+        }
+      }
+    }
+    __syncthreads();   // buf is restaged by the next tile
   }
 }
 
@@ -287,7 +426,7 @@ __global__ void k_mlp_reduce(GradParts g, float *grad) {
 }
 
 struct NetPtrs {
-  __bf16 *W[5], *WT[5];
+  __bf16 *W[5], *WT[5], *WF[4];
   float *bias[5];
 };
 
@@ -298,6 +437,10 @@ __device__ __forceinline__ void write_copies(const NetPtrs &n, int p, float v) {
     const int kpad = l == 0 ? MLP_K0 : MLP_H;
     n.W[l][(size_t)o * kpad + i] = (__bf16)v;
     if (l >= 1) n.WT[l][(size_t)i * (l == 4 ? 32 : MLP_H) + o] = (__bf16)v;
+    if (l <= 3) {   // A-fragment order (MlpWeights::WF)
+      const int lane = o % 16 + 16 * ((i % 32) / 8);
+      n.WF[l][((size_t)((o / 16) * (kpad / 32) + i / 32) * 64 + lane) * 8 + i % 8] = (__bf16)v;
+    }
   } else {
     n.bias[l][p - mlp_boff(l)] = v;
   }
@@ -539,6 +682,7 @@ struct m2c_mlp {
 static NetPtrs net_ptrs(const MlpNet &n) {
   NetPtrs p;
   for (int l = 0; l < 5; l++) { p.W[l] = n.W[l]; p.WT[l] = n.WT[l]; p.bias[l] = n.bias[l]; }
+  for (int l = 0; l < 4; l++) p.WF[l] = n.WF[l];
   return p;
 }
 
@@ -563,6 +707,7 @@ static int mlp_alloc_params(m2c_mlp *m) {
   }
   size_t total = 4 * fp;
   for (int l = 0; l < 5; l++) total += wbytes[l] + wtbytes[l] + bbytes[l];
+  for (int l = 0; l < 4; l++) total += wbytes[l];   // the fragment-ordered copies
   char *d = nullptr;
   MCHK(hipMalloc(&d, total));
   MCHK(hipMemset(d, 0, total));
@@ -576,6 +721,7 @@ static int mlp_alloc_params(m2c_mlp *m) {
     m->net.WT[l] = l == 0 ? nullptr : (__bf16 *)d; d += wtbytes[l];
     m->net.bias[l] = (float *)d; d += bbytes[l];
   }
+  for (int l = 0; l < 4; l++) { m->net.WF[l] = (__bf16 *)d; d += wbytes[l]; }
   m->net.adam_t = 0;
   return M2C_OK;
 }
@@ -1057,9 +1203,9 @@ int m2c_mlp_launch_infer(const MlpWeights &w, hipStream_t stream, const float *X
   FwdArgs a;
   memset(&a, 0, sizeof(a));
   a.X = X; a.Y = Y; a.n = n_cap; a.n_dev = n_dev;
-  const int tiles = (n_cap + MLP_TM - 1) / MLP_TM;
-  const int grid = tiles < 2048 ? tiles : 2048;
-  hipLaunchKernelGGL((k_mlp_fwd<0>), dim3(grid), dim3(256), 0, stream, w, a);
+  const int tiles = (n_cap + MLP_TI - 1) / MLP_TI;
+  const int grid = tiles < 512 ? tiles : 512;   // two resident tiles on each of the 256 CUs
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, stream, w, a);
   return hipGetLastError() == hipSuccess ? M2C_OK : M2C_ERR_HIP;
 }
 

@@ -29,17 +29,22 @@ static_assert(mlp_boff(4) + MLP_OUT == MLP_NPARAM, "parameter layout");
 struct MlpWeights {
   const __bf16 *W[5];    // W[0] [256][32], W[1..3] [256][256], W[4] [16][256] (zero padding)
   const __bf16 *WT[5];   // WT[1..3] [256][256], WT[4] [256][32]: transposes for the backward pass
+  // WF[0..3]: W[l] in MFMA A-fragment order for the inference kernel: the fragment of feature tile
+  // ft (rows 16 ft .. +15) and K step ks (inputs 32 ks .. +31) is 64 lanes x 8 bf16 at
+  // ((ft * K/32 + ks) * 64 + lane) * 8, lane = row % 16 + 16 * (input % 32 / 8)
+  const __bf16 *WF[4];
   const float *bias[5];          // bias[4] padded to 16
 };
 
 struct MlpNet {
   float *master, *adam_m, *adam_v, *grad;   // flat FP32, MLP_NPARAM each
-  __bf16 *W[5], *WT[5];
+  __bf16 *W[5], *WT[5], *WF[4];
   float *bias[5];
   long long adam_t;
   MlpWeights view() const {
     MlpWeights w;
     for (int l = 0; l < 5; l++) { w.W[l] = W[l]; w.WT[l] = WT[l]; w.bias[l] = bias[l]; }
+    for (int l = 0; l < 4; l++) w.WF[l] = WF[l];
     return w;
   }
 };

@@ -51,6 +51,24 @@ def test_forward_matches_oracle():
     net.close()
 
 
+def test_forward_over_many_tiles_matches_oracle():
+    # more rows than the persistent grid's 512 tiles of 128 hold in one round, with a ragged tail
+    p = rand_params(7)
+    net = M.Mlp(seed=0)
+    net.set_params(p)
+    n = 512 * 128 + 4 * 128 + 37
+    X = np.random.default_rng(8).normal(size=(n, 10)).astype(np.float32)
+    Yg = net.forward(X)
+    Ye, _ = R.forward(p.astype(np.float64), X.astype(np.float64), emulate=True)
+    scale = 1.0 + np.abs(Ye).max()
+    err = np.abs(Yg - Ye).max()
+    print(f"  forward over {n} rows: max |gpu - oracle(bf16)| {err:.3e}")
+    assert np.all(np.isfinite(Yg)) and err <= 2e-4 * scale
+    # rows are independent: a row's output does not depend on its tile or neighbours
+    assert np.array_equal(net.forward(X[-300:]), Yg[-300:])
+    net.close()
+
+
 def test_gradient_matches_oracle():
     p = rand_params(3)
     net = M.Mlp(seed=0)
