@@ -381,7 +381,7 @@ def bench_learned(M, W, args):
                        "flop_per_row": MLP_FLOP_PER_SAMPLE},
             "roofline": {"bound": "mfma", "achieved": inf_tflops, "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": inf_tflops / BF16_MFMA_PEAK_TFLOPS, "traffic": None,
-                         "kernel": "m2c::k_mlp_fwd<0> (fused 5-layer MLP)", "avg_launch_ms": ms_inf},
+                         "kernel": "m2c::k_mlp_infer (fused 5-layer MLP, 128-row tiles)", "avg_launch_ms": ms_inf},
             "training": {"batch": B, "ms_per_step": ms_train, "samples_per_s": B / (ms_train * 1e-3),
                          "tflops_3x_forward": train_tflops},
             "interface_problems": {"n": nb, "exact_batch_ms": 1e3 * (t1 - t0), "network_batch_ms": 1e3 * (t2 - t1),

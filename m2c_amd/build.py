@@ -38,6 +38,10 @@ FP32_FAST = ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 # SIMD; the scheduler that maximises instruction-level parallelism shortens the two-material
 # launch by ~2 % (rocprofv3, 384x192x192 shock-bubble) and leaves the results unchanged.
 ILP_SCHED = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+# The mixed-form two-material unit runs three 256-thread blocks per CU (168 VGPRs, a few hundred
+# bytes of spills): 2.02 -> 1.96 ms per stage at the 384x192x192 shock-bubble. The single-gas FP32
+# unit is faster at two (2.20 against 2.35 ms at 256^3).
+BLOCKS3 = ["-DM2C_STAGE_BLOCKS=3"]
 # the batched runner: its host unit, and one unit per stage variant holding both tile shapes
 BATCH_VARIANTS = [("hllc_sg", 2, 0, "double", 0), ("hllc_gen", 2, -1, "double", 1), ("roe_gen", 0, -1, "double", 1),
                   ("llf_gen", 1, -1, "double", 1), ("hllc_sg_f32", 2, 0, "float", 0),
@@ -49,7 +53,8 @@ UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), (
                                         f"-DM2C_MP={mp}"] + (FP32_FAST if tc == "float" else []))
     for t, f, e, tc, mp in BATCH_VARIANTS] + [
     (f"stage_{t}", "stage_inst.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}"]
-     + (["-DM2C_MP=1"] if t in MP_VARIANTS else []) + (FP32_FAST if tc == "float" else []) + (ILP_SCHED if t in ("hllc_sg", "hllc_sg_mp") else []))
+     + (["-DM2C_MP=1"] if t in MP_VARIANTS else []) + (FP32_FAST if tc == "float" else []) + (ILP_SCHED if t in ("hllc_sg", "hllc_sg_mp") else [])
+     + (BLOCKS3 if t == "hllc_sg_mp_f32" else []))
     for t, f, e, tc in VARIANTS]
 # host-only C++ (no device code), compiled with the host compiler
 HOST_UNITS = [("input_deck", "input_deck.cpp"), ("output", "output.cpp"), ("mesh", "mesh.cpp")]
