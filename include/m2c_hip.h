@@ -274,6 +274,34 @@ int m2c_wall_face_flux_batch(m2c_ctx *ctx, int n, const int *axis, const int *si
  * M2C_ERR_UNSUPPORTED. Host arrays. */
 int m2c_set_embedded_faces(m2c_ctx *ctx, int n, const int *face, const double *wall, int normal_mode, int recon);
 
+/* One embedded surface as the embedded-boundary operator hands it to the fluid solver: the
+ * triangulated surface (TriangulatedSurface: node velocities Udot, triangles elems, unit normals
+ * elemNorm) and its intersection points with mesh edges (EmbeddedBoundaryDataSet.h:34-60: the
+ * triangle tid, barycentric coordinates xi within it, and dist, the distance from the edge's first
+ * node to the crossing). Host arrays. */
+typedef struct {
+  int nnodes, nelems, npoints;
+  const double *udot;        /* [nnodes][3] */
+  const int *elems;          /* [nelems][3] node indices */
+  const double *elem_norm;   /* [nelems][3] */
+  const int *point_tid;      /* [npoints] */
+  const double *point_xi;    /* [npoints][3] */
+  const double *point_dist;  /* [npoints] */
+} m2c_surface;
+
+/* The wall records of mesh faces, as SpaceOperator::FindEdgeSurfaceIntersections
+ * (SpaceOperator.cpp:3434-3500) derives them for ComputeAdvectionFluxes: for face t and surface s,
+ * xf[t*nsurf+s] and xb[t*nsurf+s] are the indices of the surface's first intersection point along the
+ * face's edge seen from its minus and from its plus cell (-1: none; the EmbeddedBoundaryDataSet's
+ * XForward / XBackward). Over the surfaces the forward crossing with the smallest dist and the
+ * backward one with the largest are taken (the first surface wins ties); each gives the triangle's
+ * normal and the barycentric interpolation of its nodes' velocities. cut[t] = 1 and wall[12t..12t+11]
+ * = nwallf, vwallf, nwallb, vwallb (m2c_set_embedded_faces' layout) for a face with a crossing;
+ * cut[t] = 0 and the record left alone without one. M2C_ERR_ARG for an index out of range and for a
+ * face with a crossing in one direction only (where the reference asserts). Host code. */
+int m2c_edge_surface_walls(int nsurf, const m2c_surface *surf, int nfaces, const int *xf, const int *xb, int *cut,
+                           double *wall);
+
 /* ---- many independent instances (batched) ------------------------------------------------
  * A batch holds n instances, each its own grid, boundaries, state and CFL-adaptive time step; one
  * flux, precision and RK2/RK3 scheme for all, and one material table (the materials an instance

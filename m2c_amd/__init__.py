@@ -182,6 +182,50 @@ def mesh_1d(x0, xmax, n=0, points=None):
     return x, w
 
 
+class Surface(C.Structure):
+    """m2c_surface: one embedded surface (triangles, normals, node velocities) and its edge crossings."""
+    _fields_ = [("nnodes", C.c_int), ("nelems", C.c_int), ("npoints", C.c_int),
+                ("udot", C.POINTER(C.c_double)), ("elems", C.POINTER(C.c_int)), ("elem_norm", C.POINTER(C.c_double)),
+                ("point_tid", C.POINTER(C.c_int)), ("point_xi", C.POINTER(C.c_double)),
+                ("point_dist", C.POINTER(C.c_double))]
+
+
+def _surface_structs(surfaces, struct):
+    """ctypes structs for dicts with udot [n][3], elems [m][3], elem_norm [m][3], tid [p], xi [p][3],
+    dist [p]; the arrays are kept alive in the returned list."""
+    keep, out = [], (struct * max(len(surfaces), 1))()
+    for i, sf in enumerate(surfaces):
+        a = dict(udot=np.ascontiguousarray(sf["udot"], dtype=np.float64).reshape(-1, 3),
+                 elems=np.ascontiguousarray(sf["elems"], dtype=np.int32).reshape(-1, 3),
+                 elem_norm=np.ascontiguousarray(sf["elem_norm"], dtype=np.float64).reshape(-1, 3),
+                 tid=np.ascontiguousarray(sf["tid"], dtype=np.int32).reshape(-1),
+                 xi=np.ascontiguousarray(sf["xi"], dtype=np.float64).reshape(-1, 3),
+                 dist=np.ascontiguousarray(sf["dist"], dtype=np.float64).reshape(-1))
+        keep.append(a)
+        D, I = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        out[i] = struct(len(a["udot"]), len(a["elems"]), len(a["tid"]), a["udot"].ctypes.data_as(D),
+                        a["elems"].ctypes.data_as(I), a["elem_norm"].ctypes.data_as(D), a["tid"].ctypes.data_as(I),
+                        a["xi"].ctypes.data_as(D), a["dist"].ctypes.data_as(D))
+    return out, keep
+
+
+def edge_surface_walls(surfaces, xf, xb):
+    """FindEdgeSurfaceIntersections over faces (m2c_edge_surface_walls): surfaces as dicts (see
+    _surface_structs), xf / xb [faces][surfaces] crossing indices (-1 none). Returns (cut [faces] bool,
+    walls [faces][12]: nwallf, vwallf, nwallb, vwallb; rows of uncut faces are zero)."""
+    xf = np.ascontiguousarray(xf, dtype=np.int32).reshape(-1, len(surfaces))
+    xb = np.ascontiguousarray(xb, dtype=np.int32).reshape(-1, len(surfaces))
+    n = len(xf)
+    arr, keep = _surface_structs(surfaces, Surface)
+    cut = np.zeros(n, np.int32)
+    wall = np.zeros((n, 12))
+    _chk(lib().m2c_edge_surface_walls(len(surfaces), arr, n, xf.ctypes.data_as(C.POINTER(C.c_int)),
+                                      xb.ctypes.data_as(C.POINTER(C.c_int)), cut.ctypes.data_as(C.POINTER(C.c_int)),
+                                      _dp(wall)), "m2c_edge_surface_walls")
+    del keep
+    return cut.astype(bool), wall
+
+
 _lib = None
 
 
@@ -227,6 +271,8 @@ def lib():
         L.m2c_batch_download.argtypes = [Pb, C.c_int, D, C.POINTER(C.c_int32)]
         L.m2c_batch_upload_levelset.argtypes = [Pb, C.c_int, D]
         L.m2c_mesh_1d.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, D, C.c_int, D, D, C.POINTER(C.c_int)]
+        L.m2c_edge_surface_walls.argtypes = [C.c_int, C.POINTER(Surface), C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int), C.POINTER(C.c_int), D]
         L.m2c_batch_download_levelset.argtypes = [Pb, C.c_int, D]
         L.m2c_batch_run.argtypes = [Pb, C.c_int, C.c_int, D]
         L.m2c_batch_status.argtypes = [Pb, C.c_int, C.POINTER(Status)]

@@ -57,7 +57,8 @@ UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), (
      + (BLOCKS3 if t == "hllc_sg_mp_f32" else []))
     for t, f, e, tc in VARIANTS]
 # host-only C++ (no device code), compiled with the host compiler
-HOST_UNITS = [("input_deck", "input_deck.cpp"), ("output", "output.cpp"), ("mesh", "mesh.cpp")]
+HOST_UNITS = [("input_deck", "input_deck.cpp"), ("output", "output.cpp"), ("mesh", "mesh.cpp"),
+              ("embedded_walls", "embedded_walls.cpp")]
 CXX = os.environ.get("CXX", "g++")
 
 

@@ -70,6 +70,24 @@ typedef struct {
 int orc_mesh_1d(double x0, double xmax, int n, int npts, const double *pts, int cap, double *x, double *dx,
                 int *nout);
 
+/* one embedded surface: node velocities, triangles, unit normals, and its edge crossings (triangle,
+ * barycentric coordinates, distance from the edge's first node) */
+typedef struct {
+  int nnodes, nelems, npoints;
+  const double *udot;
+  const int *elems;
+  const double *elem_norm;
+  const int *point_tid;
+  const double *point_xi;
+  const double *point_dist;
+} orc_surface;
+
+/* wall records of nfaces mesh faces from the forward / backward crossing indices xf, xb
+ * ([face][surface], -1 none): cut[t] and wall[12t..] = nwallf, vwallf, nwallb, vwallb (oracle/embedded_walls.c);
+ * -1 for an index out of range or a crossing seen from one side only */
+int orc_edge_surface_walls(int nsurf, const orc_surface *surf, int nfaces, const int *xf, const int *xb, int *cut,
+                           double *wall);
+
 typedef struct orc_ctx orc_ctx;
 
 orc_ctx *orc_create(const orc_config *cfg);

@@ -180,6 +180,7 @@ def lib():
             getattr(L, f).restype = C.c_double
         L.orc_counters.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.orc_mesh_1d.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, D, C.c_int, D, D, I]
+        L.orc_edge_surface_walls.argtypes = [C.c_int, C.c_void_p, C.c_int, I, I, I, D]
         _lib = L
     return _lib
 
@@ -197,6 +198,37 @@ def mesh_1d(x0, xmax, n=0, control_points=None):
     dx = np.zeros(cnt.value)
     L.orc_mesh_1d(x0, xmax, n, pts.size // 2, pp, cnt.value, _dp(x), _dp(dx), C.byref(cnt))
     return x, dx
+
+
+class Surface(C.Structure):
+    _fields_ = [("nnodes", C.c_int), ("nelems", C.c_int), ("npoints", C.c_int),
+                ("udot", C.POINTER(C.c_double)), ("elems", C.POINTER(C.c_int)), ("elem_norm", C.POINTER(C.c_double)),
+                ("point_tid", C.POINTER(C.c_int)), ("point_xi", C.POINTER(C.c_double)),
+                ("point_dist", C.POINTER(C.c_double))]
+
+
+def edge_surface_walls(surfaces, xf, xb):
+    """FindEdgeSurfaceIntersections restated (oracle/embedded_walls.c): surfaces as dicts with udot,
+    elems, elem_norm, tid, xi, dist; xf / xb [faces][surfaces]. Returns (cut bool, walls [faces][12])."""
+    keep = []
+    arr = (Surface * max(len(surfaces), 1))()
+    D, I = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    for i, sf in enumerate(surfaces):
+        a = [np.ascontiguousarray(sf["udot"], dtype=np.float64), np.ascontiguousarray(sf["elems"], dtype=np.int32),
+             np.ascontiguousarray(sf["elem_norm"], dtype=np.float64), np.ascontiguousarray(sf["tid"], dtype=np.int32),
+             np.ascontiguousarray(sf["xi"], dtype=np.float64), np.ascontiguousarray(sf["dist"], dtype=np.float64)]
+        keep.append(a)
+        arr[i] = Surface(a[0].size // 3, a[1].size // 3, a[3].size, a[0].ctypes.data_as(D), a[1].ctypes.data_as(I),
+                         a[2].ctypes.data_as(D), a[3].ctypes.data_as(I), a[4].ctypes.data_as(D), a[5].ctypes.data_as(D))
+    xf = np.ascontiguousarray(xf, dtype=np.int32).reshape(-1, len(surfaces))
+    xb = np.ascontiguousarray(xb, dtype=np.int32).reshape(-1, len(surfaces))
+    n = len(xf)
+    cut = np.zeros(n, np.int32)
+    wall = np.zeros((n, 12))
+    rc = lib().orc_edge_surface_walls(len(surfaces), C.cast(arr, C.c_void_p), n, _ip(xf), _ip(xb), _ip(cut), _dp(wall))
+    if rc != 0:
+        raise ValueError("orc_edge_surface_walls: index out of range or one-sided crossing")
+    return cut.astype(bool), wall
 
 
 def _dp(a):

@@ -116,3 +116,66 @@ def test_walls_removed_and_rejected():
     with pytest.raises(M.M2CError):
         s2.set_embedded_faces([[0, 4, 4, 4]], [np.r_[1.0, 0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 0]])
     s2.close()
+
+
+def piston_case(nx=40, ny=6, nz=5, xw=25.3, uw=-0.2):
+    """Gas left of a plane wall x = xw moving at uw along x (into the gas for uw < 0), the cells
+    beyond it inactive; the wall given as a triangulated surface (two triangles) and its crossings
+    with the x-edges between the last gas cell and the first solid cell of every row, as an
+    embedded-boundary operator hands them over (triangle, barycentric coordinates, distance from the
+    edge's first node)."""
+    gas = M.material(M.EOS_SG, [1.4, 0.0])
+    settings = dict(nx=nx, ny=ny, nz=nz, box=(0.0, float(nx), 0.0, float(ny), 0.0, float(nz)),
+                    bc=[M.BC_INLET, M.BC_OUTLET, M.BC_SLIPWALL, M.BC_SLIPWALL, M.BC_SYMMETRY, M.BC_SYMMETRY],
+                    mats=[gas, gas], inactive_id=1, inlet=(1.0, 0.0, 0.0, 0.0, 1.0), outlet=(1.0, 0.0, 0.0, 0.0, 1.0),
+                    flux=M.FLUX_HLLC, limiter=M.LIM_VANALBADA, ts=M.TS_RK2, cfl=0.5)
+    x = np.arange(nx) + 0.5
+    ID = np.broadcast_to((x > xw).astype(np.int32), (nz, ny, nx)).copy()
+    rng = np.random.default_rng(4)
+    V = np.zeros((nz, ny, nx, 5))
+    V[..., 0] = 1.0 + 0.02 * rng.uniform(-1, 1, (nz, ny, nx))
+    V[..., 4] = 1.0 + 0.02 * rng.uniform(-1, 1, (nz, ny, nx))
+    V[ID == 1, 1:4] = 0.0
+    nodes = np.array([[xw, 0.0, 0.0], [xw, ny, 0.0], [xw, ny, nz], [xw, 0.0, nz]])
+    elems = np.array([[0, 1, 2], [0, 2, 3]])
+    surf = dict(udot=np.tile([uw, 0.0, 0.0], (4, 1)), elems=elems, elem_norm=[[-1.0, 0.0, 0.0]] * 2,
+                tid=[], xi=[], dist=[])
+    i0 = int(np.argmax(x > xw))
+    faces, tid, xi, dist = [], [], [], []
+    for k in range(nz):
+        for j in range(ny):
+            p = np.array([j + 0.5, k + 0.5])
+            for t, e in enumerate(elems):
+                a, b, c = (nodes[e][:, 1:] - 0.0)
+                T = np.array([b - a, c - a]).T
+                l1, l2 = np.linalg.solve(T, p - a)
+                if l1 >= 0 and l2 >= 0 and l1 + l2 <= 1:
+                    tid.append(t)
+                    xi.append([1.0 - l1 - l2, l1, l2])
+                    break
+            dist.append(xw - x[i0 - 1])
+            faces.append([0, i0, j, k])
+    surf.update(tid=tid, xi=xi, dist=dist)
+    idx = np.arange(len(faces)).reshape(-1, 1)
+    return settings, V, ID, np.array(faces, np.int32), [surf], idx, idx
+
+
+def test_walls_from_surface_crossings_bitwise():
+    settings, V0, ID0, faces, surfaces, xf, xb = piston_case()
+    cut, walls = M.edge_surface_walls(surfaces, xf, xb)
+    co, wo = po.edge_surface_walls(surfaces, xf, xb)
+    assert cut.all() and np.array_equal(walls, wo)
+    assert np.allclose(walls[:, 3:6], [-0.2, 0.0, 0.0], rtol=0, atol=1e-15)
+    assert np.array_equal(walls[:, 0:3], np.tile([-1.0, 0.0, 0.0], (len(faces), 1)))
+    print(f"case: piston 40x6x5, {len(faces)} cut faces from surface crossings, 12 steps", flush=True)
+    s = solver_for(settings, faces, walls, 0, 1, V0, ID0)
+    st = s.run(1e300, 12, sync_every=0)
+    V, _ = s.download()
+    s.close()
+    o, Vo, IDo = oracle_for(settings, faces, walls, 0, 1, V0, ID0)
+    ost, t, n = o.run(Vo, IDo, 1e300, 12)
+    assert st.error == 0 and ost >= 0 and n == 12 and st.t == t
+    Vo = Vo[1:-1, 1:-1, 1:-1]
+    assert np.array_equal(V, Vo), f"max diff {np.max(np.abs(V - Vo))}"
+    # the moving wall compresses the gas next to it
+    assert V[:, :, 24, 0].mean() > 1.02
