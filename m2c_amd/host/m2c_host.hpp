@@ -23,6 +23,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/m2c_hip.h"
@@ -200,10 +201,40 @@ class SpaceOperator {
     check(m2c_set_embedded_faces(c_.get(), (int)(face.size() / 4), face.data(), wall.data(), riemann_normal, recon),
           "m2c_set_embedded_faces");
   }
+  // SpaceOperator::FindEdgeSurfaceIntersections over a list of faces: xf / xb hold, per face and
+  // surface, the index of the surface's forward / backward crossing (-1 none); cut and wall (12 per
+  // face, SetEmbeddedFaces' layout) are resized and filled. Host code, no device.
+  static void FindEdgeSurfaceIntersections(const std::vector<m2c_surface> &surfaces, const std::vector<int> &xf,
+                                           const std::vector<int> &xb, std::vector<int> &cut,
+                                           std::vector<double> &wall) {
+    const size_t ns = surfaces.size();
+    if (ns == 0 || xf.size() != xb.size() || xf.size() % ns)
+      throw std::runtime_error("FindEdgeSurfaceIntersections: xf and xb need one entry per face and surface");
+    const int n = (int)(xf.size() / ns);
+    cut.assign((size_t)n, 0);
+    wall.assign((size_t)n * 12, 0.0);
+    check(m2c_edge_surface_walls((int)ns, surfaces.data(), n, xf.data(), xb.data(), cut.data(), wall.data()),
+          "m2c_edge_surface_walls");
+  }
 
  private:
   Context &c_;
 };
+
+// MeshGenerator::ComputeMesh1DUniform / ComputeMesh1DNonUniform: the centres and widths of one axis
+// (n uniform cells, or the cells of (coordinate, width) control points). Host code.
+inline void ComputeMesh1D(double x0, double xmax, int n, const std::vector<std::pair<double, double>> &points,
+                          std::vector<double> &x, std::vector<double> &dx) {
+  std::vector<double> pts;
+  for (const auto &p : points) { pts.push_back(p.first); pts.push_back(p.second); }
+  int m = 0;
+  check(m2c_mesh_1d(x0, xmax, n, (int)points.size(), pts.empty() ? nullptr : pts.data(), 0, nullptr, nullptr, &m),
+        "m2c_mesh_1d");
+  x.assign((size_t)m, 0.0);
+  dx.assign((size_t)m, 0.0);
+  check(m2c_mesh_1d(x0, xmax, n, (int)points.size(), pts.empty() ? nullptr : pts.data(), m, x.data(), dx.data(), &m),
+        "m2c_mesh_1d");
+}
 
 class LevelSetOperator {
  public:
