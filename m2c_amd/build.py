@@ -47,7 +47,8 @@ BATCH_VARIANTS = [("hllc_sg", 2, 0, "double", 0), ("hllc_gen", 2, -1, "double", 
                   ("llf_gen", 1, -1, "double", 1), ("hllc_sg_f32", 2, 0, "float", 0),
                   ("hllc_gen_f32", 2, -1, "float", 1), ("hllc_sg_mp", 2, 0, "double", 1),
                   ("hllc_sg_mp_f32", 2, 0, "float", 1)]
-UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("batch", "batch.hip", []),
+UNITS = [("m2c_hip", "m2c_hip.hip", []), ("multiphase", "multiphase.hip", []), ("iface_multi", "iface_multi.hip", []),
+         ("batch", "batch.hip", []),
          ("mlp", "mlp.hip", [])] + [
     (f"bstage_{t}", "batch_stage.hip", [f"-DM2C_FLUX={f}", f"-DM2C_EK={e}", f"-DM2C_TAG={t}", f"-DM2C_TC={tc}",
                                         f"-DM2C_MP={mp}"] + (FP32_FAST if tc == "float" else []))

@@ -142,6 +142,9 @@ struct m2c_batch {
   std::vector<m2c_ctx *> mpc;
   std::vector<int> mp_list;
   int *d_mp_list = nullptr;
+  // their exact interface solves in one launch per stage (null: one launch per instance)
+  IfaceInst *d_iface = nullptr;
+  int n_iface = 0, iface_blocks = 0, iface_flux = -1;
   long long steps = 0;          // time steps taken (the reinitialisation schedule)
   // captured steps: every step (single-material batches) or, with two-material instances, a step
   // without reinitialisation, and the parts before and after the host's reinitialisation retries
@@ -239,8 +242,16 @@ static bool reinit_step(const m2c_batch *b, long long time_step) {
 // head of its end-of-step tasks (IDs of swept nodes, the first reinitialisation attempt)
 static int enqueue_batch_head(m2c_batch *b, long long time_step) {
   for (int s = 0; s < b->nstage; s++) {
-    for (int i : b->mp_list)
-      if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_before(b->mpc[i], s)) return M2C_ERR_HIP;
+    if (b->d_iface) {   // face lists per instance (constant-memory tables), then all their solves at once
+      for (int i : b->mp_list)
+        if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_before_list(b->mpc[i], s)) return M2C_ERR_HIP;
+      if (m2c_mp_launch_iface_multi(b->stream, b->d_iface, b->n_iface, b->iface_blocks, b->iface_flux, s == 0 ? 0 : 1, s,
+                                    s == 0 ? 1 : 0))
+        return M2C_ERR_HIP;
+    } else {
+      for (int i : b->mp_list)
+        if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_before(b->mpc[i], s)) return M2C_ERR_HIP;
+    }
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->q);
     const int in_cons = s == 0 ? 0 : 1;
     const int mode = s == b->nstage - 1 ? 2 : 1;
@@ -457,6 +468,29 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
       a2.S = U2; a2.Vn = V; a2.Us = U2; a2.Out = V; a2.a = 2.0 / 3.0; a2.b = 1.0 / 3.0; a2.c = 2.0 / 3.0;
     }
   }
+  if (anymp) {   // one multi-instance interface pass per stage when every instance can join it
+    std::vector<IfaceInst> tab;
+    int blocks = 0, flux = -1;
+    bool all = true;
+    for (int i : b->mp_list) {
+      IfaceInst t;
+      int ok = 0, fl = 0;
+      if (int rc = m2c_ctx_batch_iface_inst(b->mpc[i], &t, &ok, &fl)) { m2c_batch_destroy(b); return rc; }
+      if (!ok || (t.nblk > 0 && flux >= 0 && fl != flux)) { all = false; break; }
+      if (t.nblk == 0) continue;
+      flux = fl;
+      t.blk0 = blocks;
+      blocks += t.nblk;
+      tab.push_back(t);
+    }
+    if (all && !tab.empty()) {
+      BHIP_D(hipMalloc(&b->d_iface, sizeof(IfaceInst) * tab.size()));
+      BHIP_D(hipMemcpy(b->d_iface, tab.data(), sizeof(IfaceInst) * tab.size(), hipMemcpyHostToDevice));
+      b->n_iface = (int)tab.size();
+      b->iface_blocks = blocks;
+      b->iface_flux = flux;
+    }
+  }
   if (int rc = batch_upload_tables(b)) { m2c_batch_destroy(b); return rc; }
   if (int rc = batch_layout(b, 8, 0)) { m2c_batch_destroy(b); return rc; }
   b->nblocks = 512;
@@ -484,6 +518,7 @@ int m2c_batch_destroy(m2c_batch *b) {
   drop_graphs(b);
   hipFree(b->V); hipFree(b->U1); hipFree(b->U2); hipFree(b->stage_buf); hipFree(b->backup);
   hipFree(b->ID); hipFree(b->clk); hipFree(b->st); hipFree(b->dinst); hipFree(b->q); hipFree(b->d_mp_list);
+  hipFree(b->d_iface);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return M2C_OK;

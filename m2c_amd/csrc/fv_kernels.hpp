@@ -127,12 +127,22 @@ struct StageArgs {
 
 // Each translation unit that launches kernels holds its own copy of these tables (internal
 // linkage) and exports an upload function for it. The batched translation unit (M2C_BATCHED)
-// reads the grid of the instance a block is working on from LDS instead.
+// reads the grid of the instance a block is working on from LDS instead; the multi-instance
+// interface unit (M2C_IFACE_MULTI) reads grid and materials from LDS copies of its instance's.
+#ifdef M2C_IFACE_MULTI
+static __shared__ DevEos s_ieos[8];
+#define M2C_EOS s_ieos
+#else
 static __constant__ DevEos c_eos[8];
+#define M2C_EOS c_eos
+#endif
 #ifdef M2C_BATCHED
 static __shared__ Grid s_bgrid;
 #define M2C_G s_bgrid
 #define M2C_BATCHED_LAYOUT 1
+#elif defined(M2C_IFACE_MULTI)
+static __shared__ Grid s_igrid;
+#define M2C_G s_igrid
 #else
 static __constant__ Grid c_grid;
 #define M2C_G c_grid
@@ -145,9 +155,9 @@ static __constant__ Grid c_grid;
 template <int EK, bool MP> __device__ __forceinline__ decltype(auto) mat_of(int id) {
   if constexpr (EK == EOS_SG) {
     DevEos m{};
-    const DevEos &a = c_eos[0];
+    const DevEos &a = M2C_EOS[0];
     if constexpr (MP) {
-      const DevEos &b = c_eos[1];
+      const DevEos &b = M2C_EOS[1];
       const bool o = id != 0;
       m.rhomin = o ? b.rhomin : a.rhomin; m.pmin = o ? b.pmin : a.pmin;
       m.rhomax = o ? b.rhomax : a.rhomax; m.pmax = o ? b.pmax : a.pmax;
@@ -160,7 +170,7 @@ template <int EK, bool MP> __device__ __forceinline__ decltype(auto) mat_of(int 
     m.type = EOS_SG;
     return m;
   } else {
-    return (c_eos[id]);
+    return (M2C_EOS[id]);
   }
 }
 
@@ -379,7 +389,7 @@ __device__ __forceinline__ int face_flux(const TC *Vm, const TC *Vp, const TC *c
   if (MP && idm != idp && idm != g.inactive_id && idp != g.inactive_id) {
     if (g.mp_flux == 2) {   // LLF at the material interface (SpaceOperator.cpp:2493-2501)
       const TC *am = g.mp_recon ? Vm : cm, *ap = g.mp_recon ? Vp : cp;
-      flux_llf2<TC, -1, -1, DIR>(c_eos[idm], c_eos[idp], am, ap, F);
+      flux_llf2<TC, -1, -1, DIR>(M2C_EOS[idm], M2C_EOS[idp], am, ap, F);
       return 0;
     }
     return 1;

@@ -2192,6 +2192,44 @@ int m2c_ctx_batch_before(m2c_ctx *c, int s) {
   return stage_before(c, s, p, 0);
 }
 
+int m2c_ctx_batch_iface_inst(m2c_ctx *c, IfaceInst *out, int *ok, int *flux) {
+  if (!c || !out || !ok || !flux || !c->bound) return M2C_ERR_ARG;
+  memset(out, 0, sizeof(*out));
+  const MpLaunch &L = c->L;
+  *flux = L.flux;
+  // an instance without interfaces takes no part (nblk = 0)
+  *ok = !c->iface || (!L.sur_on && (L.flux == FLUX_HLLC || L.flux == FLUX_ROE || L.flux == FLUX_LLF) &&
+                      nstages(c) <= 3 && L.riemann_threads >= 64 && L.riemann_threads % 64 == 0);
+  out->grid = c->bound->g;
+  out->mp = c->bound->mp;
+  for (int m = 0; m < 8; m++) out->eos[m] = c->bound->eos[m];
+  for (int s = 0; s < nstages(c) && s < 3; s++) {
+    StageSpec p;
+    stage_spec(c, s, p);
+    out->S[s] = p.S;
+    if (p.in_cons != (s == 0 ? 0 : 1)) *ok = 0;
+  }
+  out->ID = L.ID;
+  out->faces = L.faces;
+  out->ms = L.ms;
+  out->Fi = L.Fi;
+  out->rsV = L.rsV;
+  out->rsMeta = L.rsMeta;
+  out->scratch = L.scratch;
+  out->st = L.st;
+  out->clk = L.clk;
+  out->phin = L.phin;
+  out->store = c->ls ? 1 : 0;
+  out->nblk = c->iface ? L.riemann_threads / 64 : 0;
+  return M2C_OK;
+}
+
+int m2c_ctx_batch_before_list(m2c_ctx *c, int s) {
+  if (s == 0 && (c->iface || c->ls) && mp_chk(m2c_mp_step_begin(c->L))) return M2C_ERR_HIP;
+  if (!c->iface) return M2C_OK;
+  return mp_chk(m2c_mp_launch_iface_list(c->L, 0, s == 0 ? 1 : 0));
+}
+
 int m2c_ctx_batch_after(m2c_ctx *c, int s) {
   StageSpec p;
   stage_spec(c, s, p);

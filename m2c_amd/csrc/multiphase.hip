@@ -33,22 +33,31 @@
 #include "sg_star.hpp"
 
 namespace m2c {
+#ifdef M2C_IFACE_MULTI
+namespace imulti {   // a second copy of the interface code: its own symbols
+#endif
 
+#ifdef M2C_IFACE_MULTI
+static __shared__ MpParams s_imp;
+#define M2C_MP s_imp
+#else
 static __constant__ MpParams c_mp;
+#define M2C_MP c_mp
+#endif
 
 // ---------------------------------------------------------------- helpers
 
 __device__ __forceinline__ long long cidx3(int i, int j, int k) {
-  return ((long long)k * c_grid.ny + j) * c_grid.nx + i;
+  return ((long long)k * M2C_G.ny + j) * M2C_G.nx + i;
 }
 
-__device__ __forceinline__ double XC(int d, int i) { return c_mp.xc[d][i + 1]; }
+__device__ __forceinline__ double XC(int d, int i) { return M2C_MP.xc[d][i + 1]; }
 
 // min(dx, dy, dz) of cell (i,j,k): the reinitialisation's pseudo-time step / cfl and the width of
 // its sign function (LevelSetReinitializer.cpp:600-603, 988-989)
 __device__ __forceinline__ double cell_min_width(int i, int j, int k) {
-  if (!c_mp.stretched) return c_mp.min_dx;
-  return smin(c_mp.wc[0][i + 1], smin(c_mp.wc[1][j + 1], c_mp.wc[2][k + 1]));
+  if (!M2C_MP.stretched) return M2C_MP.min_dx;
+  return smin(M2C_MP.wc[0][i + 1], smin(M2C_MP.wc[1][j + 1], M2C_MP.wc[2][k + 1]));
 }
 
 // wave-aggregated append; every lane of the wavefront must call it
@@ -72,7 +81,7 @@ __device__ __forceinline__ double bits_to_d(unsigned long long b) { return __lon
 
 // material ID of (possibly ghost) cell (i,j,k): the image cell's (MultiPhaseOperator.cpp:796-819)
 __device__ __forceinline__ int cell_id(const int8_t *ID, int i, int j, int k) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   int ii = clampi(image_idx(i, g.nx, i < 0 ? g.bc[0] == BC_PERIODIC : g.bc[1] == BC_PERIODIC), 0, g.nx - 1);
   int jj = clampi(image_idx(j, g.ny, j < 0 ? g.bc[2] == BC_PERIODIC : g.bc[3] == BC_PERIODIC), 0, g.ny - 1);
   int kk = clampi(image_idx(k, g.nz, k < 0 ? g.bc[4] == BC_PERIODIC : g.bc[5] == BC_PERIODIC), 0, g.nz - 1);
@@ -83,7 +92,7 @@ __device__ __forceinline__ int cell_id(const int8_t *ID, int i, int j, int k) {
 // LevelSetOperator.cpp:934-1033 (1 zero-Neumann, 2 linear extrapolation, 3 non-negative,
 // 7 periodic)
 __device__ inline double phi_at(const double *phi, int i, int j, int k) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   int dim, n, q;
   long long stride;
   if (i < 0 || i >= g.nx) { dim = 0; n = g.nx; q = i; stride = 1; }
@@ -93,7 +102,7 @@ __device__ inline double phi_at(const double *phi, int i, int j, int k) {
   // (i,j,k) is a ghost along `dim` only; c is its (virtual) linear index
   const long long c = cidx3(i, j, k);
   const int hi = q >= n;
-  const int bc = c_mp.ls_bc[2 * dim + hi];
+  const int bc = M2C_MP.ls_bc[2 * dim + hi];
   int im;
   if (bc == BC_PERIODIC) im = hi ? 0 : n - 1;
   else im = hi ? n - 1 : 0;
@@ -118,13 +127,13 @@ __device__ inline double phi_at(const double *phi, int i, int j, int k) {
 // phi at (i,j,k) for the interface normal: periodic axes are wrapped first, then at most one index
 // is outside the box and takes the face-ghost rule of phi_at
 __device__ inline double phi_nrm(const double *phi, int i, int j, int k) {
-  const Grid &g = c_grid;
-  if (i < 0 && c_mp.ls_bc[0] == BC_PERIODIC) i += g.nx;
-  else if (i >= g.nx && c_mp.ls_bc[1] == BC_PERIODIC) i -= g.nx;
-  if (j < 0 && c_mp.ls_bc[2] == BC_PERIODIC) j += g.ny;
-  else if (j >= g.ny && c_mp.ls_bc[3] == BC_PERIODIC) j -= g.ny;
-  if (k < 0 && c_mp.ls_bc[4] == BC_PERIODIC) k += g.nz;
-  else if (k >= g.nz && c_mp.ls_bc[5] == BC_PERIODIC) k -= g.nz;
+  const Grid &g = M2C_G;
+  if (i < 0 && M2C_MP.ls_bc[0] == BC_PERIODIC) i += g.nx;
+  else if (i >= g.nx && M2C_MP.ls_bc[1] == BC_PERIODIC) i -= g.nx;
+  if (j < 0 && M2C_MP.ls_bc[2] == BC_PERIODIC) j += g.ny;
+  else if (j >= g.ny && M2C_MP.ls_bc[3] == BC_PERIODIC) j -= g.ny;
+  if (k < 0 && M2C_MP.ls_bc[4] == BC_PERIODIC) k += g.nz;
+  else if (k >= g.nz && M2C_MP.ls_bc[5] == BC_PERIODIC) k -= g.nz;
   return phi_at(phi, i, j, k);
 }
 
@@ -142,7 +151,7 @@ __device__ __forceinline__ double cdiff_local(double p0, double p1, double p2, d
 // the ghost layer) in the two cells, interpolated to the face.
 template <int D>
 __device__ void grad_at_face(const double *phi, const int q[3], double *dir) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const int n[3] = {g.nx, g.ny, g.nz};
   int m[3] = {q[0], q[1], q[2]};
   m[D] -= 1;
@@ -152,7 +161,7 @@ __device__ void grad_at_face(const double *phi, const int q[3], double *dir) {
     return phi_nrm(phi, r[0], r[1], r[2]);
   };
   dir[D] = (P(q, D, 0) - P(m, D, 0)) / (XC(D, q[D]) - XC(D, m[D]));
-  const double x_half = XC(D, q[D]) - 0.5 * c_mp.wc[D][q[D] + 1];
+  const double x_half = XC(D, q[D]) - 0.5 * M2C_MP.wc[D][q[D] + 1];
   const double dd = XC(D, q[D]) - XC(D, m[D]);
   const double c1 = (XC(D, q[D]) - x_half) / dd;
   const double c2 = (x_half - XC(D, m[D])) / dd;
@@ -186,7 +195,7 @@ __device__ void grad_at_face(const double *phi, const int q[3], double *dir) {
 // and normalises. myid: the plus cell's material, nbid: the minus cell's.
 __device__ void interface_normal(int D, int i, int j, int k, int myid, int nbid, const double *phi, double *dir) {
   dir[0] = dir[1] = dir[2] = 0.0;
-  const int mode = c_mp.riemann_normal;
+  const int mode = M2C_MP.riemann_normal;
   if (mode == 1 || !phi) {
     dir[D] = 1.0;
     return;
@@ -207,7 +216,7 @@ __device__ void interface_normal(int D, int i, int j, int k, int myid, int nbid,
 
 // linear index of (i,j,k) with each coordinate clamped into the box
 __device__ __forceinline__ long long clamped_idx(int i, int j, int k) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   return cidx3(clampi(i, 0, g.nx - 1), clampi(j, 0, g.ny - 1), clampi(k, 0, g.nz - 1));
 }
 
@@ -239,7 +248,7 @@ __device__ __forceinline__ void block_max_to(unsigned long long *dst, double v, 
 // calls f(i, j, k, idx, inner) for this thread's cells; inner = all six face neighbours are in the box
 template <class F>
 __device__ __forceinline__ void sweep_cells(F &&f) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const int i = blockIdx.x * SWEEP_BX + threadIdx.x, j = blockIdx.y * SWEEP_BY + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
   const int k0 = blockIdx.z * SWEEP_KB, k1 = min(g.nz, k0 + SWEEP_KB);
@@ -259,12 +268,12 @@ __device__ __forceinline__ void sweep_cells(F &&f) {
 struct Nb6 { double im, ip, jm, jp, km, kp; };
 __device__ __forceinline__ Nb6 phi_nb6(const double *phi, int i, int j, int k, long long idx, bool inner) {
   Nb6 n;
-  if (c_mp.ls_neumann) {   // zero-Neumann ghosts are the clamped image cells
+  if (M2C_MP.ls_neumann) {   // zero-Neumann ghosts are the clamped image cells
     n.im = phi[clamped_idx(i - 1, j, k)]; n.ip = phi[clamped_idx(i + 1, j, k)];
     n.jm = phi[clamped_idx(i, j - 1, k)]; n.jp = phi[clamped_idx(i, j + 1, k)];
     n.km = phi[clamped_idx(i, j, k - 1)]; n.kp = phi[clamped_idx(i, j, k + 1)];
   } else if (inner) {
-    const long long sy = c_grid.nx, sz = (long long)c_grid.nx * c_grid.ny;
+    const long long sy = M2C_G.nx, sz = (long long)M2C_G.nx * M2C_G.ny;
     n.im = phi[idx - 1]; n.ip = phi[idx + 1];
     n.jm = phi[idx - sy]; n.jp = phi[idx + sy];
     n.km = phi[idx - sz]; n.kp = phi[idx + sz];
@@ -279,9 +288,9 @@ __device__ __forceinline__ Nb6 phi_nb6(const double *phi, int i, int j, int k, l
 // velocity of a cell of the stage field S (primitive, or conservative -> ConservativeToPrimitive)
 template <int IN_CONS>
 __device__ __forceinline__ void cell_velocity(const double *S, const int8_t *ID, long long idx, double *u) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   if (IN_CONS) {
-    const DevEos &m = c_eos[ID[idx]];
+    const DevEos &m = M2C_EOS[ID[idx]];
     if (m.type == EOS_DUMMY) { u[0] = u[1] = u[2] = 0.0; return; }
     double invRho = 1.0 / S[idx];
     u[0] = S[g.N + idx] * invRho;
@@ -309,7 +318,7 @@ __device__ __forceinline__ W4 fd_w(bool uni, const double *u, const double *tab)
 template <int NB>
 __device__ inline double ls_residual(const double *phi, int i, int j, int k, long long idx, bool inner2,
                                      const double *u) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const int NX = g.nx, NY = g.ny, NZ = g.nz;
   const long long sy = NX, sz = (long long)NX * NY;
   auto P = [&](int ii, int jj, int kk) -> double {
@@ -320,17 +329,17 @@ __device__ inline double ls_residual(const double *phi, int i, int j, int k, lon
   const double p0 = phi[idx];
   double a, b, cc, d, e, f;
   if (i - 2 >= -1) {
-    const W4 w = fd_w(inner2 && c_mp.fd_uni[0], c_mp.fdl_u[0], c_mp.fdl[0] + 4 * i);
+    const W4 w = fd_w(inner2 && M2C_MP.fd_uni[0], M2C_MP.fdl_u[0], M2C_MP.fdl[0] + 4 * i);
     a = w.w0 * P(i - 2, j, k) + w.w1 * P(i - 1, j, k) + w.w2 * p0 + w.w3 * P(i + 1, j, k);
   } else
     a = (p0 - P(i - 1, j, k)) / (XC(0, i) - XC(0, i - 1));
   if (i + 2 <= NX) {
-    const W4 w = fd_w(inner2 && c_mp.fd_uni[0], c_mp.fdr_u[0], c_mp.fdr[0] + 4 * i);
+    const W4 w = fd_w(inner2 && M2C_MP.fd_uni[0], M2C_MP.fdr_u[0], M2C_MP.fdr[0] + 4 * i);
     b = w.w0 * P(i - 1, j, k) + w.w1 * p0 + w.w2 * P(i + 1, j, k) + w.w3 * P(i + 2, j, k);
   } else
     b = (P(i + 1, j, k) - p0) / (XC(0, i + 1) - XC(0, i));
   if (j - 2 >= -1) {
-    const W4 w = fd_w(inner2 && c_mp.fd_uni[1], c_mp.fdl_u[1], c_mp.fdl[1] + 4 * j);
+    const W4 w = fd_w(inner2 && M2C_MP.fd_uni[1], M2C_MP.fdl_u[1], M2C_MP.fdl[1] + 4 * j);
     cc = w.w0 * P(i, j - 2, k) + w.w1 * P(i, j - 1, k) + w.w2 * p0 + w.w3 * P(i, j + 1, k);
   } else
     cc = (p0 - P(i, j - 1, k)) / (XC(1, j) - XC(1, j - 1));
@@ -338,18 +347,18 @@ __device__ inline double ls_residual(const double *phi, int i, int j, int k, lon
     // the right-biased y coefficients are third order only where the column index i < NY-1
     // (GradientCalculatorFD3.cpp:415 tests i, not j)
     const bool third = i < NY - 1 && j < NY - 1;
-    const W4 w = third ? fd_w(inner2 && c_mp.fd_uni[1], c_mp.fdr_u[1], c_mp.fdr[1] + 4 * j)
-                       : fd_w(inner2 && c_mp.fd_uni_r1y, c_mp.fdr1y_u, c_mp.fdr1y + 4 * j);
+    const W4 w = third ? fd_w(inner2 && M2C_MP.fd_uni[1], M2C_MP.fdr_u[1], M2C_MP.fdr[1] + 4 * j)
+                       : fd_w(inner2 && M2C_MP.fd_uni_r1y, M2C_MP.fdr1y_u, M2C_MP.fdr1y + 4 * j);
     d = w.w0 * P(i, j - 1, k) + w.w1 * p0 + w.w2 * P(i, j + 1, k) + w.w3 * P(i, j + 2, k);
   } else
     d = (P(i, j + 1, k) - p0) / (XC(1, j + 1) - XC(1, j));
   if (k - 2 >= -1) {
-    const W4 w = fd_w(inner2 && c_mp.fd_uni[2], c_mp.fdl_u[2], c_mp.fdl[2] + 4 * k);
+    const W4 w = fd_w(inner2 && M2C_MP.fd_uni[2], M2C_MP.fdl_u[2], M2C_MP.fdl[2] + 4 * k);
     e = w.w0 * P(i, j, k - 2) + w.w1 * P(i, j, k - 1) + w.w2 * p0 + w.w3 * P(i, j, k + 1);
   } else
     e = (p0 - P(i, j, k - 1)) / (XC(2, k) - XC(2, k - 1));
   if (k + 2 <= NZ) {
-    const W4 w = fd_w(inner2 && c_mp.fd_uni[2], c_mp.fdr_u[2], c_mp.fdr[2] + 4 * k);
+    const W4 w = fd_w(inner2 && M2C_MP.fd_uni[2], M2C_MP.fdr_u[2], M2C_MP.fdr[2] + 4 * k);
     f = w.w0 * P(i, j, k - 1) + w.w1 * p0 + w.w2 * P(i, j, k + 1) + w.w3 * P(i, j, k + 2);
   } else
     f = (P(i, j, k + 1) - p0) / (XC(2, k + 1) - XC(2, k));
@@ -385,7 +394,7 @@ __global__ void k_mp_fb_reset(MpState *ms) { ms->nfb = 0; }
 __global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *faces, MpState *ms, const Clock *clk,
                                                      int ignore_active) {
   if (!ignore_active && !step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   if (g.mp_flux == 2) return;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   const int i = blockIdx.x * SWEEP_BX + threadIdx.x, j = blockIdx.y * SWEEP_BY + threadIdx.y;
@@ -419,7 +428,7 @@ __global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *
 // both materials are stiffened gases with admissible states
 __device__ __forceinline__ bool sur_sides(const double *vm, int idm, const double *vp, int idp, int d, SgSide &L,
                                           SgSide &R) {
-  const DevEos &ml = c_eos[idm], &mr = c_eos[idp];
+  const DevEos &ml = M2C_EOS[idm], &mr = M2C_EOS[idp];
   if (ml.type != EOS_SG || mr.type != EOS_SG) return false;
   L.rho = vm[0]; L.u = vm[1 + d]; L.p = vm[4]; L.gam = ml.gam; L.pinf = ml.pstiff;
   R.rho = vp[0]; R.u = vp[1 + d]; R.p = vp[4]; R.gam = mr.gam; R.pinf = mr.pstiff;
@@ -472,7 +481,7 @@ __global__ __launch_bounds__(256) void k_iface_feat(const double *S, const int8_
                                                      const MpState *ms, float *X, int cap, const Clock *clk,
                                                      int ignore_active) {
   if (!ignore_active && !step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   const unsigned n = min(ms->nface, (unsigned)cap);
   for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
@@ -505,7 +514,7 @@ __global__ __launch_bounds__(256) void k_iface_feat(const double *S, const int8_
 template <int DIR, int IN_CONS>
 __device__ void face_states(const double *S, const int8_t *ID, int i, int j, int k, const double *vm1, int idm1,
                             const double *vp0, int idp0, double *fm, double *fp, const double *crphi) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
   const int gn = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
   const bool per = g.bc[2 * DIR] == BC_PERIODIC;
@@ -544,8 +553,8 @@ template <int FLUX, int DIR>
 __device__ __forceinline__ void iface_fluxes(RSolver &S, const double *vm, const double *Vsm, int idm,
                                              const double *Vsp, const double *vp, int idp, const double *Vmid,
                                              int midid, double *lf1, double *lf2) {
-  if (c_grid.mp_flux == 1) {   // EXACT: physical flux of the state at xi = 0 to both sides
-    phys_flux<double, -1, DIR>(c_eos[midid], Vmid, lf1);
+  if (M2C_G.mp_flux == 1) {   // EXACT: physical flux of the state at xi = 0 to both sides
+    phys_flux<double, -1, DIR>(M2C_EOS[midid], Vmid, lf1);
 #pragma unroll
     for (int p = 0; p < 5; p++) lf2[p] = lf1[p];
   } else if (FLUX == FLUX_GODUNOV) {   // the numerical flux is itself an exact solve per side
@@ -563,23 +572,25 @@ __device__ __forceinline__ void iface_fluxes(RSolver &S, const double *vm, const
 // network's checked solution where it is accepted (trivial faces and faces beyond the prediction
 // buffer go to the exact solver inline) and lists the rest in fbl; PASS 2 runs the exact solver on
 // that list, so waves of the expensive iterative solver hold only the faces that need it.
+// The pass over the listed faces as block blk of nblk blocks of 64 threads (k_riemann: the launch's
+// grid; the multi-instance pass: the blocks that serve one instance).
 template <int FLUX, int IN_CONS, int PASS>
-__global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *ID, const unsigned *faces,
-                                                 MpState *ms, double *Fi, double *rsV, unsigned long long *rsMeta,
-                                                 int store, double *scratch, Status *st, const Clock *clk,
-                                                 int ignore_active, const float *pred, int sur_cap, int sur_nmax,
-                                                 double sur_tol, unsigned *fbl, const double *phin) {
+__device__ __forceinline__ void riemann_faces(const double *S, const int8_t *ID, const unsigned *faces, MpState *ms,
+                                              double *Fi, double *rsV, unsigned long long *rsMeta, int store,
+                                              double *scratch, Status *st, const Clock *clk, int ignore_active,
+                                              const float *pred, int sur_cap, int sur_nmax, double sur_tol,
+                                              unsigned *fbl, const double *phin, const int nblk, const int blk) {
   if (!ignore_active && !step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-  const int T = gridDim.x * blockDim.x;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int T = nblk * blockDim.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const unsigned n = PASS == 2 ? ms->nfb : ms->nface;
   RSolver S_;
-  S_.mat = c_eos;
-  S_.cfg = c_mp.rcfg;
+  S_.mat = M2C_EOS;
+  S_.cfg = M2C_MP.rcfg;
   S_.overflow = 0;
-  const int cap = c_mp.path_cap;
+  const int cap = M2C_MP.path_cap;
   S_.path1.base = scratch + tid;
   S_.path1.cap = cap;
   S_.path1.stride = T;
@@ -606,9 +617,9 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
     fetch_cell<IN_CONS, -1>(S, ID, i - (dir == 0), j - (dir == 1), k - (dir == 2), vm, idm, fl);
     fetch_cell<IN_CONS, -1>(S, ID, i, j, k, vp, idp, fl);
     flags |= fl;
-    if (c_grid.mp_recon) {   // linear reconstruction at the interface: vr of the minus cell, vl of the plus cell
+    if (M2C_G.mp_recon) {   // linear reconstruction at the interface: vr of the minus cell, vl of the plus cell
       double fm[5], fp[5];
-      const double *crphi = c_grid.conrec_depth > 0.0 ? phin : nullptr;
+      const double *crphi = M2C_G.conrec_depth > 0.0 ? phin : nullptr;
       if (dir == 0) face_states<0, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, crphi);
       else if (dir == 1) face_states<1, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, crphi);
       else face_states<2, IN_CONS>(S, ID, i, j, k, vm, idm, vp, idp, fm, fp, crphi);
@@ -642,8 +653,8 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
     }
     if (err < 0) flags |= ST_BAD_STATE;
     if (err) nerr++;
-    eos_clip<double, -1>(c_eos[idm], Vsm);
-    eos_clip<double, -1>(c_eos[idp], Vsp);
+    eos_clip<double, -1>(M2C_EOS[idm], Vsm);
+    eos_clip<double, -1>(M2C_EOS[idp], Vsp);
     if (store && err == 0) {
 #pragma unroll
       for (int p = 0; p < 5; p++) {
@@ -671,6 +682,16 @@ __global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *I
   if (flags & (ST_BAD_STATE | ST_RIEMANN_PATH | ST_CLIP_INTERMEDIATE)) atomicOr(&st->bits, flags);
 }
 
+template <int FLUX, int IN_CONS, int PASS>
+__global__ __launch_bounds__(64) void k_riemann(const double *S, const int8_t *ID, const unsigned *faces,
+                                                 MpState *ms, double *Fi, double *rsV, unsigned long long *rsMeta,
+                                                 int store, double *scratch, Status *st, const Clock *clk,
+                                                 int ignore_active, const float *pred, int sur_cap, int sur_nmax,
+                                                 double sur_tol, unsigned *fbl, const double *phin) {
+  riemann_faces<FLUX, IN_CONS, PASS>(S, ID, faces, ms, Fi, rsV, rsMeta, store, scratch, st, clk, ignore_active, pred,
+                                     sur_cap, sur_nmax, sur_tol, fbl, phin, (int)gridDim.x, (int)blockIdx.x);
+}
+
 // ---------------------------------------------------------------- level-set stages
 
 // out = t + (coef*dt) R(phiS; velocity of S), with t = 0*out + 1*X (LS_FIRST), a*X + b*Y (LS_AXBY)
@@ -685,7 +706,7 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *S, const int8_t 
                                                    double coef, const double *phin, int8_t *IDnew, unsigned *rem,
                                                    MpState *ms, const Clock *clk) {
   if (!step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const double cdt = coef * clk->dt;
   sweep_cells([&](int i, int j, int k, long long idx, bool) {
     bool swept_out = false;
@@ -714,7 +735,7 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *S, const int8_t 
       int id = IDn[idx];
       int nid = id;
       if (id != g.inactive_id && !(v * pn > 0.0)) {
-        if (v < 0) nid = c_mp.ls_matid;
+        if (v < 0) nid = M2C_MP.ls_matid;
         else swept_out = true;
       }
       IDnew[idx] = (int8_t)nid;
@@ -759,7 +780,7 @@ __device__ void block_sort(unsigned long long *s, int n) {
 __global__ __launch_bounds__(1024) void k_id_remaining_seq(const unsigned *rem, MpState *ms, int8_t *IDnew,
                                                             uint8_t *tagR, Status *st, const Clock *clk) {
   if (!step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   __shared__ unsigned long long s[MP_SORT_CAP];
   __shared__ uint8_t gone[MP_SORT_CAP];
   const int n = (int)ms->nrem;
@@ -797,7 +818,7 @@ __global__ __launch_bounds__(1024) void k_id_remaining_seq(const unsigned *rem, 
               long long nb = cidx3(ni, nj, nk);
               if (tagR[nb]) continue;
               int nid = IDnew[nb];
-              if (nid == g.inactive_id || nid == c_mp.ls_matid) continue;
+              if (nid == g.inactive_id || nid == M2C_MP.ls_matid) continue;
               if (best < 0 || nid < best) best = nid;
             }
         if (best < 0) continue;
@@ -831,7 +852,7 @@ __global__ void k_reinit_begin(MpState *ms, const Clock *clk, int retry) {
   ReinitState &R = ms->rs;
   if (!retry) {
     int ts = clk->steps + 1;
-    R.active = step_active(clk) && c_mp.reinit_freq > 0 && ts % c_mp.reinit_freq == 0;
+    R.active = step_active(clk) && M2C_MP.reinit_freq > 0 && ts % M2C_MP.reinit_freq == 0;
     R.detected = 0;
   }
   R.retry = 0;
@@ -840,7 +861,7 @@ __global__ void k_reinit_begin(MpState *ms, const Clock *clk, int retry) {
   R.accepted = 0;
   R.acc_res_bits = 0ull;
   R.acc_dphi_bits = 0ull;
-  R.iter = c_mp.reinit_maxits;
+  R.iter = M2C_MP.reinit_maxits;
   R.maxres_bits = 0ull;
   R.dphi_bits = 0x7FF0000000000000ull;   // +inf
 }
@@ -871,7 +892,7 @@ __global__ __launch_bounds__(256) void k_reinit_tag(const double *phi, uint8_t *
 }
 
 __global__ void k_reinit_restore(double *phi, const double *phibk, const MpState *ms) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
        idx += (long long)gridDim.x * blockDim.x)
     phi[idx] = 0.0 * phi[idx] + 1.0 * phibk[idx];
@@ -888,11 +909,11 @@ __device__ __forceinline__ T reinit_rhs_t(T p, const NbT<T> &n, int tg, T sg, in
   if (tg != 0) return T(0);
   const T dt = cfl * T(cell_min_width(i, j, k));
   T a, b, c, d, e, f;
-  if (c_mp.dx_pow2[0]) { a = (p - n.im) * T(c_mp.inv_dx[0]); b = (n.ip - p) * T(c_mp.inv_dx[0]); }
+  if (M2C_MP.dx_pow2[0]) { a = (p - n.im) * T(M2C_MP.inv_dx[0]); b = (n.ip - p) * T(M2C_MP.inv_dx[0]); }
   else { a = (p - n.im) / T(XC(0, i) - XC(0, i - 1)); b = (n.ip - p) / T(XC(0, i + 1) - XC(0, i)); }
-  if (c_mp.dx_pow2[1]) { c = (p - n.jm) * T(c_mp.inv_dx[1]); d = (n.jp - p) * T(c_mp.inv_dx[1]); }
+  if (M2C_MP.dx_pow2[1]) { c = (p - n.jm) * T(M2C_MP.inv_dx[1]); d = (n.jp - p) * T(M2C_MP.inv_dx[1]); }
   else { c = (p - n.jm) / T(XC(1, j) - XC(1, j - 1)); d = (n.jp - p) / T(XC(1, j + 1) - XC(1, j)); }
-  if (c_mp.dx_pow2[2]) { e = (p - n.km) * T(c_mp.inv_dx[2]); f = (n.kp - p) * T(c_mp.inv_dx[2]); }
+  if (M2C_MP.dx_pow2[2]) { e = (p - n.km) * T(M2C_MP.inv_dx[2]); f = (n.kp - p) * T(M2C_MP.inv_dx[2]); }
   else { e = (p - n.km) / T(XC(2, k) - XC(2, k - 1)); f = (n.kp - p) / T(XC(2, k + 1) - XC(2, k)); }
   T ap = smax(a, T(0)), am = smin(a, T(0)), bp = smax(b, T(0)), bm = smin(b, T(0));
   T cp = smax(c, T(0)), cm = smin(c, T(0)), dp = smax(d, T(0)), dm = smin(d, T(0));
@@ -944,7 +965,7 @@ __global__ __launch_bounds__(RF_NT) void k_reinit_fused(T *__restrict__ P0, T *_
                                                          const T *__restrict__ sign, MpState *ms) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || Rs.done) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const T *__restrict__ in = Rs.cur ? P1 : P0;
   T *__restrict__ out = Rs.cur ? P0 : P1;
   const T cfl = T(Rs.cfl);
@@ -1108,20 +1129,20 @@ __global__ void k_reinit_decide(MpState *ms, int it) {
   const double res = bits_to_d(Rs.acc_res_bits), dphi = bits_to_d(Rs.acc_dphi_bits);
   Rs.acc_res_bits = 0ull;
   Rs.acc_dphi_bits = 0ull;
-  if (res < c_mp.reinit_tol) { Rs.done = 1; Rs.iter = it; return; }
+  if (res < M2C_MP.reinit_tol) { Rs.done = 1; Rs.iter = it; return; }
   Rs.cur ^= 1;
   Rs.accepted += 1;
   Rs.dphi_bits = __double_as_longlong(dphi);
-  if (dphi < c_mp.reinit_tol) { Rs.done = 1; Rs.iter = it; }
+  if (dphi < M2C_MP.reinit_tol) { Rs.done = 1; Rs.iter = it; }
 }
 
 // the failsafe of LevelSetReinitializer.cpp:144-169 after the fused iterations
 __global__ void k_reinit_end_fused(MpState *ms, Status *st) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected) return;
-  if (!Rs.done) Rs.iter = c_mp.reinit_maxits;
+  if (!Rs.done) Rs.iter = M2C_MP.reinit_maxits;
   ms->reinit_iters += Rs.iter;
-  if (Rs.iter < c_mp.reinit_maxits) return;
+  if (Rs.iter < M2C_MP.reinit_maxits) return;
   if (bits_to_d(Rs.dphi_bits) < 0.02) return;
   if (Rs.cfl < 0.02) {
     atomicOr(&st->bits, (unsigned)ST_REINIT_FAIL);
@@ -1135,7 +1156,7 @@ __global__ void k_reinit_end_fused(MpState *ms, Status *st) {
 __global__ void k_reinit_to_f32(const double *phi, float *phif, const double *sign, float *signf, const MpState *ms) {
   const ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
        idx += (long long)gridDim.x * blockDim.x) {
     phif[idx] = (float)phi[idx];
@@ -1148,7 +1169,7 @@ __global__ void k_reinit_settle_f32(double *phi, const float *phif0, const float
   const ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || !Rs.accepted) return;
   const float *src = Rs.cur ? phif1 : phif0;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
        idx += (long long)gridDim.x * blockDim.x)
     phi[idx] = (double)src[idx];
@@ -1158,7 +1179,7 @@ __global__ void k_reinit_settle_f32(double *phi, const float *phif0, const float
 __global__ void k_reinit_settle(double *phi, const double *P1, const MpState *ms) {
   const ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || !Rs.cur) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.N;
        idx += (long long)gridDim.x * blockDim.x)
     phi[idx] = P1[idx];
@@ -1170,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_reinit_A(const double *__restrict__ phi
                                                    const double *__restrict__ sign, MpState *ms, int it) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || Rs.done) return;
-  if (it > 0 && bits_to_d(Rs.dphi_bits) < c_mp.reinit_tol) {   // converged after iteration it-1
+  if (it > 0 && bits_to_d(Rs.dphi_bits) < M2C_MP.reinit_tol) {   // converged after iteration it-1
     if (first_thread()) { Rs.done = 1; Rs.iter = it - 1; }
     return;
   }
@@ -1194,7 +1215,7 @@ __global__ __launch_bounds__(256) void k_reinit_B(const double *__restrict__ phi
                                                    const double *__restrict__ sign, MpState *ms, int it) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected || Rs.done) return;
-  if (bits_to_d(Rs.maxres_bits) < c_mp.reinit_tol) {
+  if (bits_to_d(Rs.maxres_bits) < M2C_MP.reinit_tol) {
     if (first_thread()) { Rs.done = 1; Rs.iter = it; }
     return;
   }
@@ -1237,11 +1258,11 @@ __global__ void k_reinit_end(MpState *ms, Status *st) {
   ReinitState &Rs = ms->rs;
   if (!Rs.active || !Rs.detected) return;
   if (!Rs.done) {
-    if (bits_to_d(Rs.dphi_bits) < c_mp.reinit_tol) Rs.iter = c_mp.reinit_maxits - 1;
-    else Rs.iter = c_mp.reinit_maxits;
+    if (bits_to_d(Rs.dphi_bits) < M2C_MP.reinit_tol) Rs.iter = M2C_MP.reinit_maxits - 1;
+    else Rs.iter = M2C_MP.reinit_maxits;
   }
   ms->reinit_iters += Rs.iter;
-  if (Rs.iter < c_mp.reinit_maxits) return;
+  if (Rs.iter < M2C_MP.reinit_maxits) return;
   if (bits_to_d(Rs.dphi_bits) < 0.02) return;
   if (Rs.cfl < 0.02) {
     atomicOr(&st->bits, (unsigned)ST_REINIT_FAIL);
@@ -1256,7 +1277,7 @@ __global__ void k_reinit_end(MpState *ms, Status *st) {
 __global__ __launch_bounds__(256) void k_changed_list(const int8_t *IDn, const int8_t *IDnew, unsigned *chg,
                                                        int *cidx, MpState *ms, const Clock *clk) {
   if (!step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < g.N; base += (long long)gridDim.x * blockDim.x) {
     long long idx = base + threadIdx.x;
     bool pred = idx < g.N && IDnew[idx] != IDn[idx] && IDnew[idx] != g.inactive_id;
@@ -1287,7 +1308,7 @@ struct NbView {
 __device__ int local_update(const NbView &nv, const double *rsV, const unsigned long long *rsMeta,
                             unsigned long long stamp, int i, int j, int k, int id, bool upwind, double *v, int round,
                             bool *dep) {
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   int counter = 0;
   double sum_weight = 0.0;
@@ -1357,15 +1378,15 @@ __device__ int local_update(const NbView &nv, const double *rsV, const unsigned 
 
 __device__ __forceinline__ void load_v(const double *V, long long idx, double *v) {
 #pragma unroll
-  for (int p = 0; p < 5; p++) v[p] = V[p * c_grid.N + idx];
+  for (int p = 0; p < 5; p++) v[p] = V[p * M2C_G.N + idx];
 }
 
 // upwinded pass, then the pass without upwinding (MultiPhaseOperator.cpp:881-892)
 __device__ int update_cell(const NbView &nv, const double *rsV, const unsigned long long *rsMeta,
                            unsigned long long stamp, long long idx, int id, double *v, int round, bool *dep) {
-  const int i = (int)(idx % c_grid.nx);
-  const long long r = idx / c_grid.nx;
-  const int j = (int)(r % c_grid.ny), k = (int)(r / c_grid.ny);
+  const int i = (int)(idx % M2C_G.nx);
+  const long long r = idx / M2C_G.nx;
+  const int j = (int)(r % M2C_G.ny), k = (int)(r / M2C_G.ny);
   load_v(nv.V, idx, v);
   int cnt = local_update(nv, rsV, rsMeta, stamp, i, j, k, id, true, v, round, dep);
   if (cnt == -2) return -2;
@@ -1469,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_state_commit(double *V, const unsigned 
       const long long idx = chg[e];
       if (cnt[e] > 0) {
 #pragma unroll
-        for (int p = 0; p < 5; p++) V[p * c_grid.N + idx] = Vnew[(long long)5 * e + p];
+        for (int p = 0; p < 5; p++) V[p * M2C_G.N + idx] = Vnew[(long long)5 * e + p];
       } else {
         un = true;
       }
@@ -1485,7 +1506,7 @@ __global__ __launch_bounds__(1024) void k_fix_unresolved(double *V, const int8_t
                                                           const int8_t *cnt, const unsigned *unres, MpState *ms,
                                                           Status *st, const Clock *clk) {
   if (!step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   __shared__ unsigned long long s[MP_SORT_CAP];
   const int n = (int)ms->nunres;
   if (n == 0) return;
@@ -1561,7 +1582,7 @@ __global__ __launch_bounds__(1024) void k_fix_unresolved(double *V, const int8_t
             }
         if (sum_weight > 0) { v[0] = density / sum_weight; break; }
       }
-      if (sum_weight == 0) v[0] = c_mp.failsafe_density[myid];
+      if (sum_weight == 0) v[0] = M2C_MP.failsafe_density[myid];
     }
     for (int p = 0; p < 5; p++) V[p * g.N + me] = v[p];
   }
@@ -1572,7 +1593,7 @@ __global__ __launch_bounds__(1024) void k_fix_unresolved(double *V, const int8_t
 __global__ __launch_bounds__(256) void k_state_finish(double *V, int8_t *IDn, const int8_t *IDnew, const unsigned *chg,
                                                        int *cidx, MpState *ms, Status *st, const Clock *clk) {
   if (!step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const unsigned n = ms->nchg;
   unsigned flags = 0;
   unsigned long long nclip = 0;
@@ -1581,7 +1602,7 @@ __global__ __launch_bounds__(256) void k_state_finish(double *V, int8_t *IDn, co
     const int id = IDnew[idx];
     double v[5];
     load_v(V, idx, v);
-    const DevEos &m = c_eos[id];
+    const DevEos &m = M2C_EOS[id];
     if (eos_clip<double, -1>(m, v)) nclip++;
     if (eos_bad<double, -1>(m, v)) flags |= ST_BAD_STATE;
 #pragma unroll
@@ -1602,15 +1623,15 @@ __global__ __launch_bounds__(64) void k_riemann_batch(int n, const double *dir, 
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= nthreads) return;
   RSolver S_;
-  S_.mat = c_eos;
-  S_.cfg = c_mp.rcfg;
+  S_.mat = M2C_EOS;
+  S_.cfg = M2C_MP.rcfg;
   S_.overflow = 0;
-  const int cap = c_mp.path_cap;
+  const int cap = M2C_MP.path_cap;
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
   S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
   S_.path3.n = 0;
   for (int t = tid; t < n; t += nthreads) {
-    if (idm[t] < 0 || idm[t] >= c_grid.nmat || idp[t] < 0 || idp[t] >= c_grid.nmat) {
+    if (idm[t] < 0 || idm[t] >= M2C_G.nmat || idp[t] < 0 || idp[t] >= M2C_G.nmat) {
       // material index outside the table (callers passing device arrays are not checked on the host)
 #pragma unroll
       for (int q = 0; q < 5; q++) Vs[5 * t + q] = Vsm[5 * t + q] = Vsp[5 * t + q] = 0.0;
@@ -1641,10 +1662,10 @@ __global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir,
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= nthreads) return;
   RSolver S_;
-  S_.mat = c_eos;
-  S_.cfg = c_mp.rcfg;
+  S_.mat = M2C_EOS;
+  S_.cfg = M2C_MP.rcfg;
   S_.overflow = 0;
-  const int cap = c_mp.path_cap;
+  const int cap = M2C_MP.path_cap;
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
   S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
   S_.path3.n = 0;
@@ -1660,7 +1681,7 @@ __global__ __launch_bounds__(64) void k_onesided_batch(int n, const double *dir,
     if (FACE) {   // an invalid input state (r < 0) leaves the flux at zero
       double f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (r >= 0) {
-        eos_clip<double, -1>(c_eos[m], vsm);
+        eos_clip<double, -1>(M2C_EOS[m], vsm);
         const bool minus = fb[t] > 0;
         const double *a = minus ? v : vsm, *b = minus ? vsm : v;
         if (FLUX == FLUX_GODUNOV) {
@@ -1698,7 +1719,7 @@ __device__ __forceinline__ int wall_side(RSolver &S, int fb, int mode, const dou
 #pragma unroll
   for (int p = 0; p < 5; p++) F[p] = 0.0;
   if (r >= 0) {
-    eos_clip<double, -1>(c_eos[m], vsm);
+    eos_clip<double, -1>(M2C_EOS[m], vsm);
     const double *a = fb > 0 ? v : vsm, *b = fb > 0 ? vsm : v;
     if (FLUX == FLUX_GODUNOV) godunov_flux<DIR>(S, a, b, m, F);
     else numerical_flux<FLUX, -1, DIR>(a, b, m, F);
@@ -1720,14 +1741,14 @@ __global__ __launch_bounds__(64) void k_wall_faces(const double *S, const int8_t
                                                    long long Nf, double *scratch, Status *st, const Clock *clk,
                                                    int ignore_active) {
   if (!ignore_active && !step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const int T = gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   RSolver S_;
-  S_.mat = c_eos;
-  S_.cfg = c_mp.rcfg;
+  S_.mat = M2C_EOS;
+  S_.cfg = M2C_MP.rcfg;
   S_.overflow = 0;
-  const int cap = c_mp.path_cap;
+  const int cap = M2C_MP.path_cap;
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = T; S_.path1.n = 0;
   S_.path3.base = scratch + (long long)3 * cap * T + tid; S_.path3.cap = cap; S_.path3.stride = T; S_.path3.n = 0;
   unsigned flags = 0;
@@ -1791,14 +1812,14 @@ __global__ __launch_bounds__(64) void k_godunov_faces(const double *S, const int
                                                        double *scratch, Status *st, const Clock *clk,
                                                        int ignore_active, const double *crphi) {
   if (!ignore_active && !step_active(clk)) return;
-  const Grid &g = c_grid;
+  const Grid &g = M2C_G;
   const int T = gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   RSolver S_;
-  S_.mat = c_eos;
-  S_.cfg = c_mp.rcfg;
+  S_.mat = M2C_EOS;
+  S_.cfg = M2C_MP.rcfg;
   S_.overflow = 0;
-  const int cap = c_mp.path_cap;
+  const int cap = M2C_MP.path_cap;
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = T; S_.path1.n = 0;
   S_.path3.base = scratch + (long long)3 * cap * T + tid; S_.path3.cap = cap; S_.path3.stride = T; S_.path3.n = 0;
   constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
@@ -1834,10 +1855,10 @@ __global__ __launch_bounds__(64) void k_godunov_batch(int n, const double *Vm, c
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= nthreads) return;
   RSolver S_;
-  S_.mat = c_eos;
-  S_.cfg = c_mp.rcfg;
+  S_.mat = M2C_EOS;
+  S_.cfg = M2C_MP.rcfg;
   S_.overflow = 0;
-  const int cap = c_mp.path_cap;
+  const int cap = M2C_MP.path_cap;
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
   S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
   S_.path3.n = 0;
@@ -1891,10 +1912,10 @@ __global__ __launch_bounds__(64) void k_sur_batch(int n, const int *dir, const d
   }
   const int nl = (int)*nfb;
   RSolver S_;
-  S_.mat = c_eos;
-  S_.cfg = c_mp.rcfg;
+  S_.mat = M2C_EOS;
+  S_.cfg = M2C_MP.rcfg;
   S_.overflow = 0;
-  const int cap = c_mp.path_cap;
+  const int cap = M2C_MP.path_cap;
   S_.path1.base = scratch + tid; S_.path1.cap = cap; S_.path1.stride = nthreads; S_.path1.n = 0;
   S_.path3.base = scratch + (long long)3 * cap * nthreads + tid; S_.path3.cap = cap; S_.path3.stride = nthreads;
   S_.path3.n = 0;
@@ -1913,7 +1934,60 @@ __global__ __launch_bounds__(64) void k_sur_batch(int n, const int *dir, const d
 
 static int hchk(hipError_t e) { return e == hipSuccess ? 0 : 2; }
 
+#ifdef M2C_IFACE_MULTI
+// ---------------------------------------------------------------- multi-instance interface pass
+//
+// Each block finds its instance, copies the instance's grid, multiphase parameters and materials
+// into the LDS tables the interface code reads in this unit, and runs the instance's pass as one
+// of its nblk blocks (so a thread solves the faces, and uses the scratch slice, it would in the
+// instance's own launch of nblk blocks).
+template <typename T> __device__ __forceinline__ void lds_copy(T *dst, const T *src) {
+  static_assert(sizeof(T) % 4 == 0, "word copy");
+  unsigned *d = reinterpret_cast<unsigned *>(dst);
+  const unsigned *q = reinterpret_cast<const unsigned *>(src);
+  for (int w = threadIdx.x; w < (int)(sizeof(T) / 4); w += blockDim.x) d[w] = q[w];
+}
+
+template <int FLUX, int IN_CONS>
+__global__ __launch_bounds__(64) void k_riemann_multi(const IfaceInst *tab, int ninst, int stage, int store) {
+  int inst = 0;
+  while (inst + 1 < ninst && (int)blockIdx.x >= tab[inst + 1].blk0) inst++;
+  const IfaceInst &I = tab[inst];
+  lds_copy(&s_igrid, &I.grid);
+  lds_copy(&s_imp, &I.mp);
+  for (int m = 0; m < 8; m++) lds_copy(&s_ieos[m], &I.eos[m]);
+  __syncthreads();
+  const int blk = (int)blockIdx.x - I.blk0;
+  if (blk < 0 || blk >= I.nblk) return;
+  riemann_faces<FLUX, IN_CONS, 0>(I.S[stage], I.ID, I.faces, I.ms, I.Fi, I.rsV, I.rsMeta, store && I.store, I.scratch,
+                                  I.st, I.clk, 0, nullptr, 0, 0, 0.0, nullptr, I.phin, I.nblk, blk);
+}
+#endif
+
+#ifdef M2C_IFACE_MULTI
+}  // namespace imulti
+#endif
 }  // namespace m2c
+
+#ifdef M2C_IFACE_MULTI
+using namespace m2c;
+using namespace m2c::imulti;
+
+extern "C" int m2c_mp_launch_iface_multi(hipStream_t s, const IfaceInst *dtab, int ninst, int nblk_total, int flux,
+                                         int in_cons, int stage, int store) {
+  if (ninst <= 0 || nblk_total <= 0 || stage < 0 || stage > 2) return 1;
+  const dim3 gr(nblk_total), bl(64);
+#define M2C_RM(F)                                                                                               \
+  if (in_cons) hipLaunchKernelGGL((k_riemann_multi<F, 1>), gr, bl, 0, s, dtab, ninst, stage, store);           \
+  else hipLaunchKernelGGL((k_riemann_multi<F, 0>), gr, bl, 0, s, dtab, ninst, stage, store)
+  if (flux == FLUX_HLLC) { M2C_RM(FLUX_HLLC); }
+  else if (flux == FLUX_ROE) { M2C_RM(FLUX_ROE); }
+  else if (flux == FLUX_LLF) { M2C_RM(FLUX_LLF); }
+  else return 1;
+#undef M2C_RM
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+#else
 
 using namespace m2c;
 
@@ -1943,8 +2017,7 @@ static int nblocks(long long n, int bs, int cap) {
   return (int)b;
 }
 
-extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_cons, int store, int ignore_active,
-                                   int relist) {
+extern "C" int m2c_mp_launch_iface_list(const MpLaunch &L, int ignore_active, int relist) {
   if (relist) {
     hipLaunchKernelGGL(k_mp_face_reset, dim3(1), dim3(1), 0, L.stream, L.ms);
     hipLaunchKernelGGL(k_iface_list, sweep_grid(L.nx + 1, L.ny + 1, L.nz + 1), dim3(SWEEP_BX, SWEEP_BY), 0, L.stream,
@@ -1952,6 +2025,12 @@ extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_co
   } else {
     hipLaunchKernelGGL(k_mp_fb_reset, dim3(1), dim3(1), 0, L.stream, L.ms);
   }
+  return hchk(hipGetLastError());
+}
+
+extern "C" int m2c_mp_launch_iface(const MpLaunch &L, const double *S, int in_cons, int store, int ignore_active,
+                                   int relist) {
+  if (int rc = m2c_mp_launch_iface_list(L, ignore_active, relist)) return rc;
   dim3 gr(L.riemann_threads / 64), bl(64);
   const float *pred = nullptr;
   if (L.sur_on) {   // network inputs and predictions of the listed faces (count read on the device)
@@ -2170,3 +2249,4 @@ extern "C" int m2c_mp_launch_state_update(const MpLaunch &L, double *V, int roun
   hipLaunchKernelGGL(k_state_finish, grc, bl, 0, L.stream, V, L.ID, L.IDnew, L.chg, L.cidx, L.ms, L.st, L.clk);
   return hchk(hipGetLastError());
 }
+#endif  // M2C_IFACE_MULTI

@@ -93,9 +93,38 @@ struct MpLaunch {
   MlpWeights sur_w;
 };
 
+// One instance of the multi-instance interface pass (several contexts' exact interface solves in
+// one launch, each block serving one instance): the tables a context's own pass reads from
+// constant memory, and its buffers. Stage s reads S[s] (primitive for s = 0, conservative after).
+struct IfaceInst {
+  Grid grid;
+  MpParams mp;
+  DevEos eos[8];
+  const double *S[3];
+  const int8_t *ID;
+  const unsigned *faces;
+  MpState *ms;
+  double *Fi, *rsV;
+  unsigned long long *rsMeta;
+  double *scratch;
+  Status *st;
+  const Clock *clk;
+  const double *phin;
+  int store;                   // keeps the Riemann solutions (a level set follows the interface)
+  int blk0, nblk;              // the launch's blocks blk0 .. blk0 + nblk - 1 serve this instance
+};
+
 }  // namespace m2c
 
 extern "C" {
+// the multi-instance interface pass over ninst instances (dtab: device copy) at stage `stage`
+// (in_cons: its input is conservative), nblk_total blocks of 64 threads; `store`: the step's first
+// stage (instances with a level set keep their Riemann solutions)
+// the interface-face list of a stage (relist: build it; else keep the step's list and reset the
+// fallback count), the first half of m2c_mp_launch_iface
+int m2c_mp_launch_iface_list(const m2c::MpLaunch &L, int ignore_active, int relist);
+int m2c_mp_launch_iface_multi(hipStream_t s, const m2c::IfaceInst *dtab, int ninst, int nblk_total, int flux,
+                              int in_cons, int stage, int store);
 int m2c_mp_launch_sur_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
                             const int *idp, float *X, float *Y, const m2c::MlpWeights &w, int nmax, double tol,
                             double *Vs, int *id, double *Vsm, double *Vsp, int *status, int *path, double *scratch,
