@@ -144,7 +144,7 @@ struct m2c_batch {
   int *d_mp_list = nullptr;
   // their exact interface solves in one launch per stage (null: one launch per instance)
   IfaceInst *d_iface = nullptr;
-  int n_iface = 0, iface_blocks = 0, iface_flux = -1;
+  int n_iface = 0, iface_blocks = 0, list_blocks = 0, iface_flux = -1;
   long long steps = 0;          // time steps taken (the reinitialisation schedule)
   // captured steps: every step (single-material batches) or, with two-material instances, a step
   // without reinitialisation, and the parts before and after the host's reinitialisation retries
@@ -242,9 +242,11 @@ static bool reinit_step(const m2c_batch *b, long long time_step) {
 // head of its end-of-step tasks (IDs of swept nodes, the first reinitialisation attempt)
 static int enqueue_batch_head(m2c_batch *b, long long time_step) {
   for (int s = 0; s < b->nstage; s++) {
-    if (b->d_iface) {   // face lists per instance (constant-memory tables), then all their solves at once
+    if (b->d_iface) {   // every instance's face list, then every instance's interface solves, one launch each
       for (int i : b->mp_list)
-        if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_before_list(b->mpc[i], s)) return M2C_ERR_HIP;
+        if (s == 0 && m2c_ctx_batch_step_begin(b->mpc[i])) return M2C_ERR_HIP;
+      if (m2c_mp_launch_iface_list_multi(b->stream, b->d_iface, b->n_iface, b->list_blocks, s == 0 ? 1 : 0))
+        return M2C_ERR_HIP;
       if (m2c_mp_launch_iface_multi(b->stream, b->d_iface, b->n_iface, b->iface_blocks, b->iface_flux, s == 0 ? 0 : 1, s,
                                     s == 0 ? 1 : 0))
         return M2C_ERR_HIP;
@@ -470,7 +472,7 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
   }
   if (anymp) {   // one multi-instance interface pass per stage when every instance can join it
     std::vector<IfaceInst> tab;
-    int blocks = 0, flux = -1;
+    int blocks = 0, lblocks = 0, flux = -1;
     bool all = true;
     for (int i : b->mp_list) {
       IfaceInst t;
@@ -481,6 +483,8 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
       flux = fl;
       t.blk0 = blocks;
       blocks += t.nblk;
+      t.lblk0 = lblocks;
+      lblocks += t.lnblk;
       tab.push_back(t);
     }
     if (all && !tab.empty()) {
@@ -488,6 +492,7 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
       BHIP_D(hipMemcpy(b->d_iface, tab.data(), sizeof(IfaceInst) * tab.size(), hipMemcpyHostToDevice));
       b->n_iface = (int)tab.size();
       b->iface_blocks = blocks;
+      b->list_blocks = lblocks;
       b->iface_flux = flux;
     }
   }

@@ -25,8 +25,8 @@ int m2c_ctx_batch_before(m2c_ctx *c, int s);
 // the instance's part of a multi-instance interface pass: *ok = 0 when its pass cannot join one
 // (the learned solver, the Godunov flux); *flux its flux
 int m2c_ctx_batch_iface_inst(m2c_ctx *c, m2c::IfaceInst *out, int *ok, int *flux);
-// stage s's work before the fluid stage except the interface Riemann solves (step start, face list)
-int m2c_ctx_batch_before_list(m2c_ctx *c, int s);
+// the step's start of the instance's interface and level-set counters (its first stage)
+int m2c_ctx_batch_step_begin(m2c_ctx *c);
 int m2c_ctx_batch_after(m2c_ctx *c, int s);
 int m2c_ctx_batch_end_head(m2c_ctx *c, long long time_step);
 int m2c_ctx_batch_end_retries(m2c_ctx *c, long long time_step);

@@ -2221,13 +2221,13 @@ int m2c_ctx_batch_iface_inst(m2c_ctx *c, IfaceInst *out, int *ok, int *flux) {
   out->phin = L.phin;
   out->store = c->ls ? 1 : 0;
   out->nblk = c->iface ? L.riemann_threads / 64 : 0;
+  out->lnblk = m2c_mp_iface_list_blocks(c->grid.nx, c->grid.ny, c->grid.nz);
   return M2C_OK;
 }
 
-int m2c_ctx_batch_before_list(m2c_ctx *c, int s) {
-  if (s == 0 && (c->iface || c->ls) && mp_chk(m2c_mp_step_begin(c->L))) return M2C_ERR_HIP;
-  if (!c->iface) return M2C_OK;
-  return mp_chk(m2c_mp_launch_iface_list(c->L, 0, s == 0 ? 1 : 0));
+int m2c_ctx_batch_step_begin(m2c_ctx *c) {
+  if ((c->iface || c->ls) && mp_chk(m2c_mp_step_begin(c->L))) return M2C_ERR_HIP;
+  return M2C_OK;
 }
 
 int m2c_ctx_batch_after(m2c_ctx *c, int s) {

@@ -391,19 +391,19 @@ __global__ void k_mp_fb_reset(MpState *ms) { ms->nfb = 0; }
 // sides; code = dir*Nf + face_key
 // One thread per (i,j) column of the box extended by one cell on the high sides, walking k; cell
 // (i,j,k) owns the three faces on its low sides.
-__global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *faces, MpState *ms, const Clock *clk,
-                                                     int ignore_active) {
+__device__ __forceinline__ void iface_list_block(const int8_t *ID, unsigned *faces, MpState *ms, const Clock *clk,
+                                                 int ignore_active, int bx, int by, int bz) {
   if (!ignore_active && !step_active(clk)) return;
   const Grid &g = M2C_G;
   if (g.mp_flux == 2) return;
   const long long Nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-  const int i = blockIdx.x * SWEEP_BX + threadIdx.x, j = blockIdx.y * SWEEP_BY + threadIdx.y;
+  const int i = bx * SWEEP_BX + threadIdx.x, j = by * SWEEP_BY + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
   auto id_at = [&](int a, int b, int c) -> int {
     if (a >= 0 && a < g.nx && b >= 0 && b < g.ny && c >= 0 && c < g.nz) return ID[cidx3(a, b, c)];
     return cell_id(ID, a, b, c);
   };
-  const int k0 = blockIdx.z * SWEEP_KB, k1 = min(g.nz + 1, k0 + SWEEP_KB);
+  const int k0 = bz * SWEEP_KB, k1 = min(g.nz + 1, k0 + SWEEP_KB);
   for (int k = k0; k < k1; k++) {
     const long long key = ((long long)k * (g.ny + 1) + j) * (g.nx + 1) + i;
     const int idp = id_at(i, j, k);
@@ -420,6 +420,11 @@ __global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *
       if (pred) faces[slot] = (unsigned)(dir * Nf + key);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_iface_list(const int8_t *ID, unsigned *faces, MpState *ms, const Clock *clk,
+                                                     int ignore_active) {
+  iface_list_block(ID, faces, ms, clk, ignore_active, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
 // ---------------------------------------------------------------- learned interface solver
@@ -1948,6 +1953,28 @@ template <typename T> __device__ __forceinline__ void lds_copy(T *dst, const T *
   for (int w = threadIdx.x; w < (int)(sizeof(T) / 4); w += blockDim.x) d[w] = q[w];
 }
 
+// the face lists of every instance: counters reset (one thread per instance), then the list sweep,
+// block b of the launch serving the instance whose list blocks hold it
+__global__ void k_iface_reset_multi(const IfaceInst *tab, int ninst, int relist) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ninst) return;
+  MpState *ms = tab[t].ms;
+  if (relist) ms->nface = 0;
+  ms->nfb = 0;
+}
+
+__global__ __launch_bounds__(256) void k_iface_list_multi(const IfaceInst *tab, int ninst) {
+  int inst = 0;
+  while (inst + 1 < ninst && (int)blockIdx.x >= tab[inst + 1].lblk0) inst++;
+  const IfaceInst &I = tab[inst];
+  lds_copy(&s_igrid, &I.grid);
+  __syncthreads();
+  const int b = (int)blockIdx.x - I.lblk0;
+  if (b < 0 || b >= I.lnblk) return;
+  const int nbx = (s_igrid.nx + SWEEP_BX) / SWEEP_BX, nby = (s_igrid.ny + SWEEP_BY) / SWEEP_BY;
+  iface_list_block(I.ID, I.faces, I.ms, I.clk, 0, b % nbx, (b / nbx) % nby, b / (nbx * nby));
+}
+
 template <int FLUX, int IN_CONS>
 __global__ __launch_bounds__(64) void k_riemann_multi(const IfaceInst *tab, int ninst, int stage, int store) {
   int inst = 0;
@@ -1972,6 +1999,15 @@ __global__ __launch_bounds__(64) void k_riemann_multi(const IfaceInst *tab, int 
 #ifdef M2C_IFACE_MULTI
 using namespace m2c;
 using namespace m2c::imulti;
+
+extern "C" int m2c_mp_launch_iface_list_multi(hipStream_t s, const IfaceInst *dtab, int ninst, int nlist_blocks,
+                                              int relist) {
+  if (ninst <= 0 || nlist_blocks <= 0) return 1;
+  hipLaunchKernelGGL(k_iface_reset_multi, dim3((ninst + 63) / 64), dim3(64), 0, s, dtab, ninst, relist);
+  if (relist)
+    hipLaunchKernelGGL(k_iface_list_multi, dim3(nlist_blocks), dim3(SWEEP_BX, SWEEP_BY), 0, s, dtab, ninst);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
 
 extern "C" int m2c_mp_launch_iface_multi(hipStream_t s, const IfaceInst *dtab, int ninst, int nblk_total, int flux,
                                          int in_cons, int stage, int store) {
@@ -2015,6 +2051,11 @@ static int nblocks(long long n, int bs, int cap) {
   if (b < 1) b = 1;
   if (b > cap) b = cap;
   return (int)b;
+}
+
+extern "C" int m2c_mp_iface_list_blocks(int nx, int ny, int nz) {
+  const dim3 g = sweep_grid(nx + 1, ny + 1, nz + 1);
+  return (int)(g.x * g.y * g.z);
 }
 
 extern "C" int m2c_mp_launch_iface_list(const MpLaunch &L, int ignore_active, int relist) {

@@ -102,7 +102,7 @@ struct IfaceInst {
   DevEos eos[8];
   const double *S[3];
   const int8_t *ID;
-  const unsigned *faces;
+  unsigned *faces;
   MpState *ms;
   double *Fi, *rsV;
   unsigned long long *rsMeta;
@@ -112,6 +112,7 @@ struct IfaceInst {
   const double *phin;
   int store;                   // keeps the Riemann solutions (a level set follows the interface)
   int blk0, nblk;              // the launch's blocks blk0 .. blk0 + nblk - 1 serve this instance
+  int lblk0, lnblk;            // likewise in the face-list launch (m2c_mp_iface_list_blocks of its grid)
 };
 
 }  // namespace m2c
@@ -123,6 +124,11 @@ extern "C" {
 // the interface-face list of a stage (relist: build it; else keep the step's list and reset the
 // fallback count), the first half of m2c_mp_launch_iface
 int m2c_mp_launch_iface_list(const m2c::MpLaunch &L, int ignore_active, int relist);
+// blocks of one instance's face-list sweep (grid nx x ny x nz)
+int m2c_mp_iface_list_blocks(int nx, int ny, int nz);
+// the face lists of the ninst instances of dtab (relist: rebuilt; else only the fallback counts
+// reset), nlist_blocks blocks in all
+int m2c_mp_launch_iface_list_multi(hipStream_t s, const m2c::IfaceInst *dtab, int ninst, int nlist_blocks, int relist);
 int m2c_mp_launch_iface_multi(hipStream_t s, const m2c::IfaceInst *dtab, int ninst, int nblk_total, int flux,
                               int in_cons, int stage, int store);
 int m2c_mp_launch_sur_batch(hipStream_t s, int n, const int *dir, const double *Vm, const int *idm, const double *Vp,
