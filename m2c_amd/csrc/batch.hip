@@ -136,6 +136,12 @@ struct m2c_batch {
   Status *st = nullptr;
   BatchInst *dinst = nullptr;
   TileQueue *q = nullptr;
+  // with two-material instances: the single-material instances' stage launch apart, with the
+  // variant without the material-interface code (variant_s >= 0), each group its own instance
+  // array (tiles numbered within the group) and queue
+  int variant_s = -1, n_s = 0, n_m = 0;
+  BatchInst *dinst_s = nullptr, *dinst_m = nullptr;
+  TileQueue *q_s = nullptr, *q_m = nullptr;
   unsigned ntiles = 0;
   int kchunk = 8, nblocks = 512, shape = 0;   // shape: 0 = 32 x 8 column tiles, 1 = 16 x 16
   // two-material instances: their contexts (null for a single-material instance) and their indices
@@ -205,17 +211,40 @@ static int batch_layout(m2c_batch *b, int kchunk, int shape) {
   }
   BHIP(hipMemcpy(b->dinst, b->hinst.data(), sizeof(BatchInst) * b->n, hipMemcpyHostToDevice));
   BHIP(hipMemcpy(b->q, &hq, sizeof(hq), hipMemcpyHostToDevice));
+  if (b->variant_s >= 0) {   // the two groups' arrays and queues
+    for (int grp = 0; grp < 2; grp++) {
+      std::vector<BatchInst> h;
+      unsigned tg = 0;
+      for (int i = 0; i < b->n; i++) {
+        if ((b->mpc[i] != nullptr) != (grp == 1)) continue;
+        BatchInst B = b->hinst[i];
+        B.tile0 = tg;
+        tg += (unsigned)(B.gx * B.gy * B.nzc);
+        h.push_back(B);
+      }
+      TileQueue gq;
+      for (int x = 0; x < NXCD; x++) {
+        gq.start[x] = (unsigned)((unsigned long long)tg * x / NXCD);
+        gq.end[x] = (unsigned)((unsigned long long)tg * (x + 1) / NXCD);
+        gq.head[x] = gq.start[x];
+      }
+      BHIP(hipMemcpy(grp ? b->dinst_m : b->dinst_s, h.data(), sizeof(BatchInst) * h.size(), hipMemcpyHostToDevice));
+      BHIP(hipMemcpy(grp ? b->q_m : b->q_s, &gq, sizeof(gq), hipMemcpyHostToDevice));
+    }
+  }
   drop_graphs(b);
   return M2C_OK;
 }
 
-static int launch_stage(m2c_batch *b, int in_cons, int mode, int stage) {
+static int launch_variant(m2c_batch *b, int variant, const BatchInst *dinst, int n, TileQueue *q, int in_cons,
+                          int mode, int stage) {
   // the single-instance context's variants: Roe and LLF run the generic-EOS kernels; 9 and 10 are
   // the stiffened-gas variants with the material-interface code (two-material instances)
   const int sh = b->shape;
   int rc;
-  switch (b->variant) {
-#define M2C_BL(tag) m2c_bstage_launch_##tag(sh, in_cons, mode, b->nblocks, b->stream, b->dinst, b->n, b->q, stage)
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, q);
+  switch (variant) {
+#define M2C_BL(tag) m2c_bstage_launch_##tag(sh, in_cons, mode, b->nblocks, b->stream, dinst, n, q, stage)
   case 0: rc = M2C_BL(hllc_sg); break;
   case 1: rc = M2C_BL(hllc_gen); break;
   case 2: rc = M2C_BL(roe_gen); break;
@@ -227,6 +256,12 @@ static int launch_stage(m2c_batch *b, int in_cons, int mode, int stage) {
 #undef M2C_BL
   }
   return rc ? M2C_ERR_HIP : M2C_OK;
+}
+
+static int launch_stage(m2c_batch *b, int in_cons, int mode, int stage) {
+  if (b->variant_s < 0) return launch_variant(b, b->variant, b->dinst, b->n, b->q, in_cons, mode, stage);
+  if (int rc = launch_variant(b, b->variant_s, b->dinst_s, b->n_s, b->q_s, in_cons, mode, stage)) return rc;
+  return launch_variant(b, b->variant, b->dinst_m, b->n_m, b->q_m, in_cons, mode, stage);
 }
 
 static bool reinit_step(const m2c_batch *b, long long time_step) {
@@ -254,7 +289,6 @@ static int enqueue_batch_head(m2c_batch *b, long long time_step) {
       for (int i : b->mp_list)
         if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_before(b->mpc[i], s)) return M2C_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->q);
     const int in_cons = s == 0 ? 0 : 1;
     const int mode = s == b->nstage - 1 ? 2 : 1;
     if (int rc = launch_stage(b, in_cons, mode, s)) return rc;
@@ -385,7 +419,7 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
     if (b->hinst[i].grid.has_interfaces || c.ls_matid >= 0) b->mp_list.push_back(i);
     if (c.nmat > nmat_max) { nmat_max = c.nmat; memcpy(b->eos, eos, sizeof(eos)); }
   }
-  // with two-material instances every instance runs the variant with the material-interface code
+  // with two-material instances the batch's variant is the one with the material-interface code
   // (the stiffened-gas ones: 9 for FP64, 10 for the mixed form; the run-time-EOS variants have it)
   const bool anymp = !b->mp_list.empty();
   for (int i = 0; i < n; i++) {
@@ -397,6 +431,11 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
   }
   // HLLC (0, 1, 9), Roe (2) and LLF (3) in FP64, HLLC in the mixed form (4, 5, 10)
   if (b->variant < 0 || (b->variant > 5 && b->variant != 9 && b->variant != 10)) { delete b; return M2C_ERR_UNSUPPORTED; }
+  // the single-material instances of a batch with two-material ones keep the stiffened-gas variant
+  // without the interface code in a launch of their own (the interface code costs them ~40 %)
+  b->n_m = (int)b->mp_list.size();
+  b->n_s = n - b->n_m;
+  if (anymp && b->n_s > 0 && (b->variant == 9 || b->variant == 10)) b->variant_s = b->variant == 9 ? 0 : 4;
   std::vector<char> two(n, 0);
   for (int i : b->mp_list) two[i] = 1;
   for (int i = 0; i < n; i++) {
@@ -427,6 +466,12 @@ int m2c_batch_create(const m2c_config *cfgs, int n, m2c_batch **out) {
   BHIP_D(hipMemset(b->st, 0, sizeof(Status) * n));
   BHIP_D(hipMalloc(&b->dinst, sizeof(BatchInst) * n));
   BHIP_D(hipMalloc(&b->q, sizeof(TileQueue)));
+  if (b->variant_s >= 0) {
+    BHIP_D(hipMalloc(&b->dinst_s, sizeof(BatchInst) * b->n_s));
+    BHIP_D(hipMalloc(&b->dinst_m, sizeof(BatchInst) * b->n_m));
+    BHIP_D(hipMalloc(&b->q_s, sizeof(TileQueue)));
+    BHIP_D(hipMalloc(&b->q_m, sizeof(TileQueue)));
+  }
   BHIP_D(hipMalloc(&b->d_mp_list, sizeof(int) * std::max<size_t>(1, b->mp_list.size())));
   if (anymp)
     BHIP_D(hipMemcpy(b->d_mp_list, b->mp_list.data(), sizeof(int) * b->mp_list.size(), hipMemcpyHostToDevice));
@@ -524,6 +569,7 @@ int m2c_batch_destroy(m2c_batch *b) {
   hipFree(b->V); hipFree(b->U1); hipFree(b->U2); hipFree(b->stage_buf); hipFree(b->backup);
   hipFree(b->ID); hipFree(b->clk); hipFree(b->st); hipFree(b->dinst); hipFree(b->q); hipFree(b->d_mp_list);
   hipFree(b->d_iface);
+  hipFree(b->dinst_s); hipFree(b->dinst_m); hipFree(b->q_s); hipFree(b->q_m);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return M2C_OK;
