@@ -306,7 +306,7 @@ static int enqueue_batch_tail(m2c_batch *b) {
   for (int i : b->mp_list)
     if (m2c_ctx_batch_bind(b->mpc[i]) || m2c_ctx_batch_end_tail(b->mpc[i])) return M2C_ERR_HIP;
   if (!b->mp_list.empty())
-    hipLaunchKernelGGL(k_batch_cfl, dim3(64, (unsigned)b->mp_list.size()), dim3(256), 0, b->stream, b->dinst,
+    hipLaunchKernelGGL(k_batch_cfl, dim3(256, (unsigned)b->mp_list.size()), dim3(256), 0, b->stream, b->dinst,
                        b->d_mp_list, b->clk, b->st);
   hipLaunchKernelGGL(k_batch_clock_finalize, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->clk, b->n, 1);
   return M2C_OK;
@@ -320,7 +320,7 @@ static int batch_retries(m2c_batch *b, long long time_step) {
 
 static void enqueue_batch_cfl(m2c_batch *b, int maxsteps_more) {
   hipLaunchKernelGGL(k_batch_clock_set, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->clk, b->n, maxsteps_more);
-  hipLaunchKernelGGL(k_batch_cfl, dim3(64, b->n), dim3(256), 0, b->stream, b->dinst, (const int *)nullptr, b->clk,
+  hipLaunchKernelGGL(k_batch_cfl, dim3(256, b->n), dim3(256), 0, b->stream, b->dinst, (const int *)nullptr, b->clk,
                      b->st);
   hipLaunchKernelGGL(k_batch_clock_finalize, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->clk, b->n, 0);
 }
