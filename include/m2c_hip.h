@@ -308,8 +308,9 @@ int m2c_edge_surface_walls(int nsurf, const m2c_surface *surf, int nfaces, const
  * defines are the leading entries of the first instance's); the flux HLLC, or Roe or LLF in FP64,
  * else M2C_ERR_UNSUPPORTED. An instance is single-material, or two-material: materials meeting at
  * interface faces (exact Riemann solves there), with a level set (advection, reinitialisation,
- * material-ID and state update of swept cells). Every RK stage of every instance runs in one
- * persistent launch draining per-XCD tile queues; the two-material instances' interface passes,
+ * material-ID and state update of swept cells). Every RK stage runs as persistent launches draining
+ * per-XCD tile queues: one for all instances, or, when two-material instances are present, one for
+ * the single-material instances and one for the two-material ones; the latter's interface passes,
  * level-set stages and end-of-step tasks run around those launches on the batch's stream. A time
  * step is captured once in a hipGraph and replayed (with two-material instances, a
  * reinitialisation step is replayed as two graphs with the host's check of the reinitialisation
